@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""MEMD throughput / latency benchmark on one MI355X.
+
+Metric: MEMD epochs/s with p50/p99 latency on 32 ch x 4096 samples, K = 64
+directions, the reference's schedule (9 output slots, 10 sifts per IMF, fix
+mode, reference extrema window), FP64.  A step is one batched launch sequence
+that decomposes `--batch` independent synthetic epochs already resident in HBM;
+`value` is epochs per second over the timed steps.  The plain per-epoch path
+(one memd_run_f64 call per epoch) and the FP64 CPU baseline (the host oracle on
+this machine's cores, a bounded sample) are reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--batch B]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "memd-gpu_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64, help="epochs per step")
+    ap.add_argument("--plain-epochs", type=int, default=12)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--T", type=int, default=4096)
+    ap.add_argument("--K", type=int, default=64)
+    ap.add_argument("--n-imf", type=int, default=9)
+    ap.add_argument("--sifts", type=int, default=10)
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    return ap.parse_args()
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, float), q)) if len(xs) else None
+
+
+def cpu_baseline(n, T, K, n_imf, S, budget_s):
+    """The FP64 oracle (OpenMP over channels, bit-identical to its serial path)
+    on the same workload, whole epochs, until the budget is spent."""
+    from oracle import memd_oracle
+    import synth
+    threads = max(1, min(16, os.cpu_count() or 1))
+    V = memd_oracle.directions(n, K)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        memd_oracle.memd(synth.sines_epoch(n, T, 5000 + done), V, n_imf, S, threads=threads, trace=False)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 8:
+            break
+    return {"value": done / el, "unit": "epochs/s", "cores": threads, "kind": "port",
+            "sample": f"{done} whole epochs of the same workload ({el:.1f} s), FP64 oracle, "
+                      f"OpenMP over channels with {threads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(epochs shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import memd_amd
+    import synth
+
+    n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
+    dev = "cuda:0"
+    ctx = memd_amd.MemdContext(0)
+    V = ctx.directions(n, K)
+    ys = [torch.from_numpy(synth.sines_epoch(n, T, 1000 + i)).to(dev) for i in range(B)]
+    outs = [torch.empty((N_imf, n, T), dtype=torch.float64, device=dev) for _ in range(B)]
+    eps = [{"y": ys[i], "V": V, "n_imf": N_imf, "S": S, "out": outs[i]} for i in range(B)]
+
+    for _ in range(a.warmup):
+        ctx.run_batch(eps)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s0 = time.perf_counter()
+        ctx.run_batch(eps)
+        lat.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    value = B * a.steps / total
+
+    # plain per-epoch path (one call per epoch), latency per epoch
+    plain_lat = []
+    ctx.run(ys[0], V, N_imf, S, out=outs[0])
+    torch.cuda.synchronize()
+    for i in range(a.plain_epochs):
+        s0 = time.perf_counter()
+        ctx.run(ys[i % B], V, N_imf, S, out=outs[i % B])
+        plain_lat.append(time.perf_counter() - s0)
+    plain_eps = len(plain_lat) / sum(plain_lat)
+
+    roofline, kernels = None, None
+    if not a.no_profile:
+        _, info = ctx.run_batch(eps, profile=True)
+        pr = info["profile"]
+        kernels = {k: round(v * 1e3, 2) for k, v in pr["ms_per_launch"].items()}   # us per launch
+        steps = max(pr["steps"], 1)
+        W = (T + 63) // 64
+        # k_eval per launch: cur read+write (16 B/sample), one 32-B coefficient record per
+        # knot-channel, 4 B per knot position, 12 B per flag word + prefix per (epoch, direction, envelope)
+        bytes_eval = (16 * pr["samples"] + 32 * pr["knot_channels"] + 4 * pr["knots"]) / steps \
+            + 12 * 2 * K * W * B
+        flops_eval = (15 * pr["eval_points"] + 2 * pr["samples"]) / steps
+        t_eval = pr["ms_per_launch"]["k_eval"] * 1e-3
+        gbs = bytes_eval / t_eval / 1e9
+        tfs = flops_eval / t_eval / 1e12
+        if gbs / HBM_PEAK_GBS >= tfs / FP64_PEAK_TFLOPS:
+            roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}
+        else:
+            roofline = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tfs / FP64_PEAK_TFLOPS, 4), "traffic": None}
+        roofline.update({"kernel": "k_eval", "launch_us": round(t_eval * 1e6, 2),
+                         "alg_bytes_per_launch": int(bytes_eval), "alg_flops_per_launch": int(flops_eval),
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_frac": round(tfs / FP64_PEAK_TFLOPS, 4)})
+
+    cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S, a.cpu_budget_s)
+    ctx.close()
+    line = {
+        "metric": "MEMD epochs/s (32ch x 4096, K=64); batched vs plain GPU vs CPU",
+        "value": round(value, 2), "unit": "epochs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(total / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"memd_{n}ch_x{T}_K{K}_fix{N_imf}x{S}", "n": n, "T": T, "K": K,
+                   "N_imf": N_imf, "S": S, "mode": "fix", "window": "refwarp32", "batch_epochs": B,
+                   "generator": "sines 5/20/60 Hz, N(0,1)/sqrt(n) mixing, N(0,0.1^2) noise"},
+        "latency_ms": {"batched_p50": round(pct(lat, 50) * 1e3, 3), "batched_p99": round(pct(lat, 99) * 1e3, 3),
+                       "plain_p50": round(pct(plain_lat, 50) * 1e3, 3),
+                       "plain_p99": round(pct(plain_lat, 99) * 1e3, 3)},
+        "plain_gpu_epochs_per_s": round(plain_eps, 2),
+        "batched_over_plain": round(value / plain_eps, 2),
+        "roofline": roofline,
+        "kernel_us_per_launch": kernels,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * memd_amd.h -- C ABI of the MI355X (gfx950) MEMD engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point that
+ * replaces a reference interface cites it (file:line into EEGLab-Pannon/MEMD-GPU).
+ *
+ * Arithmetic is FP64 end to end (the *_f32 entry points convert at the
+ * boundary).  Results are bit-identical to the FP64 host oracle that restates
+ * the reference's algorithm; see the engine's design notes for the contract.
+ *
+ * Threading: a memd_ctx is not thread-safe; use one per host thread.  All
+ * device pointers must belong to the context's device.
+ */
+#ifndef MEMD_AMD_H
+#define MEMD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum memd_status {
+    MEMD_OK = 0,
+    MEMD_ERR_ARG = 1,        /* invalid shape / argument */
+    MEMD_ERR_HIP = 2,        /* HIP runtime failure (message in memd_last_error) */
+    MEMD_ERR_NOMEM = 3,      /* device allocation failed */
+    MEMD_ERR_COORDS = 4,     /* sample coordinates are not 0..T-1 */
+    MEMD_ERR_UNSUPPORTED = 5,/* shape outside the engine's limits (e.g. n > 256) */
+    MEMD_ERR_ADMISSION = 6,  /* request refused by the HBM admission budget */
+    MEMD_ERR_NOT_FOUND = 7   /* unknown request id */
+} memd_status;
+
+/* Sifting policy. */
+enum {
+    MEMD_MODE_FIX = 0,  /* exactly S sifts per IMF (memReducedEpochs_dev.cu:1176-1179) */
+    MEMD_MODE_STOP = 1  /* memd.m stop criterion [sd, sd2, tol], at most S sifts per IMF */
+};
+/* Extrema window. */
+enum {
+    MEMD_WINDOW_REFWARP32 = 0, /* the reference's coverage: interior t <= 240*ceil(T/242)
+                                  (memReducedEpochs_dev.cu:77-141 launched at :1191-1193) */
+    MEMD_WINDOW_FULL = 1       /* every interior sample */
+};
+
+typedef struct memd_opts {
+    int32_t mode;            /* MEMD_MODE_* (default FIX) */
+    int32_t window;          /* MEMD_WINDOW_* (default REFWARP32) */
+    double sd, sd2, tol;     /* stop thresholds (default 0.075, 0.75, 0.075) */
+    int32_t use_graph;       /* 1: replay one captured sift step as a hipGraph (default 1) */
+    /* optional diagnostics (NULL = off) */
+    int32_t* d_knot_counts;  /* device [max_passes][K][2]: E+ / E- per envelope pass */
+    int32_t max_passes;
+    int32_t* h_sift_counts;  /* host [N_imf]: sifts per IMF */
+    int32_t* h_n_imf_out;    /* host: output slots used (IMFs + residual) */
+} memd_opts;
+
+typedef struct memd_ctx memd_ctx;
+
+/* Context: owns the stream binding, the cached device workspace and the
+ * direction cache.  device < 0 selects the current device. */
+memd_status memd_ctx_create(int device, memd_ctx** out);
+memd_status memd_ctx_destroy(memd_ctx* ctx);
+/* Run on the given hipStream_t (passed as void*); NULL = legacy default stream. */
+memd_status memd_ctx_set_stream(memd_ctx* ctx, void* hip_stream);
+/* Message of the last failure on this context (never NULL). */
+const char* memd_last_error(memd_ctx* ctx);
+void memd_opts_default(memd_opts* o);
+/* Library version string. */
+const char* memd_version(void);
+
+/*
+ * Direction vectors, FP64, written to device d_V [K][n].
+ * Replaces GenerateDirectionVector<T>(SignalDim, NumDirVector, d_DirectionVector)
+ * (memReducedEpochs_dev.cu:1029-1072).  h_primes == NULL uses the reference's
+ * table [-K, 2, 3, 5, ...] (:1033-1046); a host array of n primes selects the
+ * MEX semantics (memdMatlabCuda.cu:1131-1137: h[k][0] = (k+1.5)/(-P[0])).
+ */
+memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes,
+                                double* d_V);
+
+/*
+ * One epoch, all pointers on the device.
+ * Replaces template memd<coord_t,real_t>(SignalDim, SignalLength, NumDirVector,
+ * num_IMFs, max_iter, d_x, d_y, d_DirectionVectors, d_IMFs)
+ * (memReducedEpochs_dev.cu:1074-1075): d_x [T] sample coordinates (NULL, or
+ * exactly 0..T-1 as every reference caller passes, :1504-1506), d_y [n][T],
+ * d_V [K][n], d_IMFs [N_imf][n][T] with the residual in the last used slot.
+ * elapsed_s (optional) receives the device time of the sifting loop, like the
+ * reference's return value (:1336-1337,1367).  Blocking.
+ */
+memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S,
+                         const int32_t* d_x, const double* d_y, const double* d_V, double* d_IMFs,
+                         const memd_opts* opts, double* elapsed_s);
+
+/* Same call with the reference's memd<int,float> argument types
+ * (memReducedEpochs_dev.cu:1522); converts to FP64 on the device. */
+memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S,
+                         const int32_t* d_x, const float* d_y, const float* d_V, float* d_IMFs,
+                         const memd_opts* opts, double* elapsed_s);
+
+/*
+ * MEX argument layout (memdMatlabCuda.cu:996-1039, assistMatlabCuda.m:3-17):
+ * prhs[0..3] = n, K, N_imf, S (host int32); prhs[4] = data (device, numel
+ * values, memory [n][T], T = numel/n); prhs[5] = sample index (device int32
+ * [T]); prhs[6] = primes (device int32 [n], P[0] used as -divisor);
+ * plhs[0] = d_out (device [N_imf][n][T]).  Directions are generated from the
+ * supplied primes and cached per (n, K, primes).
+ */
+memd_status memd_mex_compat_f32(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_imf, int32_t S,
+                                size_t numel, const float* d_data, const int32_t* d_index,
+                                const int32_t* d_primes, float* d_out);
+memd_status memd_mex_compat_f64(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_imf, int32_t S,
+                                size_t numel, const double* d_data, const int32_t* d_index,
+                                const int32_t* d_primes, double* d_out);
+
+/*
+ * Batched: many independent epochs (own n, T, K, N_imf, S) advanced together,
+ * one launch per kernel per sift step over all (epoch, direction) work items.
+ * No reference counterpart (the reference runs one epoch per call,
+ * memReducedEpochs_dev.cu:1074).  Per-epoch arithmetic is unchanged, so each
+ * epoch's result is bit-identical to memd_run_f64 on it alone.
+ */
+typedef struct memd_epoch {
+    size_t n, T, K, N_imf, S;
+    const double* d_y;     /* [n][T] */
+    const double* d_V;     /* [K][n] */
+    double* d_IMFs;        /* [N_imf][n][T] */
+    int32_t* h_sift_counts;/* optional host [N_imf] */
+    int32_t* h_n_imf_out;  /* optional host */
+} memd_epoch;
+
+memd_status memd_run_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t count,
+                               const memd_opts* opts, double* elapsed_s);
+
+/*
+ * Diagnostics (no reference counterpart): runs the batch once with direct
+ * launches and HIP events between kernels, and returns the average device time
+ * per launch of every kernel plus the algorithmic work counters the roofline
+ * figures are computed from.  Results are the same as memd_run_batch_f64's.
+ * Kernel order: 0 flags, 1 compact, 2 solve1, 3 reduced, 4 solve3, 5 eval,
+ * 6 advance, 7 init.
+ */
+#define MEMD_NUM_KERNELS 8
+typedef struct memd_kernel_profile {
+    double ms_per_launch[MEMD_NUM_KERNELS];
+    int64_t launches[MEMD_NUM_KERNELS];
+    int64_t steps;          /* sift steps launched */
+    int64_t knots;          /* sum over epochs, passes, directions, envelopes of E */
+    int64_t knot_channels;  /* same, times the epoch's channel count */
+    int64_t eval_points;    /* sum over epochs and passes of nvalid * n * T */
+    int64_t samples;        /* sum over epochs and passes of n * T */
+} memd_kernel_profile;
+
+memd_status memd_profile_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t count,
+                                   const memd_opts* opts, memd_kernel_profile* out);
+
+/* Device bytes the engine needs for one epoch of this shape (workspace only). */
+size_t memd_workspace_bytes(size_t n, size_t T, size_t K);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEMD_AMD_H */

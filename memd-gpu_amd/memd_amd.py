@@ -1,0 +1,300 @@
+"""Python mirror of the MEMD engine's host interface (ctypes over libmemd_amd.so).
+
+The names and argument meaning follow the reference's host API:
+
+* ``GenerateDirectionVector(SignalDim, NumDirVector, d_DirectionVector)``
+  (memReducedEpochs_dev.cu:1029-1072)
+* ``memd(SignalDim, SignalLength, NumDirVector, num_IMFs, max_iter, d_x, d_y,
+  d_DirectionVectors, d_IMFs)`` returning elapsed seconds
+  (memReducedEpochs_dev.cu:1074-1075, 1367)
+* ``memdMatlabCuda(signalDim, numDirVector, numIMFs, maxiter, data, samplesIndex,
+  primesSet, outputData)`` (memdMatlabCuda.cu:996-1039, assistMatlabCuda.m:17)
+
+Device buffers are torch tensors on a ROCm device (torch is only the allocator
+here).  Every call goes through the C ABI declared in include/memd_amd.h; there
+is no CPU fallback: if the shared library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmemd_amd.so")
+
+MEMD_OK = 0
+STATUS_NAMES = {0: "MEMD_OK", 1: "MEMD_ERR_ARG", 2: "MEMD_ERR_HIP", 3: "MEMD_ERR_NOMEM",
+                4: "MEMD_ERR_COORDS", 5: "MEMD_ERR_UNSUPPORTED", 6: "MEMD_ERR_ADMISSION",
+                7: "MEMD_ERR_NOT_FOUND"}
+MODE_FIX, MODE_STOP = 0, 1
+WINDOW_REFWARP32, WINDOW_FULL = 0, 1
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_i32 = ctypes.c_int32
+
+
+class MemdOpts(ctypes.Structure):
+    _fields_ = [("mode", _i32), ("window", _i32), ("sd", ctypes.c_double), ("sd2", ctypes.c_double),
+                ("tol", ctypes.c_double), ("use_graph", _i32), ("d_knot_counts", _vp),
+                ("max_passes", _i32), ("h_sift_counts", ctypes.POINTER(_i32)),
+                ("h_n_imf_out", ctypes.POINTER(_i32))]
+
+
+class MemdEpoch(ctypes.Structure):
+    _fields_ = [("n", _sz), ("T", _sz), ("K", _sz), ("N_imf", _sz), ("S", _sz),
+                ("d_y", _vp), ("d_V", _vp), ("d_IMFs", _vp),
+                ("h_sift_counts", ctypes.POINTER(_i32)), ("h_n_imf_out", ctypes.POINTER(_i32))]
+
+
+class MemdKernelProfile(ctypes.Structure):
+    _fields_ = [("ms_per_launch", ctypes.c_double * 8), ("launches", ctypes.c_int64 * 8),
+                ("steps", ctypes.c_int64), ("knots", ctypes.c_int64), ("knot_channels", ctypes.c_int64),
+                ("eval_points", ctypes.c_int64), ("samples", ctypes.c_int64)]
+
+
+#: kernel order of MemdKernelProfile.ms_per_launch
+KERNEL_NAMES = ("k_flags", "k_compact", "k_solve1", "k_reduced", "k_solve3", "k_eval", "k_advance", "k_init")
+
+
+class MemdError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+#: every symbol include/memd_amd.h declares (checked by the CPU test-suite)
+EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_last_error",
+           "memd_opts_default", "memd_version", "memd_directions_f64", "memd_run_f64", "memd_run_f32",
+           "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
+           "memd_workspace_bytes")
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmemd_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `make -C memd-gpu_amd` "
+                          "(or __graft_entry__.build()); there is no fallback path")
+    L = ctypes.CDLL(path)
+    L.memd_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.memd_ctx_destroy.argtypes = [_vp]
+    L.memd_ctx_set_stream.argtypes = [_vp, _vp]
+    L.memd_last_error.argtypes = [_vp]
+    L.memd_last_error.restype = ctypes.c_char_p
+    L.memd_version.restype = ctypes.c_char_p
+    L.memd_opts_default.argtypes = [ctypes.POINTER(MemdOpts)]
+    L.memd_opts_default.restype = None
+    L.memd_workspace_bytes.argtypes = [_sz, _sz, _sz]
+    L.memd_workspace_bytes.restype = _sz
+    L.memd_directions_f64.argtypes = [_vp, _sz, _sz, ctypes.POINTER(_i32), _vp]
+    L.memd_run_f64.argtypes = [_vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, _vp, _vp,
+                               ctypes.POINTER(MemdOpts), ctypes.POINTER(ctypes.c_double)]
+    L.memd_run_f32.argtypes = L.memd_run_f64.argtypes
+    L.memd_mex_compat_f32.argtypes = [_vp, _i32, _i32, _i32, _i32, _sz, _vp, _vp, _vp, _vp]
+    L.memd_mex_compat_f64.argtypes = L.memd_mex_compat_f32.argtypes
+    L.memd_run_batch_f64.argtypes = [_vp, ctypes.POINTER(MemdEpoch), _sz, ctypes.POINTER(MemdOpts),
+                                     ctypes.POINTER(ctypes.c_double)]
+    L.memd_profile_batch_f64.argtypes = [_vp, ctypes.POINTER(MemdEpoch), _sz, ctypes.POINTER(MemdOpts),
+                                         ctypes.POINTER(MemdKernelProfile)]
+    for name in EXPORTS:
+        if name not in ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def default_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, use_graph=True):
+    o = MemdOpts()
+    load_library().memd_opts_default(ctypes.byref(o))
+    o.mode, o.window, o.sd, o.sd2, o.tol, o.use_graph = mode, window, sd, sd2, tol, int(bool(use_graph))
+    return o
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _require_device(t, dtype, name):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class MemdContext:
+    """One engine context (stream binding, cached workspace, direction cache)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.memd_ctx_create(device, ctypes.byref(h))
+        if rc != MEMD_OK:
+            raise MemdError(rc, "memd_ctx_create failed (no HIP device?)")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.memd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != MEMD_OK:
+            raise MemdError(rc, self.lib.memd_last_error(self.h).decode())
+
+    def set_stream(self, stream):
+        """Bind a torch.cuda.Stream (or None for the legacy default stream)."""
+        self._check(self.lib.memd_ctx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
+
+    # ---- GenerateDirectionVector (memReducedEpochs_dev.cu:1029-1072)
+    def directions(self, n, K, primes=None, out=None):
+        import torch
+        if out is None:
+            out = torch.empty((K, n), dtype=torch.float64, device=f"cuda:{self.device}")
+        _require_device(out, torch.float64, "d_DirectionVector")
+        p = None
+        if primes is not None:
+            arr = (ctypes.c_int32 * n)(*[int(v) for v in primes])
+            p = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int32))
+        self._check(self.lib.memd_directions_f64(self.h, n, K, p, _ptr(out)))
+        return out
+
+    # ---- memd (memReducedEpochs_dev.cu:1074-1368)
+    def run(self, y, V, n_imf, S, x=None, out=None, mode=MODE_FIX, window=WINDOW_REFWARP32,
+            sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0):
+        """Decompose one epoch y [n][T] (float64 or float32 device tensor).
+
+        Returns (imfs [n_imf][n][T], info) with info = elapsed_s, sift_counts,
+        n_imf_out and, when trace_passes > 0, knot_counts [passes][K][2]."""
+        import torch
+        n, T = y.shape
+        K = V.shape[0]
+        f32 = y.dtype == torch.float32
+        dt = torch.float32 if f32 else torch.float64
+        _require_device(y, dt, "d_y")
+        _require_device(V, dt, "d_DirectionVectors")
+        if out is None:
+            out = torch.empty((n_imf, n, T), dtype=dt, device=y.device)
+        _require_device(out, dt, "d_IMFs")
+        if x is not None:
+            _require_device(x, torch.int32, "d_x")
+        o = default_opts(mode, window, sd, sd2, tol, use_graph)
+        sc = (ctypes.c_int32 * n_imf)()
+        nout = ctypes.c_int32(0)
+        o.h_sift_counts = ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32))
+        o.h_n_imf_out = ctypes.pointer(nout)
+        kc = None
+        if trace_passes:
+            kc = torch.zeros((trace_passes, K, 2), dtype=torch.int32, device=y.device)
+            o.d_knot_counts = kc.data_ptr()
+            o.max_passes = trace_passes
+        el = ctypes.c_double(0.0)
+        fn = self.lib.memd_run_f32 if f32 else self.lib.memd_run_f64
+        self._check(fn(self.h, n, T, K, n_imf, S, _ptr(x), _ptr(y), _ptr(V), _ptr(out), ctypes.byref(o),
+                       ctypes.byref(el)))
+        info = {"elapsed_s": el.value, "sift_counts": np.array(sc[:], np.int32), "n_imf_out": nout.value}
+        if kc is not None:
+            info["knot_counts"] = kc
+        return out, info
+
+    # ---- mexFunction argument layout (memdMatlabCuda.cu:996-1039)
+    def mex_compat(self, signalDim, numDirVector, numIMFs, maxiter, data, samplesIndex, primesSet, outputData):
+        import torch
+        f32 = data.dtype == torch.float32
+        dt = torch.float32 if f32 else torch.float64
+        _require_device(data, dt, "data")
+        _require_device(outputData, dt, "outputData")
+        _require_device(primesSet, torch.int32, "primesSet")
+        if samplesIndex is not None:
+            _require_device(samplesIndex, torch.int32, "samplesIndex")
+        fn = self.lib.memd_mex_compat_f32 if f32 else self.lib.memd_mex_compat_f64
+        self._check(fn(self.h, int(signalDim), int(numDirVector), int(numIMFs), int(maxiter), data.numel(),
+                       _ptr(data), _ptr(samplesIndex), _ptr(primesSet), _ptr(outputData)))
+        return outputData
+
+    # ---- batched epochs (no reference counterpart)
+    def run_batch(self, epochs, mode=MODE_FIX, window=WINDOW_REFWARP32, use_graph=True, profile=False):
+        """epochs: list of dicts with y, V, n_imf, S (and optional out).
+        Returns (list of imfs tensors, info); with profile=True info["profile"]
+        holds per-kernel average launch times (ms) and the work counters."""
+        import torch
+        arr = (MemdEpoch * len(epochs))()
+        outs, scs, nouts = [], [], []
+        for i, ep in enumerate(epochs):
+            y, V = ep["y"], ep["V"]
+            _require_device(y, torch.float64, "d_y")
+            _require_device(V, torch.float64, "d_V")
+            n, T = y.shape
+            out = ep.get("out")
+            if out is None:
+                out = torch.empty((ep["n_imf"], n, T), dtype=torch.float64, device=y.device)
+            sc = (ctypes.c_int32 * ep["n_imf"])()
+            no = ctypes.c_int32(0)
+            arr[i] = MemdEpoch(n, T, V.shape[0], ep["n_imf"], ep["S"], y.data_ptr(), V.data_ptr(),
+                               out.data_ptr(), ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32)),
+                               ctypes.pointer(no))
+            outs.append(out)
+            scs.append(sc)
+            nouts.append(no)
+        o = default_opts(mode, window, use_graph=use_graph)
+        el = ctypes.c_double(0.0)
+        if profile:
+            pr = MemdKernelProfile()
+            self._check(self.lib.memd_profile_batch_f64(self.h, arr, len(epochs), ctypes.byref(o), ctypes.byref(pr)))
+        else:
+            self._check(self.lib.memd_run_batch_f64(self.h, arr, len(epochs), ctypes.byref(o), ctypes.byref(el)))
+        info = {"elapsed_s": el.value, "sift_counts": [np.array(s[:], np.int32) for s in scs],
+                "n_imf_out": [v.value for v in nouts]}
+        if profile:
+            info["profile"] = {
+                "ms_per_launch": dict(zip(KERNEL_NAMES, pr.ms_per_launch[:])),
+                "launches": dict(zip(KERNEL_NAMES, pr.launches[:])),
+                "steps": pr.steps, "knots": pr.knots, "knot_channels": pr.knot_channels,
+                "eval_points": pr.eval_points, "samples": pr.samples}
+        return outs, info
+
+
+# ---- module-level functions with the reference's names -----------------------
+
+_default_ctx = None
+
+
+def _ctx():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = MemdContext(0)
+    return _default_ctx
+
+
+def GenerateDirectionVector(SignalDim, NumDirVector, d_DirectionVector):
+    """memReducedEpochs_dev.cu:1029-1072: fill d_DirectionVector [K][n] in place."""
+    _ctx().directions(SignalDim, NumDirVector, out=d_DirectionVector)
+
+
+def memd(SignalDim, SignalLength, NumDirVector, num_IMFs, max_iter, d_x, d_y, d_DirectionVectors, d_IMFs):
+    """memReducedEpochs_dev.cu:1074-1075: decompose d_y into d_IMFs; returns seconds."""
+    if tuple(d_y.shape) != (SignalDim, SignalLength):
+        raise ValueError("d_y must be [SignalDim][SignalLength]")
+    if tuple(d_DirectionVectors.shape) != (NumDirVector, SignalDim):
+        raise ValueError("d_DirectionVectors must be [NumDirVector][SignalDim]")
+    _, info = _ctx().run(d_y, d_DirectionVectors, num_IMFs, max_iter, x=d_x, out=d_IMFs)
+    return info["elapsed_s"]
+
+
+def memdMatlabCuda(signalDim, numDirVector, numIMFs, maxiter, data, samplesIndex, primesSet, outputData):
+    """memdMatlabCuda.cu:996-1039: returns the IMF array (memory [N_imf][n][T])."""
+    return _ctx().mex_compat(signalDim, numDirVector, numIMFs, maxiter, data, samplesIndex, primesSet, outputData)

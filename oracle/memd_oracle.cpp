@@ -1,0 +1,593 @@
+/*
+ * FP64 host oracle for MEMD -- TEST INFRASTRUCTURE, NOT THE PRODUCT.
+ *
+ * Only tests/, the smoke check and the benchmark's CPU-baseline leg load this
+ * library, and only as the checker.  The HIP engine never calls it.
+ *
+ * What it restates (file:line into the reference, EEGLab-Pannon/MEMD-GPU):
+ *   directions          memReducedEpochs_dev.cu:934-1072 (MEX twin memdMatlabCuda.cu:915-978)
+ *   projection          memReducedEpochs_dev.cu:1182-1184 (cublasSgemm, P = V * cur)
+ *   extrema + window    memReducedEpochs_dev.cu:77-141, launch geometry :1191-1193
+ *   exclusive scan      memReducedEpochs_dev.cu:143-382 (knot slot = #flags before t)
+ *   compaction          memReducedEpochs_dev.cu:384-445 (E = scan[T-1] + 1)
+ *   end-knot clamp      memReducedEpochs_dev.cu:447-486 (max), :545-584 (min)
+ *   spline system       memReducedEpochs_dev.cu:704-736 (natural end conditions)
+ *   tridiagonal solve   memReducedEpochs_dev.cu:1227-1241 (cusparseSgtsv2_nopivot, one
+ *                       system per direction, n right-hand sides)
+ *   coefficients        memReducedEpochs_dev.cu:775-793
+ *   evaluation          memReducedEpochs_dev.cu:795-830
+ *   mean + update       memReducedEpochs_dev.cu:832-864
+ *   IMF loop            memReducedEpochs_dev.cu:1174-1179, 1328-1338
+ *
+ * Arithmetic contract (shared bit-for-bit with the HIP kernels):
+ *   - FP64 throughout; compiled with -ffp-contract=off, so a*b+c is two roundings
+ *     unless written as fma() below.
+ *   - projection: acc = fma(V[k][c], cur[c][t], acc), c ascending, acc starts at 0.
+ *   - envelope evaluation: fma(fma(fma(d,tau,c),tau,b),tau,a) (Horner with fma).
+ *   - tridiagonal: the blocked Thomas algorithm of tridiag_* below (block length
+ *     SOLVER_BLOCK), replacing the closed cuSPARSE gtsv2 solver whose internal
+ *     order is not published.  Exact in exact arithmetic; pinned against dense
+ *     solves and scipy's natural CubicSpline by the golden tests.
+ *   - mean over directions: sum = sum + (e_max + e_min) * 0.5 in ascending k over
+ *     valid directions, then cur - sum / nvalid (0/0 -> NaN as the reference).
+ *
+ * Stop mode (not in the reference, which only has fix mode) follows the public
+ * memd.m criterion: sx(t) = |mean(t)|_2 / amp(t), stop when
+ * !((frac(sx > sd) > tol || any(sx > sd2)) && any(nem > 2)); an IMF is not started
+ * when every direction has fewer than 3 interior extrema.  PARITY UNPINNED against
+ * the reference (memd.m is not available here); only GPU-vs-oracle agreement is tested.
+ */
+#include "memd_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int SOLVER_BLOCK = 32;   // rows per block, separator included
+
+inline double fma_(double a, double b, double c) { return std::fma(a, b, c); }
+
+/* ---------------------------------------------------------------- directions */
+
+bool is_composite(int v) {   // memReducedEpochs_dev.cu:1014-1027 (returns 1 for composite)
+    for (int i = 2; i <= v / 2; ++i)
+        if (v % i == 0) return true;
+    return false;
+}
+
+/* ------------------------------------------------------------ tridiagonal core */
+
+/* Factor rows p..q of (lo,di,up) as one Thomas block: den_p = di_p,
+ * den_u = fma(-lo_u, cp_{u-1}, di_u); rd_u = 1/den_u; cp_u = up_u * rd_u. */
+void tridiag_factor(int p, int q, const double* lo, const double* di, const double* up,
+                    double* rd, double* cp) {
+    for (int u = p; u <= q; ++u) {
+        double den = (u == p) ? di[u] : fma_(-lo[u], cp[u - 1], di[u]);
+        rd[u] = 1.0 / den;
+        cp[u] = up[u] * rd[u];
+    }
+}
+
+/* Solve rows p..q with the factors above: z_p = g_p*rd_p,
+ * z_u = fma(-lo_u, z_{u-1}, g_u) * rd_u, x_q = z_q, x_u = fma(-cp_u, x_{u+1}, z_u). */
+void tridiag_apply(int p, int q, const double* lo, const double* rd, const double* cp,
+                   const double* g, double* z, double* x) {
+    z[p] = g[p] * rd[p];
+    for (int u = p + 1; u <= q; ++u) z[u] = fma_(-lo[u], z[u - 1], g[u]) * rd[u];
+    x[q] = z[q];
+    for (int u = q - 1; u >= p; --u) x[u] = fma_(-cp[u], x[u + 1], z[u]);
+}
+
+/* Reverse (bottom-up) factor of rows p..q: den'_q = di_q,
+ * den'_u = fma(-up_u, cl_{u+1}, di_u); rdr_u = 1/den'_u; cl_u = lo_u * rdr_u. */
+void tridiag_factor_rev(int p, int q, const double* lo, const double* di, const double* up,
+                        double* rdr, double* cl) {
+    for (int u = q; u >= p; --u) {
+        double den = (u == q) ? di[u] : fma_(-up[u], cl[u + 1], di[u]);
+        rdr[u] = 1.0 / den;
+        cl[u] = lo[u] * rdr[u];
+    }
+}
+
+/* Last entry of the block solution from the forward sweep alone:
+ * x_q = z_q with z as in tridiag_apply. */
+double sweep_fwd_last(int p, int q, const double* lo, const double* rd, const double* g) {
+    double z = g[p] * rd[p];
+    for (int u = p + 1; u <= q; ++u) z = fma_(-lo[u], z, g[u]) * rd[u];
+    return z;
+}
+
+/* First entry of the block solution from the reverse sweep alone:
+ * z'_q = g_q*rdr_q, z'_u = fma(-up_u, z'_{u+1}, g_u) * rdr_u, x_p = z'_p. */
+double sweep_rev_first(int p, int q, const double* up, const double* rdr, const double* g) {
+    double z = g[q] * rdr[q];
+    for (int u = q - 1; u >= p; --u) z = fma_(-up[u], z, g[u]) * rdr[u];
+    return z;
+}
+
+/* Blocked Thomas (shared by all right-hand sides of one direction).
+ * Rows 1..N; x[0] = x[N+1] = 0 are the natural end conditions.
+ * Separators sit at rows jL (j = 1..J, J = (N-1)/L); block b holds rows
+ * bL+1 .. min((b+1)L-1, N) (the last block ends at N).
+ *   phase 1: per block, the first/last entries of A_b^{-1} r (y), of the left
+ *            spike A_b^{-1}(lo_p e_first) (alpha) and of the right spike
+ *            A_b^{-1}(up_q e_last) (beta): last entries from the forward sweep,
+ *            first entries from the reverse sweep;
+ *   phase 2: the J-row reduced tridiagonal system for the separator values,
+ *            solved by forward sweep + back substitution;
+ *   phase 3: every block re-solved (forward sweep + back substitution) with the
+ *            separator values folded into its first / last right-hand side.
+ * With N <= L there is one block and the solve is plain Thomas. */
+struct Blocked {
+    int N = 0, L = SOLVER_BLOCK, J = 0;
+    const double *lo = nullptr, *di = nullptr, *up = nullptr;
+    std::vector<double> rd, cp, rdr, cl;        // per row: forward / reverse block-local factors
+    std::vector<double> ap, aq, bp, bq;         // spikes per block (first / last entry)
+    std::vector<double> Rlo, Rdi, Rup, Rrd, Rcp; // reduced system rows 1..J
+    std::vector<double> z, x, g;                // scratch
+
+    int bp_(int b) const { return b * L + 1; }
+    int bq_(int b) const { return (b < J) ? (b + 1) * L - 1 : N; }
+
+    void factor(int N_, const double* lo_, const double* di_, const double* up_) {
+        N = N_; lo = lo_; di = di_; up = up_;
+        J = (N >= 1) ? (N - 1) / L : 0;
+        rd.assign(N + 2, 0.0); cp.assign(N + 2, 0.0); rdr.assign(N + 2, 0.0); cl.assign(N + 2, 0.0);
+        z.assign(N + 2, 0.0); x.assign(N + 2, 0.0); g.assign(N + 2, 0.0);
+        if (N <= 0) return;
+        for (int b = 0; b <= J; ++b) tridiag_factor(bp_(b), bq_(b), lo, di, up, rd.data(), cp.data());
+        if (J == 0) return;
+        for (int b = 0; b <= J; ++b) tridiag_factor_rev(bp_(b), bq_(b), lo, di, up, rdr.data(), cl.data());
+        ap.assign(J + 1, 0.0); aq.assign(J + 1, 0.0); bp.assign(J + 1, 0.0); bq.assign(J + 1, 0.0);
+        for (int b = 0; b <= J; ++b) {
+            int p = bp_(b), q = bq_(b);
+            if (b >= 1) {   // left spike: A_b^{-1} (lo_p e_first)
+                for (int u = p; u <= q; ++u) g[u] = 0.0;
+                g[p] = lo[p];
+                aq[b] = sweep_fwd_last(p, q, lo, rd.data(), g.data());
+                ap[b] = sweep_rev_first(p, q, up, rdr.data(), g.data());
+            }
+            if (b < J) {    // right spike: A_b^{-1} (up_q e_last)
+                for (int u = p; u <= q; ++u) g[u] = 0.0;
+                g[q] = up[q];
+                bq[b] = sweep_fwd_last(p, q, lo, rd.data(), g.data());
+                bp[b] = sweep_rev_first(p, q, up, rdr.data(), g.data());
+            }
+        }
+        // reduced system (rows 1..J); coefficients of X_{j-1}, X_j, X_{j+1}
+        Rlo.assign(J + 2, 0.0); Rdi.assign(J + 2, 0.0); Rup.assign(J + 2, 0.0);
+        Rrd.assign(J + 2, 0.0); Rcp.assign(J + 2, 0.0);
+        for (int j = 1; j <= J; ++j) {
+            int s = j * L;
+            Rlo[j] = (j >= 2) ? -(lo[s] * aq[j - 1]) : 0.0;
+            Rdi[j] = fma_(-up[s], ap[j], fma_(-lo[s], bq[j - 1], di[s]));
+            Rup[j] = (j <= J - 1) ? -(up[s] * bp[j]) : 0.0;
+        }
+        tridiag_factor(1, J, Rlo.data(), Rdi.data(), Rup.data(), Rrd.data(), Rcp.data());
+    }
+
+    /* r: rows 1..N (index 0 and N+1 ignored); out: x[0..N+1] with x[0]=x[N+1]=0 */
+    void solve(const double* r, double* out) {
+        out[0] = 0.0; out[N + 1] = 0.0;
+        if (N <= 0) return;
+        if (J == 0) {
+            tridiag_apply(1, N, lo, rd.data(), cp.data(), r, z.data(), out);
+            return;
+        }
+        std::vector<double> yp(J + 1), yq(J + 1), D(J + 2, 0.0), Zr(J + 2), Xs(J + 2, 0.0);
+        for (int b = 0; b <= J; ++b) {   // phase 1
+            int p = bp_(b), q = bq_(b);
+            yq[b] = sweep_fwd_last(p, q, lo, rd.data(), r);
+            yp[b] = sweep_rev_first(p, q, up, rdr.data(), r);
+        }
+        for (int j = 1; j <= J; ++j) {   // phase 2
+            int s = j * L;
+            D[j] = fma_(-up[s], yp[j], fma_(-lo[s], yq[j - 1], r[s]));
+        }
+        tridiag_apply(1, J, Rlo.data(), Rrd.data(), Rcp.data(), D.data(), Zr.data(), Xs.data());
+        for (int b = 0; b <= J; ++b) {   // phase 3
+            int p = bp_(b), q = bq_(b);
+            for (int u = p; u <= q; ++u) g[u] = r[u];
+            if (b >= 1) g[p] = fma_(-lo[p], Xs[b], g[p]);
+            if (b < J) g[q] = fma_(-up[q], Xs[b + 1], g[q]);
+            tridiag_apply(p, q, lo, rd.data(), cp.data(), g.data(), z.data(), x.data());
+            for (int u = p; u <= q; ++u) out[u] = x[u];
+        }
+        for (int j = 1; j <= J; ++j) out[j * L] = Xs[j];
+    }
+};
+
+/* ----------------------------------------------------------- spline per env */
+
+/* Shared (per direction, per envelope) spline system: h, lo, di, up from the
+ * knot positions, natural rows 0 and m-1 (memReducedEpochs_dev.cu:704-736). */
+struct KnotSystem {
+    int E = 0;
+    std::vector<double> h, lo, di, up;
+    Blocked solver;
+    void build(int E_, const int32_t* X) {
+        E = E_;
+        h.assign(std::max(E - 1, 1), 0.0);
+        for (int i = 0; i + 1 < E; ++i) h[i] = (double)(X[i + 1] - X[i]);
+        int N = E - 2;
+        lo.assign(E + 1, 0.0); di.assign(E + 1, 0.0); up.assign(E + 1, 0.0);
+        for (int u = 1; u <= N; ++u) {
+            lo[u] = h[u - 1];
+            di[u] = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
+            up[u] = h[u];
+        }
+        solver.factor(N, lo.data(), di.data(), up.data());
+    }
+    /* one channel: Y (clamped, E values) -> coefficients [E-1][4] */
+    void channel(const double* Y, double* coef, std::vector<double>& sl, std::vector<double>& r,
+                 std::vector<double>& M) {
+        int N = E - 2;
+        sl.resize(std::max(E - 1, 1));
+        r.assign(E + 1, 0.0);
+        M.assign(E + 1, 0.0);
+        for (int i = 0; i + 1 < E; ++i) sl[i] = (Y[i + 1] - Y[i]) / h[i];
+        for (int u = 1; u <= N; ++u) r[u] = 3.0 * (sl[u] - sl[u - 1]);
+        solver.solve(r.data(), M.data());   // M[0] = M[E-1] = 0
+        for (int j = 0; j + 1 < E; ++j) {
+            double hj = h[j];
+            coef[4 * j + 0] = Y[j];
+            coef[4 * j + 1] = sl[j] - (hj * (2.0 * M[j] + M[j + 1])) / 3.0;
+            coef[4 * j + 2] = M[j];
+            coef[4 * j + 3] = (M[j + 1] - M[j]) / (3.0 * hj);
+        }
+    }
+};
+
+inline double horner(const double* cf, double tau) {
+    return fma_(fma_(fma_(cf[3], tau, cf[2]), tau, cf[1]), tau, cf[0]);
+}
+
+void clamp_ends(int E, const int32_t* X, double* Y, int env) {
+    // memReducedEpochs_dev.cu:451-485 (max) / :549-583 (min): only when scan[T-1] > 4, i.e. E >= 6
+    if (E < 6) return;
+    {
+        double s = (Y[2] - Y[1]) / (double)(X[2] - X[1]);
+        double tv = Y[1] - s * (double)(X[1] - X[0]);
+        if (env == 0 ? (tv > Y[0]) : (tv < Y[0])) Y[0] = tv;
+    }
+    {
+        int m = E - 1;
+        double s = (Y[m - 1] - Y[m - 2]) / (double)(X[m - 1] - X[m - 2]);
+        double tv = Y[m - 1] + s * (double)(X[m] - X[m - 1]);
+        if (env == 0 ? (tv > Y[m]) : (tv < Y[m])) Y[m] = tv;
+    }
+}
+
+int window_limit(int T, int window) {
+    if (window == ORACLE_WINDOW_FULL) return T - 2;
+    // grid of ceil(T/242) blocks x 8 warps; warp w tests samples 30w+1 .. 30w+30
+    long G = ((long)T + 241) / 242;
+    long lim = 240 * G;
+    return (int)std::min<long>(T - 2, lim);
+}
+
+int extrema(int T, const double* P, int env, int window, int32_t* X) {
+    int lim = window_limit(T, window);
+    int E = 0;
+    X[E++] = 0;
+    for (int t = 1; t <= lim; ++t) {
+        bool f = (env == 0) ? (P[t] > P[t - 1] && P[t] > P[t + 1])
+                            : (P[t] < P[t - 1] && P[t] < P[t + 1]);
+        if (f) X[E++] = t;
+    }
+    if (T - 1 > 0) X[E++] = T - 1;
+    return E;
+}
+
+struct Workspace {
+    std::vector<double> P;
+    std::vector<int32_t> X[2];
+    int E[2] = {0, 0};
+};
+
+/* Envelopes of cur for all directions; accumulate mean into sum (ascending k),
+ * optional memd.m amplitude.  nthreads>1 parallelises over channels / samples. */
+void envelope_pass(int n, int T, int K, const double* cur, const double* V, int window,
+                   bool want_amp, int nthreads, double* sum, double* ampsum, double* nvalid_out,
+                   int32_t* counts) {
+    std::vector<double> Pk(T);
+    std::vector<int32_t> X[2];
+    X[0].resize(T); X[1].resize(T);
+    KnotSystem sys[2];
+    std::vector<double> diff;
+    if (want_amp) diff.assign((size_t)n * T, 0.0);
+    for (size_t i = 0; i < (size_t)n * T; ++i) sum[i] = 0.0;
+    if (want_amp) for (int t = 0; t < T; ++t) ampsum[t] = 0.0;
+    double nvalid = 0.0;
+    (void)nthreads;
+    for (int k = 0; k < K; ++k) {
+        const double* Vk = V + (size_t)k * n;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+        for (int t = 0; t < T; ++t) {
+            double acc = 0.0;
+            for (int c = 0; c < n; ++c) acc = fma_(Vk[c], cur[(size_t)c * T + t], acc);
+            Pk[t] = acc;
+        }
+        int E[2];
+        for (int e = 0; e < 2; ++e) E[e] = extrema(T, Pk.data(), e, window, X[e].data());
+        if (counts) { counts[2 * k] = E[0]; counts[2 * k + 1] = E[1]; }
+        bool valid = (E[0] > 3) && (E[1] > 3);   // memReducedEpochs_dev.cu:840,857
+        if (!valid) continue;
+        nvalid = nvalid + 1.0;
+        for (int e = 0; e < 2; ++e) sys[e].build(E[e], X[e].data());
+#pragma omp parallel num_threads(nthreads) if (nthreads > 1)
+        {
+            std::vector<double> Y[2], coef[2], sl, r, M, env[2];
+            for (int e = 0; e < 2; ++e) {
+                Y[e].resize(E[e]); coef[e].resize(4 * (size_t)std::max(E[e] - 1, 1)); env[e].resize(T);
+            }
+#pragma omp for schedule(static)
+            for (int c = 0; c < n; ++c) {
+                const double* cc = cur + (size_t)c * T;
+                for (int e = 0; e < 2; ++e) {
+                    for (int i = 0; i < E[e]; ++i) Y[e][i] = cc[X[e][i]];
+                    clamp_ends(E[e], X[e].data(), Y[e].data(), e);
+                    sys[e].channel(Y[e].data(), coef[e].data(), sl, r, M);
+                    // evaluation: interval j with X_j < t <= X_{j+1}; t = 0 uses j = 0
+                    int j = 0;
+                    const int32_t* Xe = X[e].data();
+                    for (int t = 0; t < T; ++t) {
+                        while (j + 1 < E[e] - 1 && Xe[j + 1] < t) ++j;
+                        double tau = (double)(t - Xe[j]);
+                        env[e][t] = horner(&coef[e][4 * (size_t)j], tau);
+                    }
+                }
+                double* sc = sum + (size_t)c * T;
+                for (int t = 0; t < T; ++t) sc[t] = sc[t] + (env[0][t] + env[1][t]) * 0.5;
+                if (want_amp) {
+                    double* dc = diff.data() + (size_t)c * T;
+                    for (int t = 0; t < T; ++t) dc[t] = env[0][t] - env[1][t];
+                }
+            }
+        }
+        if (want_amp) {
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+            for (int t = 0; t < T; ++t) {
+                double s = 0.0;
+                for (int c = 0; c < n; ++c) {
+                    double d = diff[(size_t)c * T + t];
+                    s = s + d * d;
+                }
+                ampsum[t] = ampsum[t] + std::sqrt(s) * 0.5;
+            }
+        }
+    }
+    *nvalid_out = nvalid;
+}
+
+}  // namespace
+
+/* ============================================================= C interface */
+
+extern "C" {
+
+int oracle_solver_block(void) { return SOLVER_BLOCK; }
+int oracle_window_limit(int T, int window) { return window_limit(T, window); }
+
+double oracle_radical_inverse(int base, unsigned i) {
+    // memReducedEpochs_dev.cu:934-946, in FP64 instead of float
+    double digit, radical, inv;
+    digit = radical = 1.0 / (double)base;
+    inv = 0.0;
+    while (i) {
+        inv = inv + digit * (double)(i % (unsigned)base);
+        digit = digit * radical;
+        i /= (unsigned)base;
+    }
+    return inv;
+}
+
+int oracle_prime_table(int n, int K, int32_t* P) {
+    // memReducedEpochs_dev.cu:1033-1046
+    if (n < 1 || K < 1) return -1;
+    int cnt = 1, cand = 2;
+    while (cnt < n) {
+        if (!is_composite(cand)) P[cnt++] = cand;
+        ++cand;
+    }
+    P[0] = -K;
+    return 0;
+}
+
+int oracle_directions(int n, int K, const int32_t* primes, double* V) {
+    if (n < 1 || K < 1) return -1;
+    std::vector<int32_t> P(n);
+    if (primes) {
+        for (int j = 0; j < n; ++j) P[j] = primes[j];
+    } else {
+        oracle_prime_table(n, K, P.data());
+    }
+    if (P[0] == 0) return -1;
+    for (int j = 1; j < n; ++j) if (P[j] < 2) return -1;
+    std::vector<double> b(n), tht(n > 1 ? n - 1 : 1);
+    for (int k = 0; k < K; ++k) {
+        // Hammersley point (memReducedEpochs_dev.cu:992-1005)
+        b[0] = ((double)k + 1.5) / (double)(-P[0]);
+        for (int j = 1; j < n; ++j) b[j] = oracle_radical_inverse(P[j], (unsigned)(k + 1));
+        // Sequence2Vector (memReducedEpochs_dev.cu:948-990)
+        for (int i = 0; i < n; ++i) b[i] = 2.0 * b[i] - 1.0;
+        for (int i = 0; i < n - 1; ++i) {
+            double tem = 0.0;
+            for (int j = n - 1; j > i; --j) tem = tem + b[j] * b[j];
+            tht[i] = std::atan2(std::sqrt(tem), b[i]);
+        }
+        double* v = V + (size_t)k * n;
+        for (int i = 0; i < n; ++i) {
+            if (i == 0) { v[i] = 1.0; continue; }
+            double tem = 1.0;
+            for (int j = 0; j < i; ++j) tem = tem * std::sin(tht[j]);
+            v[i] = tem;
+        }
+        for (int i = 0; i < n - 1; ++i) v[i] = v[i] * std::cos(tht[i]);
+    }
+    return 0;
+}
+
+void oracle_project(int n, int T, int K, const double* cur, const double* V, double* P) {
+    for (int k = 0; k < K; ++k)
+        for (int t = 0; t < T; ++t) {
+            double acc = 0.0;
+            for (int c = 0; c < n; ++c) acc = fma_(V[(size_t)k * n + c], cur[(size_t)c * T + t], acc);
+            P[(size_t)k * T + t] = acc;
+        }
+}
+
+int oracle_extrema(int T, const double* Pk, int env, int window, int32_t* X) {
+    return extrema(T, Pk, env, window, X);
+}
+
+void oracle_clamp(int E, const int32_t* X, double* Y, int env) { clamp_ends(E, X, Y, env); }
+
+void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, double* M) {
+    KnotSystem sys;
+    sys.build(E, X);
+    std::vector<double> sl, r, Mv, coef(4 * (size_t)std::max(E - 1, 1));
+    for (int c = 0; c < nrhs; ++c) {
+        sys.channel(Y + (size_t)c * E, coef.data(), sl, r, Mv);
+        for (int i = 0; i < E; ++i) M[(size_t)c * E + i] = Mv[i];
+    }
+}
+
+void oracle_spline_coefs(int E, const int32_t* X, const double* Y, const double* M, double* coef) {
+    for (int j = 0; j + 1 < E; ++j) {
+        double hj = (double)(X[j + 1] - X[j]);
+        double sl = (Y[j + 1] - Y[j]) / hj;
+        coef[4 * j + 0] = Y[j];
+        coef[4 * j + 1] = sl - (hj * (2.0 * M[j] + M[j + 1])) / 3.0;
+        coef[4 * j + 2] = M[j];
+        coef[4 * j + 3] = (M[j + 1] - M[j]) / (3.0 * hj);
+    }
+}
+
+void oracle_envelope(int T, int E, const int32_t* X, const double* coef, double* env) {
+    int j = 0;
+    for (int t = 0; t < T; ++t) {
+        while (j + 1 < E - 1 && X[j + 1] < t) ++j;
+        env[t] = horner(&coef[4 * (size_t)j], (double)(t - X[j]));
+    }
+}
+
+void oracle_tridiag_blocked(int N, const double* lo, const double* di, const double* up,
+                            const double* r, double* x) {
+    Blocked b;
+    b.factor(N, lo, di, up);
+    b.solve(r, x);
+}
+
+void oracle_envelope_mean(int n, int T, int K, const double* cur, const double* V,
+                          const oracle_opts* o, double* mean, double* amp, int32_t* counts) {
+    int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
+    int nth = (o && o->threads > 1) ? o->threads : 1;
+    std::vector<double> ampsum(T);
+    double nv = 0.0;
+    envelope_pass(n, T, K, cur, V, window, amp != nullptr, nth, mean, ampsum.data(), &nv, counts);
+    for (size_t i = 0; i < (size_t)n * T; ++i) mean[i] = mean[i] / nv;
+    if (amp) for (int t = 0; t < T; ++t) amp[t] = ampsum[t] / nv;
+}
+
+int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const double* V,
+                double* imfs, const oracle_opts* o, oracle_trace* tr) {
+    if (n < 1 || T < 4 || K < 1 || N_imf < 1 || S < 1) return -1;
+    int mode = o ? o->mode : ORACLE_MODE_FIX;
+    int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
+    int nth = (o && o->threads > 1) ? o->threads : 1;
+    double sd = o ? o->sd : 0.075, sd2 = o ? o->sd2 : 0.75, tol = o ? o->tol : 0.075;
+    size_t nT = (size_t)n * T;
+    std::vector<double> running(y, y + nT), sum(nT), ampsum(T), q(T);
+    std::vector<int32_t> counts((size_t)2 * K);
+    int pass = 0;
+    auto record = [&]() {
+        if (tr && tr->knot_counts && pass < tr->max_sifts)
+            std::memcpy(tr->knot_counts + (size_t)pass * 2 * K, counts.data(), sizeof(int32_t) * 2 * K);
+        ++pass;
+    };
+    if (tr && tr->sift_counts) for (int i = 0; i < N_imf; ++i) tr->sift_counts[i] = 0;
+    int imf = 0;
+    if (mode == ORACLE_MODE_FIX) {
+        // memReducedEpochs_dev.cu:1176-1338
+        for (; imf < N_imf - 1; ++imf) {
+            double* cur = imfs + (size_t)imf * nT;
+            std::memcpy(cur, running.data(), sizeof(double) * nT);
+            for (int s = 0; s < S; ++s) {
+                double nv = 0.0;
+                envelope_pass(n, T, K, cur, V, window, false, nth, sum.data(), nullptr, &nv, counts.data());
+                record();
+                for (size_t i = 0; i < nT; ++i) cur[i] = cur[i] - sum[i] / nv;
+            }
+            if (tr && tr->sift_counts) tr->sift_counts[imf] = S;
+            for (size_t i = 0; i < nT; ++i) running[i] = running[i] - cur[i];
+        }
+    } else {
+        bool finished = false;
+        for (; imf < N_imf - 1 && !finished; ++imf) {
+            double* cur = imfs + (size_t)imf * nT;
+            std::memcpy(cur, running.data(), sizeof(double) * nT);
+            int nsift = 0;
+            for (;;) {
+                double nv = 0.0;
+                envelope_pass(n, T, K, cur, V, window, true, nth, sum.data(), ampsum.data(), &nv,
+                              counts.data());
+                record();
+                if (nsift == 0) {   // memd.m stop_emd: every direction < 3 interior extrema
+                    bool all_small = true;
+                    for (int k = 0; k < K; ++k)
+                        if ((counts[2 * k] - 2) + (counts[2 * k + 1] - 2) >= 3) all_small = false;
+                    if (all_small) { finished = true; break; }
+                }
+                bool stop;
+                if (nv == 0.0) {
+                    stop = true;   // memd.m: no valid direction -> zero mean, stop
+                } else {
+                    bool amp_nonzero = true;
+                    for (int t = 0; t < T; ++t) {
+                        double a = ampsum[t] / nv;
+                        ampsum[t] = a;
+                        if (!(a != 0.0)) amp_nonzero = false;
+                    }
+                    long cnt1 = 0;
+                    bool any2 = false;
+                    for (int t = 0; t < T; ++t) {
+                        double s = 0.0;
+                        for (int c = 0; c < n; ++c) {
+                            double m = sum[(size_t)c * T + t] / nv;
+                            s = s + m * m;
+                        }
+                        double sx = std::sqrt(s);
+                        if (amp_nonzero) sx = sx / ampsum[t];
+                        if (sx > sd) ++cnt1;
+                        if (sx > sd2) any2 = true;
+                    }
+                    bool anynem = false;
+                    for (int k = 0; k < K; ++k)
+                        if ((counts[2 * k] - 2) + (counts[2 * k + 1] - 2) > 2) anynem = true;
+                    stop = !((((double)cnt1 / (double)T) > tol || any2) && anynem);
+                }
+                if (stop) break;
+                for (size_t i = 0; i < nT; ++i) cur[i] = cur[i] - sum[i] / nv;
+                ++nsift;
+                if (nsift == S) break;
+            }
+            if (finished) break;
+            if (tr && tr->sift_counts) tr->sift_counts[imf] = nsift;
+            for (size_t i = 0; i < nT; ++i) running[i] = running[i] - cur[i];
+        }
+    }
+    std::memcpy(imfs + (size_t)imf * nT, running.data(), sizeof(double) * nT);
+    if (tr && tr->n_imf_out) *tr->n_imf_out = imf + 1;
+    if (tr && tr->n_passes) *tr->n_passes = pass;
+    return 0;
+}
+
+}  // extern "C"

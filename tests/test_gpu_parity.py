@@ -1,0 +1,179 @@
+"""HIP engine vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Bar: bit-exact.  The engine and the oracle share one arithmetic contract
+(FP64, contraction off, fma where both write it), so knot counts per direction
+per sift, sift counts, IMF count and every IMF sample must be identical.  The
+numeric tolerance the north star allows (1e-9 relative L2) is asserted too, as
+a weaker second line.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD_DIR = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(a, dtype=None):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0") if dtype is None else \
+        torch.from_numpy(np.ascontiguousarray(a, dtype)).to("cuda:0")
+
+
+def _rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _check_epoch(ctx, oracle, y, K, n_imf, S, window=0, primes=None, use_graph=True):
+    n, T = y.shape
+    V = oracle.directions(n, K, primes)
+    passes = (n_imf - 1) * S
+    imfs_o, info_o = oracle.memd(y, V, n_imf, S, window=window)
+    imfs_g, info_g = ctx.run(_dev(y), _dev(V), n_imf, S, window=window, use_graph=use_graph,
+                             trace_passes=max(passes, 1))
+    got = imfs_g.cpu().numpy()
+    assert info_g["n_imf_out"] == info_o["n_imf_out"]
+    assert np.array_equal(info_g["sift_counts"], info_o["sift_counts"])
+    kc = info_g["knot_counts"].cpu().numpy()[:passes]
+    assert np.array_equal(kc, info_o["knot_counts"]), "knot counts differ"
+    assert _rel_l2(got, imfs_o) <= 1e-9
+    mism = np.argwhere(got != imfs_o)
+    assert mism.size == 0, f"{len(mism)} samples differ, first at {mism[0].tolist()}"
+    return got
+
+
+def test_directions_bit_exact(ctx, oracle):
+    for n, K in [(1, 4), (8, 16), (32, 64), (128, 256)]:
+        got = ctx.directions(n, K).cpu().numpy()
+        assert np.array_equal(got, oracle.directions(n, K))
+    P = np.array([2, 3, 5, 7, 11, 13, 17, 19], np.int32)
+    assert np.array_equal(ctx.directions(8, 16, primes=P).cpu().numpy(), oracle.directions(8, 16, P))
+
+
+@pytest.mark.parametrize("n,T,K,n_imf,S,seed", [
+    (8, 1000, 16, 9, 10, 0),      # the 8 x 1000, K=16 host case, reference schedule
+    (3, 300, 6, 4, 3, 1),
+    (1, 500, 4, 5, 4, 2),         # n = 1 reduces to univariate EMD
+    (5, 777, 9, 3, 3, 3),         # ragged sizes
+    (33, 2049, 7, 3, 2, 4),       # channel count not a power of two
+    (2, 5000, 3, 3, 2, 5),        # many solver blocks (E ~ 1600 knots)
+])
+def test_run_matches_oracle_sines(ctx, oracle, n, T, K, n_imf, S, seed):
+    import synth
+    _check_epoch(ctx, oracle, synth.sines_epoch(n, T, seed), K, n_imf, S)
+
+
+def test_headline_shape_full_schedule(ctx, oracle):
+    # 32 ch x 4096 samples, K = 64, the reference's 9 slots x 10 sifts
+    import synth
+    y = synth.sines_epoch(32, 4096, 7)
+    got = _check_epoch(ctx, oracle, y, 64, 9, 10)
+    np.testing.assert_allclose(got.sum(0), y, rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+def test_full_window(ctx, oracle):
+    import synth
+    _check_epoch(ctx, oracle, synth.sines_epoch(4, 4096, 8), 8, 3, 3, window=1)
+
+
+def test_graph_and_direct_launch_agree(ctx, oracle):
+    import synth
+    y = synth.eeg_epoch(8, 3000, 9)
+    a = _check_epoch(ctx, oracle, y, 16, 4, 3, use_graph=True)
+    b = _check_epoch(ctx, oracle, y, 16, 4, 3, use_graph=False)
+    assert np.array_equal(a, b)
+
+
+def test_eeg_sample_reduced_schedule(ctx, oracle):
+    # the reference's own input file, 32 x 30504, K = 64 (2 IMFs x 2 sifts to keep the oracle fast)
+    import synth
+    y = synth.load_eeg_sample(os.path.join(GOLD_DIR, "eeg_sample_32x30504.npz")).astype(np.float64)
+    _check_epoch(ctx, oracle, y, 64, 3, 2)
+
+
+def test_constant_epoch_gives_nan_like_reference(ctx, oracle):
+    # no interior extrema -> no valid direction -> 0/0 (memReducedEpochs_dev.cu:862)
+    y = np.ones((4, 256))
+    V = oracle.directions(4, 8)
+    imfs_o, _ = oracle.memd(y, V, 3, 2)
+    imfs_g, _ = ctx.run(_dev(y), _dev(V), 3, 2)
+    got = imfs_g.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(imfs_o))
+    assert np.isnan(got[0]).all()
+
+
+def test_f32_entry_point(ctx, oracle):
+    import synth
+    torch = _torch()
+    y32 = synth.sines_epoch(8, 1000, 10).astype(np.float32)
+    V32 = oracle.directions(8, 16).astype(np.float32)
+    imfs_o, _ = oracle.memd(y32.astype(np.float64), V32.astype(np.float64), 5, 4)
+    out, _ = ctx.run(_dev(y32), _dev(V32), 5, 4)
+    assert out.dtype == torch.float32
+    assert np.array_equal(out.cpu().numpy(), imfs_o.astype(np.float32))
+
+
+def test_reference_named_entry_points(oracle):
+    import torch
+    import memd_amd
+    import synth
+    n, T, K, N_imf, S = 8, 1000, 16, 4, 3
+    y = synth.sines_epoch(n, T, 11)
+    d_V = torch.empty((K, n), dtype=torch.float64, device="cuda:0")
+    memd_amd.GenerateDirectionVector(n, K, d_V)
+    d_x = torch.arange(T, dtype=torch.int32, device="cuda:0")
+    d_IMFs = torch.empty((N_imf, n, T), dtype=torch.float64, device="cuda:0")
+    secs = memd_amd.memd(n, T, K, N_imf, S, d_x, _dev(y), d_V, d_IMFs)
+    assert secs > 0
+    imfs_o, _ = oracle.memd(y, oracle.directions(n, K), N_imf, S)
+    assert np.array_equal(d_IMFs.cpu().numpy(), imfs_o)
+
+
+def test_mex_layout(ctx, oracle):
+    # assistMatlabCuda.m: single data, int32 index 0..T-1, primes(2000)(1:n), output T x n x N_imf
+    import torch
+    import synth
+    n, T, K, N_imf, S = 6, 900, 12, 4, 3
+    y32 = synth.sines_epoch(n, T, 12).astype(np.float32)
+    primes = np.array([2, 3, 5, 7, 11, 13], np.int32)
+    out = torch.zeros((N_imf, n, T), dtype=torch.float32, device="cuda:0")
+    idx = torch.arange(T, dtype=torch.int32, device="cuda:0")
+    ctx.mex_compat(n, K, N_imf, S, _dev(y32), idx, _dev(primes), out)
+    V = oracle.directions(n, K, primes)
+    imfs_o, _ = oracle.memd(y32.astype(np.float64), V, N_imf, S)
+    assert np.array_equal(out.cpu().numpy(), imfs_o.astype(np.float32))
+
+
+def test_bad_coordinates_rejected(ctx, oracle):
+    import torch
+    import memd_amd
+    y = np.random.default_rng(0).standard_normal((2, 100))
+    V = oracle.directions(2, 4)
+    x = torch.arange(100, dtype=torch.int32, device="cuda:0")
+    x[5] = 7
+    with pytest.raises(memd_amd.MemdError) as ei:
+        ctx.run(_dev(y), _dev(V), 3, 2, x=x)
+    assert ei.value.status == 4
+
+
+def test_batch_bit_identical_to_alone(ctx, oracle):
+    import synth
+    eps = []
+    for i, (n, T, K) in enumerate([(8, 1200, 16), (16, 1000, 32), (4, 2500, 8), (8, 1100, 16)]):
+        y = synth.eeg_epoch(n, T, 100 + i)
+        eps.append({"y": _dev(y), "V": _dev(oracle.directions(n, K)), "n_imf": 4, "S": 3})
+    outs, info = ctx.run_batch(eps)
+    for ep, out in zip(eps, outs):
+        alone, _ = ctx.run(ep["y"], ep["V"], 4, 3)
+        assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
+    # and the first one against the oracle
+    y0 = eps[0]["y"].cpu().numpy()
+    imfs_o, _ = oracle.memd(y0, eps[0]["V"].cpu().numpy(), 4, 3)
+    assert np.array_equal(outs[0].cpu().numpy(), imfs_o)

@@ -1,0 +1,146 @@
+"""Pins the FP64 oracle against independent answers (CPU only).
+
+Golden vectors come from tests/golden/make_golden.py (numpy/scipy, different
+algorithms from the oracle); the rest are closed forms and known values.
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "golden_small.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLD, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ---- directions (memReducedEpochs_dev.cu:934-1072) ---------------------------
+
+def test_radical_inverse_known_values(oracle):
+    known = {(2, 1): 0.5, (2, 2): 0.25, (2, 3): 0.75, (2, 4): 0.125, (2, 5): 0.625,
+             (3, 1): 1 / 3, (3, 2): 2 / 3, (3, 3): 1 / 9, (3, 4): 4 / 9, (5, 7): 2 / 5 + 1 / 25}
+    for (b, i), v in known.items():
+        assert oracle.radical_inverse(b, i) == pytest.approx(v, abs=1e-15)
+
+
+def test_prime_table_reference_rule(oracle):
+    # memReducedEpochs_dev.cu:1033-1046: P[0] = -K, then 2,3,5,7,...
+    P = oracle.prime_table(8, 64)
+    assert P.tolist() == [-64, 2, 3, 5, 7, 11, 13, 17]
+
+
+@pytest.mark.parametrize("n,K", [(1, 4), (2, 8), (3, 5), (8, 16), (32, 64)])
+def test_directions_closed_form(oracle, gold, n, K):
+    V = oracle.directions(n, K)
+    np.testing.assert_allclose(V, gold[f"dirs_ref_{n}_{K}"], rtol=0, atol=2e-14)
+    np.testing.assert_allclose(np.linalg.norm(V, axis=1), 1.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,K", [(3, 7), (32, 64)])
+def test_directions_mex_primes(oracle, gold, n, K):
+    P = gold[f"dirs_mex_{n}_{K}_primes"]
+    V = oracle.directions(n, K, primes=P)
+    np.testing.assert_allclose(V, gold[f"dirs_mex_{n}_{K}"], rtol=0, atol=2e-14)
+    # the MEX path differs from the C++ path (P[0] = 2 instead of -K)
+    assert not np.allclose(V, oracle.directions(n, K))
+
+
+def test_direction_of_single_channel_is_one(oracle):
+    assert np.array_equal(oracle.directions(1, 9), np.ones((9, 1)))
+
+
+# ---- projection ---------------------------------------------------------------
+
+def test_projection_matches_matmul(oracle):
+    rng = np.random.default_rng(3)
+    cur = rng.standard_normal((7, 333))
+    V = rng.standard_normal((5, 7))
+    np.testing.assert_allclose(oracle.project(cur, V), V @ cur, rtol=1e-14, atol=1e-14)
+
+
+# ---- extrema window (memReducedEpochs_dev.cu:77-141, :1191-1193) ---------------
+
+@pytest.mark.parametrize("T", [5, 100, 242, 483, 484, 1000, 4096, 8193])
+def test_extrema_match_kernel_simulation(oracle, gold, T):
+    P = gold[f"ext_P_{T}"]
+    assert np.array_equal(oracle.extrema(P, 0), gold[f"ext_max_{T}"])
+    assert np.array_equal(oracle.extrema(P, 1), gold[f"ext_min_{T}"])
+
+
+@pytest.mark.parametrize("T,gap", [(1000, 0), (4096, 14), (8193, 31), (16384, 62), (30504, 22),
+                                   (60000, 478), (241665, 1903)])
+def test_window_gap_lengths(oracle, T, gap):
+    assert (T - 2) - oracle.window_limit(T, oracle.WINDOW_REFWARP32) == gap
+    assert oracle.window_limit(T, oracle.WINDOW_FULL) == T - 2
+
+
+def test_full_window_tests_every_interior_sample(oracle):
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal(4096)
+    want = np.r_[0, 1 + np.flatnonzero((P[1:-1] > P[:-2]) & (P[1:-1] > P[2:])), 4095]
+    assert np.array_equal(oracle.extrema(P, 0, oracle.WINDOW_FULL), want)
+
+
+# ---- clamp + natural spline (memReducedEpochs_dev.cu:447-584, 704-830) --------
+
+@pytest.mark.parametrize("case", range(5))
+def test_clamp_and_envelope_match_scipy(oracle, gold, case):
+    X, Y, T = gold[f"spl_X_{case}"], gold[f"spl_Y_{case}"], int(gold[f"spl_T_{case}"])
+    for e in range(2):
+        Yc = oracle.clamp(X, Y, e)
+        np.testing.assert_allclose(Yc, gold[f"spl_Yc_{case}_{e}"], rtol=1e-11, atol=1e-12)  # polyfit lstsq rounding
+        M = oracle.spline_solve(X, Yc[None, :])[0]
+        env = oracle.envelope(T, X, oracle.spline_coefs(X, Yc, M))
+        ref = gold[f"spl_env_{case}_{e}"]
+        np.testing.assert_allclose(env, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+def test_clamp_only_from_six_knots(oracle):
+    # scan[T-1] > 4 <=> E >= 6 (memReducedEpochs_dev.cu:454)
+    X = np.array([0, 2, 5, 9, 14], np.int32)
+    Y = np.array([-10.0, 1.0, 2.0, 3.0, -10.0])
+    assert np.array_equal(oracle.clamp(X, Y, 0), Y)
+    X6 = np.array([0, 2, 5, 9, 14, 20], np.int32)
+    Y6 = np.array([-10.0, 1.0, 2.0, 5.0, 3.0, -10.0])
+    got = oracle.clamp(X6, Y6, 0)
+    assert got[0] == pytest.approx(1.0 - (1.0 / 3.0) * 2)      # line through knots 1,2 at X0
+    assert got[-1] == pytest.approx(3.0 + (-2.0 / 5.0) * 6)    # line through knots 3,4 at X5
+    # min envelope only lowers
+    got_min = oracle.clamp(X6, Y6, 1)
+    assert got_min[0] == -10.0 and got_min[-1] == -10.0
+
+
+@pytest.mark.parametrize("N", [1, 2, 31, 32, 33, 63, 64, 65, 200, 1000])
+def test_blocked_tridiagonal_matches_dense(oracle, gold, N):
+    lo, di, up, r = gold[f"tri_{N}"]
+    x = oracle.tridiag_blocked(lo, di, up, r)
+    ref = gold[f"tri_x_{N}"]
+    assert x[0] == 0.0 and x[N + 1] == 0.0
+    np.testing.assert_allclose(x[1:N + 1], ref, rtol=0, atol=4e-15 * np.abs(ref).max())
+
+
+def test_blocked_solver_block_boundaries_shift_nothing(oracle):
+    # a planted-error check: the residual of the blocked solve must be at rounding level
+    rng = np.random.default_rng(11)
+    for N in [95, 96, 97, 500]:
+        h = rng.integers(1, 20, N + 1).astype(float)
+        lo = np.r_[0.0, h[:N], 0.0]
+        up = np.r_[0.0, h[1:N + 1], 0.0]
+        di = np.r_[0.0, 2.0 * (h[:N] + h[1:N + 1]), 0.0]
+        r = np.r_[0.0, rng.standard_normal(N), 0.0]
+        x = oracle.tridiag_blocked(lo, di, up, r)
+        res = di[1:N + 1] * x[1:N + 1] + lo[1:N + 1] * x[0:N] + up[1:N + 1] * x[2:N + 2] - r[1:N + 1]
+        assert np.abs(res).max() < 1e-13
+
+
+# ---- one envelope-mean pass (memReducedEpochs_dev.cu:1181-1318) ---------------
+
+def test_envelope_mean_pass_matches_independent_pipeline(oracle, gold):
+    y, V = gold["pass_y"], gold["pass_V"]
+    mean, _, counts = oracle.envelope_mean(y, V)
+    assert np.array_equal(counts, gold["pass_counts"])
+    ref = gold["pass_mean"]
+    np.testing.assert_allclose(mean, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
