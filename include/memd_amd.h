@@ -139,10 +139,11 @@ memd_status memd_run_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t c
  * launches and HIP events between kernels, and returns the average device time
  * per launch of every kernel plus the algorithmic work counters the roofline
  * figures are computed from.  Results are the same as memd_run_batch_f64's.
- * Kernel order: 0 flags, 1 compact, 2 solve1, 3 reduced, 4 solve3, 5 eval,
- * 6 advance, 7 init.
+ * Kernel slots: 0 flags, 1 compact, 2 factor, 3 knots, 4 block, 5 reduced,
+ * 6 coef, 7 eval (stop mode: eval_stop), 8 advance (stop mode: stop_decide),
+ * 9 stop_apply (stop mode only), 10 init.
  */
-#define MEMD_NUM_KERNELS 8
+#define MEMD_NUM_KERNELS 11
 typedef struct memd_kernel_profile {
     double ms_per_launch[MEMD_NUM_KERNELS];
     int64_t launches[MEMD_NUM_KERNELS];

@@ -12,6 +12,8 @@
 #include "engine.h"
 
 #include <cstdarg>
+#include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -33,7 +35,7 @@ struct memd_ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
     int* flag = nullptr;            // device word for the coordinate check
-    hipEvent_t pev[8] = {};         // per-kernel timing events (profiling entry point)
+    hipEvent_t pev[kNumStepKernels + 1] = {};   // per-kernel timing events (profiling entry point)
 };
 
 namespace {
@@ -69,7 +71,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, mask, pre, E, X, coef, bend, bspk, sep, mean, tstat, red, sift, st, total;
+    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf) {
@@ -84,10 +86,12 @@ EpochLayout layout_for(int n, int T, int K, int N_imf) {
     L.pre = take((size_t)K * 2 * W * 4);
     L.E = take((size_t)K * 2 * 4);
     L.X = take((size_t)K * 2 * Ecap * 4);
+    L.fac = take((size_t)K * 2 * 6 * Ecap * 8);
+    L.redsys = take((size_t)K * 2 * 3 * nb * 8);
+    L.yknot = take((size_t)K * 2 * n * Ecap * 8);
+    L.ysol = take((size_t)K * 2 * n * Ecap * 8);
     L.coef = take((size_t)K * 2 * n * Ecap * 32);
-    L.bend = take((size_t)K * 2 * nb * n * 16);
-    L.bspk = take((size_t)K * 2 * nb * 32);
-    L.sep = take((size_t)K * 2 * nb * n * 16);
+    L.sep = take((size_t)K * 2 * n * nb * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
     L.red = take(64);
@@ -135,6 +139,16 @@ memd_status validate_shape(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N
 
 hipStream_t work_stream(memd_ctx* ctx) { return ctx->user; }
 
+// Evaluation-kernel tile variant: MEMD_EVAL_VARIANT (tuning runs) or the default.
+int eval_variant_choice() {
+    const char* v = std::getenv("MEMD_EVAL_VARIANT");
+    if (v && *v) {
+        int i = std::atoi(v);
+        if (i >= 0 && i < kNumEvalVariants) return i;
+    }
+    return kEvalDefault;
+}
+
 // Core: run a set of epochs (all already FP64 and on the device) to completion.
 memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
                        memd_kernel_profile* prof = nullptr) {
@@ -142,14 +156,16 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     if (opts) o = *opts; else memd_opts_default(&o);
     if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP)
         return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
-    if (o.mode == MEMD_MODE_STOP)
-        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode is not available in this build");
     if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
         return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
     const int count = (int)in.size();
     if (count == 0) return MEMD_OK;
     if (o.d_knot_counts && count != 1)
         return fail(ctx, MEMD_ERR_ARG, "knot-count tracing is only available for single-epoch calls");
+    int nmax = 0;
+    for (const EpochIn& I : in) nmax = I.n > nmax ? I.n : nmax;
+    if (o.mode == MEMD_MODE_STOP && nmax > kStopMaxChannels)
+        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %d)", kStopMaxChannels, nmax);
 
     // ---- workspace plan
     std::vector<EpochLayout> lay(count);
@@ -162,10 +178,42 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     }
     size_t desc_at = total;
     total += align_up(sizeof(EpochDesc) * count);
-    size_t off_at = total;
-    total += align_up(sizeof(int) * (size_t)(count + 1) * 6);
+    // block tables of the fixed-size launches and the solver work lists
+    const int variant = eval_variant_choice();
+    std::vector<int> nblk(kNumMaps, 0);
+    std::vector<long> lcap(kNumLists, 0);
+    {
+        std::vector<EpochDesc> tmp(count);
+        for (int e = 0; e < count; ++e) {
+            tmp[e].n = in[e].n; tmp[e].T = in[e].T; tmp[e].K = in[e].K;
+            tmp[e].Ecap = knot_capacity(in[e].T);
+            tmp[e].nbmax = solver_block_capacity(tmp[e].Ecap);
+            tmp[e].mode = o.mode;
+            tmp[e].coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+            for (int f = 0; f < kNumMaps; ++f) {
+                long v = (long)nblk[f] + blocks_for(f, tmp[e], variant);
+                if (v > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
+                nblk[f] = (int)v;
+            }
+            for (int l = 0; l < kNumLists; ++l) lcap[l] += list_capacity(l, tmp[e]);
+        }
+    }
+    size_t tab_at[kNumMaps], list_at[kNumLists];
+    for (int f = 0; f < kNumMaps; ++f) {
+        tab_at[f] = total;
+        total += align_up(sizeof(int2) * (size_t)nblk[f]);
+    }
+    for (int l = 0; l < kNumLists; ++l) {
+        if (lcap[l] > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
+        list_at[l] = total;
+        total += align_up(sizeof(int4) * (size_t)lcap[l]);
+    }
+    size_t cnt_at = total;
+    total += align_up(sizeof(int) * kNumLists);
     size_t stats_at = total;
     total += align_up(sizeof(unsigned long long) * 4);
+    size_t ndone_at = total;
+    total += align_up(sizeof(int));
     memd_status rs = ensure_arena(ctx, total);
     if (rs != MEMD_OK) return rs;
 
@@ -186,55 +234,71 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         d.pre = (int*)(b + L.pre);
         d.E = (int*)(b + L.E);
         d.X = (int*)(b + L.X);
+        d.fac = (double*)(b + L.fac);
+        d.red = (double*)(b + L.redsys);
+        d.yknot = (double*)(b + L.yknot);
+        d.ysol = (double*)(b + L.ysol);
         d.coef = (double*)(b + L.coef);
-        d.bend = (double*)(b + L.bend);
-        d.bspk = (double*)(b + L.bspk);
         d.sep = (double*)(b + L.sep);
         d.mean = (double*)(b + L.mean);
         d.tstat = (double*)(b + L.tstat);
-        d.red = (int*)(b + L.red);
+        d.stopred = (int*)(b + L.red);
         d.d_trace = o.d_knot_counts;
         d.max_passes = o.d_knot_counts ? o.max_passes : 0;
         d.sift_counts = (int*)(b + L.sift);
         d.st = (EpochState*)(b + L.st);
         d.stats = prof ? (unsigned long long*)(ctx->arena + stats_at) : nullptr;
+        d.ndone = (int*)(ctx->arena + ndone_at);
+        d.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
     }
-    // block prefix maps: init, flags, compact, solve, reduced, eval
-    std::vector<int> offs((size_t)(count + 1) * 6, 0);
-    int (*fns[6])(const EpochDesc&) = {blocks_init, blocks_flags, blocks_compact, blocks_solve, blocks_reduced, blocks_eval};
-    for (int f = 0; f < 6; ++f) {
-        long acc = 0;
-        for (int e = 0; e < count; ++e) {
-            offs[(size_t)f * (count + 1) + e] = (int)acc;
-            acc += fns[f](eds[e]);
-        }
-        if (acc > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
-        offs[(size_t)f * (count + 1) + count] = (int)acc;
-    }
+    std::vector<int2> tabs;
     hipStream_t s = work_stream(ctx);
+    int* ndone_dev = (int*)(ctx->arena + ndone_at);
+    HIP_TRY(ctx, hipMemsetAsync(ndone_dev, 0, sizeof(int), s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + desc_at, eds.data(), sizeof(EpochDesc) * count, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + off_at, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice, s));
+    for (int f = 0; f < kNumMaps; ++f) {
+        size_t at0 = tabs.size();
+        for (int e = 0; e < count; ++e) {
+            int nb = blocks_for(f, eds[e], variant);
+            for (int b = 0; b < nb; ++b) tabs.push_back(make_int2(e, b));
+        }
+        (void)at0;
+    }
+    {
+        size_t pos = 0;
+        for (int f = 0; f < kNumMaps; ++f) {
+            if (nblk[f] > 0)
+                HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + tab_at[f], tabs.data() + pos, sizeof(int2) * nblk[f],
+                                            hipMemcpyHostToDevice, s));
+            pos += nblk[f];
+        }
+    }
 
     StepPlan plan{};
     plan.eps = (const EpochDesc*)(ctx->arena + desc_at);
     plan.count = count;
-    const int* od = (const int*)(ctx->arena + off_at);
-    plan.off_init = od;
-    plan.off_flags = od + (count + 1);
-    plan.off_compact = od + 2 * (count + 1);
-    plan.off_solve = od + 3 * (count + 1);
-    plan.off_reduced = od + 4 * (count + 1);
-    plan.off_eval = od + 5 * (count + 1);
-    plan.blocks_init = offs[count];
-    plan.blocks_flags = offs[(size_t)1 * (count + 1) + count];
-    plan.blocks_compact = offs[(size_t)2 * (count + 1) + count];
-    plan.blocks_solve = offs[(size_t)3 * (count + 1) + count];
-    plan.blocks_reduced = offs[(size_t)4 * (count + 1) + count];
-    plan.blocks_eval = offs[(size_t)5 * (count + 1) + count];
+    for (int f = 0; f < kNumMaps; ++f) {
+        plan.tab[f] = (const int2*)(ctx->arena + tab_at[f]);
+        plan.blocks[f] = nblk[f];
+    }
+    plan.wl.count = (int*)(ctx->arena + cnt_at);
+    for (int l = 0; l < kNumLists; ++l) {
+        plan.wl.item[l] = (int4*)(ctx->arena + list_at[l]);
+        plan.wl.cap[l] = (int)lcap[l];
+        long g = lcap[l] < kListGrid ? lcap[l] : kListGrid;
+        plan.grid[l] = (int)(g > 0 ? g : 1);
+    }
+    plan.mode = o.mode;
+    plan.nmax = nmax;
+    plan.eval_variant = variant;
+    plan.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
 
+    // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
+    // passes (S subtractions plus the deciding pass per IMF); the host replays
+    // steps in chunks and stops early once every epoch reports done.
     long steps = 0;
     for (int e = 0; e < count; ++e) {
-        long st = (long)(in[e].N_imf - 1) * in[e].S;
+        long st = (long)(in[e].N_imf - 1) * (o.mode == MEMD_MODE_STOP ? in[e].S + 1 : in[e].S);
         if (st > steps) steps = st;
     }
 
@@ -247,20 +311,28 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         HIP_TRY(ctx, hipEventSynchronize(ctx->pev[1]));
         float ms = 0.f;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->pev[0], ctx->pev[1]));
-        prof->ms_per_launch[7] = ms;
-        prof->launches[7] = 1;
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        prof->ms_per_launch[kNumStepKernels] = ms;
+        prof->launches[kNumStepKernels] = 1;
+        double acc[kNumStepKernels] = {};
         HIP_TRY(ctx, hipEventRecord(ctx->t0, s));
+        long ran = 0;
         for (long i = 0; i < steps; ++i) {
             HIP_TRY(ctx, launch_step_timed(plan, s, ctx->pev));
-            HIP_TRY(ctx, hipEventSynchronize(ctx->pev[7]));
-            for (int kk = 0; kk < 7; ++kk) {
+            HIP_TRY(ctx, hipEventSynchronize(ctx->pev[kNumStepKernels]));
+            ++ran;
+            for (int kk = 0; kk < kNumStepKernels; ++kk) {
                 HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->pev[kk], ctx->pev[kk + 1]));
                 acc[kk] += ms;
             }
+            if (o.mode == MEMD_MODE_STOP) {
+                int nd = 0;
+                HIP_TRY(ctx, hipMemcpy(&nd, ndone_dev, sizeof(int), hipMemcpyDeviceToHost));
+                if (nd >= count) break;
+            }
         }
+        steps = ran;
         HIP_TRY(ctx, hipEventRecord(ctx->t1, s));
-        for (int kk = 0; kk < 7; ++kk) {
+        for (int kk = 0; kk < kNumStepKernels; ++kk) {
             prof->launches[kk] = steps;
             prof->ms_per_launch[kk] = steps ? acc[kk] / (double)steps : 0.0;
         }
@@ -277,9 +349,12 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     if (o.use_graph && steps > 1) {
         // capture one step on the owned stream; the plan lives in the arena, so the
         // key is the arena address plus the launch geometry
-        char key[256];
-        snprintf(key, sizeof key, "%p/%d/%d/%d/%d/%d/%d", (void*)ctx->arena, count, plan.blocks_flags,
-                 plan.blocks_compact, plan.blocks_solve, plan.blocks_reduced, plan.blocks_eval);
+        std::string key = std::to_string((uintptr_t)ctx->arena) + "/" + std::to_string(count) + "/" +
+                          std::to_string(plan.mode) + "/" + std::to_string(plan.nmax) + "/" +
+                          std::to_string(plan.eval_variant);
+        for (int f = 0; f < kNumMaps; ++f) key += "/" + std::to_string(plan.blocks[f]);
+        for (int l = 0; l < kNumLists; ++l)
+            key += "/" + std::to_string(plan.wl.cap[l]) + ":" + std::to_string(plan.grid[l]);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) {
                 HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
@@ -302,13 +377,39 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             }
             ctx->graph_key = key;
         }
-        HIP_TRY(ctx, hipEventRecord(ctx->ev_in, s));
-        HIP_TRY(ctx, hipStreamWaitEvent(ctx->own, ctx->ev_in, 0));
-        for (long i = 0; i < steps; ++i) HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->own));
-        HIP_TRY(ctx, hipEventRecord(ctx->ev_out, ctx->own));
-        HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_out, 0));
-    } else {
-        for (long i = 0; i < steps; ++i) HIP_TRY(ctx, launch_step(plan, s));
+    }
+    {
+        const bool graph = o.use_graph && steps > 1;
+        auto launch_steps = [&](long nsteps) -> memd_status {
+            if (graph) {
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_in, s));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->own, ctx->ev_in, 0));
+                for (long i = 0; i < nsteps; ++i) HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->own));
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_out, ctx->own));
+                HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_out, 0));
+            } else {
+                for (long i = 0; i < nsteps; ++i) HIP_TRY(ctx, launch_step(plan, s));
+            }
+            return MEMD_OK;
+        };
+        if (o.mode == MEMD_MODE_STOP) {
+            // epochs finish after data-dependent pass counts: replay in chunks and
+            // stop once the device-side done counter covers the batch
+            long left = steps;
+            while (left > 0) {
+                long chunk = left < 4 ? left : 4;
+                memd_status r2 = launch_steps(chunk);
+                if (r2 != MEMD_OK) return r2;
+                left -= chunk;
+                int nd = 0;
+                HIP_TRY(ctx, hipMemcpyAsync(&nd, ndone_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                if (nd >= count) break;
+            }
+        } else {
+            memd_status r2 = launch_steps(steps);
+            if (r2 != MEMD_OK) return r2;
+        }
     }
     HIP_TRY(ctx, hipEventRecord(ctx->t1, s));
     }
@@ -424,11 +525,11 @@ memd_status memd_ctx_create(int device, memd_ctx** out) {
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&ctx->t0) != hipSuccess || hipEventCreate(&ctx->t1) != hipSuccess ||
-        hipMalloc((void**)&ctx->flag, sizeof(int)) != hipSuccess) {
+        hipMalloc((void**)&ctx->flag, sizeof(int)) != hipSuccess || configure_kernels() != hipSuccess) {
         memd_ctx_destroy(ctx);
         return MEMD_ERR_HIP;
     }
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i <= kNumStepKernels; ++i) {
         if (hipEventCreate(&ctx->pev[i]) == hipSuccess) continue;
         memd_ctx_destroy(ctx);
         return MEMD_ERR_HIP;
@@ -450,7 +551,7 @@ memd_status memd_ctx_destroy(memd_ctx* ctx) {
     if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
     if (ctx->t0) (void)hipEventDestroy(ctx->t0);
     if (ctx->t1) (void)hipEventDestroy(ctx->t1);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i <= kNumStepKernels; ++i)
         if (ctx->pev[i]) (void)hipEventDestroy(ctx->pev[i]);
     delete ctx;
     return MEMD_OK;

@@ -13,12 +13,37 @@ namespace memd {
 constexpr int kSolverBlock = 32;
 // Samples per projection/extrema tile (4 wavefronts of 64).
 constexpr int kFlagTile = 256;
-// Channels per lane in the fused evaluate/mean/update kernel.
-constexpr int kEvalCPL = 4;
+// Fused evaluate/mean/update kernel: one wavefront per 64 samples x CPL
+// channels with U directions' records in flight; (CPL, U) is a tuning choice
+// that never changes the arithmetic.
+struct EvalVariant { int cpl, unroll; };
+constexpr EvalVariant kEvalVariants[] = {{4, 1}, {4, 2}, {2, 4}, {8, 1}, {2, 2}, {1, 4}, {0, 0}};
+constexpr int kNumEvalVariants = 7;
+// Variant kEvalFused builds the spline records of each 64-sample tile in LDS
+// (no record array in HBM); tiles of 64 samples x kEvalFusedCG channels.
+constexpr int kEvalFused = 6;
+constexpr int kEvalFusedCG = 16;
+constexpr int kEvalMaxInt = 36;   // >= intervals meeting a 64-sample tile (<= 34)
+constexpr int kEvalDefault = kEvalFused;
+// Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
+constexpr int kFlagLdsMaxCh = 64;
+// Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
+constexpr int kKnotChunk = 256;
+constexpr int kKnotCh = 4;
+// Coefficient kernel: items (channel, interval) per thread.
+constexpr int kCoefItems = 16;
+// Persistent grid size of the list-driven solver kernels (workgroups).
+constexpr long kListGrid = 2048;
+enum : int { kFacRd = 0, kFacCp = 1, kFacAl = 2, kFacBe = 3, kFacIh = 4, kFacI3h = 5 };
 // Largest channel count: one solver workgroup of 256 lanes holds >= 1 block.
 constexpr int kMaxChannels = 256;
 
 enum : int { kModeFix = 0, kModeStop = 1 };
+// Stop-mode decisions: subtract the mean (and finish the IMF after S sifts),
+// finish the IMF without subtracting, or end the decomposition.
+enum : int { kActContinue = 0, kActFinishImf = 1, kActEnd = 2, kActNone = 3 };
+// Stop mode keeps per-sample channel sums in LDS: 64 samples x n channels.
+constexpr int kStopMaxChannels = 128;
 enum : int { kWindowRefWarp32 = 0, kWindowFull = 1 };
 
 // Per-epoch device state, advanced on the device after every sift step.
@@ -28,8 +53,9 @@ struct EpochState {
     int done;       // 1 once the residual has been written
     int pass;       // envelope passes performed (trace index)
     int n_imf_out;  // output slots used (IMFs + residual), valid when done
-    int action;     // stop mode: decision of the last step (see kernels.hip)
-    int pad[2];
+    int action;     // stop mode: decision of the last step (kActContinue, ...)
+    int act_imf;    // stop mode: IMF index the decision applies to
+    int pad;
 };
 
 // Everything a kernel needs to know about one epoch.  Lives in device memory.
@@ -47,56 +73,70 @@ struct EpochDesc {
     int* pre;              // [K][2][W] knots before each word
     int* E;                // [K][2] knot counts
     int* X;                // [K][2][Ecap] knot sample positions
-    double* coef;          // [K][2][n][Ecap][4] spline (a,b,c,d) per interval
-    double* bend;          // [K][2][nbmax][n][2] block solution ends (first, last)
-    double* bspk;          // [K][2][nbmax][4] spikes (alpha_first, alpha_last, beta_first, beta_last)
-    double* sep;           // [K][2][nbmax][n][2] reduced system (z, cp) then x in slot 0
+    double* fac;           // [K][2][6][Ecap] per knot row, shared by channels:
+                           //   rd, cp (block factors), al, be (spikes), ih = 1/h, i3h = 1/(3h)
+    double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
+    double* yknot;         // [K][2][n][Ecap] knot values Y (end knots clamped)
+    double* ysol;          // [K][2][n][Ecap] per channel: rhs r; block rows then hold y
+    double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
+    double* sep;           // [K][2][n][nbmax] separator values (z, then X_j)
     double* mean;          // [n][T] stop mode: envelope mean
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
-    int* red;              // [4] stop mode: reduction scratch
+    int* stopred;          // [4] stop mode: reduction scratch
     int* d_trace;          // optional [max_passes][K][2]
     int max_passes;
     int* sift_counts;      // device [N_imf]
     EpochState* st;
     unsigned long long* stats; // optional [4]: knots, knot*channels, eval points, samples
+    int* ndone;                // shared by the batch: epochs finished (stop mode)
+    int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
 };
 
-// Work map for one kernel: block b of the launch belongs to the epoch e with
-// off[e] <= b < off[e+1].
-struct WorkMap {
-    const int* off;
-    int count;
+// Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its
+// local block tab[b].y.
+enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kNumMaps };
+
+// List-driven launches: k_compact appends the exact work items (epoch, ke, sub)
+// of every valid direction; persistent grids drain the lists.
+enum : int { kListFactor = 0, kListKnots, kListBlock, kListReduced, kListCoef, kNumLists };
+
+struct WorkLists {
+    int4* item[kNumLists];
+    int* count;             // [kNumLists], cleared at the end of every step
+    int cap[kNumLists];
 };
 
 struct StepPlan {
     const EpochDesc* eps;   // device array
     int count;
-    // block prefix arrays (device), one per kernel
-    const int* off_init;
-    const int* off_flags;
-    const int* off_compact;
-    const int* off_solve;
-    const int* off_reduced;
-    const int* off_eval;
-    int blocks_init, blocks_flags, blocks_compact, blocks_solve, blocks_reduced, blocks_eval;
-    int any_stop;
+    const int2* tab[kNumMaps];   // device block tables
+    int blocks[kNumMaps];
+    WorkLists wl;
+    int grid[kNumLists];    // persistent grid size per list
+    int mode;               // kModeFix or kModeStop (all epochs of one call)
+    int nmax;               // largest channel count in the batch
+    int eval_variant;       // index into kEvalVariants
+    int coef_records;       // launch k_coef (the evaluation reads the record array)
 };
 
 // Host-side block counts for one epoch (must match the kernels' decoding).
-int blocks_init(const EpochDesc& e);
-int blocks_flags(const EpochDesc& e);
-int blocks_compact(const EpochDesc& e);
-int blocks_solve(const EpochDesc& e);
-int blocks_reduced(const EpochDesc& e);
-int blocks_eval(const EpochDesc& e);
+int blocks_for(int map, const EpochDesc& e, int eval_variant);
+// Worst-case work items of one epoch in a list.
+long list_capacity(int list, const EpochDesc& e);
+// Kernel slots of one sift step (profiling order): flags, compact, factor,
+// knots, block, reduced, coef, eval | eval_stop, advance | stop_decide,
+// stop_apply (stop mode only); slot kNumStepKernels is the one-off init.
+constexpr int kNumStepKernels = 10;
 int solver_blocks_per_wg(int n);
 int knot_capacity(int T);
 int solver_block_capacity(int Ecap);
 
+// Per-device kernel attributes (large dynamic LDS); call once per context.
+hipError_t configure_kernels();
 hipError_t launch_init(const StepPlan& p, hipStream_t s);
 hipError_t launch_step(const StepPlan& p, hipStream_t s);
 // Same launches with an event recorded after each kernel (ev[0] before the first).
-hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev /*[8]*/);
+hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev /*[kNumStepKernels+1]*/);
 hipError_t launch_check_coords(const int32_t* x, int T, int* flag, hipStream_t s);
 hipError_t launch_f32_to_f64(const float* in, double* out, size_t count, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* in, float* out, size_t count, hipStream_t s);

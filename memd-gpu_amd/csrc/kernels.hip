@@ -1,21 +1,26 @@
 // gfx950 kernels of the MEMD engine.
 //
 // One sift step of every active epoch is a fixed sequence of launches, each
-// covering all epochs of the batch (block -> epoch through a prefix map):
+// covering all epochs of the batch:
 //
 //   flags    projection of the current signal onto every direction, fused with
 //            strict-extremum tests and wave64 ballots -> one 64-bit flag word per
 //            64 samples per (direction, envelope)
 //            [reference: cublasSgemm :1182-1184, find_extrema_shfl_max/min :77-141]
-//   compact  popcount prefix over the flag words and expansion into knot lists
+//   compact  popcount prefix over the flag words, expansion into knot lists, and
+//            the exact work items of the solver kernels for valid directions
 //            [prescan_* :143-281, select_extrema_* :384-445]
-//   solve1   blocked tridiagonal, phase 1: block ends and spikes per (block, channel)
-//   reduced  phase 2: separator system per (direction, envelope), channels as lanes
-//   solve3   phase 3: block re-solve + spline coefficients per interval
+//   factor   blocked tridiagonal, shared part: block factors, interval
+//            reciprocals, spike vectors, reduced-system factorisation
+//   knots    knot values (end knots clamped) and right-hand sides, coalesced
+//   block    phase 1: per-channel block solves (fma chains in registers)
+//   reduced  phase 2: separator values, channels as lanes
+//   coef     phase 3 (second derivatives M) and the spline record (a,b,c,d) of
+//            every interval of every channel, coalesced
 //            [setBoundary* :447-584, tridiagonal_setup :704-736,
 //             cusparseSgtsv2_nopivot :1227-1241, spline_coefficients :775-793]
-//   eval     spline evaluation of both envelopes for every direction, mean over
-//            valid directions and the sift update, fused; envelopes stay in
+//   eval     evaluation of both envelopes for every direction, mean over valid
+//            directions and the sift update, fused; envelopes never leave
 //            registers [interpolate :795-830, averageUppperLower :832-844,
 //            averageDirection :846-864, updateSignal :866-874]
 //   advance  per-epoch state machine (IMF / sift counters)
@@ -23,21 +28,16 @@
 // Line numbers refer to memReducedEpochs_dev.cu.  All arithmetic is FP64 with
 // contraction disabled (-ffp-contract=off); fma() appears exactly where the
 // oracle's contract puts it, so results are bit-identical to the oracle.
+//
+// Fixed-size launches (init, flags, compact, eval, stop kernels) find their
+// epoch through a per-block table; the solver kernels are persistent grids that
+// drain work lists filled by k_compact with the exact number of items, so no
+// workgroup is spent on knots that do not exist.
 #include "engine.h"
 
 namespace memd {
 
 namespace {
-
-__device__ __forceinline__ int find_epoch(const int* off, int count, int block) {
-    // largest e with off[e] <= block (off is ascending, off[0] = 0)
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= block) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ int window_limit(int T, int window) {
     if (window == kWindowFull) return T - 2;
@@ -65,7 +65,6 @@ struct KnotRow {
     int env;             // 0 = max envelope, 1 = min envelope
 
     __device__ __forceinline__ double raw(int u) const { return cur[X[u]]; }
-    __device__ __forceinline__ double h(int u) const { return (double)(X[u + 1] - X[u]); }
     __device__ double Y(int u) const {
         if (E >= 6 && u == 0) {
             double y0 = raw(0), y1 = raw(1), y2 = raw(2);
@@ -86,29 +85,54 @@ struct KnotRow {
     }
 };
 
+// Knot system geometry: rows u = 1..N (N = E-2), separators at u = jL
+// (j = 1..J, J = (N-1)/L), block b = rows bL+1 .. (b<J ? (b+1)L-1 : N).
+struct BlockGeom {
+    int N, J, p, q;
+};
+
+__device__ __forceinline__ BlockGeom block_geom(int E, int b) {
+    BlockGeom g;
+    g.N = E - 2;
+    g.J = (g.N - 1) / kSolverBlock;
+    g.p = b * kSolverBlock + 1;
+    g.q = (b < g.J) ? (b + 1) * kSolverBlock - 1 : g.N;
+    return g;
+}
+
+__device__ __forceinline__ double* fac_row(const EpochDesc& ed, int ke, int field) {
+    return ed.fac + ((size_t)ke * 6 + field) * ed.Ecap;
+}
+
+__device__ __forceinline__ void zero_counters(const WorkLists& wl) {
+    for (int i = 0; i < kNumLists; ++i) wl.count[i] = 0;
+}
+
 // ---------------------------------------------------------------- init
 
-__global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps, const int* __restrict__ off,
-                                              int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
-    int local = blockIdx.x - off[e];
+__global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab,
+                                              WorkLists wl) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
     size_t nT = (size_t)ed.n * ed.T;
-    size_t i = (size_t)local * 256 + threadIdx.x;
+    size_t i = (size_t)br.y * 256 + threadIdx.x;
     if (i < nT) {
         double v = ed.y[i];
         ed.running[i] = v;
         ed.imfs[i] = v;   // slot 0 holds the signal being sifted (or the residual if N_imf == 1)
     }
-    if (local == 0 && threadIdx.x == 0) {
+    if (br.y == 0 && threadIdx.x == 0) {
         EpochState* st = ed.st;
         st->imf = 0;
         st->nsift = 0;
         st->pass = 0;
-        st->action = 0;
+        st->action = kActNone;
+        st->act_imf = 0;
         st->done = (ed.N_imf <= 1) ? 1 : 0;
         st->n_imf_out = (ed.N_imf <= 1) ? 1 : 0;
+        if (ed.N_imf <= 1) atomicAdd(ed.ndone, 1);
         for (int i2 = 0; i2 < ed.N_imf; ++i2) ed.sift_counts[i2] = 0;
+        if (br.x == 0) zero_counters(wl);
     }
 }
 
@@ -121,17 +145,18 @@ __device__ __forceinline__ double project_at(const double* __restrict__ Vk, cons
     return acc;
 }
 
-__global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict__ eps,
-                                                     const int* __restrict__ off, int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
+// Wide-channel fallback (n > kFlagLdsMaxCh): one workgroup per (direction, tile),
+// signal read through the caches.
+__global__ __launch_bounds__(kFlagTile) void k_flags_global(const EpochDesc* __restrict__ eps,
+                                                     const int2* __restrict__ tab) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
-    int local = blockIdx.x - off[e];
     const int n = ed.n, T = ed.T;
     int tiles = (T + kFlagTile - 1) / kFlagTile;
-    int k = local / tiles;
-    int tile = local - k * tiles;
+    int k = br.y / tiles;
+    int tile = br.y - k * tiles;
     const double* cur = cur_slot(ed, st->imf);
     const double* Vk = ed.V + (size_t)k * n;
     __shared__ double Ps[kFlagTile + 2];
@@ -163,376 +188,552 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
     }
 }
 
-// ------------------------------------------------------------ compaction
 
-__global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ eps,
-                                                 const int* __restrict__ off, int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
+// One workgroup per (epoch, 256-sample tile) and all directions: the tile of
+// the current signal (plus one sample each side) is staged in LDS once, then
+// every direction's projection, extremum tests and ballots run from LDS.
+__global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict__ eps,
+                                                     const int2* __restrict__ tab) {
+    extern __shared__ double curT[];   // [n][kFlagTile + 2]
+    __shared__ double Ps[kFlagTile + 2];
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
-    int local = blockIdx.x - off[e];   // = k*2 + env
-    const int W = ed.W;
-    const unsigned long long* m = ed.mask + (size_t)local * W;
-    int* pre = ed.pre + (size_t)local * W;
-    int* X = ed.X + (size_t)local * ed.Ecap;
-    int tid = threadIdx.x;
-    int per = (W + 255) / 256;
-    int w0 = tid * per;
-    int w1 = min(w0 + per, W);
-    int mine = 0;
-    for (int w = w0; w < w1; ++w) mine += __popcll(m[w]);
-    __shared__ int sc[256];
-    sc[tid] = mine;
+    const int n = ed.n, T = ed.T, K = ed.K;
+    constexpr int TW = kFlagTile + 2;
+    const int tile = br.y;
+    const int t0 = tile * kFlagTile;
+    const double* cur = cur_slot(ed, st->imf);
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < n * TW; idx += kFlagTile) {
+        int c = idx / TW, i = idx - c * TW;
+        int t = t0 - 1 + i;
+        curT[idx] = (t >= 0 && t < T) ? cur[(size_t)c * T + t] : 0.0;
+    }
     __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {   // inclusive Hillis-Steele scan of 256 partials
-        int v = (tid >= d) ? sc[tid - d] : 0;
+    const int lim = window_limit(T, ed.window);
+    const int t = t0 + tid;
+    const int lane = tid & 63;
+    const int w = (t0 >> 6) + (tid >> 6);
+    for (int k = 0; k < K; ++k) {
+        const double* Vk = ed.V + (size_t)k * n;
+        // positions t0-1+i for i = tid and (tid < 2) i = 256 + tid
+        double acc = 0.0;
+        for (int c = 0; c < n; ++c) acc = fma(Vk[c], curT[c * TW + tid], acc);
+        Ps[tid] = acc;
+        if (tid < 2) {
+            double acc2 = 0.0;
+            for (int c = 0; c < n; ++c) acc2 = fma(Vk[c], curT[c * TW + kFlagTile + tid], acc2);
+            Ps[kFlagTile + tid] = acc2;
+        }
         __syncthreads();
-        sc[tid] += v;
+        bool fmax = false, fmin = false;
+        if (t < T) {
+            if (t == 0 || t == T - 1) {
+                fmax = fmin = true;
+            } else if (t <= lim) {
+                double p = Ps[tid + 1], a = Ps[tid], b = Ps[tid + 2];
+                fmax = (p > a) && (p > b);
+                fmin = (p < a) && (p < b);
+            }
+        }
+        unsigned long long mmax = __ballot(fmax);
+        unsigned long long mmin = __ballot(fmin);
+        if (lane == 0 && w < ed.W) {
+            ed.mask[((size_t)k * 2 + 0) * ed.W + w] = mmax;
+            ed.mask[((size_t)k * 2 + 1) * ed.W + w] = mmin;
+        }
         __syncthreads();
     }
-    int base = sc[tid] - mine;
-    for (int w = w0; w < w1; ++w) {
-        unsigned long long bits = m[w];
-        pre[w] = base;
-        while (bits) {
-            int b = __ffsll((long long)bits) - 1;
-            X[base++] = (w << 6) + b;
-            bits &= bits - 1;
+}
+
+// ------------------------------------------------------------ compaction
+
+// Work items one direction-envelope with E knots contributes to each solver list.
+__device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
+    const int J = (E - 3) / kSolverBlock;
+    const int bpw = 256 / ed.n;
+    m[kListFactor] = 1;
+    m[kListKnots] = (E + kKnotChunk - 1) / kKnotChunk;
+    m[kListBlock] = (J + 1 + bpw - 1) / bpw;
+    m[kListReduced] = J > 0 ? 1 : 0;
+    m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * (E - 1) + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
+}
+
+// One workgroup per (epoch, direction): both envelopes' knot lists, then the
+// work items of a valid direction appended to the solver lists.
+__global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab,
+                                                 WorkLists wl) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
+    const EpochState* st = ed.st;
+    if (st->done) return;
+    const int k = br.y;
+    const int W = ed.W;
+    const int tid = threadIdx.x;
+    __shared__ int sc[256];
+    __shared__ int sE[2];
+    for (int env = 0; env < 2; ++env) {
+        const int ke = 2 * k + env;
+        const unsigned long long* m = ed.mask + (size_t)ke * W;
+        int* pre = ed.pre + (size_t)ke * W;
+        int* X = ed.X + (size_t)ke * ed.Ecap;
+        int per = (W + 255) / 256;
+        int w0 = tid * per;
+        int w1 = min(w0 + per, W);
+        int mine = 0;
+        for (int w = w0; w < w1; ++w) mine += __popcll(m[w]);
+        __syncthreads();
+        sc[tid] = mine;
+        __syncthreads();
+        for (int d = 1; d < 256; d <<= 1) {   // inclusive Hillis-Steele scan of 256 partials
+            int v = (tid >= d) ? sc[tid - d] : 0;
+            __syncthreads();
+            sc[tid] += v;
+            __syncthreads();
+        }
+        int base = sc[tid] - mine;
+        for (int w = w0; w < w1; ++w) {
+            unsigned long long bits = m[w];
+            pre[w] = base;
+            while (bits) {
+                int b = __ffsll((long long)bits) - 1;
+                X[base++] = (w << 6) + b;
+                bits &= bits - 1;
+            }
+        }
+        if (tid == 255) {
+            int Etot = sc[255];
+            sE[env] = Etot;
+            ed.E[ke] = Etot;
+            if (ed.d_trace && st->pass < ed.max_passes)
+                ed.d_trace[(size_t)st->pass * 2 * ed.K + ke] = Etot;
+            if (ed.stats) {
+                atomicAdd(&ed.stats[0], (unsigned long long)Etot);
+                atomicAdd(&ed.stats[1], (unsigned long long)Etot * (unsigned long long)ed.n);
+            }
         }
     }
-    if (tid == 255) {
-        int Etot = sc[255];
-        ed.E[local] = Etot;
-        if (ed.stats) {
-            atomicAdd(&ed.stats[0], (unsigned long long)Etot);
-            atomicAdd(&ed.stats[1], (unsigned long long)Etot * (unsigned long long)ed.n);
+    __syncthreads();
+    if (!(sE[0] > 3 && sE[1] > 3)) return;
+    __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
+    if (tid < 2) {
+        int m[kNumLists];
+        list_sizes(ed, sE[tid], m);
+        for (int l = 0; l < kNumLists; ++l) {
+            s_m[tid][l] = m[l];
+            s_base[tid][l] = atomicAdd(&wl.count[l], m[l]);
         }
-        if (ed.d_trace && st->pass < ed.max_passes)
-            ed.d_trace[(size_t)st->pass * 2 * ed.K + local] = Etot;
     }
+    __syncthreads();
+    for (int env = 0; env < 2; ++env)
+        for (int l = 0; l < kNumLists; ++l)
+            for (int i = tid; i < s_m[env][l]; i += 256) {
+                int at = s_base[env][l] + i;
+                if (at < wl.cap[l]) wl.item[l][at] = make_int4(br.x, 2 * k + env, i, 0);
+            }
 }
 
 // ------------------------------------------------- blocked tridiagonal solver
 //
-// Knot system of one (direction, envelope): rows u = 1..N (N = E-2) with
+// Knot system of one (direction, envelope): rows u = 1..N with
 // lo_u = h_{u-1}, di_u = 2(X_{u+1}-X_{u-1}), up_u = h_u and, per channel,
-// r_u = 3 (sl_u - sl_{u-1}), sl_u = (Y_{u+1}-Y_u)/h_u; M_0 = M_{E-1} = 0.
-// Blocks of L = kSolverBlock rows: separators at u = jL (j = 1..J, J = (N-1)/L),
-// block b = rows bL+1 .. (b<J ? (b+1)L-1 : N).
+// r_u = 3 (sl_u - sl_{u-1}), sl_u = (Y_{u+1}-Y_u) * ih_u; M_0 = M_{E-1} = 0.
+// Same arithmetic as the oracle's struct Blocked.
 
-struct BlockGeom {
-    int N, J, p, q;
-};
-
-__device__ __forceinline__ BlockGeom block_geom(int E, int b) {
-    BlockGeom g;
-    g.N = E - 2;
-    g.J = (g.N - 1) / kSolverBlock;
-    g.p = b * kSolverBlock + 1;
-    g.q = (b < g.J) ? (b + 1) * kSolverBlock - 1 : g.N;
-    return g;
-}
-
-__device__ __forceinline__ void solver_lane(const EpochDesc& ed, int local, int& k, int& env, int& b, int& c,
-                                            bool& active) {
-    int bpw = 256 / ed.n;
-    int wgs = (ed.nbmax + bpw - 1) / bpw;
-    int ke = local / wgs;          // k*2 + env
-    int wg = local - ke * wgs;
-    k = ke >> 1;
-    env = ke & 1;
-    int slot = threadIdx.x / ed.n;
-    c = threadIdx.x - slot * ed.n;
-    b = wg * bpw + slot;
-    active = slot < bpw;
-}
-
-// Phase 1: per (block, channel) the first/last entries of A_b^{-1} r; per
-// block also the spike ends (written by channel 0).  Forward sweep gives the
-// last entries, reverse sweep the first ones.
-__global__ __launch_bounds__(256) void k_solve1(const EpochDesc* __restrict__ eps,
-                                                const int* __restrict__ off, int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
-    const EpochState* st = ed.st;
-    if (st->done) return;
-    int k, env, b, c;
-    bool active;
-    solver_lane(ed, blockIdx.x - off[e], k, env, b, c, active);
-    if (!active || !dir_valid(ed.E, k)) return;
-    int ke = 2 * k + env;
-    int E = ed.E[ke];
-    BlockGeom g = block_geom(E, b);
-    if (g.J == 0 || b > g.J) return;
-    KnotRow kr{ed.X + (size_t)ke * ed.Ecap, cur_slot(ed, st->imf) + (size_t)c * ed.T, E, env};
-    const int p = g.p, q = g.q;
-    const bool left = b >= 1, right = b < g.J;
-
-    // forward sweep u = p..q
-    double cp = 0.0, zy = 0.0, za = 0.0, zb = 0.0;
-    double yprev = kr.Y(p - 1), ycur = kr.Y(p);
-    double slprev = (ycur - yprev) / kr.h(p - 1);
-    for (int u = p; u <= q; ++u) {
-        double ynext = kr.Y(u + 1);
-        double hu = kr.h(u), hl = kr.h(u - 1);
-        double sl = (ynext - ycur) / hu;
-        double r = 3.0 * (sl - slprev);
-        double lo = hl, up = hu;
-        double di = (double)(2 * ((kr.X[u] - kr.X[u - 1]) + (kr.X[u + 1] - kr.X[u])));
-        double ga = (u == p) ? lo : 0.0;
-        double gb = (u == q) ? up : 0.0;
-        double den = (u == p) ? di : fma(-lo, cp, di);
-        double rd = 1.0 / den;
-        cp = up * rd;
-        if (u == p) {
-            zy = r * rd; za = ga * rd; zb = gb * rd;
-        } else {
-            zy = fma(-lo, zy, r) * rd;
-            za = fma(-lo, za, ga) * rd;
-            zb = fma(-lo, zb, gb) * rd;
-        }
-        ycur = ynext;
-        slprev = sl;
-    }
-    double yq = zy, aq = za, bq = zb;
-
-    // reverse sweep u = q..p
-    double cl = 0.0;
-    double ry = 0.0, ra = 0.0, rb = 0.0;
-    double ynext = kr.Y(q + 1);
-    ycur = kr.Y(q);
-    double slnext = (ynext - ycur) / kr.h(q);
-    for (int u = q; u >= p; --u) {
-        double yp = kr.Y(u - 1);
-        double hl = kr.h(u - 1), hu = kr.h(u);
-        double sl = (ycur - yp) / hl;
-        double r = 3.0 * (slnext - sl);
-        double lo = hl, up = hu;
-        double di = (double)(2 * ((kr.X[u] - kr.X[u - 1]) + (kr.X[u + 1] - kr.X[u])));
-        double ga = (u == p) ? lo : 0.0;
-        double gb = (u == q) ? up : 0.0;
-        double den = (u == q) ? di : fma(-up, cl, di);
-        double rd = 1.0 / den;
-        cl = lo * rd;
-        if (u == q) {
-            ry = r * rd; ra = ga * rd; rb = gb * rd;
-        } else {
-            ry = fma(-up, ry, r) * rd;
-            ra = fma(-up, ra, ga) * rd;
-            rb = fma(-up, rb, gb) * rd;
-        }
-        ycur = yp;
-        slnext = sl;
-    }
-    size_t bi = (size_t)ke * ed.nbmax + b;
-    double* be = ed.bend + (bi * ed.n + c) * 2;
-    be[0] = ry;   // first entry
-    be[1] = yq;   // last entry
-    if (c == 0) {
-        double* sp = ed.bspk + bi * 4;
-        sp[0] = left ? ra : 0.0;
-        sp[1] = left ? aq : 0.0;
-        sp[2] = right ? rb : 0.0;
-        sp[3] = right ? bq : 0.0;
-    }
-}
-
-// Phase 2: reduced system of the J separators, one lane per channel.
-__global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ eps,
-                                                 const int* __restrict__ off, int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
-    const EpochState* st = ed.st;
-    if (st->done) return;
-    int ke = blockIdx.x - off[e];
-    int k = ke >> 1, env = ke & 1;
-    int c = threadIdx.x;
-    if (c >= ed.n || !dir_valid(ed.E, k)) return;
-    int E = ed.E[ke];
-    int N = E - 2;
-    int J = (N - 1) / kSolverBlock;
-    if (J == 0) return;
-    KnotRow kr{ed.X + (size_t)ke * ed.Ecap, cur_slot(ed, st->imf) + (size_t)c * ed.T, E, env};
-    const double* sp = ed.bspk + (size_t)ke * ed.nbmax * 4;
-    const double* be = ed.bend + (size_t)ke * ed.nbmax * ed.n * 2;
-    double* sv = ed.sep + (size_t)ke * ed.nbmax * ed.n * 2;
-    double cp = 0.0, z = 0.0;
-    for (int j = 1; j <= J; ++j) {
-        int s = j * kSolverBlock;
-        double ym = kr.Y(s - 1), y0 = kr.Y(s), yp = kr.Y(s + 1);
-        double hl = kr.h(s - 1), hu = kr.h(s);
-        double r = 3.0 * ((yp - y0) / hu - (y0 - ym) / hl);
-        double lo = hl, up = hu;
-        double di = (double)(2 * ((kr.X[s] - kr.X[s - 1]) + (kr.X[s + 1] - kr.X[s])));
-        const double* spL = sp + (size_t)(j - 1) * 4;   // block j-1
-        const double* spR = sp + (size_t)j * 4;         // block j
-        double Rlo = (j >= 2) ? -(lo * spL[1]) : 0.0;
-        double Rdi = fma(-up, spR[0], fma(-lo, spL[3], di));
-        double Rup = (j <= J - 1) ? -(up * spR[2]) : 0.0;
-        double yqL = be[((size_t)(j - 1) * ed.n + c) * 2 + 1];
-        double ypR = be[((size_t)j * ed.n + c) * 2 + 0];
-        double D = fma(-up, ypR, fma(-lo, yqL, r));
-        double den = (j == 1) ? Rdi : fma(-Rlo, cp, Rdi);
-        double rd = 1.0 / den;
-        cp = Rup * rd;
-        z = (j == 1) ? D * rd : fma(-Rlo, z, D) * rd;
-        double* slot = sv + ((size_t)j * ed.n + c) * 2;
-        slot[0] = z;
-        slot[1] = cp;
-    }
-    double x = z;   // x_J = z_J
-    for (int j = J - 1; j >= 1; --j) {
-        double* slot = sv + ((size_t)j * ed.n + c) * 2;
-        x = fma(-slot[1], x, slot[0]);
-        slot[0] = x;
-    }
-}
-
-// Phase 3: re-solve every block with its separator values and emit the spline
-// coefficients of the intervals it owns (bL .. q).  With J == 0 this is the
-// whole (single-block) solve.
-__global__ __launch_bounds__(256) void k_solve3(const EpochDesc* __restrict__ eps,
-                                                const int* __restrict__ off, int count) {
+// Shared factors of one (direction, envelope): per block the forward factors,
+// the reciprocals of the intervals it owns and its two spike vectors; then the
+// reduced system's factorisation (one lane, J steps).
+__global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
-    const EpochState* st = ed.st;
-    if (st->done) return;
-    int k, env, b, c;
-    bool active;
-    solver_lane(ed, blockIdx.x - off[e], k, env, b, c, active);
-    if (!active || !dir_valid(ed.E, k)) return;
-    int ke = 2 * k + env;
-    int E = ed.E[ke];
-    BlockGeom g = block_geom(E, b);
-    if (b > g.J) return;
-    KnotRow kr{ed.X + (size_t)ke * ed.Ecap, cur_slot(ed, st->imf) + (size_t)c * ed.T, E, env};
-    const int p = g.p, q = g.q;
-    const double* sv = ed.sep + (size_t)ke * ed.nbmax * ed.n * 2;
-    double XL = (b >= 1) ? sv[((size_t)b * ed.n + c) * 2] : 0.0;        // M at row p-1
-    double XR = (b < g.J) ? sv[((size_t)(b + 1) * ed.n + c) * 2] : 0.0; // M at row q+1
+    const int cnt = min(wl.count[kListFactor], wl.cap[kListFactor]);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int4 it = wl.item[kListFactor][w];
+        const EpochDesc& ed = eps[it.x];
+        const int ke = it.y;
+        const int E = ed.E[ke];
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
+        double* rd = fac_row(ed, ke, kFacRd);
+        double* cp = fac_row(ed, ke, kFacCp);
+        double* al = fac_row(ed, ke, kFacAl);
+        double* be = fac_row(ed, ke, kFacBe);
+        double* ih = fac_row(ed, ke, kFacIh);
+        double* i3h = fac_row(ed, ke, kFacI3h);
+        const int J = block_geom(E, 0).J;
+        for (int b = threadIdx.x; b <= J; b += blockDim.x) {
+            BlockGeom g = block_geom(E, b);
+            const int p = g.p, q = g.q;
+            for (int j = p - 1; j <= q; ++j) {   // intervals owned by this block
+                double h = (double)(X[j + 1] - X[j]);
+                ih[j] = 1.0 / h;
+                i3h[j] = 1.0 / (3.0 * h);
+            }
+            double cprev = 0.0;
+            for (int u = p; u <= q; ++u) {
+                double lo = (double)(X[u] - X[u - 1]), up = (double)(X[u + 1] - X[u]);
+                double di = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
+                double den = (u == p) ? di : fma(-lo, cprev, di);
+                double r = 1.0 / den;
+                cprev = up * r;
+                rd[u] = r;
+                cp[u] = cprev;
+            }
+            if (J == 0) continue;
+            if (b >= 1) {   // left spike A_b^{-1}(lo_p e_first): forward sweep into al, back substitution
+                double z = 0.0;
+                for (int u = p; u <= q; ++u) {
+                    double lo = (double)(X[u] - X[u - 1]);
+                    double gv = (u == p) ? lo : 0.0;
+                    z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
+                    al[u] = z;
+                }
+                double x = al[q];
+                for (int u = q - 1; u >= p; --u) {
+                    x = fma(-cp[u], x, al[u]);
+                    al[u] = x;
+                }
+            }
+            if (b < J) {    // right spike A_b^{-1}(up_q e_last)
+                double z = 0.0;
+                for (int u = p; u <= q; ++u) {
+                    double lo = (double)(X[u] - X[u - 1]);
+                    double gv = (u == q) ? (double)(X[u + 1] - X[u]) : 0.0;
+                    z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
+                    be[u] = z;
+                }
+                double x = be[q];
+                for (int u = q - 1; u >= p; --u) {
+                    x = fma(-cp[u], x, be[u]);
+                    be[u] = x;
+                }
+            }
+        }
+        __syncthreads();
+        if (J > 0 && threadIdx.x == 0) {
+            double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
+            double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
+            double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
+            double cprev = 0.0;
+            for (int j = 1; j <= J; ++j) {
+                int s = j * L;
+                double lo = (double)(X[s] - X[s - 1]), up = (double)(X[s + 1] - X[s]);
+                double di = (double)(2 * ((X[s] - X[s - 1]) + (X[s + 1] - X[s])));
+                double rlo = (j >= 2) ? -(lo * al[s - 1]) : 0.0;
+                double rdi = fma(-up, al[s + 1], fma(-lo, be[s - 1], di));
+                double rup = (j <= J - 1) ? -(up * be[s + 1]) : 0.0;
+                double den = (j == 1) ? rdi : fma(-rlo, cprev, rdi);
+                double r = 1.0 / den;
+                cprev = rup * r;
+                Rlo[j] = rlo;
+                Rrd[j] = r;
+                Rcp[j] = cprev;
+            }
+        }
+        __syncthreads();
+    }
+}
 
-    double zarr[L], cparr[L];
-    double cp = 0.0, z = 0.0;
-    double yprev = kr.Y(p - 1), ycur = kr.Y(p);
-    double slprev = (ycur - yprev) / kr.h(p - 1);
+// Knot values and right-hand sides, per (direction, envelope) and 256-knot
+// chunk: Y_i (end knots clamped) and r_i = 3 (sl_i - sl_{i-1}) with
+// sl_i = (Y_{i+1} - Y_i) * ih_i, for every channel; lanes run over knots.
+__global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restrict__ eps, WorkLists wl) {
+    constexpr int CH = kKnotCh;
+    __shared__ double Ys[CH][kKnotChunk + 2];
+    const int cnt = min(wl.count[kListKnots], wl.cap[kListKnots]);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int4 it = wl.item[kListKnots][w];
+        const EpochDesc& ed = eps[it.x];
+        const EpochState* st = ed.st;
+        const int ke = it.y, env = ke & 1;
+        const int E = ed.E[ke];
+        const int i0 = it.z * kKnotChunk;
+        const int n = ed.n;
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
+        const double* ihv = fac_row(ed, ke, kFacIh);
+        const double* cur = cur_slot(ed, st->imf);
+        const int l = threadIdx.x;
+        const int i = i0 + l;
+        for (int c0 = 0; c0 < n; c0 += CH) {
+            __syncthreads();
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-        int u = p + i;
-        if (u <= q) {
-            double ynext = kr.Y(u + 1);
-            double hu = kr.h(u), hl = kr.h(u - 1);
-            double sl = (ynext - ycur) / hu;
-            double gv = 3.0 * (sl - slprev);
-            double lo = hl, up = hu;
-            double di = (double)(2 * ((kr.X[u] - kr.X[u - 1]) + (kr.X[u + 1] - kr.X[u])));
-            if (u == p && b >= 1) gv = fma(-lo, XL, gv);
-            if (u == q && b < g.J) gv = fma(-up, XR, gv);
-            double den = (u == p) ? di : fma(-lo, cp, di);
-            double rd = 1.0 / den;
-            cp = up * rd;
-            z = (u == p) ? gv * rd : fma(-lo, z, gv) * rd;
-            zarr[i] = z;
-            cparr[i] = cp;
-            ycur = ynext;
-            slprev = sl;
+            for (int cc = 0; cc < CH; ++cc) {
+                int c = c0 + cc;
+                if (c >= n) break;
+                KnotRow kr{X, cur + (size_t)c * ed.T, E, env};
+                if (i < E) Ys[cc][l + 1] = kr.Y(i);
+                if (l == 0 && i0 >= 1) Ys[cc][0] = kr.Y(i0 - 1);
+                if (l == kKnotChunk - 1 && i0 + kKnotChunk < E) Ys[cc][kKnotChunk + 1] = kr.Y(i0 + kKnotChunk);
+            }
+            __syncthreads();
+            if (i < E) {
+                double ihi = (i <= E - 2) ? ihv[i] : 0.0;
+                double ihm = (i >= 1) ? ihv[i - 1] : 0.0;
+#pragma unroll
+                for (int cc = 0; cc < CH; ++cc) {
+                    int c = c0 + cc;
+                    if (c >= n) break;
+                    size_t at = ((size_t)ke * n + c) * ed.Ecap + i;
+                    ed.yknot[at] = Ys[cc][l + 1];
+                    if (i >= 1 && i <= E - 2) {
+                        double sl = (Ys[cc][l + 2] - Ys[cc][l + 1]) * ihi;
+                        double slm = (Ys[cc][l + 1] - Ys[cc][l]) * ihm;
+                        ed.ysol[at] = 3.0 * (sl - slm);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Phase 1: y = A_b^{-1} r per (block, channel), fma chains from registers.
+// The block factors (rd, cp, lo) are shared by the block's channels and staged
+// in LDS; each lane loads its block's right-hand sides up front.  Rows of
+// separators keep their r for the reduced system.
+__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
+    constexpr int L = kSolverBlock;
+    __shared__ double s_rd[8][L], s_cp[8][L], s_lo[8][L];
+    const int cnt = min(wl.count[kListBlock], wl.cap[kListBlock]);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int4 it = wl.item[kListBlock][w];
+        const EpochDesc& ed = eps[it.x];
+        const int ke = it.y;
+        const int n = ed.n;
+        const int bpw = 256 / n;
+        const int E = ed.E[ke];
+        const int J = block_geom(E, 0).J;
+        const int b0 = it.z * bpw;
+        const int nb = min(bpw, J - b0 + 1);
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
+        const double* rdv = fac_row(ed, ke, kFacRd);
+        const double* cpv = fac_row(ed, ke, kFacCp);
+        for (int pass0 = 0; pass0 < nb; pass0 += 8) {
+            int npass = min(8, nb - pass0);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < npass * L; idx += 256) {
+                int sb = idx / L, i = idx - sb * L;
+                BlockGeom g = block_geom(E, b0 + pass0 + sb);
+                int u = min(g.p + i, g.q);
+                s_rd[sb][i] = rdv[u];
+                s_cp[sb][i] = cpv[u];
+                s_lo[sb][i] = (double)(X[u] - X[u - 1]);
+            }
+            __syncthreads();
+            int slot = threadIdx.x / n;
+            int c = threadIdx.x - slot * n;
+            int sb = slot - pass0;
+            if (slot < bpw && sb >= 0 && sb < npass) {
+                BlockGeom g = block_geom(E, b0 + slot);
+                const int p = g.p, q = g.q;
+                double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
+                double z[L];
+#pragma unroll
+                for (int i = 0; i < L; ++i) z[i] = row[min(p + i, q)];
+#pragma unroll
+                for (int i = 0; i < L; ++i)
+                    z[i] = (i == 0) ? z[0] * s_rd[sb][0] : fma(-s_lo[sb][i], z[i > 0 ? i - 1 : 0], z[i]) * s_rd[sb][i];
+                double x = 0.0;
+#pragma unroll
+                for (int i = L - 1; i >= 0; --i) {
+                    int u = p + i;
+                    if (u <= q) {
+                        x = (u == q) ? z[i] : fma(-s_cp[sb][i], x, z[i]);
+                        row[u] = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Phase 2: separator values, one lane per channel (no division: the reduced
+// factorisation comes from k_factor).
+__global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
+    constexpr int L = kSolverBlock;
+    const int cnt = min(wl.count[kListReduced], wl.cap[kListReduced]);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int4 it = wl.item[kListReduced][w];
+        const EpochDesc& ed = eps[it.x];
+        const int ke = it.y;
+        const int n = ed.n;
+        const int E = ed.E[ke];
+        const int J = (E - 3) / L;
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
+        const double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
+        const double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
+        const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
+            double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
+            double z = 0.0;
+            for (int j = 1; j <= J; ++j) {
+                int s = j * L;
+                double lo = (double)(X[s] - X[s - 1]), up = (double)(X[s + 1] - X[s]);
+                double D = fma(-up, row[s + 1], fma(-lo, row[s - 1], row[s]));
+                z = (j == 1) ? D * Rrd[j] : fma(-Rlo[j], z, D) * Rrd[j];
+                sv[j] = z;
+            }
+            double x = z;
+            for (int j = J - 1; j >= 1; --j) {
+                x = fma(-Rcp[j], x, sv[j]);
+                sv[j] = x;
+            }
         }
     }
-    // back substitution, emitting interval u's coefficients as soon as M_u is known
-    double* cf = ed.coef + ((size_t)ke * ed.n + c) * (size_t)ed.Ecap * 4;
-    double xnext = XR;   // M_{u+1}
-    double ynext = kr.Y(q + 1);
+}
+
+// Phase 3 + coefficients: for every interval j of every channel, M_j and
+// M_{j+1} (x_u = fma(-X_{b+1}, be_u, fma(-X_b, al_u, y_u)) inside blocks, the
+// separator values at separator rows, 0 at the two end knots) and the spline
+// record (a, b, c, d) of memReducedEpochs_dev.cu:775-793 with the shared
+// reciprocals.  Lanes run over consecutive intervals of one channel, so the
+// record rows are written coalesced; each item is 256*kCoefItems (channel,
+// interval) pairs, four per thread at a time with their loads issued together.
+__device__ __forceinline__ double knot_M(const double* row, const double* sv, const double* al,
+                                         const double* be, int u, int E, int J) {
+    constexpr int L = kSolverBlock;
+    if (u == 0 || u == E - 1) return 0.0;
+    if (u % L == 0 && u / L <= J) return sv[u / L];
+    int b = (u - 1) / L;
+    double m = row[u];
+    if (b >= 1) m = fma(-sv[b], al[u], m);
+    if (b < J) m = fma(-sv[b + 1], be[u], m);
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
+    constexpr int L = kSolverBlock;
+    const int cnt = min(wl.count[kListCoef], wl.cap[kListCoef]);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int4 it = wl.item[kListCoef][w];
+        const EpochDesc& ed = eps[it.x];
+        const int ke = it.y;
+        const int n = ed.n;
+        const int E = ed.E[ke];
+        const int J = (E - 3) / L;
+        const int nI = E - 1;
+        const int items = n * nI;
+        const int base = it.z * 256 * kCoefItems;
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
+        const double* al = fac_row(ed, ke, kFacAl);
+        const double* be = fac_row(ed, ke, kFacBe);
+        const double* ihv = fac_row(ed, ke, kFacIh);
+        const double* i3hv = fac_row(ed, ke, kFacI3h);
+        double4* out = reinterpret_cast<double4*>(ed.coef);
+        for (int r0 = 0; r0 < kCoefItems; r0 += 4) {
+            double y0[4], y1[4], m0[4], m1[4], h[4], ih[4], i3h[4];
+            int cs[4], js[4];
+            bool ok[4];
 #pragma unroll
-    for (int i = L - 1; i >= 0; --i) {
-        int u = p + i;
-        if (u <= q) {
-            double x = (u == q) ? zarr[i] : fma(-cparr[i], xnext, zarr[i]);
-            double yu = kr.Y(u);
-            double hu = kr.h(u);
-            double sl = (ynext - yu) / hu;
-            double* o = cf + (size_t)u * 4;
-            o[0] = yu;
-            o[1] = sl - (hu * (2.0 * x + xnext)) / 3.0;
-            o[2] = x;
-            o[3] = (xnext - x) / (3.0 * hu);
-            xnext = x;
-            ynext = yu;
+            for (int a = 0; a < 4; ++a) {
+                int idx = base + (r0 + a) * 256 + threadIdx.x;
+                ok[a] = idx < items;
+                int id2 = ok[a] ? idx : 0;
+                int c = id2 / nI;
+                int j = id2 - c * nI;
+                cs[a] = c;
+                js[a] = j;
+                const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
+                const double* yk = ed.yknot + ((size_t)ke * n + c) * ed.Ecap;
+                const double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
+                y0[a] = yk[j];
+                y1[a] = yk[j + 1];
+                m0[a] = knot_M(row, sv, al, be, j, E, J);
+                m1[a] = knot_M(row, sv, al, be, j + 1, E, J);
+                h[a] = (double)(X[j + 1] - X[j]);
+                ih[a] = ihv[j];
+                i3h[a] = i3hv[j];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                if (!ok[a]) continue;
+                double sl = (y1[a] - y0[a]) * ih[a];
+                double4 r;
+                r.x = y0[a];
+                r.y = sl - (h[a] * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
+                r.z = m0[a];
+                r.w = (m1[a] - m0[a]) * i3h[a];
+                out[((size_t)ke * n + cs[a]) * ed.Ecap + js[a]] = r;
+            }
         }
-    }
-    {   // interval p-1 (from separator row bL, or from knot 0)
-        int u = p - 1;
-        double x = XL;
-        double yu = kr.Y(u);
-        double hu = kr.h(u);
-        double sl = (ynext - yu) / hu;
-        double* o = cf + (size_t)u * 4;
-        o[0] = yu;
-        o[1] = sl - (hu * (2.0 * x + xnext)) / 3.0;
-        o[2] = x;
-        o[3] = (xnext - x) / (3.0 * hu);
     }
 }
 
 // --------------------------------------------- fused evaluate + mean + update
 
-__device__ __forceinline__ double horner(const double* __restrict__ cf, double tau) {
-    double2 ab = *reinterpret_cast<const double2*>(cf);
-    double2 cd = *reinterpret_cast<const double2*>(cf + 2);
-    return fma(fma(fma(cd.y, tau, cd.x), tau, ab.y), tau, ab.x);
-}
-
-__device__ __forceinline__ int knot_interval(const EpochDesc& ed, int ke, int w, int lane, int E) {
+__device__ __forceinline__ int knot_interval(const EpochDesc& ed, int ke, int t, int E) {
     // j = #knots strictly before t, minus one, clamped to [0, E-2]
+    int w = t >> 6, bit = t & 63;
     unsigned long long m = ed.mask[(size_t)ke * ed.W + w];
-    int before = ed.pre[(size_t)ke * ed.W + w] + __popcll(m & ((1ull << lane) - 1ull));
+    int before = ed.pre[(size_t)ke * ed.W + w] + __popcll(m & ((1ull << bit) - 1ull));
     int j = before - 1;
     j = j < 0 ? 0 : j;
     j = j > E - 2 ? E - 2 : j;
     return j;
 }
 
-// Fix mode: cur -= sum_valid((e+ + e-)/2) / nvalid, and on the S-th sift of an
-// IMF also running -= cur and the next slot <- running.
-__global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps,
-                                             const int* __restrict__ off, int count) {
-    int e = find_epoch(off, count, blockIdx.x);
-    const EpochDesc& ed = eps[e];
+__device__ __forceinline__ double horner4(const double4& r, double tau) {
+    return fma(fma(fma(r.w, tau, r.z), tau, r.y), tau, r.x);
+}
+
+// One wavefront = 64 consecutive samples x CPL channels.  Directions are
+// processed U at a time: all their spline records are loaded before any is
+// used, then the means are added in ascending direction order.
+template <int CPL, int U>
+__global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
     const int n = ed.n, T = ed.T, K = ed.K;
-    int local = blockIdx.x - off[e];
-    int groups = (n + kEvalCPL - 1) / kEvalCPL;
-    int tile = local / groups;
-    int cg = local - tile * groups;
-    int lane = threadIdx.x;
-    int t = tile * 64 + lane;
-    int c0 = cg * kEvalCPL;
-    double acc[kEvalCPL];
+    int groups = (n + CPL - 1) / CPL;
+    int tile = br.y / groups;
+    int cg = br.y - tile * groups;
+    const int lane = threadIdx.x;
+    const int t = tile * 64 + lane;
+    const int tq = min(t, T - 1);   // lookups stay in range on a ragged last tile
+    const int c0 = cg * CPL;
+    const double4* coef = reinterpret_cast<const double4*>(ed.coef);
+    double acc[CPL];
 #pragma unroll
-    for (int i = 0; i < kEvalCPL; ++i) acc[i] = 0.0;
+    for (int i = 0; i < CPL; ++i) acc[i] = 0.0;
     double nv = 0.0;
-    for (int k = 0; k < K; ++k) {
-        if (!dir_valid(ed.E, k)) continue;
-        nv = nv + 1.0;
-        int E0 = ed.E[2 * k], E1 = ed.E[2 * k + 1];
-        int j0 = knot_interval(ed, 2 * k, tile, lane, E0);
-        int j1 = knot_interval(ed, 2 * k + 1, tile, lane, E1);
-        double tau0 = (double)(t - ed.X[(size_t)(2 * k) * ed.Ecap + j0]);
-        double tau1 = (double)(t - ed.X[(size_t)(2 * k + 1) * ed.Ecap + j1]);
-        const double* cf0 = ed.coef + ((size_t)(2 * k) * n) * (size_t)ed.Ecap * 4 + (size_t)j0 * 4;
-        const double* cf1 = ed.coef + ((size_t)(2 * k + 1) * n) * (size_t)ed.Ecap * 4 + (size_t)j1 * 4;
+    for (int k0 = 0; k0 < K; k0 += U) {
+        double4 r[U][2][CPL];
+        double tau[U][2];
+        bool ok[U];
 #pragma unroll
-        for (int i = 0; i < kEvalCPL; ++i) {
-            int c = c0 + i;
-            if (c < n) {
-                double ep = horner(cf0 + (size_t)c * ed.Ecap * 4, tau0);
-                double em = horner(cf1 + (size_t)c * ed.Ecap * 4, tau1);
+        for (int u = 0; u < U; ++u) {
+            int k = min(k0 + u, K - 1);
+            ok[u] = (k0 + u < K) && dir_valid(ed.E, k);
+#pragma unroll
+            for (int env = 0; env < 2; ++env) {
+                int ke = 2 * k + env;
+                int E = max(ed.E[ke], 2);
+                int j = knot_interval(ed, ke, tq, E);
+                tau[u][env] = (double)(tq - ed.X[(size_t)ke * ed.Ecap + j]);
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) {
+                    int c = min(c0 + i, n - 1);
+                    r[u][env][i] = coef[((size_t)ke * n + c) * ed.Ecap + j];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!ok[u]) continue;
+            nv = nv + 1.0;
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                double ep = horner4(r[u][0][i], tau[u][0]);
+                double em = horner4(r[u][1][i], tau[u][1]);
                 acc[i] = acc[i] + (ep + em) * 0.5;
             }
         }
     }
-    if (ed.stats && local == 0 && lane == 0) {
+    if (ed.stats && br.y == 0 && lane == 0) {
         atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
         atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
     }
@@ -541,24 +742,154 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps,
     bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
     double* nxt = cur_slot(ed, st->imf + 1);
 #pragma unroll
-    for (int i = 0; i < kEvalCPL; ++i) {
+    for (int i = 0; i < CPL; ++i) {
         int c = c0 + i;
         if (c < n) {
             size_t idx = (size_t)c * T + t;
             double v = cur[idx] - acc[i] / nv;
             cur[idx] = v;
             if (last) {
-                double r = ed.running[idx] - v;
-                ed.running[idx] = r;
-                nxt[idx] = r;
+                double rr = ed.running[idx] - v;
+                ed.running[idx] = rr;
+                nxt[idx] = rr;
             }
         }
     }
 }
 
-// Fix-mode state advance: one thread per epoch.
-__global__ void k_advance(const EpochDesc* __restrict__ eps, int count) {
+// Fused records + evaluation (fix mode).  One workgroup = one 64-sample tile
+// x 16 channels.  Per valid direction, phase A builds in LDS the spline record
+// (a,b,c,d) of every interval meeting the tile, for both envelopes, from M
+// (phase-3 formula), the knot values and the shared reciprocals -- the same
+// operations as k_coef, so the records are bit-identical; phase B evaluates
+// both envelopes (each lane walks 4 consecutive samples of one channel) and
+// adds the mean in ascending direction order.
+__global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict__ eps,
+                                                   const int2* __restrict__ tab) {
+    constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, L = kSolverBlock;
+    __shared__ double4 rec[2][CG][NI];
+    __shared__ int xk[2][NI + 1];
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
+    const EpochState* st = ed.st;
+    if (st->done) return;
+    const int n = ed.n, T = ed.T, K = ed.K;
+    const int groups = (n + CG - 1) / CG;
+    const int tile = br.y / groups;
+    const int cg = br.y - tile * groups;
+    const int t0 = tile * TL;
+    const int tlast = min(t0 + TL - 1, T - 1);
+    const int tid = threadIdx.x;
+    const int q = tid & 15;          // sample quad
+    const int cl = tid >> 4;         // channel in group
+    const int c = cg * CG + cl;
+    const int ts = t0 + 4 * q;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double ep[4] = {0.0, 0.0, 0.0, 0.0};
+    double nv = 0.0;
+    for (int k = 0; k < K; ++k) {
+        if (!dir_valid(ed.E, k)) continue;
+        nv = nv + 1.0;
+        int jlo[2], nint[2], Ek[2];
+#pragma unroll
+        for (int env = 0; env < 2; ++env) {
+            Ek[env] = ed.E[2 * k + env];
+            jlo[env] = knot_interval(ed, 2 * k + env, t0, Ek[env]);
+            nint[env] = knot_interval(ed, 2 * k + env, tlast, Ek[env]) - jlo[env] + 1;
+        }
+        // phase A
+        const int per0 = nint[0] * CG;
+        const int total = per0 + nint[1] * CG;
+        for (int idx = tid; idx < total; idx += 256) {
+            int env = idx < per0 ? 0 : 1;
+            int rem = env == 0 ? idx : idx - per0;
+            int ni = nint[env];
+            int cl2 = rem / ni;
+            int i = rem - cl2 * ni;
+            int c2 = cg * CG + cl2;
+            int ke = 2 * k + env;
+            const int* X = ed.X + (size_t)ke * ed.Ecap;
+            int j = jlo[env] + i;
+            if (cl2 == 0) {
+                xk[env][i] = X[j];
+                if (i == ni - 1) xk[env][ni] = X[j + 1];
+            }
+            if (c2 >= n) continue;
+            const int E = Ek[env];
+            const int J = (E - 3) / L;
+            const double* row = ed.ysol + ((size_t)ke * n + c2) * ed.Ecap;
+            const double* yk = ed.yknot + ((size_t)ke * n + c2) * ed.Ecap;
+            const double* sv = ed.sep + ((size_t)ke * n + c2) * ed.nbmax;
+            const double* al = fac_row(ed, ke, kFacAl);
+            const double* be = fac_row(ed, ke, kFacBe);
+            double y0 = yk[j], y1 = yk[j + 1];
+            double m0 = knot_M(row, sv, al, be, j, E, J);
+            double m1 = knot_M(row, sv, al, be, j + 1, E, J);
+            double h = (double)(X[j + 1] - X[j]);
+            double sl = (y1 - y0) * fac_row(ed, ke, kFacIh)[j];
+            double4 r;
+            r.x = y0;
+            r.y = sl - (h * (2.0 * m0 + m1)) * (1.0 / 3.0);
+            r.z = m0;
+            r.w = (m1 - m0) * fac_row(ed, ke, kFacI3h)[j];
+            rec[env][cl2][i] = r;
+        }
+        __syncthreads();
+        // phase B
+        if (c < n && ts <= tlast) {
+#pragma unroll
+            for (int env = 0; env < 2; ++env) {
+                int i = knot_interval(ed, 2 * k + env, ts, Ek[env]) - jlo[env];
+                int xj = xk[env][i], xn = xk[env][i + 1];
+                double4 r = rec[env][cl][i];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    int t = ts + s;
+                    if (t <= tlast) {
+                        if (t > xn) {
+                            ++i;
+                            xj = xn;
+                            xn = xk[env][i + 1];
+                            r = rec[env][cl][i];
+                        }
+                        double v = horner4(r, (double)(t - xj));
+                        if (env == 0) ep[s] = v;
+                        else acc[s] = acc[s] + (ep[s] + v) * 0.5;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (ed.stats && br.y == 0 && tid == 0) {
+        atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
+        atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
+    }
+    if (c >= n) return;
+    double* cur = cur_slot(ed, st->imf);
+    const bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
+    double* nxt = cur_slot(ed, st->imf + 1);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int t = ts + s;
+        if (t <= tlast) {
+            size_t idx = (size_t)c * T + t;
+            double v = cur[idx] - acc[s] / nv;
+            cur[idx] = v;
+            if (last) {
+                double rr = ed.running[idx] - v;
+                ed.running[idx] = rr;
+                nxt[idx] = rr;
+            }
+        }
+    }
+}
+
+// Fix-mode state advance: one thread per epoch; also clears the work lists
+// for the next step.
+__global__ void k_advance(const EpochDesc* __restrict__ eps, int count, WorkLists wl) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) zero_counters(wl);
     if (e >= count) return;
     const EpochDesc& ed = eps[e];
     EpochState* st = ed.st;
@@ -572,8 +903,174 @@ __global__ void k_advance(const EpochDesc* __restrict__ eps, int count) {
         if (st->imf == ed.N_imf - 1) {
             st->done = 1;
             st->n_imf_out = ed.N_imf;
+            atomicAdd(ed.ndone, 1);
         }
     }
+}
+
+// ------------------------------------------------------------ stop mode
+//
+// memd.m-style criterion (not in the reference, which only runs fixed sift
+// counts; see the oracle's header).  Per sample t: mean(c,t) = sum_valid
+// ((e+ + e-)/2) / nvalid, amp(t) = sum_valid sqrt(sum_c (e+ - e-)^2)/2 / nvalid,
+// q(t) = sum_c mean(c,t)^2, all sums in ascending order.
+
+// One wavefront per 64 samples; every lane owns one sample and walks the
+// channels in order, so the channel sums match the oracle bit for bit.
+__global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab) {
+    extern __shared__ double sums[];   // [n][64]
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
+    const EpochState* st = ed.st;
+    if (st->done) return;
+    const int n = ed.n, T = ed.T, K = ed.K;
+    const int lane = threadIdx.x;
+    const int t = br.y * 64 + lane;
+    const int tq = min(t, T - 1);
+    const double4* coef = reinterpret_cast<const double4*>(ed.coef);
+    for (int c = 0; c < n; ++c) sums[c * 64 + lane] = 0.0;
+    double amps = 0.0, nv = 0.0;
+    for (int k = 0; k < K; ++k) {
+        if (!dir_valid(ed.E, k)) continue;
+        nv = nv + 1.0;
+        int ke0 = 2 * k, ke1 = 2 * k + 1;
+        int j0 = knot_interval(ed, ke0, tq, ed.E[ke0]);
+        int j1 = knot_interval(ed, ke1, tq, ed.E[ke1]);
+        double tau0 = (double)(tq - ed.X[(size_t)ke0 * ed.Ecap + j0]);
+        double tau1 = (double)(tq - ed.X[(size_t)ke1 * ed.Ecap + j1]);
+        double s2 = 0.0;
+        for (int c = 0; c < n; ++c) {
+            double ep = horner4(coef[((size_t)ke0 * n + c) * ed.Ecap + j0], tau0);
+            double em = horner4(coef[((size_t)ke1 * n + c) * ed.Ecap + j1], tau1);
+            double d = ep - em;
+            s2 = s2 + d * d;
+            sums[c * 64 + lane] = sums[c * 64 + lane] + (ep + em) * 0.5;
+        }
+        amps = amps + sqrt(s2) * 0.5;
+    }
+    if (t >= T) return;
+    double q = 0.0;
+    for (int c = 0; c < n; ++c) {
+        double m = (nv > 0.0) ? sums[c * 64 + lane] / nv : 0.0;
+        ed.mean[(size_t)c * T + t] = m;
+        q = q + m * m;
+    }
+    ed.tstat[2 * (size_t)t + 0] = q;
+    ed.tstat[2 * (size_t)t + 1] = (nv > 0.0) ? amps / nv : 0.0;
+    if (t == 0) ed.stopred[0] = (int)nv;
+}
+
+// One workgroup per epoch: the stop decision and the state transition.
+__global__ __launch_bounds__(256) void k_stop_decide(const EpochDesc* __restrict__ eps, int count) {
+    int e = blockIdx.x;
+    if (e >= count) return;
+    const EpochDesc& ed = eps[e];
+    EpochState* st = ed.st;
+    __shared__ int s_done;
+    __shared__ int s_zero, s_cnt, s_any2;
+    if (threadIdx.x == 0) {
+        s_done = st->done;
+        s_zero = 0;
+        s_cnt = 0;
+        s_any2 = 0;
+    }
+    __syncthreads();
+    if (s_done) return;
+    const int T = ed.T, K = ed.K;
+    const bool nvzero = ed.stopred[0] == 0;
+    // every amp(t) nonzero? (memd.m divides by amp only then)
+    int zero = 0;
+    for (int t = threadIdx.x; t < T; t += blockDim.x)
+        if (!(ed.tstat[2 * (size_t)t + 1] != 0.0)) zero = 1;
+    if (zero) atomicOr(&s_zero, 1);
+    __syncthreads();
+    const bool amp_nonzero = (s_zero == 0);
+    int cnt = 0, any2 = 0;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        double sx = sqrt(ed.tstat[2 * (size_t)t + 0]);
+        if (amp_nonzero) sx = sx / ed.tstat[2 * (size_t)t + 1];
+        if (sx > ed.sd) ++cnt;
+        if (sx > ed.sd2) any2 = 1;
+    }
+    atomicAdd(&s_cnt, cnt);
+    if (any2) atomicOr(&s_any2, 1);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    bool all_small = true, anynem = false;
+    for (int k = 0; k < K; ++k) {
+        int nem = (ed.E[2 * k] - 2) + (ed.E[2 * k + 1] - 2);
+        if (nem >= 3) all_small = false;
+        if (nem > 2) anynem = true;
+    }
+    const int imf = st->imf;
+    st->act_imf = imf;
+    st->pass += 1;
+    if (st->nsift == 0 && all_small) {   // memd.m stop_emd: no IMF left to extract
+        st->action = kActEnd;
+        st->done = 1;
+        st->n_imf_out = imf + 1;
+        atomicAdd(ed.ndone, 1);
+        return;
+    }
+    bool stop;
+    if (nvzero) {
+        stop = true;
+    } else {
+        stop = !(((((double)s_cnt / (double)T) > ed.tol) || s_any2) && anynem);
+    }
+    bool finish;
+    if (stop) {
+        st->action = kActFinishImf;
+        finish = true;
+    } else {
+        st->action = kActContinue;
+        st->nsift += 1;
+        finish = (st->nsift == ed.S);
+    }
+    if (finish) {
+        ed.sift_counts[imf] = st->nsift;
+        st->nsift = 0;
+        st->imf = imf + 1;
+        if (imf + 1 == ed.N_imf - 1) {
+            st->done = 1;
+            st->n_imf_out = ed.N_imf;
+            atomicAdd(ed.ndone, 1);
+        }
+    }
+    // k_stop_apply reads "IMF finished" as imf != act_imf
+}
+
+// Elementwise application of the decision (grid = the init table: n*T per epoch).
+__global__ __launch_bounds__(256) void k_stop_apply(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
+    const EpochState* st = ed.st;
+    int act = st->action;
+    if (act == kActEnd || act == kActNone) return;
+    size_t nT = (size_t)ed.n * ed.T;
+    size_t i = (size_t)br.y * 256 + threadIdx.x;
+    if (i >= nT) return;
+    const int imf = st->act_imf;
+    const bool finish = st->imf != imf;
+    double* cur = cur_slot(ed, imf);
+    double v = cur[i];
+    if (act == kActContinue) {
+        v = v - ed.mean[i];
+        cur[i] = v;
+    }
+    if (finish) {
+        double r = ed.running[i] - v;
+        ed.running[i] = r;
+        cur_slot(ed, imf + 1)[i] = r;
+    }
+}
+
+// End of a stop-mode step: epochs already done decide nothing in the next step,
+// and the work lists are cleared.
+__global__ void k_stop_reset(const EpochDesc* __restrict__ eps, int count, WorkLists wl) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) zero_counters(wl);
+    if (e < count && eps[e].st->done) eps[e].st->action = kActNone;
 }
 
 // ------------------------------------------------------------ misc kernels
@@ -604,50 +1101,122 @@ int solver_block_capacity(int Ecap) {
 }
 int solver_blocks_per_wg(int n) { return 256 / n; }
 
-int blocks_init(const EpochDesc& e) { return (int)(((size_t)e.n * e.T + 255) / 256); }
-int blocks_flags(const EpochDesc& e) { return e.K * ((e.T + kFlagTile - 1) / kFlagTile); }
-int blocks_compact(const EpochDesc& e) { return e.K * 2; }
-int blocks_solve(const EpochDesc& e) {
-    int bpw = solver_blocks_per_wg(e.n);
-    return e.K * 2 * ((e.nbmax + bpw - 1) / bpw);
+int blocks_for(int map, const EpochDesc& e, int eval_variant) {
+    switch (map) {
+    case kMapInit: return (int)(((size_t)e.n * e.T + 255) / 256);
+    case kMapFlags:
+        if (e.n <= kFlagLdsMaxCh) return (e.T + kFlagTile - 1) / kFlagTile;
+        return e.K * ((e.T + kFlagTile - 1) / kFlagTile);
+    case kMapCompact: return e.K;
+    case kMapEval:
+        if (e.mode == kModeStop) return (e.T + 63) / 64;
+        if (eval_variant == kEvalFused) return ((e.T + 63) / 64) * ((e.n + kEvalFusedCG - 1) / kEvalFusedCG);
+        return ((e.T + 63) / 64) * ((e.n + kEvalVariants[eval_variant].cpl - 1) / kEvalVariants[eval_variant].cpl);
+    }
+    return 0;
 }
-int blocks_reduced(const EpochDesc& e) { return e.K * 2; }
-int blocks_eval(const EpochDesc& e) { return ((e.T + 63) / 64) * ((e.n + kEvalCPL - 1) / kEvalCPL); }
+
+long list_capacity(int list, const EpochDesc& e) {
+    const long K2 = 2L * e.K;
+    const long J = (e.Ecap - 3) / kSolverBlock;
+    const long bpw = solver_blocks_per_wg(e.n);
+    switch (list) {
+    case kListFactor: return K2;
+    case kListKnots: return K2 * ((e.Ecap + kKnotChunk - 1) / kKnotChunk);
+    case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
+    case kListReduced: return K2;
+    case kListCoef:
+        if (!e.coef_records) return 0;
+        return K2 * (((long)e.n * (e.Ecap - 1) + 256L * kCoefItems - 1) / (256L * kCoefItems));
+    }
+    return 0;
+}
+
+#define MEMD_TABLE_LAUNCH(kern, map, threads, lds) \
+    hipLaunchKernelGGL(kern, dim3(p.blocks[map]), dim3(threads), lds, s, p.eps, p.tab[map])
+#define MEMD_LIST_LAUNCH(kern, list, threads) \
+    hipLaunchKernelGGL(kern, dim3(p.grid[list]), dim3(threads), 0, s, p.eps, p.wl)
 
 hipError_t launch_init(const StepPlan& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_init, dim3(p.blocks_init), dim3(256), 0, s, p.eps, p.off_init, p.count);
+    hipLaunchKernelGGL(k_init, dim3(p.blocks[kMapInit]), dim3(256), 0, s, p.eps, p.tab[kMapInit], p.wl);
     return hipGetLastError();
 }
 
+static void launch_eval(const StepPlan& p, hipStream_t s) {
+    switch (p.eval_variant) {
+    case 1: MEMD_TABLE_LAUNCH((k_eval<4, 2>), kMapEval, 64, 0); break;
+    case 2: MEMD_TABLE_LAUNCH((k_eval<2, 4>), kMapEval, 64, 0); break;
+    case 3: MEMD_TABLE_LAUNCH((k_eval<8, 1>), kMapEval, 64, 0); break;
+    case 4: MEMD_TABLE_LAUNCH((k_eval<2, 2>), kMapEval, 64, 0); break;
+    case 5: MEMD_TABLE_LAUNCH((k_eval<1, 4>), kMapEval, 64, 0); break;
+    case kEvalFused: MEMD_TABLE_LAUNCH(k_eval_fused, kMapEval, 256, 0); break;
+    default: MEMD_TABLE_LAUNCH((k_eval<4, 1>), kMapEval, 64, 0); break;
+    }
+}
+
+static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
+    switch (which) {
+    case 0:
+        if (p.nmax <= kFlagLdsMaxCh)
+            MEMD_TABLE_LAUNCH(k_flags, kMapFlags, kFlagTile, (size_t)p.nmax * (kFlagTile + 2) * sizeof(double));
+        else
+            MEMD_TABLE_LAUNCH(k_flags_global, kMapFlags, kFlagTile, 0);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(256), 0, s, p.eps, p.tab[kMapCompact], p.wl);
+        break;
+    case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 256); break;
+    case 3: MEMD_LIST_LAUNCH(k_knots, kListKnots, kKnotChunk); break;
+    case 4: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
+    case 5: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
+    case 6:
+        if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
+        break;
+    case 7:
+        if (p.mode == kModeStop)
+            MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
+        else
+            launch_eval(p, s);
+        break;
+    case 8:
+        if (p.mode == kModeStop)
+            hipLaunchKernelGGL(k_stop_decide, dim3(p.count), dim3(256), 0, s, p.eps, p.count);
+        else
+            hipLaunchKernelGGL(k_advance, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count, p.wl);
+        break;
+    default:
+        if (p.mode == kModeStop) {
+            MEMD_TABLE_LAUNCH(k_stop_apply, kMapInit, 256, 0);
+            hipLaunchKernelGGL(k_stop_reset, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count, p.wl);
+        }
+        break;
+    }
+}
+
 hipError_t launch_step(const StepPlan& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_flags, dim3(p.blocks_flags), dim3(kFlagTile), 0, s, p.eps, p.off_flags, p.count);
-    hipLaunchKernelGGL(k_compact, dim3(p.blocks_compact), dim3(256), 0, s, p.eps, p.off_compact, p.count);
-    hipLaunchKernelGGL(k_solve1, dim3(p.blocks_solve), dim3(256), 0, s, p.eps, p.off_solve, p.count);
-    hipLaunchKernelGGL(k_reduced, dim3(p.blocks_reduced), dim3(256), 0, s, p.eps, p.off_reduced, p.count);
-    hipLaunchKernelGGL(k_solve3, dim3(p.blocks_solve), dim3(256), 0, s, p.eps, p.off_solve, p.count);
-    hipLaunchKernelGGL(k_eval, dim3(p.blocks_eval), dim3(64), 0, s, p.eps, p.off_eval, p.count);
-    hipLaunchKernelGGL(k_advance, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count);
+    for (int i = 0; i < kNumStepKernels; ++i) launch_kernel(i, p, s);
     return hipGetLastError();
 }
 
 hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
     hipError_t e;
     if ((e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_flags, dim3(p.blocks_flags), dim3(kFlagTile), 0, s, p.eps, p.off_flags, p.count);
-    if ((e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(p.blocks_compact), dim3(256), 0, s, p.eps, p.off_compact, p.count);
-    if ((e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve1, dim3(p.blocks_solve), dim3(256), 0, s, p.eps, p.off_solve, p.count);
-    if ((e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_reduced, dim3(p.blocks_reduced), dim3(256), 0, s, p.eps, p.off_reduced, p.count);
-    if ((e = hipEventRecord(ev[4], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_solve3, dim3(p.blocks_solve), dim3(256), 0, s, p.eps, p.off_solve, p.count);
-    if ((e = hipEventRecord(ev[5], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_eval, dim3(p.blocks_eval), dim3(64), 0, s, p.eps, p.off_eval, p.count);
-    if ((e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_advance, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count);
-    if ((e = hipEventRecord(ev[7], s)) != hipSuccess) return e;
+    for (int i = 0; i < kNumStepKernels; ++i) {
+        launch_kernel(i, p, s);
+        if ((e = hipEventRecord(ev[i + 1], s)) != hipSuccess) return e;
+    }
     return hipGetLastError();
+}
+
+hipError_t configure_kernels() {
+    // the LDS-staged projection kernel holds up to 64 x 258 doubles (132 KB)
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flags),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       kFlagLdsMaxCh * (kFlagTile + 2) * (int)sizeof(double));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_stop),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               kStopMaxChannels * 64 * (int)sizeof(double));
 }
 
 hipError_t launch_check_coords(const int32_t* x, int T, int* flag, hipStream_t s) {

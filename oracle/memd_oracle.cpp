@@ -24,10 +24,14 @@
  *     unless written as fma() below.
  *   - projection: acc = fma(V[k][c], cur[c][t], acc), c ascending, acc starts at 0.
  *   - envelope evaluation: fma(fma(fma(d,tau,c),tau,b),tau,a) (Horner with fma).
- *   - tridiagonal: the blocked Thomas algorithm of tridiag_* below (block length
+ *   - tridiagonal: the blocked Thomas algorithm of struct Blocked below (block length
  *     SOLVER_BLOCK), replacing the closed cuSPARSE gtsv2 solver whose internal
  *     order is not published.  Exact in exact arithmetic; pinned against dense
  *     solves and scipy's natural CubicSpline by the golden tests.
+ *   - slopes and coefficients (memReducedEpochs_dev.cu:729-731, 775-793) use the
+ *     per-interval reciprocals ih = 1/h and i3h = 1/(3h), shared by all channels:
+ *     sl = (Y_{i+1}-Y_i)*ih, b = sl - (h*(2M_i+M_{i+1}))*(1/3), d = (M_{i+1}-M_i)*i3h
+ *     (the reference divides; the results differ by at most an ulp per operation).
  *   - mean over directions: sum = sum + (e_max + e_min) * 0.5 in ascending k over
  *     valid directions, then cur - sum / nvalid (0/0 -> NaN as the reference).
  *
@@ -85,53 +89,27 @@ void tridiag_apply(int p, int q, const double* lo, const double* rd, const doubl
     for (int u = q - 1; u >= p; --u) x[u] = fma_(-cp[u], x[u + 1], z[u]);
 }
 
-/* Reverse (bottom-up) factor of rows p..q: den'_q = di_q,
- * den'_u = fma(-up_u, cl_{u+1}, di_u); rdr_u = 1/den'_u; cl_u = lo_u * rdr_u. */
-void tridiag_factor_rev(int p, int q, const double* lo, const double* di, const double* up,
-                        double* rdr, double* cl) {
-    for (int u = q; u >= p; --u) {
-        double den = (u == q) ? di[u] : fma_(-up[u], cl[u + 1], di[u]);
-        rdr[u] = 1.0 / den;
-        cl[u] = lo[u] * rdr[u];
-    }
-}
-
-/* Last entry of the block solution from the forward sweep alone:
- * x_q = z_q with z as in tridiag_apply. */
-double sweep_fwd_last(int p, int q, const double* lo, const double* rd, const double* g) {
-    double z = g[p] * rd[p];
-    for (int u = p + 1; u <= q; ++u) z = fma_(-lo[u], z, g[u]) * rd[u];
-    return z;
-}
-
-/* First entry of the block solution from the reverse sweep alone:
- * z'_q = g_q*rdr_q, z'_u = fma(-up_u, z'_{u+1}, g_u) * rdr_u, x_p = z'_p. */
-double sweep_rev_first(int p, int q, const double* up, const double* rdr, const double* g) {
-    double z = g[q] * rdr[q];
-    for (int u = q - 1; u >= p; --u) z = fma_(-up[u], z, g[u]) * rdr[u];
-    return z;
-}
-
 /* Blocked Thomas (shared by all right-hand sides of one direction).
  * Rows 1..N; x[0] = x[N+1] = 0 are the natural end conditions.
  * Separators sit at rows jL (j = 1..J, J = (N-1)/L); block b holds rows
- * bL+1 .. min((b+1)L-1, N) (the last block ends at N).
- *   phase 1: per block, the first/last entries of A_b^{-1} r (y), of the left
- *            spike A_b^{-1}(lo_p e_first) (alpha) and of the right spike
- *            A_b^{-1}(up_q e_last) (beta): last entries from the forward sweep,
- *            first entries from the reverse sweep;
- *   phase 2: the J-row reduced tridiagonal system for the separator values,
- *            solved by forward sweep + back substitution;
- *   phase 3: every block re-solved (forward sweep + back substitution) with the
- *            separator values folded into its first / last right-hand side.
- * With N <= L there is one block and the solve is plain Thomas. */
+ * p_b = bL+1 .. q_b = (b<J ? (b+1)L-1 : N).
+ *   factor : per block, forward factors (rd, cp) and the two spike vectors
+ *            alpha = A_b^{-1}(lo_p e_first) (b >= 1) and
+ *            beta  = A_b^{-1}(up_q e_last)  (b < J), each by tridiag_apply;
+ *   phase 1: per block and right-hand side, y = A_b^{-1} r by tridiag_apply;
+ *   phase 2: the J-row reduced tridiagonal for the separator values X_j,
+ *            built from the block-end entries of y, alpha, beta and solved by
+ *            tridiag_apply;
+ *   phase 3: x_u = fma(-X_{b+1}, beta_u, fma(-X_b, alpha_u, y_u)), each term
+ *            present only where its spike exists.
+ * With N <= L there is one block and x = y (plain Thomas). */
 struct Blocked {
     int N = 0, L = SOLVER_BLOCK, J = 0;
     const double *lo = nullptr, *di = nullptr, *up = nullptr;
-    std::vector<double> rd, cp, rdr, cl;        // per row: forward / reverse block-local factors
-    std::vector<double> ap, aq, bp, bq;         // spikes per block (first / last entry)
+    std::vector<double> rd, cp;                  // per row, block-local forward factors
+    std::vector<double> al, be;                  // spike vectors, per row of their block
     std::vector<double> Rlo, Rdi, Rup, Rrd, Rcp; // reduced system rows 1..J
-    std::vector<double> z, x, g;                // scratch
+    std::vector<double> z, g, y;                 // scratch
 
     int bp_(int b) const { return b * L + 1; }
     int bq_(int b) const { return (b < J) ? (b + 1) * L - 1 : N; }
@@ -139,26 +117,23 @@ struct Blocked {
     void factor(int N_, const double* lo_, const double* di_, const double* up_) {
         N = N_; lo = lo_; di = di_; up = up_;
         J = (N >= 1) ? (N - 1) / L : 0;
-        rd.assign(N + 2, 0.0); cp.assign(N + 2, 0.0); rdr.assign(N + 2, 0.0); cl.assign(N + 2, 0.0);
-        z.assign(N + 2, 0.0); x.assign(N + 2, 0.0); g.assign(N + 2, 0.0);
+        rd.assign(N + 2, 0.0); cp.assign(N + 2, 0.0);
+        al.assign(N + 2, 0.0); be.assign(N + 2, 0.0);
+        z.assign(N + 2, 0.0); g.assign(N + 2, 0.0); y.assign(N + 2, 0.0);
         if (N <= 0) return;
         for (int b = 0; b <= J; ++b) tridiag_factor(bp_(b), bq_(b), lo, di, up, rd.data(), cp.data());
         if (J == 0) return;
-        for (int b = 0; b <= J; ++b) tridiag_factor_rev(bp_(b), bq_(b), lo, di, up, rdr.data(), cl.data());
-        ap.assign(J + 1, 0.0); aq.assign(J + 1, 0.0); bp.assign(J + 1, 0.0); bq.assign(J + 1, 0.0);
         for (int b = 0; b <= J; ++b) {
             int p = bp_(b), q = bq_(b);
-            if (b >= 1) {   // left spike: A_b^{-1} (lo_p e_first)
+            if (b >= 1) {   // left spike
                 for (int u = p; u <= q; ++u) g[u] = 0.0;
                 g[p] = lo[p];
-                aq[b] = sweep_fwd_last(p, q, lo, rd.data(), g.data());
-                ap[b] = sweep_rev_first(p, q, up, rdr.data(), g.data());
+                tridiag_apply(p, q, lo, rd.data(), cp.data(), g.data(), z.data(), al.data());
             }
-            if (b < J) {    // right spike: A_b^{-1} (up_q e_last)
+            if (b < J) {    // right spike
                 for (int u = p; u <= q; ++u) g[u] = 0.0;
                 g[q] = up[q];
-                bq[b] = sweep_fwd_last(p, q, lo, rd.data(), g.data());
-                bp[b] = sweep_rev_first(p, q, up, rdr.data(), g.data());
+                tridiag_apply(p, q, lo, rd.data(), cp.data(), g.data(), z.data(), be.data());
             }
         }
         // reduced system (rows 1..J); coefficients of X_{j-1}, X_j, X_{j+1}
@@ -166,9 +141,10 @@ struct Blocked {
         Rrd.assign(J + 2, 0.0); Rcp.assign(J + 2, 0.0);
         for (int j = 1; j <= J; ++j) {
             int s = j * L;
-            Rlo[j] = (j >= 2) ? -(lo[s] * aq[j - 1]) : 0.0;
-            Rdi[j] = fma_(-up[s], ap[j], fma_(-lo[s], bq[j - 1], di[s]));
-            Rup[j] = (j <= J - 1) ? -(up[s] * bp[j]) : 0.0;
+            int qL = s - 1, pR = s + 1;   // last row of block j-1, first row of block j
+            Rlo[j] = (j >= 2) ? -(lo[s] * al[qL]) : 0.0;
+            Rdi[j] = fma_(-up[s], al[pR], fma_(-lo[s], be[qL], di[s]));
+            Rup[j] = (j <= J - 1) ? -(up[s] * be[pR]) : 0.0;
         }
         tridiag_factor(1, J, Rlo.data(), Rdi.data(), Rup.data(), Rrd.data(), Rcp.data());
     }
@@ -181,24 +157,22 @@ struct Blocked {
             tridiag_apply(1, N, lo, rd.data(), cp.data(), r, z.data(), out);
             return;
         }
-        std::vector<double> yp(J + 1), yq(J + 1), D(J + 2, 0.0), Zr(J + 2), Xs(J + 2, 0.0);
-        for (int b = 0; b <= J; ++b) {   // phase 1
-            int p = bp_(b), q = bq_(b);
-            yq[b] = sweep_fwd_last(p, q, lo, rd.data(), r);
-            yp[b] = sweep_rev_first(p, q, up, rdr.data(), r);
-        }
+        std::vector<double> D(J + 2, 0.0), Zr(J + 2), Xs(J + 2, 0.0);
+        for (int b = 0; b <= J; ++b)     // phase 1
+            tridiag_apply(bp_(b), bq_(b), lo, rd.data(), cp.data(), r, z.data(), y.data());
         for (int j = 1; j <= J; ++j) {   // phase 2
             int s = j * L;
-            D[j] = fma_(-up[s], yp[j], fma_(-lo[s], yq[j - 1], r[s]));
+            D[j] = fma_(-up[s], y[s + 1], fma_(-lo[s], y[s - 1], r[s]));
         }
         tridiag_apply(1, J, Rlo.data(), Rrd.data(), Rcp.data(), D.data(), Zr.data(), Xs.data());
         for (int b = 0; b <= J; ++b) {   // phase 3
             int p = bp_(b), q = bq_(b);
-            for (int u = p; u <= q; ++u) g[u] = r[u];
-            if (b >= 1) g[p] = fma_(-lo[p], Xs[b], g[p]);
-            if (b < J) g[q] = fma_(-up[q], Xs[b + 1], g[q]);
-            tridiag_apply(p, q, lo, rd.data(), cp.data(), g.data(), z.data(), x.data());
-            for (int u = p; u <= q; ++u) out[u] = x[u];
+            for (int u = p; u <= q; ++u) {
+                double x = y[u];
+                if (b >= 1) x = fma_(-Xs[b], al[u], x);
+                if (b < J) x = fma_(-Xs[b + 1], be[u], x);
+                out[u] = x;
+            }
         }
         for (int j = 1; j <= J; ++j) out[j * L] = Xs[j];
     }
@@ -207,15 +181,25 @@ struct Blocked {
 /* ----------------------------------------------------------- spline per env */
 
 /* Shared (per direction, per envelope) spline system: h, lo, di, up from the
- * knot positions, natural rows 0 and m-1 (memReducedEpochs_dev.cu:704-736). */
+ * knot positions, natural rows 0 and m-1 (memReducedEpochs_dev.cu:704-736),
+ * and the per-interval reciprocals ih = 1/h, i3h = 1/(3h) that every channel
+ * shares (slopes and the d coefficient multiply by them). */
+constexpr double kThird = 1.0 / 3.0;
+
 struct KnotSystem {
     int E = 0;
-    std::vector<double> h, lo, di, up;
+    std::vector<double> h, ih, i3h, lo, di, up;
     Blocked solver;
     void build(int E_, const int32_t* X) {
         E = E_;
         h.assign(std::max(E - 1, 1), 0.0);
-        for (int i = 0; i + 1 < E; ++i) h[i] = (double)(X[i + 1] - X[i]);
+        ih.assign(std::max(E - 1, 1), 0.0);
+        i3h.assign(std::max(E - 1, 1), 0.0);
+        for (int i = 0; i + 1 < E; ++i) {
+            h[i] = (double)(X[i + 1] - X[i]);
+            ih[i] = 1.0 / h[i];
+            i3h[i] = 1.0 / (3.0 * h[i]);
+        }
         int N = E - 2;
         lo.assign(E + 1, 0.0); di.assign(E + 1, 0.0); up.assign(E + 1, 0.0);
         for (int u = 1; u <= N; ++u) {
@@ -232,15 +216,14 @@ struct KnotSystem {
         sl.resize(std::max(E - 1, 1));
         r.assign(E + 1, 0.0);
         M.assign(E + 1, 0.0);
-        for (int i = 0; i + 1 < E; ++i) sl[i] = (Y[i + 1] - Y[i]) / h[i];
+        for (int i = 0; i + 1 < E; ++i) sl[i] = (Y[i + 1] - Y[i]) * ih[i];
         for (int u = 1; u <= N; ++u) r[u] = 3.0 * (sl[u] - sl[u - 1]);
         solver.solve(r.data(), M.data());   // M[0] = M[E-1] = 0
         for (int j = 0; j + 1 < E; ++j) {
-            double hj = h[j];
             coef[4 * j + 0] = Y[j];
-            coef[4 * j + 1] = sl[j] - (hj * (2.0 * M[j] + M[j + 1])) / 3.0;
+            coef[4 * j + 1] = sl[j] - (h[j] * (2.0 * M[j] + M[j + 1])) * kThird;
             coef[4 * j + 2] = M[j];
-            coef[4 * j + 3] = (M[j + 1] - M[j]) / (3.0 * hj);
+            coef[4 * j + 3] = (M[j + 1] - M[j]) * i3h[j];
         }
     }
 };
@@ -463,11 +446,12 @@ void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, dou
 void oracle_spline_coefs(int E, const int32_t* X, const double* Y, const double* M, double* coef) {
     for (int j = 0; j + 1 < E; ++j) {
         double hj = (double)(X[j + 1] - X[j]);
-        double sl = (Y[j + 1] - Y[j]) / hj;
+        double ih = 1.0 / hj, i3h = 1.0 / (3.0 * hj);
+        double sl = (Y[j + 1] - Y[j]) * ih;
         coef[4 * j + 0] = Y[j];
-        coef[4 * j + 1] = sl - (hj * (2.0 * M[j] + M[j + 1])) / 3.0;
+        coef[4 * j + 1] = sl - (hj * (2.0 * M[j] + M[j + 1])) * kThird;
         coef[4 * j + 2] = M[j];
-        coef[4 * j + 3] = (M[j + 1] - M[j]) / (3.0 * hj);
+        coef[4 * j + 3] = (M[j + 1] - M[j]) * i3h;
     }
 }
 

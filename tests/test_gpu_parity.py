@@ -31,18 +31,21 @@ def _rel_l2(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def _check_epoch(ctx, oracle, y, K, n_imf, S, window=0, primes=None, use_graph=True):
+def _check_epoch(ctx, oracle, y, K, n_imf, S, window=0, primes=None, use_graph=True, mode=0):
     n, T = y.shape
     V = oracle.directions(n, K, primes)
-    passes = (n_imf - 1) * S
-    imfs_o, info_o = oracle.memd(y, V, n_imf, S, window=window)
-    imfs_g, info_g = ctx.run(_dev(y), _dev(V), n_imf, S, window=window, use_graph=use_graph,
-                             trace_passes=max(passes, 1))
+    imfs_o, info_o = oracle.memd(y, V, n_imf, S, window=window, mode=mode)
+    passes = info_o["n_passes"]
+    cap = (n_imf - 1) * (S + 1) + 1
+    imfs_g, info_g = ctx.run(_dev(y), _dev(V), n_imf, S, window=window, use_graph=use_graph, mode=mode,
+                             trace_passes=cap)
     got = imfs_g.cpu().numpy()
     assert info_g["n_imf_out"] == info_o["n_imf_out"]
     assert np.array_equal(info_g["sift_counts"], info_o["sift_counts"])
     kc = info_g["knot_counts"].cpu().numpy()[:passes]
     assert np.array_equal(kc, info_o["knot_counts"]), "knot counts differ"
+    used = info_o["n_imf_out"]
+    got, imfs_o = got[:used], imfs_o[:used]
     assert _rel_l2(got, imfs_o) <= 1e-9
     mism = np.argwhere(got != imfs_o)
     assert mism.size == 0, f"{len(mism)} samples differ, first at {mism[0].tolist()}"
@@ -76,6 +79,34 @@ def test_headline_shape_full_schedule(ctx, oracle):
     y = synth.sines_epoch(32, 4096, 7)
     got = _check_epoch(ctx, oracle, y, 64, 9, 10)
     np.testing.assert_allclose(got.sum(0), y, rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+@pytest.mark.parametrize("gen,n,T,K,n_imf,S,seed", [
+    ("sines", 8, 1000, 16, 12, 50, 20),   # the 8 x 1000, K=16 host case with [0.075, 0.75, 0.075]
+    ("sines", 1, 700, 4, 10, 30, 21),     # univariate
+    ("eeg", 4, 2000, 8, 12, 20, 22),
+    ("sines", 3, 64, 6, 8, 10, 23),       # short: the decomposition ends early (too few extrema)
+    ("eeg", 32, 4096, 64, 12, 10, 24),    # headline shape
+])
+def test_stop_mode_matches_oracle(ctx, oracle, gen, n, T, K, n_imf, S, seed):
+    import synth
+    y = synth.sines_epoch(n, T, seed) if gen == "sines" else synth.eeg_epoch(n, T, seed)
+    _check_epoch(ctx, oracle, y, K, n_imf, S, mode=1)
+
+
+def test_stop_mode_batch_matches_alone(ctx, oracle):
+    import synth
+    eps = []
+    for i, (n, T, K) in enumerate([(8, 1500, 16), (4, 900, 8), (16, 1200, 32)]):
+        y = synth.eeg_epoch(n, T, 300 + i)
+        eps.append({"y": _dev(y), "V": _dev(oracle.directions(n, K)), "n_imf": 10, "S": 15})
+    outs, info = ctx.run_batch(eps, mode=1)
+    for i, (ep, out) in enumerate(zip(eps, outs)):
+        alone, ainfo = ctx.run(ep["y"], ep["V"], 10, 15, mode=1)
+        used = ainfo["n_imf_out"]
+        assert info["n_imf_out"][i] == used
+        assert np.array_equal(info["sift_counts"][i], ainfo["sift_counts"])
+        assert np.array_equal(out.cpu().numpy()[:used], alone.cpu().numpy()[:used])
 
 
 def test_full_window(ctx, oracle):
