@@ -43,7 +43,22 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration) and exit")
     return ap.parse_args()
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = rec.get(workload, {}).get(kernel)
+    return int(ent["bytes_per_launch"]) if ent else None
 
 
 def pct(xs, q):
@@ -79,6 +94,16 @@ def main():
     import memd_amd
     import synth
 
+    if a.copy_bytes:
+        # known-size copy: reads and writes copy_bytes each (calibrates FETCH_SIZE / WRITE_SIZE)
+        src = torch.ones(a.copy_bytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.copy_bytes}), flush=True)
+        return
+
     n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
     dev = "cuda:0"
     ctx = memd_amd.MemdContext(0)
@@ -102,14 +127,16 @@ def main():
 
     # plain per-epoch path (one call per epoch), latency per epoch
     plain_lat = []
-    ctx.run(ys[0], V, N_imf, S, out=outs[0])
-    torch.cuda.synchronize()
-    for i in range(a.plain_epochs):
-        s0 = time.perf_counter()
-        ctx.run(ys[i % B], V, N_imf, S, out=outs[i % B])
-        plain_lat.append(time.perf_counter() - s0)
-    plain_eps = len(plain_lat) / sum(plain_lat)
+    if a.plain_epochs > 0:
+        ctx.run(ys[0], V, N_imf, S, out=outs[0])
+        torch.cuda.synchronize()
+        for i in range(a.plain_epochs):
+            s0 = time.perf_counter()
+            ctx.run(ys[i % B], V, N_imf, S, out=outs[i % B])
+            plain_lat.append(time.perf_counter() - s0)
+    plain_eps = len(plain_lat) / sum(plain_lat) if plain_lat else None
 
+    workload = f"memd_{n}ch_x{T}_K{K}_fix{N_imf}x{S}_b{B}"
     roofline, kernels = None, None
     if not a.no_profile:
         _, info = ctx.run_batch(eps, profile=True)
@@ -117,12 +144,15 @@ def main():
         kernels = {k: round(v * 1e3, 2) for k, v in pr["ms_per_launch"].items()}   # us per launch
         steps = max(pr["steps"], 1)
         W = (T + 63) // 64
-        # k_eval per launch: cur read+write (16 B/sample), one 32-B coefficient record per
-        # knot-channel, 4 B per knot position, 12 B per flag word + prefix per (epoch, direction, envelope)
-        bytes_eval = (16 * pr["samples"] + 32 * pr["knot_channels"] + 4 * pr["knots"]) / steps \
+        # fused record+evaluation kernel, per launch: cur read+write (16 B/sample); knot value
+        # and block solution per knot-channel (16 B); per knot the position, the two spike
+        # entries and the two reciprocals (36 B); flag word + prefix per (epoch, direction,
+        # envelope, 64 samples) (12 B).  Flops: two 3-fma Horners + mean per evaluation point
+        # (15), record build per knot-channel (10), divide + subtract per sample (2).
+        bytes_eval = (16 * pr["samples"] + 16 * pr["knot_channels"] + 36 * pr["knots"]) / steps \
             + 12 * 2 * K * W * B
-        flops_eval = (15 * pr["eval_points"] + 2 * pr["samples"]) / steps
-        t_eval = pr["ms_per_launch"]["k_eval"] * 1e-3
+        flops_eval = (15 * pr["eval_points"] + 10 * pr["knot_channels"] + 2 * pr["samples"]) / steps
+        t_eval = pr["ms_per_launch"]["k_eval"] * 1e-3   # slot of the fused record+evaluation kernel
         gbs = bytes_eval / t_eval / 1e9
         tfs = flops_eval / t_eval / 1e12
         if gbs / HBM_PEAK_GBS >= tfs / FP64_PEAK_TFLOPS:
@@ -131,25 +161,26 @@ def main():
         else:
             roofline = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tfs / FP64_PEAK_TFLOPS, 4), "traffic": None}
-        roofline.update({"kernel": "k_eval", "launch_us": round(t_eval * 1e6, 2),
+        roofline["traffic"] = measured_traffic(workload, "k_eval_fused")
+        roofline.update({"kernel": "k_eval_fused", "launch_us": round(t_eval * 1e6, 2),
                          "alg_bytes_per_launch": int(bytes_eval), "alg_flops_per_launch": int(flops_eval),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_frac": round(tfs / FP64_PEAK_TFLOPS, 4)})
 
     cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S, a.cpu_budget_s)
+    pct_or_none = lambda xs, q: round(pct(xs, q) * 1e3, 3) if xs else None
     ctx.close()
     line = {
         "metric": "MEMD epochs/s (32ch x 4096, K=64); batched vs plain GPU vs CPU",
         "value": round(value, 2), "unit": "epochs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(total / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"memd_{n}ch_x{T}_K{K}_fix{N_imf}x{S}", "n": n, "T": T, "K": K,
+        "config": {"workload": workload, "n": n, "T": T, "K": K,
                    "N_imf": N_imf, "S": S, "mode": "fix", "window": "refwarp32", "batch_epochs": B,
                    "generator": "sines 5/20/60 Hz, N(0,1)/sqrt(n) mixing, N(0,0.1^2) noise"},
         "latency_ms": {"batched_p50": round(pct(lat, 50) * 1e3, 3), "batched_p99": round(pct(lat, 99) * 1e3, 3),
-                       "plain_p50": round(pct(plain_lat, 50) * 1e3, 3),
-                       "plain_p99": round(pct(plain_lat, 99) * 1e3, 3)},
-        "plain_gpu_epochs_per_s": round(plain_eps, 2),
-        "batched_over_plain": round(value / plain_eps, 2),
+                       "plain_p50": pct_or_none(plain_lat, 50), "plain_p99": pct_or_none(plain_lat, 99)},
+        "plain_gpu_epochs_per_s": round(plain_eps, 2) if plain_eps else None,
+        "batched_over_plain": round(value / plain_eps, 2) if plain_eps else None,
         "roofline": roofline,
         "kernel_us_per_launch": kernels,
         "cpu_baseline": cpu,

@@ -198,6 +198,18 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             for (int l = 0; l < kNumLists; ++l) lcap[l] += list_capacity(l, tmp[e]);
         }
     }
+    // XCD affinity when the batch has enough epochs to keep every XCD busy
+    const int groups = count >= kXcds ? kXcds : 1;
+    std::vector<std::vector<long>> gcap(kNumLists, std::vector<long>(kXcds, 0));
+    for (int e = 0; e < count; ++e) {
+        EpochDesc tmp{};
+        tmp.n = in[e].n; tmp.T = in[e].T; tmp.K = in[e].K;
+        tmp.Ecap = knot_capacity(in[e].T);
+        tmp.nbmax = solver_block_capacity(tmp.Ecap);
+        tmp.mode = o.mode;
+        tmp.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+        for (int l = 0; l < kNumLists; ++l) gcap[l][groups > 1 ? e % kXcds : 0] += list_capacity(l, tmp);
+    }
     size_t tab_at[kNumMaps], list_at[kNumLists];
     for (int f = 0; f < kNumMaps; ++f) {
         tab_at[f] = total;
@@ -209,7 +221,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         total += align_up(sizeof(int4) * (size_t)lcap[l]);
     }
     size_t cnt_at = total;
-    total += align_up(sizeof(int) * kNumLists);
+    total += align_up(sizeof(int) * kNumLists * kXcds);
     size_t stats_at = total;
     total += align_up(sizeof(unsigned long long) * 4);
     size_t ndone_at = total;
@@ -257,12 +269,23 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, hipMemsetAsync(ndone_dev, 0, sizeof(int), s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + desc_at, eds.data(), sizeof(EpochDesc) * count, hipMemcpyHostToDevice, s));
     for (int f = 0; f < kNumMaps; ++f) {
-        size_t at0 = tabs.size();
+        // block order: block id b runs on XCD b mod 8; place epoch e's blocks on
+        // XCD e mod 8 (falling back to any queue once its own is drained)
+        std::vector<std::vector<int2>> q(groups);
         for (int e = 0; e < count; ++e) {
             int nb = blocks_for(f, eds[e], variant);
-            for (int b = 0; b < nb; ++b) tabs.push_back(make_int2(e, b));
+            for (int b = 0; b < nb; ++b) q[groups > 1 ? e % kXcds : 0].push_back(make_int2(e, b));
         }
-        (void)at0;
+        std::vector<size_t> head(groups, 0);
+        for (int b = 0; b < nblk[f]; ++b) {
+            int x = b % groups;
+            if (head[x] == q[x].size()) {
+                size_t best = 0;
+                for (int y = 0; y < groups; ++y)
+                    if (q[y].size() - head[y] > best) { best = q[y].size() - head[y]; x = y; }
+            }
+            tabs.push_back(q[x][head[x]++]);
+        }
     }
     {
         size_t pos = 0;
@@ -282,10 +305,17 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         plan.blocks[f] = nblk[f];
     }
     plan.wl.count = (int*)(ctx->arena + cnt_at);
+    plan.wl.groups = groups;
     for (int l = 0; l < kNumLists; ++l) {
         plan.wl.item[l] = (int4*)(ctx->arena + list_at[l]);
-        plan.wl.cap[l] = (int)lcap[l];
+        long at = 0;
+        for (int x = 0; x < kXcds; ++x) {
+            plan.wl.off[l][x] = (int)at;
+            plan.wl.cap[l][x] = (int)gcap[l][x];
+            at += gcap[l][x];
+        }
         long g = lcap[l] < kListGrid ? lcap[l] : kListGrid;
+        if (groups > 1) g = kListGrid;   // a multiple of kXcds, every XCD gets its share
         plan.grid[l] = (int)(g > 0 ? g : 1);
     }
     plan.mode = o.mode;
@@ -353,8 +383,11 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
                           std::to_string(plan.mode) + "/" + std::to_string(plan.nmax) + "/" +
                           std::to_string(plan.eval_variant);
         for (int f = 0; f < kNumMaps; ++f) key += "/" + std::to_string(plan.blocks[f]);
-        for (int l = 0; l < kNumLists; ++l)
-            key += "/" + std::to_string(plan.wl.cap[l]) + ":" + std::to_string(plan.grid[l]);
+        key += "/g" + std::to_string(plan.wl.groups);
+        for (int l = 0; l < kNumLists; ++l) {
+            key += "/" + std::to_string(plan.grid[l]);
+            for (int x = 0; x < kXcds; ++x) key += ":" + std::to_string(plan.wl.cap[l][x]);
+        }
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) {
                 HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));

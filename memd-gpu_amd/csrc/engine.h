@@ -100,10 +100,18 @@ enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kNumMaps };
 // of every valid direction; persistent grids drain the lists.
 enum : int { kListFactor = 0, kListKnots, kListBlock, kListReduced, kListCoef, kNumLists };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup id mod 8).
+// With >= kXcds epochs in a batch, every epoch is pinned to XCD (epoch mod 8):
+// its table blocks sit at ids = XCD (mod 8) and its work items go to that
+// XCD's sub-list, so its signal and knot data stay in one XCD's L2.
+constexpr int kXcds = 8;
+
 struct WorkLists {
     int4* item[kNumLists];
-    int* count;             // [kNumLists], cleared at the end of every step
-    int cap[kNumLists];
+    int* count;                     // [kNumLists][kXcds], cleared at the end of every step
+    int off[kNumLists][kXcds];      // sub-list start in item[list]
+    int cap[kNumLists][kXcds];
+    int groups;                     // kXcds (XCD affinity) or 1
 };
 
 struct StepPlan {

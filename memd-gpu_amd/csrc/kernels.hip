@@ -105,7 +105,24 @@ __device__ __forceinline__ double* fac_row(const EpochDesc& ed, int ke, int fiel
 }
 
 __device__ __forceinline__ void zero_counters(const WorkLists& wl) {
-    for (int i = 0; i < kNumLists; ++i) wl.count[i] = 0;
+    for (int i = 0; i < kNumLists * kXcds; ++i) wl.count[i] = 0;
+}
+
+// Persistent-grid iteration over one list: with XCD affinity, workgroup w
+// drains the sub-list of XCD w mod 8.
+struct ListCursor {
+    const int4* base;
+    int start, stride, cnt;
+};
+
+__device__ __forceinline__ ListCursor list_cursor(const WorkLists& wl, int list) {
+    ListCursor cu;
+    int g = wl.groups > 1 ? (int)(blockIdx.x % kXcds) : 0;
+    cu.base = wl.item[list] + wl.off[list][g];
+    cu.start = wl.groups > 1 ? (int)(blockIdx.x / kXcds) : (int)blockIdx.x;
+    cu.stride = wl.groups > 1 ? (int)(gridDim.x / kXcds) : (int)gridDim.x;
+    cu.cnt = min(wl.count[list * kXcds + g], wl.cap[list][g]);
+    return cu;
 }
 
 // ---------------------------------------------------------------- init
@@ -318,12 +335,13 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
     __syncthreads();
     if (!(sE[0] > 3 && sE[1] > 3)) return;
     __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
+    const int g = wl.groups > 1 ? br.x % kXcds : 0;
     if (tid < 2) {
         int m[kNumLists];
         list_sizes(ed, sE[tid], m);
         for (int l = 0; l < kNumLists; ++l) {
             s_m[tid][l] = m[l];
-            s_base[tid][l] = atomicAdd(&wl.count[l], m[l]);
+            s_base[tid][l] = m[l] ? atomicAdd(&wl.count[l * kXcds + g], m[l]) : 0;
         }
     }
     __syncthreads();
@@ -331,7 +349,7 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
         for (int l = 0; l < kNumLists; ++l)
             for (int i = tid; i < s_m[env][l]; i += 256) {
                 int at = s_base[env][l] + i;
-                if (at < wl.cap[l]) wl.item[l][at] = make_int4(br.x, 2 * k + env, i, 0);
+                if (at < wl.cap[l][g]) wl.item[l][wl.off[l][g] + at] = make_int4(br.x, 2 * k + env, i, 0);
             }
 }
 
@@ -347,9 +365,9 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
 // reduced system's factorisation (one lane, J steps).
 __global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    const int cnt = min(wl.count[kListFactor], wl.cap[kListFactor]);
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-        const int4 it = wl.item[kListFactor][w];
+    const ListCursor cu = list_cursor(wl, kListFactor);
+    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
+        const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
         const int E = ed.E[ke];
@@ -440,9 +458,9 @@ __global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ ep
 __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int CH = kKnotCh;
     __shared__ double Ys[CH][kKnotChunk + 2];
-    const int cnt = min(wl.count[kListKnots], wl.cap[kListKnots]);
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-        const int4 it = wl.item[kListKnots][w];
+    const ListCursor cu = list_cursor(wl, kListKnots);
+    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
+        const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const EpochState* st = ed.st;
         const int ke = it.y, env = ke & 1;
@@ -494,9 +512,9 @@ __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restric
 __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
     __shared__ double s_rd[8][L], s_cp[8][L], s_lo[8][L];
-    const int cnt = min(wl.count[kListBlock], wl.cap[kListBlock]);
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-        const int4 it = wl.item[kListBlock][w];
+    const ListCursor cu = list_cursor(wl, kListBlock);
+    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
+        const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
         const int n = ed.n;
@@ -552,9 +570,9 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
 // factorisation comes from k_factor).
 __global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    const int cnt = min(wl.count[kListReduced], wl.cap[kListReduced]);
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-        const int4 it = wl.item[kListReduced][w];
+    const ListCursor cu = list_cursor(wl, kListReduced);
+    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
+        const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
         const int n = ed.n;
@@ -605,9 +623,9 @@ __device__ __forceinline__ double knot_M(const double* row, const double* sv, co
 
 __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    const int cnt = min(wl.count[kListCoef], wl.cap[kListCoef]);
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-        const int4 it = wl.item[kListCoef][w];
+    const ListCursor cu = list_cursor(wl, kListCoef);
+    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
+        const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
         const int n = ed.n;
