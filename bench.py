@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--workload", choices=("m", "c1", "c3"), default="m",
+                    help="m: 32ch x 4096 K=64 batched (headline); c1: 32ch x 30504 K=64 single epochs "
+                         "vs the CPU oracle; c3: 512 uneven epochs batched vs plain")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     return ap.parse_args()
@@ -84,6 +87,96 @@ def cpu_baseline(n, T, K, n_imf, S, budget_s):
                       f"OpenMP over channels with {threads} threads"}
 
 
+def bench_c1(a):
+    """32 ch x 30504 samples (EEG-like generator), K = 64, 9 slots x 10 sifts:
+    plain per-epoch GPU path vs the FP64 CPU oracle on this host."""
+    import torch
+    import memd_amd
+    import synth
+    from oracle import memd_oracle
+    n, T, K, N_imf, S = 32, 30504, 64, 9, 10
+    ctx = memd_amd.MemdContext(0)
+    V = ctx.directions(n, K)
+    ys = [torch.from_numpy(synth.eeg_epoch(n, T, 3000 + i)).to("cuda:0") for i in range(max(a.steps, 1))]
+    out = torch.empty((N_imf, n, T), dtype=torch.float64, device="cuda:0")
+    for _ in range(a.warmup):
+        ctx.run(ys[0], V, N_imf, S, out=out)
+    torch.cuda.synchronize()
+    lat = []
+    for y in ys:
+        s0 = time.perf_counter()
+        ctx.run(y, V, N_imf, S, out=out)
+        lat.append(time.perf_counter() - s0)
+    gpu_eps = len(lat) / sum(lat)
+    ctx.close()
+    # CPU: the oracle for 2 sifts of the first IMF, scaled to the 80-sift epoch (the
+    # per-sift cost is dominated by the K*n*T envelope evaluations, the same in every sift)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    y0 = synth.eeg_epoch(n, T, 3000)
+    Vh = memd_oracle.directions(n, K)
+    t0 = time.perf_counter()
+    memd_oracle.memd(y0, Vh, 2, 2, threads=threads, trace=False)
+    per_sift = (time.perf_counter() - t0) / 2
+    cpu_eps = 1.0 / (per_sift * (N_imf - 1) * S)
+    print(json.dumps({
+        "metric": "MEMD epochs/s, 32ch x 30504, K=64 (plain GPU vs FP64 CPU)", "value": round(gpu_eps, 3),
+        "unit": "epochs/s", "n_gpus": 1, "steps": len(lat), "warmup": a.warmup,
+        "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "memd_32ch_x30504_K64_fix9x10_plain", "generator": "eeg pink+alpha, fs=512"},
+        "latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
+        "cpu_baseline": {"value": round(cpu_eps, 5), "unit": "epochs/s", "cores": threads, "kind": "port",
+                         "sample": f"2 sifts of IMF 1 of one epoch ({per_sift * 2:.1f} s) scaled to 80 sifts"},
+        "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
+
+
+def bench_c3(a):
+    """512 independent epochs, n in {8,16,32,64}, T in [1000,8000], K = 2n (EEG-like
+    generator, per-epoch seeds), 9 slots x 10 sifts: one batched call vs one call
+    per epoch (plain path timed on the first 32 epochs)."""
+    import torch
+    import memd_amd
+    import synth
+    count = 512
+    ctx = memd_amd.MemdContext(0)
+    eps, Vs = [], {}
+    for i, (n, T, K) in enumerate(synth.batch_shapes(count, 77)):
+        if (n, K) not in Vs:
+            Vs[(n, K)] = ctx.directions(n, K)
+        y = torch.from_numpy(synth.eeg_epoch(n, T, 78 + i)).to("cuda:0")
+        eps.append({"y": y, "V": Vs[(n, K)], "n_imf": 9, "S": 10,
+                    "out": torch.empty((9, n, T), dtype=torch.float64, device="cuda:0")})
+    for _ in range(a.warmup):
+        ctx.run_batch(eps)
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(max(a.steps, 1)):
+        s0 = time.perf_counter()
+        ctx.run_batch(eps)
+        lat.append(time.perf_counter() - s0)
+    batched = count * len(lat) / sum(lat)
+    plain_n = min(32, count)
+    ctx.run(eps[0]["y"], eps[0]["V"], 9, 10, out=eps[0]["out"])
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    plat = []
+    for ep in eps[:plain_n]:
+        s1 = time.perf_counter()
+        ctx.run(ep["y"], ep["V"], 9, 10, out=ep["out"])
+        plat.append(time.perf_counter() - s1)
+    plain = plain_n / (time.perf_counter() - s0)
+    ctx.close()
+    print(json.dumps({
+        "metric": "MEMD epochs/s, 512 uneven epochs (batched vs plain GPU)", "value": round(batched, 2),
+        "unit": "epochs/s", "n_gpus": 1, "steps": len(lat), "warmup": a.warmup,
+        "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "memd_512_uneven_fix9x10", "n": [8, 16, 32, 64], "T": [1000, 8000], "K": "2n"},
+        "plain_gpu_epochs_per_s": round(plain, 2), "batched_over_plain": round(batched / plain, 2),
+        "latency_ms": {"batch_p50": round(pct(lat, 50) * 1e3, 1), "plain_epoch_p50": round(pct(plat, 50) * 1e3, 3),
+                       "plain_epoch_p99": round(pct(plat, 99) * 1e3, 3)}}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -103,6 +196,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.copy_bytes}), flush=True)
         return
+
+    if a.workload == "c1":
+        return bench_c1(a)
+    if a.workload == "c3":
+        return bench_c3(a)
 
     n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
     dev = "cuda:0"

@@ -74,7 +74,7 @@ struct EpochLayout {
     size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, mean, tstat, red, sift, st, total;
 };
 
-EpochLayout layout_for(int n, int T, int K, int N_imf) {
+EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     EpochLayout L{};
     size_t W = (size_t)(T + 63) / 64;
     size_t Ecap = (size_t)knot_capacity(T);
@@ -90,7 +90,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf) {
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     L.yknot = take((size_t)K * 2 * n * Ecap * 8);
     L.ysol = take((size_t)K * 2 * n * Ecap * 8);
-    L.coef = take((size_t)K * 2 * n * Ecap * 32);
+    L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
     L.sep = take((size_t)K * 2 * n * nb * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
@@ -168,18 +168,19 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %d)", kStopMaxChannels, nmax);
 
     // ---- workspace plan
+    const int variant = eval_variant_choice();
+    const bool records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused);
     std::vector<EpochLayout> lay(count);
     std::vector<size_t> base(count);
     size_t total = 0;
     for (int e = 0; e < count; ++e) {
-        lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf);
+        lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, records);
         base[e] = total;
         total += lay[e].total;
     }
     size_t desc_at = total;
     total += align_up(sizeof(EpochDesc) * count);
     // block tables of the fixed-size launches and the solver work lists
-    const int variant = eval_variant_choice();
     std::vector<int> nblk(kNumMaps, 0);
     std::vector<long> lcap(kNumLists, 0);
     {
@@ -189,7 +190,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             tmp[e].Ecap = knot_capacity(in[e].T);
             tmp[e].nbmax = solver_block_capacity(tmp[e].Ecap);
             tmp[e].mode = o.mode;
-            tmp[e].coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+            tmp[e].coef_records = records ? 1 : 0;
             for (int f = 0; f < kNumMaps; ++f) {
                 long v = (long)nblk[f] + blocks_for(f, tmp[e], variant);
                 if (v > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
@@ -207,7 +208,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         tmp.Ecap = knot_capacity(in[e].T);
         tmp.nbmax = solver_block_capacity(tmp.Ecap);
         tmp.mode = o.mode;
-        tmp.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+        tmp.coef_records = records ? 1 : 0;
         for (int l = 0; l < kNumLists; ++l) gcap[l][groups > 1 ? e % kXcds : 0] += list_capacity(l, tmp);
     }
     size_t tab_at[kNumMaps], list_at[kNumLists];
@@ -261,7 +262,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         d.st = (EpochState*)(b + L.st);
         d.stats = prof ? (unsigned long long*)(ctx->arena + stats_at) : nullptr;
         d.ndone = (int*)(ctx->arena + ndone_at);
-        d.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+        d.coef_records = records ? 1 : 0;
     }
     std::vector<int2> tabs;
     hipStream_t s = work_stream(ctx);
@@ -321,7 +322,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     plan.mode = o.mode;
     plan.nmax = nmax;
     plan.eval_variant = variant;
-    plan.coef_records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused) ? 1 : 0;
+    plan.coef_records = records ? 1 : 0;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
     // passes (S subtractions plus the deciding pass per IMF); the host replays
@@ -534,7 +535,8 @@ void memd_opts_default(memd_opts* o) {
 
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K) {
     if (n < 1 || T < 4 || K < 1) return 0;
-    return layout_for((int)n, (int)T, (int)K, 1).total;
+    const bool records = eval_variant_choice() != kEvalFused;
+    return layout_for((int)n, (int)T, (int)K, 1, records).total;
 }
 
 memd_status memd_ctx_create(int device, memd_ctx** out) {

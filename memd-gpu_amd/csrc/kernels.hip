@@ -275,7 +275,7 @@ __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     m[kListKnots] = (E + kKnotChunk - 1) / kKnotChunk;
     m[kListBlock] = (J + 1 + bpw - 1) / bpw;
     m[kListReduced] = J > 0 ? 1 : 0;
-    m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * (E - 1) + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
+    m[kListCoef] = (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems));
 }
 
 // One workgroup per (epoch, direction): both envelopes' knot lists, then the
@@ -631,12 +631,36 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const int n = ed.n;
         const int E = ed.E[ke];
         const int J = (E - 3) / L;
-        const int nI = E - 1;
-        const int items = n * nI;
         const int base = it.z * 256 * kCoefItems;
-        const int* X = ed.X + (size_t)ke * ed.Ecap;
         const double* al = fac_row(ed, ke, kFacAl);
         const double* be = fac_row(ed, ke, kFacBe);
+        if (!ed.coef_records) {
+            // M-only: the second derivative of every knot row, in place of y
+            const int items = n * E;
+            for (int r0 = 0; r0 < kCoefItems; r0 += 4) {
+                double m[4];
+                int cs[4], us[4];
+                bool ok[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    int idx = base + (r0 + a) * 256 + threadIdx.x;
+                    ok[a] = idx < items;
+                    int id2 = ok[a] ? idx : 0;
+                    cs[a] = id2 / E;
+                    us[a] = id2 - cs[a] * E;
+                    const double* row = ed.ysol + ((size_t)ke * n + cs[a]) * ed.Ecap;
+                    const double* sv = ed.sep + ((size_t)ke * n + cs[a]) * ed.nbmax;
+                    m[a] = knot_M(row, sv, al, be, us[a], E, J);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+                    if (ok[a]) ed.ysol[((size_t)ke * n + cs[a]) * ed.Ecap + us[a]] = m[a];
+            }
+            continue;
+        }
+        const int nI = E - 1;
+        const int items = n * nI;
+        const int* X = ed.X + (size_t)ke * ed.Ecap;
         const double* ihv = fac_row(ed, ke, kFacIh);
         const double* i3hv = fac_row(ed, ke, kFacI3h);
         double4* out = reinterpret_cast<double4*>(ed.coef);
@@ -776,17 +800,23 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, 
 }
 
 // Fused records + evaluation (fix mode).  One workgroup = one 64-sample tile
-// x 16 channels.  Per valid direction, phase A builds in LDS the spline record
-// (a,b,c,d) of every interval meeting the tile, for both envelopes, from M
-// (phase-3 formula), the knot values and the shared reciprocals -- the same
-// operations as k_coef, so the records are bit-identical; phase B evaluates
-// both envelopes (each lane walks 4 consecutive samples of one channel) and
-// adds the mean in ascending direction order.
+// (one flag word) x kEvalFusedCG channels.  For each chunk of 64 directions a
+// prologue stages, per (direction, envelope), the knot count, the tile's flag
+// word and its knot prefix in LDS.  Per valid direction, phase A builds in LDS
+// the spline record (a,b,c,d) of every interval meeting the tile, for both
+// envelopes, from M (written in place by k_coef), the knot values and the
+// shared reciprocals -- the same operations as the record path, so the records
+// are bit-identical; each thread issues the loads of all its items before
+// using any.  Phase B evaluates both envelopes (each lane walks 4 consecutive
+// samples of one channel) and adds the mean in ascending direction order.
 __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
-    constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, L = kSolverBlock;
+    constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, KC = 64;
+    constexpr int MI = (2 * NI * CG + 255) / 256;   // phase-A items per thread
     __shared__ double4 rec[2][CG][NI];
     __shared__ int xk[2][NI + 1];
+    __shared__ int sE[KC][2], sPre[KC][2];
+    __shared__ unsigned long long sMask[KC][2];
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
@@ -805,79 +835,116 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
-    for (int k = 0; k < K; ++k) {
-        if (!dir_valid(ed.E, k)) continue;
-        nv = nv + 1.0;
-        int jlo[2], nint[2], Ek[2];
-#pragma unroll
-        for (int env = 0; env < 2; ++env) {
-            Ek[env] = ed.E[2 * k + env];
-            jlo[env] = knot_interval(ed, 2 * k + env, t0, Ek[env]);
-            nint[env] = knot_interval(ed, 2 * k + env, tlast, Ek[env]) - jlo[env] + 1;
-        }
-        // phase A
-        const int per0 = nint[0] * CG;
-        const int total = per0 + nint[1] * CG;
-        for (int idx = tid; idx < total; idx += 256) {
-            int env = idx < per0 ? 0 : 1;
-            int rem = env == 0 ? idx : idx - per0;
-            int ni = nint[env];
-            int cl2 = rem / ni;
-            int i = rem - cl2 * ni;
-            int c2 = cg * CG + cl2;
-            int ke = 2 * k + env;
-            const int* X = ed.X + (size_t)ke * ed.Ecap;
-            int j = jlo[env] + i;
-            if (cl2 == 0) {
-                xk[env][i] = X[j];
-                if (i == ni - 1) xk[env][ni] = X[j + 1];
-            }
-            if (c2 >= n) continue;
-            const int E = Ek[env];
-            const int J = (E - 3) / L;
-            const double* row = ed.ysol + ((size_t)ke * n + c2) * ed.Ecap;
-            const double* yk = ed.yknot + ((size_t)ke * n + c2) * ed.Ecap;
-            const double* sv = ed.sep + ((size_t)ke * n + c2) * ed.nbmax;
-            const double* al = fac_row(ed, ke, kFacAl);
-            const double* be = fac_row(ed, ke, kFacBe);
-            double y0 = yk[j], y1 = yk[j + 1];
-            double m0 = knot_M(row, sv, al, be, j, E, J);
-            double m1 = knot_M(row, sv, al, be, j + 1, E, J);
-            double h = (double)(X[j + 1] - X[j]);
-            double sl = (y1 - y0) * fac_row(ed, ke, kFacIh)[j];
-            double4 r;
-            r.x = y0;
-            r.y = sl - (h * (2.0 * m0 + m1)) * (1.0 / 3.0);
-            r.z = m0;
-            r.w = (m1 - m0) * fac_row(ed, ke, kFacI3h)[j];
-            rec[env][cl2][i] = r;
+    for (int kc0 = 0; kc0 < K; kc0 += KC) {
+        const int kcn = min(KC, K - kc0);
+        __syncthreads();
+        if (tid < 2 * kcn) {
+            int kk = tid >> 1, env = tid & 1;
+            int ke = 2 * (kc0 + kk) + env;
+            sE[kk][env] = ed.E[ke];
+            sMask[kk][env] = ed.mask[(size_t)ke * ed.W + tile];
+            sPre[kk][env] = ed.pre[(size_t)ke * ed.W + tile];
         }
         __syncthreads();
-        // phase B
-        if (c < n && ts <= tlast) {
+        for (int kk = 0; kk < kcn; ++kk) {
+            const int k = kc0 + kk;
+            if (!(sE[kk][0] > 3 && sE[kk][1] > 3)) continue;
+            nv = nv + 1.0;
+            int jlo[2], nint[2];
 #pragma unroll
             for (int env = 0; env < 2; ++env) {
-                int i = knot_interval(ed, 2 * k + env, ts, Ek[env]) - jlo[env];
-                int xj = xk[env][i], xn = xk[env][i + 1];
-                double4 r = rec[env][cl][i];
+                int E = sE[kk][env];
+                int pr = sPre[kk][env];
+                unsigned long long m = sMask[kk][env];
+                int lo = min(max(pr - 1, 0), E - 2);
+                int hi = min(max(pr + __popcll(m & ((1ull << (tlast - t0)) - 1ull)) - 1, 0), E - 2);
+                jlo[env] = lo;
+                nint[env] = hi - lo + 1;
+            }
+            // phase A: loads of every item first, then the records
+            const int per0 = nint[0] * CG;
+            const int total = per0 + nint[1] * CG;
+            double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
+            int x0[MI], x1[MI], slot[MI];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    int t = ts + s;
-                    if (t <= tlast) {
-                        if (t > xn) {
-                            ++i;
-                            xj = xn;
-                            xn = xk[env][i + 1];
-                            r = rec[env][cl][i];
+            for (int a = 0; a < MI; ++a) {
+                int idx = tid + 256 * a;
+                bool ok = idx < total;
+                int env = (idx < per0) ? 0 : 1;
+                int rem = env == 0 ? idx : idx - per0;
+                int ni = max(nint[env], 1);
+                int cl2 = rem / ni;
+                int i = rem - cl2 * ni;
+                int c2 = min(cg * CG + cl2, n - 1);
+                int ke = 2 * k + env;
+                int j = ok ? jlo[env] + i : 0;
+                const int* X = ed.X + (size_t)ke * ed.Ecap;
+                const double* M = ed.ysol + ((size_t)ke * n + c2) * ed.Ecap;
+                const double* yk = ed.yknot + ((size_t)ke * n + c2) * ed.Ecap;
+                x0[a] = X[j];
+                x1[a] = X[j + 1];
+                v0[a] = yk[j];
+                v1[a] = yk[j + 1];
+                m0[a] = M[j];
+                m1[a] = M[j + 1];
+                ih[a] = fac_row(ed, ke, kFacIh)[j];
+                i3h[a] = fac_row(ed, ke, kFacI3h)[j];
+                slot[a] = ok ? ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) : -1;
+            }
+#pragma unroll
+            for (int a = 0; a < MI; ++a) {
+                int sl0 = slot[a];
+                if (sl0 < 0) continue;
+                int sv = sl0 & 0x3fffffff;
+                int env = sv / (CG * NI);
+                int rem = sv - env * CG * NI;
+                int cl2 = rem / NI;
+                int i = rem - cl2 * NI;
+                if (cl2 == 0) {
+                    xk[env][i] = x0[a];
+                    if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
+                }
+                if (sl0 & 0x40000000) continue;   // channel beyond n (ragged group)
+                double h = (double)(x1[a] - x0[a]);
+                double slp = (v1[a] - v0[a]) * ih[a];
+                double4 r;
+                r.x = v0[a];
+                r.y = slp - (h * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
+                r.z = m0[a];
+                r.w = (m1[a] - m0[a]) * i3h[a];
+                rec[env][cl2][i] = r;
+            }
+            __syncthreads();
+            // phase B
+            if (c < n && ts <= tlast) {
+#pragma unroll
+                for (int env = 0; env < 2; ++env) {
+                    int E = sE[kk][env];
+                    unsigned long long m = sMask[kk][env];
+                    int jj = sPre[kk][env] + __popcll(m & ((1ull << (ts - t0)) - 1ull)) - 1;
+                    jj = min(max(jj, 0), E - 2);
+                    int i = jj - jlo[env];
+                    int xj = xk[env][i], xn = xk[env][i + 1];
+                    double4 r = rec[env][cl][i];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        int t = ts + s;
+                        if (t <= tlast) {
+                            if (t > xn) {
+                                ++i;
+                                xj = xn;
+                                xn = xk[env][i + 1];
+                                r = rec[env][cl][i];
+                            }
+                            double v = horner4(r, (double)(t - xj));
+                            if (env == 0) ep[s] = v;
+                            else acc[s] = acc[s] + (ep[s] + v) * 0.5;
                         }
-                        double v = horner4(r, (double)(t - xj));
-                        if (env == 0) ep[s] = v;
-                        else acc[s] = acc[s] + (ep[s] + v) * 0.5;
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (ed.stats && br.y == 0 && tid == 0) {
         atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
@@ -1143,9 +1210,7 @@ long list_capacity(int list, const EpochDesc& e) {
     case kListKnots: return K2 * ((e.Ecap + kKnotChunk - 1) / kKnotChunk);
     case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
     case kListReduced: return K2;
-    case kListCoef:
-        if (!e.coef_records) return 0;
-        return K2 * (((long)e.n * (e.Ecap - 1) + 256L * kCoefItems - 1) / (256L * kCoefItems));
+    case kListCoef: return K2 * (((long)e.n * e.Ecap + 256L * kCoefItems - 1) / (256L * kCoefItems));
     }
     return 0;
 }
@@ -1187,9 +1252,7 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     case 3: MEMD_LIST_LAUNCH(k_knots, kListKnots, kKnotChunk); break;
     case 4: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
     case 5: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
-    case 6:
-        if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
-        break;
+    case 6: MEMD_LIST_LAUNCH(k_coef, kListCoef, 256); break;
     case 7:
         if (p.mode == kModeStop)
             MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
