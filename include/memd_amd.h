@@ -29,7 +29,8 @@ typedef enum memd_status {
     MEMD_ERR_COORDS = 4,     /* sample coordinates are not 0..T-1 */
     MEMD_ERR_UNSUPPORTED = 5,/* shape outside the engine's limits (e.g. n > 256) */
     MEMD_ERR_ADMISSION = 6,  /* request refused by the HBM admission budget */
-    MEMD_ERR_NOT_FOUND = 7   /* unknown request id */
+    MEMD_ERR_NOT_FOUND = 7,  /* unknown request id */
+    MEMD_ERR_IO = 8          /* file could not be opened, read or written */
 } memd_status;
 
 /* Sifting policy. */
@@ -64,7 +65,9 @@ memd_status memd_ctx_create(int device, memd_ctx** out);
 memd_status memd_ctx_destroy(memd_ctx* ctx);
 /* Run on the given hipStream_t (passed as void*); NULL = legacy default stream. */
 memd_status memd_ctx_set_stream(memd_ctx* ctx, void* hip_stream);
-/* Message of the last failure on this context (never NULL). */
+/* Message of the last failure on this context (never NULL).  With ctx == NULL:
+ * the last failure of a context-free call (file I/O, memd_ctx_create) on this
+ * host thread. */
 const char* memd_last_error(memd_ctx* ctx);
 void memd_opts_default(memd_opts* o);
 /* Library version string. */
@@ -159,6 +162,61 @@ memd_status memd_profile_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size
 
 /* Device bytes the engine needs for one epoch of this shape (workspace only). */
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K);
+
+/*
+ * Raw sample files: float32, channel-major [n][T_file], no header -- the
+ * reference demo's input (EEGsample_32CH_30504.bin) and output (mode_matrix.bin,
+ * [N_imf][n][T]).  Replace getBinSize / readBin / writeBin
+ * (memReducedEpochs_dev.cu:896-932) as main() (:1484-1501, :1526-1527) and
+ * runBenchmark() (:1405-1426) use them.  Host memory only, no device needed;
+ * failures return MEMD_ERR_IO / MEMD_ERR_ARG (the reference exits) with the
+ * message in memd_last_error(NULL).
+ */
+/* T_file = bytes / sizeof(float) / n, floored like the reference (:1486). */
+memd_status memd_bin_length(const char* path, size_t n, size_t* T_file);
+/* h_y [n][T] <- the first T samples of each channel of an [n][T_file] file
+ * (T == T_file: the whole file, as main(); T < T_file: runBenchmark's slicing). */
+memd_status memd_read_bin_f32(const char* path, size_t n, size_t T_file, size_t T, float* h_y);
+/* Write count floats (writeBin). */
+memd_status memd_write_bin_f32(const char* path, const float* h, size_t count);
+
+/*
+ * Served continuous batching.  No reference counterpart: the reference runs one
+ * epoch per blocking call (memReducedEpochs_dev.cu:1074) and allocates its
+ * workspace inside it (:1077-1165).  A server owns a fixed HBM pool.  Between
+ * sift steps it admits queued epochs whose workspace fits the pool (first fit,
+ * FIFO order with bounded backfill), advances every active epoch by one sift
+ * step per launch sequence, and retires each epoch as soon as it finishes,
+ * returning its workspace to the pool.  Per-epoch arithmetic is unchanged, so
+ * every result is bit-identical to memd_run_f64 on that epoch alone.  The IMF
+ * stack is written into the request's d_IMFs on the device as sifting
+ * proceeds; read it once the request is MEMD_REQ_DONE.
+ */
+typedef struct memd_server memd_server;
+enum { MEMD_REQ_QUEUED = 0, MEMD_REQ_RUNNING = 1, MEMD_REQ_DONE = 2 };
+
+/* hbm_pool_bytes: workspace pool (sum of memd_workspace_bytes of the epochs
+ * active at once); max_active: cap on epochs advanced together (0 = only the
+ * pool limits it).  The options' mode, window and thresholds apply to every
+ * request; the diagnostics fields must be unset.  Uses the context's stream. */
+memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max_active, const memd_opts* opts,
+                               memd_server** out);
+memd_status memd_server_destroy(memd_server* srv);
+/* Queue one epoch.  Its device buffers and optional host outputs
+ * (h_sift_counts, h_n_imf_out, written at completion) must stay valid until the
+ * request is done.  MEMD_ERR_ADMISSION: its workspace can never fit the pool. */
+memd_status memd_server_submit(memd_server* srv, const memd_epoch* ep, int64_t* id);
+/* Admit what fits, run `steps` sift steps over the active set, retire finished
+ * epochs.  Blocking.  Optional outputs: active and queued counts afterwards. */
+memd_status memd_server_step(memd_server* srv, int32_t steps, int32_t* n_active, int32_t* n_queued);
+/* Pop up to cap ids of requests finished since the last call, in completion order. */
+memd_status memd_server_completed(memd_server* srv, int64_t* ids, int32_t cap, int32_t* n);
+/* MEMD_REQ_* state of a request; MEMD_ERR_NOT_FOUND for unknown or released ids. */
+memd_status memd_server_poll(memd_server* srv, int64_t id, int32_t* state);
+/* Forget a finished request. */
+memd_status memd_server_release(memd_server* srv, int64_t id);
+/* Pool bytes currently held by active epochs. */
+size_t memd_server_pool_used(memd_server* srv);
 
 #ifdef __cplusplus
 }

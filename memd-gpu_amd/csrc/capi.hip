@@ -8,6 +8,7 @@
 // across calls, knot counts never leave the device, and no host synchronisation
 // happens inside the sifting loop.
 #include "../../include/memd_amd.h"
+#include "binio.h"
 #include "directions.h"
 #include "engine.h"
 
@@ -50,6 +51,7 @@ memd_status fail(memd_ctx* ctx, memd_status code, const char* fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     if (ctx) ctx->err = buf;
+    else set_global_error(buf);
     return code;
 }
 
@@ -149,6 +151,164 @@ int eval_variant_choice() {
     return kEvalDefault;
 }
 
+// Descriptor fields that depend only on the epoch's shape and the options.
+EpochDesc desc_geometry(const EpochIn& I, const memd_opts& o, bool records) {
+    EpochDesc d{};
+    d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
+    d.Ecap = knot_capacity(I.T);
+    d.nbmax = solver_block_capacity(d.Ecap);
+    d.N_imf = I.N_imf; d.S = I.S; d.mode = o.mode; d.window = o.window;
+    d.sd = o.sd; d.sd2 = o.sd2; d.tol = o.tol;
+    d.y = I.y; d.V = I.V; d.imfs = I.imfs;
+    d.coef_records = records ? 1 : 0;
+    return d;
+}
+
+// Point a descriptor at its workspace (layout L starting at b).
+void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
+    d.running = (double*)(b + L.running);
+    d.mask = (unsigned long long*)(b + L.mask);
+    d.pre = (int*)(b + L.pre);
+    d.E = (int*)(b + L.E);
+    d.X = (int*)(b + L.X);
+    d.fac = (double*)(b + L.fac);
+    d.red = (double*)(b + L.redsys);
+    d.yknot = (double*)(b + L.yknot);
+    d.ysol = (double*)(b + L.ysol);
+    d.coef = (double*)(b + L.coef);
+    d.sep = (double*)(b + L.sep);
+    d.mean = (double*)(b + L.mean);
+    d.tstat = (double*)(b + L.tstat);
+    d.stopred = (int*)(b + L.red);
+    d.sift_counts = (int*)(b + L.sift);
+    d.st = (EpochState*)(b + L.st);
+}
+
+// Device region of a step plan: descriptors, block tables of the fixed-size
+// launches, the solver work lists and their counters, statistics, the done
+// counter, a table of epochs to initialise and a state snapshot.  Offsets are
+// relative to the region's base.
+struct PlanLayout {
+    size_t desc_at = 0, tab_at[kNumMaps] = {}, list_at[kNumLists] = {};
+    size_t cnt_at = 0, stats_at = 0, ndone_at = 0, fresh_at = 0, states_at = 0, total = 0;
+    int nblk[kNumMaps] = {};
+    long lcap[kNumLists] = {};
+    long gcap[kNumLists][kXcds] = {};
+    int groups = 1;
+};
+
+// grp[e] in [0, groups) is the XCD group of epoch e (groups == 1: no affinity).
+memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const std::vector<int>& grp, int groups,
+                        int variant, size_t start, PlanLayout* P) {
+    const int count = (int)eds.size();
+    *P = PlanLayout{};
+    P->groups = groups;
+    for (int e = 0; e < count; ++e) {
+        for (int f = 0; f < kNumMaps; ++f) {
+            long v = (long)P->nblk[f] + blocks_for(f, eds[e], variant);
+            if (v > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
+            P->nblk[f] = (int)v;
+        }
+        for (int l = 0; l < kNumLists; ++l) {
+            long c = list_capacity(l, eds[e]);
+            P->lcap[l] += c;
+            P->gcap[l][grp[e]] += c;
+        }
+    }
+    size_t total = start;
+    P->desc_at = total;
+    total += align_up(sizeof(EpochDesc) * (size_t)count);
+    for (int f = 0; f < kNumMaps; ++f) {
+        P->tab_at[f] = total;
+        total += align_up(sizeof(int2) * (size_t)P->nblk[f]);
+    }
+    for (int l = 0; l < kNumLists; ++l) {
+        if (P->lcap[l] > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
+        P->list_at[l] = total;
+        total += align_up(sizeof(int4) * (size_t)P->lcap[l]);
+    }
+    P->cnt_at = total;
+    total += align_up(sizeof(int) * kNumLists * kXcds);
+    P->stats_at = total;
+    total += align_up(sizeof(unsigned long long) * 4);
+    P->ndone_at = total;
+    total += align_up(sizeof(int));
+    P->fresh_at = total;
+    total += align_up(sizeof(int2) * (size_t)P->nblk[kMapInit]);
+    P->states_at = total;
+    total += align_up(sizeof(EpochState) * (size_t)count);
+    P->total = total;
+    return MEMD_OK;
+}
+
+// Upload descriptors and block tables into the region and fill the plan.  Block
+// id b runs on XCD b mod 8: epoch e's blocks are placed at ids of XCD grp[e]
+// (falling back to any queue once its own is drained).  tabs is host staging
+// that must outlive the asynchronous copies (the caller synchronises later).
+memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::vector<int>& grp,
+                       const PlanLayout& P, char* region, int mode, int variant, bool records,
+                       std::vector<int2>& tabs, hipStream_t s, StepPlan* out) {
+    const int count = (int)eds.size();
+    const int groups = P.groups;
+    // the sub-list an epoch appends to is the one its capacity was planned in
+    for (int e = 0; e < count; ++e) eds[e].xcd = grp[e];
+    HIP_TRY(ctx, hipMemcpyAsync(region + P.desc_at, eds.data(), sizeof(EpochDesc) * count, hipMemcpyHostToDevice, s));
+    tabs.clear();
+    for (int f = 0; f < kNumMaps; ++f) {
+        std::vector<std::vector<int2>> q(groups);
+        for (int e = 0; e < count; ++e) {
+            int nb = blocks_for(f, eds[e], variant);
+            for (int b = 0; b < nb; ++b) q[grp[e]].push_back(make_int2(e, b));
+        }
+        std::vector<size_t> head(groups, 0);
+        for (int b = 0; b < P.nblk[f]; ++b) {
+            int x = b % groups;
+            if (head[x] == q[x].size()) {
+                size_t best = 0;
+                for (int y = 0; y < groups; ++y)
+                    if (q[y].size() - head[y] > best) { best = q[y].size() - head[y]; x = y; }
+            }
+            tabs.push_back(q[x][head[x]++]);
+        }
+    }
+    size_t pos = 0;
+    for (int f = 0; f < kNumMaps; ++f) {
+        if (P.nblk[f] > 0)
+            HIP_TRY(ctx, hipMemcpyAsync(region + P.tab_at[f], tabs.data() + pos, sizeof(int2) * P.nblk[f],
+                                        hipMemcpyHostToDevice, s));
+        pos += P.nblk[f];
+    }
+    StepPlan plan{};
+    plan.eps = (const EpochDesc*)(region + P.desc_at);
+    plan.count = count;
+    for (int f = 0; f < kNumMaps; ++f) {
+        plan.tab[f] = (const int2*)(region + P.tab_at[f]);
+        plan.blocks[f] = P.nblk[f];
+    }
+    plan.wl.count = (int*)(region + P.cnt_at);
+    plan.wl.groups = groups;
+    for (int l = 0; l < kNumLists; ++l) {
+        plan.wl.item[l] = (int4*)(region + P.list_at[l]);
+        long at = 0;
+        for (int x = 0; x < kXcds; ++x) {
+            plan.wl.off[l][x] = (int)at;
+            plan.wl.cap[l][x] = (int)P.gcap[l][x];
+            at += P.gcap[l][x];
+        }
+        long g = P.lcap[l] < kListGrid ? P.lcap[l] : kListGrid;
+        if (groups > 1) g = kListGrid;   // a multiple of kXcds, every XCD gets its share
+        plan.grid[l] = (int)(g > 0 ? g : 1);
+    }
+    int nmax = 0;
+    for (const EpochDesc& d : eds) nmax = d.n > nmax ? d.n : nmax;
+    plan.mode = mode;
+    plan.nmax = nmax;
+    plan.eval_variant = variant;
+    plan.coef_records = records ? 1 : 0;
+    *out = plan;
+    return MEMD_OK;
+}
+
 // Core: run a set of epochs (all already FP64 and on the device) to completion.
 memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
                        memd_kernel_profile* prof = nullptr) {
@@ -167,162 +327,44 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     if (o.mode == MEMD_MODE_STOP && nmax > kStopMaxChannels)
         return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %d)", kStopMaxChannels, nmax);
 
-    // ---- workspace plan
+    // ---- workspace plan: every epoch's workspace, then the plan region
     const int variant = eval_variant_choice();
     const bool records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused);
     std::vector<EpochLayout> lay(count);
     std::vector<size_t> base(count);
+    std::vector<EpochDesc> eds(count);
     size_t total = 0;
     for (int e = 0; e < count; ++e) {
         lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, records);
         base[e] = total;
         total += lay[e].total;
-    }
-    size_t desc_at = total;
-    total += align_up(sizeof(EpochDesc) * count);
-    // block tables of the fixed-size launches and the solver work lists
-    std::vector<int> nblk(kNumMaps, 0);
-    std::vector<long> lcap(kNumLists, 0);
-    {
-        std::vector<EpochDesc> tmp(count);
-        for (int e = 0; e < count; ++e) {
-            tmp[e].n = in[e].n; tmp[e].T = in[e].T; tmp[e].K = in[e].K;
-            tmp[e].Ecap = knot_capacity(in[e].T);
-            tmp[e].nbmax = solver_block_capacity(tmp[e].Ecap);
-            tmp[e].mode = o.mode;
-            tmp[e].coef_records = records ? 1 : 0;
-            for (int f = 0; f < kNumMaps; ++f) {
-                long v = (long)nblk[f] + blocks_for(f, tmp[e], variant);
-                if (v > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
-                nblk[f] = (int)v;
-            }
-            for (int l = 0; l < kNumLists; ++l) lcap[l] += list_capacity(l, tmp[e]);
-        }
+        eds[e] = desc_geometry(in[e], o, records);
     }
     // XCD affinity when the batch has enough epochs to keep every XCD busy
     const int groups = count >= kXcds ? kXcds : 1;
-    std::vector<std::vector<long>> gcap(kNumLists, std::vector<long>(kXcds, 0));
-    for (int e = 0; e < count; ++e) {
-        EpochDesc tmp{};
-        tmp.n = in[e].n; tmp.T = in[e].T; tmp.K = in[e].K;
-        tmp.Ecap = knot_capacity(in[e].T);
-        tmp.nbmax = solver_block_capacity(tmp.Ecap);
-        tmp.mode = o.mode;
-        tmp.coef_records = records ? 1 : 0;
-        for (int l = 0; l < kNumLists; ++l) gcap[l][groups > 1 ? e % kXcds : 0] += list_capacity(l, tmp);
-    }
-    size_t tab_at[kNumMaps], list_at[kNumLists];
-    for (int f = 0; f < kNumMaps; ++f) {
-        tab_at[f] = total;
-        total += align_up(sizeof(int2) * (size_t)nblk[f]);
-    }
-    for (int l = 0; l < kNumLists; ++l) {
-        if (lcap[l] > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
-        list_at[l] = total;
-        total += align_up(sizeof(int4) * (size_t)lcap[l]);
-    }
-    size_t cnt_at = total;
-    total += align_up(sizeof(int) * kNumLists * kXcds);
-    size_t stats_at = total;
-    total += align_up(sizeof(unsigned long long) * 4);
-    size_t ndone_at = total;
-    total += align_up(sizeof(int));
-    memd_status rs = ensure_arena(ctx, total);
+    std::vector<int> grp(count);
+    for (int e = 0; e < count; ++e) grp[e] = groups > 1 ? e % kXcds : 0;
+    PlanLayout P;
+    memd_status rs = plan_layout(ctx, eds, grp, groups, variant, total, &P);
     if (rs != MEMD_OK) return rs;
-
-    std::vector<EpochDesc> eds(count);
+    rs = ensure_arena(ctx, P.total);
+    if (rs != MEMD_OK) return rs;
+    const size_t stats_at = P.stats_at;
+    int* ndone_dev = (int*)(ctx->arena + P.ndone_at);
     for (int e = 0; e < count; ++e) {
-        const EpochIn& I = in[e];
-        const EpochLayout& L = lay[e];
-        char* b = ctx->arena + base[e];
         EpochDesc& d = eds[e];
-        d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
-        d.Ecap = knot_capacity(I.T);
-        d.nbmax = solver_block_capacity(d.Ecap);
-        d.N_imf = I.N_imf; d.S = I.S; d.mode = o.mode; d.window = o.window;
-        d.sd = o.sd; d.sd2 = o.sd2; d.tol = o.tol;
-        d.y = I.y; d.V = I.V; d.imfs = I.imfs;
-        d.running = (double*)(b + L.running);
-        d.mask = (unsigned long long*)(b + L.mask);
-        d.pre = (int*)(b + L.pre);
-        d.E = (int*)(b + L.E);
-        d.X = (int*)(b + L.X);
-        d.fac = (double*)(b + L.fac);
-        d.red = (double*)(b + L.redsys);
-        d.yknot = (double*)(b + L.yknot);
-        d.ysol = (double*)(b + L.ysol);
-        d.coef = (double*)(b + L.coef);
-        d.sep = (double*)(b + L.sep);
-        d.mean = (double*)(b + L.mean);
-        d.tstat = (double*)(b + L.tstat);
-        d.stopred = (int*)(b + L.red);
+        bind_workspace(d, ctx->arena + base[e], lay[e]);
         d.d_trace = o.d_knot_counts;
         d.max_passes = o.d_knot_counts ? o.max_passes : 0;
-        d.sift_counts = (int*)(b + L.sift);
-        d.st = (EpochState*)(b + L.st);
         d.stats = prof ? (unsigned long long*)(ctx->arena + stats_at) : nullptr;
-        d.ndone = (int*)(ctx->arena + ndone_at);
-        d.coef_records = records ? 1 : 0;
+        d.ndone = ndone_dev;
     }
-    std::vector<int2> tabs;
     hipStream_t s = work_stream(ctx);
-    int* ndone_dev = (int*)(ctx->arena + ndone_at);
     HIP_TRY(ctx, hipMemsetAsync(ndone_dev, 0, sizeof(int), s));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + desc_at, eds.data(), sizeof(EpochDesc) * count, hipMemcpyHostToDevice, s));
-    for (int f = 0; f < kNumMaps; ++f) {
-        // block order: block id b runs on XCD b mod 8; place epoch e's blocks on
-        // XCD e mod 8 (falling back to any queue once its own is drained)
-        std::vector<std::vector<int2>> q(groups);
-        for (int e = 0; e < count; ++e) {
-            int nb = blocks_for(f, eds[e], variant);
-            for (int b = 0; b < nb; ++b) q[groups > 1 ? e % kXcds : 0].push_back(make_int2(e, b));
-        }
-        std::vector<size_t> head(groups, 0);
-        for (int b = 0; b < nblk[f]; ++b) {
-            int x = b % groups;
-            if (head[x] == q[x].size()) {
-                size_t best = 0;
-                for (int y = 0; y < groups; ++y)
-                    if (q[y].size() - head[y] > best) { best = q[y].size() - head[y]; x = y; }
-            }
-            tabs.push_back(q[x][head[x]++]);
-        }
-    }
-    {
-        size_t pos = 0;
-        for (int f = 0; f < kNumMaps; ++f) {
-            if (nblk[f] > 0)
-                HIP_TRY(ctx, hipMemcpyAsync(ctx->arena + tab_at[f], tabs.data() + pos, sizeof(int2) * nblk[f],
-                                            hipMemcpyHostToDevice, s));
-            pos += nblk[f];
-        }
-    }
-
     StepPlan plan{};
-    plan.eps = (const EpochDesc*)(ctx->arena + desc_at);
-    plan.count = count;
-    for (int f = 0; f < kNumMaps; ++f) {
-        plan.tab[f] = (const int2*)(ctx->arena + tab_at[f]);
-        plan.blocks[f] = nblk[f];
-    }
-    plan.wl.count = (int*)(ctx->arena + cnt_at);
-    plan.wl.groups = groups;
-    for (int l = 0; l < kNumLists; ++l) {
-        plan.wl.item[l] = (int4*)(ctx->arena + list_at[l]);
-        long at = 0;
-        for (int x = 0; x < kXcds; ++x) {
-            plan.wl.off[l][x] = (int)at;
-            plan.wl.cap[l][x] = (int)gcap[l][x];
-            at += gcap[l][x];
-        }
-        long g = lcap[l] < kListGrid ? lcap[l] : kListGrid;
-        if (groups > 1) g = kListGrid;   // a multiple of kXcds, every XCD gets its share
-        plan.grid[l] = (int)(g > 0 ? g : 1);
-    }
-    plan.mode = o.mode;
-    plan.nmax = nmax;
-    plan.eval_variant = variant;
-    plan.coef_records = records ? 1 : 0;
+    std::vector<int2> tabs;
+    rs = write_plan(ctx, eds, grp, P, ctx->arena, o.mode, variant, records, tabs, s, &plan);
+    if (rs != MEMD_OK) return rs;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
     // passes (S subtractions plus the deciding pass per IMF); the host replays
@@ -546,23 +588,32 @@ memd_status memd_ctx_create(int device, memd_ctx** out) {
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
-        return MEMD_ERR_HIP;
+        return fail(nullptr, MEMD_ERR_HIP, "no HIP device available (%s)",
+                    e != hipSuccess ? hipGetErrorString(e) : "device count is 0");
     }
     memd_ctx* ctx = new memd_ctx();
     if (device < 0) {
         if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return MEMD_ERR_HIP; }
     } else {
-        if (device >= ndev) { delete ctx; return MEMD_ERR_ARG; }
+        if (device >= ndev) {
+            delete ctx;
+            return fail(nullptr, MEMD_ERR_ARG, "device %d does not exist (%d visible)", device, ndev);
+        }
         ctx->device = device;
     }
-    if (hipSetDevice(ctx->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&ctx->t0) != hipSuccess || hipEventCreate(&ctx->t1) != hipSuccess ||
-        hipMalloc((void**)&ctx->flag, sizeof(int)) != hipSuccess || configure_kernels() != hipSuccess) {
+    const char* step = "hipSetDevice";
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he == hipSuccess) { step = "hipStreamCreateWithFlags"; he = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking); }
+    if (he == hipSuccess) { step = "hipEventCreateWithFlags"; he = hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming); }
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming);
+    if (he == hipSuccess) { step = "hipEventCreate"; he = hipEventCreate(&ctx->t0); }
+    if (he == hipSuccess) he = hipEventCreate(&ctx->t1);
+    if (he == hipSuccess) { step = "hipMalloc"; he = hipMalloc((void**)&ctx->flag, sizeof(int)); }
+    if (he == hipSuccess) { step = "hipFuncSetAttribute"; he = configure_kernels(); }
+    if (he != hipSuccess) {
         memd_ctx_destroy(ctx);
-        return MEMD_ERR_HIP;
+        return fail(nullptr, MEMD_ERR_HIP, "context set-up on device %d: %s failed: %s", device, step,
+                    hipGetErrorString(he));
     }
     for (int i = 0; i <= kNumStepKernels; ++i) {
         if (hipEventCreate(&ctx->pev[i]) == hipSuccess) continue;
@@ -593,15 +644,15 @@ memd_status memd_ctx_destroy(memd_ctx* ctx) {
 }
 
 memd_status memd_ctx_set_stream(memd_ctx* ctx, void* hip_stream) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     ctx->user = (hipStream_t)hip_stream;
     return MEMD_OK;
 }
 
-const char* memd_last_error(memd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* memd_last_error(memd_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
 
 memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes, double* d_V) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     if (n < 1 || K < 1 || !d_V) return fail(ctx, MEMD_ERR_ARG, "invalid direction arguments n=%zu K=%zu", n, K);
     if (n > (size_t)kMaxChannels || K > 65536) return fail(ctx, MEMD_ERR_UNSUPPORTED, "n or K too large");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -619,7 +670,7 @@ memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t
 
 memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S, const int32_t* d_x,
                          const double* d_y, const double* d_V, double* d_IMFs, const memd_opts* opts, double* elapsed_s) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     memd_status rs = validate_shape(ctx, n, T, K, N_imf, S);
     if (rs != MEMD_OK) return rs;
     if (!d_y || !d_V || !d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
@@ -633,7 +684,7 @@ memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_i
 
 memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S, const int32_t* d_x,
                          const float* d_y, const float* d_V, float* d_IMFs, const memd_opts* opts, double* elapsed_s) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     memd_status rs = validate_shape(ctx, n, T, K, N_imf, S);
     if (rs != MEMD_OK) return rs;
     if (!d_y || !d_V || !d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
@@ -674,7 +725,7 @@ static memd_status mex_common(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_imf
 
 memd_status memd_mex_compat_f64(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_imf, int32_t S, size_t numel,
                                 const double* d_data, const int32_t* d_index, const int32_t* d_primes, double* d_out) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     if (!d_data || !d_out) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
     size_t T = 0;
     const double* dV = nullptr;
@@ -685,7 +736,7 @@ memd_status memd_mex_compat_f64(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_i
 
 memd_status memd_mex_compat_f32(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_imf, int32_t S, size_t numel,
                                 const float* d_data, const int32_t* d_index, const int32_t* d_primes, float* d_out) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     if (!d_data || !d_out) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
     size_t T = 0;
     const double* dV = nullptr;
@@ -706,7 +757,7 @@ memd_status memd_mex_compat_f32(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_i
 
 static memd_status batch_common(memd_ctx* ctx, const memd_epoch* epochs, size_t count, const memd_opts* opts,
                                 double* elapsed_s, memd_kernel_profile* prof) {
-    if (!ctx) return MEMD_ERR_ARG;
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
     if (count == 0) return MEMD_OK;
     if (!epochs) return fail(ctx, MEMD_ERR_ARG, "null epoch array");
     std::vector<EpochIn> in(count);
@@ -732,5 +783,325 @@ memd_status memd_profile_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size
     if (!out) return fail(ctx, MEMD_ERR_ARG, "null profile output");
     return batch_common(ctx, epochs, count, opts, nullptr, out);
 }
+
+}  // extern "C"
+
+// ======================================================================= server
+//
+// Served continuous batching.  Workspaces live in one pool allocated at
+// creation; the plan region (descriptors, tables, lists) is rebuilt whenever
+// the active set changes.  Continuing epochs keep their workspace (and thus
+// their state) across rebuilds; joining epochs are initialised through a
+// table that covers only them.
+
+struct memd_server {
+    struct Req {
+        EpochIn in{};
+        EpochLayout lay{};
+        int state = MEMD_REQ_QUEUED;
+        size_t off = 0;
+        int grp = 0;
+        long weight = 0;      // n * T * K, the XCD load estimate
+        long skipped = 0;     // admission rounds this request was passed over
+    };
+    memd_ctx* ctx = nullptr;
+    memd_opts opts{};
+    int variant = 0;
+    bool records = false;
+    char* pool = nullptr;
+    size_t pool_bytes = 0, pool_used = 0;
+    std::map<size_t, size_t> free_blocks;   // offset -> bytes, coalesced
+    char* region = nullptr;
+    size_t region_bytes = 0;
+    int max_active = 0;
+    std::map<int64_t, Req> reqs;
+    std::vector<int64_t> queue;             // FIFO
+    std::vector<int64_t> active;
+    std::vector<int64_t> finished;          // completion order, popped by memd_server_completed
+    size_t finished_head = 0;
+    int64_t next_id = 1;
+    long load[kXcds] = {};
+    // current plan and its host staging (kept alive across asynchronous copies)
+    std::vector<EpochDesc> eds;
+    std::vector<int> grp;
+    std::vector<int2> tabs, fresh;
+    std::vector<EpochState> states;
+    PlanLayout P;
+    StepPlan plan{};
+};
+
+namespace {
+
+// A queued request passed over this many admission rounds blocks backfill
+// behind it until it fits.
+constexpr long kMaxSkip = 8;
+
+bool pool_alloc(memd_server* S, size_t bytes, size_t* off) {
+    for (auto it = S->free_blocks.begin(); it != S->free_blocks.end(); ++it) {
+        if (it->second < bytes) continue;
+        *off = it->first;
+        size_t rem = it->second - bytes, at = it->first + bytes;
+        S->free_blocks.erase(it);
+        if (rem) S->free_blocks[at] = rem;
+        S->pool_used += bytes;
+        return true;
+    }
+    return false;
+}
+
+void pool_free(memd_server* S, size_t off, size_t bytes) {
+    S->pool_used -= bytes;
+    auto it = S->free_blocks.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != S->free_blocks.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        S->free_blocks.erase(nx);
+    }
+    if (it != S->free_blocks.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            S->free_blocks.erase(it);
+        }
+    }
+}
+
+// Admit queued requests that fit, in FIFO order; returns the admitted ids.
+void server_admit(memd_server* S, std::vector<int64_t>& joined) {
+    std::vector<int64_t> keep;
+    bool blocked = false;
+    for (int64_t id : S->queue) {
+        memd_server::Req& r = S->reqs[id];
+        size_t off = 0;
+        bool room = !(S->max_active > 0 && (int)S->active.size() >= S->max_active);
+        if (!blocked && room && pool_alloc(S, r.lay.total, &off)) {
+            r.off = off;
+            r.state = MEMD_REQ_RUNNING;
+            int g = 0;
+            for (int x = 1; x < kXcds; ++x)
+                if (S->load[x] < S->load[g]) g = x;
+            r.grp = g;
+            S->load[g] += r.weight;
+            S->active.push_back(id);
+            joined.push_back(id);
+            continue;
+        }
+        if (!blocked && room && ++r.skipped > kMaxSkip) blocked = true;
+        keep.push_back(id);
+    }
+    S->queue.swap(keep);
+}
+
+// Rebuild the plan over the active set and initialise the joining epochs.
+memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, hipStream_t s) {
+    memd_ctx* ctx = S->ctx;
+    const int count = (int)S->active.size();
+    S->eds.resize(count);
+    S->grp.resize(count);
+    const int groups = count >= kXcds ? kXcds : 1;
+    std::map<int64_t, int> index;
+    for (int e = 0; e < count; ++e) {
+        memd_server::Req& r = S->reqs[S->active[e]];
+        S->eds[e] = desc_geometry(r.in, S->opts, S->records);
+        bind_workspace(S->eds[e], S->pool + r.off, r.lay);
+        S->grp[e] = groups > 1 ? r.grp : 0;
+        index[S->active[e]] = e;
+    }
+    memd_status rs = plan_layout(ctx, S->eds, S->grp, groups, S->variant, 0, &S->P);
+    if (rs != MEMD_OK) return rs;
+    if (S->P.total > S->region_bytes) {
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (S->region) HIP_TRY(ctx, hipFree(S->region));
+        S->region = nullptr;
+        S->region_bytes = 0;
+        size_t want = S->P.total + S->P.total / 4;
+        hipError_t e = hipMalloc((void**)&S->region, want);
+        if (e != hipSuccess) {
+            S->region = nullptr;
+            (void)hipGetLastError();
+            return fail(ctx, MEMD_ERR_NOMEM, "plan region allocation of %zu bytes failed: %s", want,
+                        hipGetErrorString(e));
+        }
+        S->region_bytes = want;
+    }
+    int* ndone = (int*)(S->region + S->P.ndone_at);
+    for (EpochDesc& d : S->eds) d.ndone = ndone;
+    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->opts.mode, S->variant, S->records, S->tabs, s, &S->plan);
+    if (rs != MEMD_OK) return rs;
+    // counters are zero between steps; a new region starts from zero too
+    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, sizeof(int) * kNumLists * kXcds, s));
+    HIP_TRY(ctx, hipMemsetAsync(ndone, 0, sizeof(int), s));
+    S->fresh.clear();
+    for (int64_t id : joined) {
+        int e = index[id];
+        int nb = blocks_for(kMapInit, S->eds[e], S->variant);
+        for (int b = 0; b < nb; ++b) S->fresh.push_back(make_int2(e, b));
+    }
+    if (!S->fresh.empty()) {
+        int2* ft = (int2*)(S->region + S->P.fresh_at);
+        HIP_TRY(ctx, hipMemcpyAsync(ft, S->fresh.data(), sizeof(int2) * S->fresh.size(), hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, launch_init_table(S->plan, ft, (int)S->fresh.size(), s));
+    }
+    return MEMD_OK;
+}
+
+// Snapshot the states and retire every finished epoch.
+memd_status server_retire(memd_server* S, hipStream_t s) {
+    memd_ctx* ctx = S->ctx;
+    const int count = (int)S->active.size();
+    S->states.resize(count);
+    EpochState* dst = (EpochState*)(S->region + S->P.states_at);
+    HIP_TRY(ctx, launch_gather_states(S->plan, dst, s));
+    HIP_TRY(ctx, hipMemcpyAsync(S->states.data(), dst, sizeof(EpochState) * count, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    std::vector<int64_t> still;
+    for (int e = 0; e < count; ++e) {
+        int64_t id = S->active[e];
+        memd_server::Req& r = S->reqs[id];
+        if (!S->states[e].done) {
+            still.push_back(id);
+            continue;
+        }
+        if (r.in.h_sift_counts)
+            HIP_TRY(ctx, hipMemcpy(r.in.h_sift_counts, S->eds[e].sift_counts, sizeof(int32_t) * r.in.N_imf,
+                                   hipMemcpyDeviceToHost));
+        if (r.in.h_n_imf_out) *r.in.h_n_imf_out = S->states[e].n_imf_out;
+        r.state = MEMD_REQ_DONE;
+        pool_free(S, r.off, r.lay.total);
+        S->load[r.grp] -= r.weight;
+        S->finished.push_back(id);
+    }
+    S->active.swap(still);
+    return MEMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max_active, const memd_opts* opts,
+                               memd_server** out) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    if (!out) return fail(ctx, MEMD_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    memd_opts o;
+    if (opts) o = *opts; else memd_opts_default(&o);
+    if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP) return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
+    if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
+        return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
+    if (o.d_knot_counts || o.h_sift_counts || o.h_n_imf_out)
+        return fail(ctx, MEMD_ERR_ARG, "per-call diagnostics are not available on a server (use the request's outputs)");
+    if (hbm_pool_bytes < kAlign || max_active < 0) return fail(ctx, MEMD_ERR_ARG, "invalid pool size or active cap");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    memd_server* S = new memd_server();
+    S->ctx = ctx;
+    S->opts = o;
+    S->variant = eval_variant_choice();
+    S->records = (o.mode == MEMD_MODE_STOP || S->variant != kEvalFused);
+    S->max_active = max_active;
+    S->pool_bytes = hbm_pool_bytes / kAlign * kAlign;
+    hipError_t e = hipMalloc((void**)&S->pool, S->pool_bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        delete S;
+        return fail(ctx, MEMD_ERR_NOMEM, "server pool allocation of %zu bytes failed: %s", hbm_pool_bytes,
+                    hipGetErrorString(e));
+    }
+    S->free_blocks[0] = S->pool_bytes;
+    *out = S;
+    return MEMD_OK;
+}
+
+memd_status memd_server_destroy(memd_server* S) {
+    if (!S) return MEMD_OK;
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(work_stream(S->ctx));
+    if (S->pool) (void)hipFree(S->pool);
+    if (S->region) (void)hipFree(S->region);
+    delete S;
+    return MEMD_OK;
+}
+
+memd_status memd_server_submit(memd_server* S, const memd_epoch* ep, int64_t* id) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    memd_ctx* ctx = S->ctx;
+    if (!ep || !id) return fail(ctx, MEMD_ERR_ARG, "null epoch or id pointer");
+    memd_status rs = validate_shape(ctx, ep->n, ep->T, ep->K, ep->N_imf, ep->S);
+    if (rs != MEMD_OK) return rs;
+    if (!ep->d_y || !ep->d_V || !ep->d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
+    if (S->opts.mode == MEMD_MODE_STOP && ep->n > (size_t)kStopMaxChannels)
+        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %zu)", kStopMaxChannels, ep->n);
+    memd_server::Req r;
+    r.in = {(int)ep->n, (int)ep->T, (int)ep->K, (int)ep->N_imf, (int)ep->S, ep->d_y, ep->d_V, ep->d_IMFs,
+            ep->h_sift_counts, ep->h_n_imf_out};
+    r.lay = layout_for(r.in.n, r.in.T, r.in.K, r.in.N_imf, S->records);
+    if (r.lay.total > S->pool_bytes)
+        return fail(ctx, MEMD_ERR_ADMISSION, "epoch needs %zu workspace bytes, the pool holds %zu", r.lay.total,
+                    S->pool_bytes);
+    r.weight = (long)r.in.n * r.in.T * r.in.K;
+    *id = S->next_id++;
+    S->reqs[*id] = r;
+    S->queue.push_back(*id);
+    return MEMD_OK;
+}
+
+memd_status memd_server_step(memd_server* S, int32_t steps, int32_t* n_active, int32_t* n_queued) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    memd_ctx* ctx = S->ctx;
+    if (steps < 0) return fail(ctx, MEMD_ERR_ARG, "negative step count");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = work_stream(ctx);
+    std::vector<int64_t> joined;
+    const size_t before = S->active.size();
+    server_admit(S, joined);
+    // the plan is rebuilt when the set changed: epochs joined now, or retired in
+    // the previous call (the active list then shrank below the plan's count)
+    if (!joined.empty() || before != (size_t)S->plan.count) {
+        memd_status rs = server_rebuild(S, joined, s);
+        if (rs != MEMD_OK) return rs;
+    }
+    if (!S->active.empty()) {
+        for (int i = 0; i < steps; ++i) HIP_TRY(ctx, launch_step(S->plan, s));
+        memd_status rs = server_retire(S, s);
+        if (rs != MEMD_OK) return rs;
+    }
+    if (n_active) *n_active = (int32_t)S->active.size();
+    if (n_queued) *n_queued = (int32_t)S->queue.size();
+    return MEMD_OK;
+}
+
+memd_status memd_server_completed(memd_server* S, int64_t* ids, int32_t cap, int32_t* n) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    if (!n || (cap > 0 && !ids)) return fail(S->ctx, MEMD_ERR_ARG, "null output pointer");
+    int32_t k = 0;
+    while (k < cap && S->finished_head < S->finished.size()) ids[k++] = S->finished[S->finished_head++];
+    if (S->finished_head == S->finished.size()) {
+        S->finished.clear();
+        S->finished_head = 0;
+    }
+    *n = k;
+    return MEMD_OK;
+}
+
+memd_status memd_server_poll(memd_server* S, int64_t id, int32_t* state) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    if (!state) return fail(S->ctx, MEMD_ERR_ARG, "null state pointer");
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(S->ctx, MEMD_ERR_NOT_FOUND, "unknown request id %lld", (long long)id);
+    *state = it->second.state;
+    return MEMD_OK;
+}
+
+memd_status memd_server_release(memd_server* S, int64_t id) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(S->ctx, MEMD_ERR_NOT_FOUND, "unknown request id %lld", (long long)id);
+    if (it->second.state != MEMD_REQ_DONE)
+        return fail(S->ctx, MEMD_ERR_ARG, "request %lld is not finished", (long long)id);
+    S->reqs.erase(it);
+    return MEMD_OK;
+}
+
+size_t memd_server_pool_used(memd_server* S) { return S ? S->pool_used : 0; }
 
 }  // extern "C"

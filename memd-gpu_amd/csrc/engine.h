@@ -90,6 +90,7 @@ struct EpochDesc {
     unsigned long long* stats; // optional [4]: knots, knot*channels, eval points, samples
     int* ndone;                // shared by the batch: epochs finished (stop mode)
     int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
+    int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
 };
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its
@@ -142,6 +143,10 @@ int solver_block_capacity(int Ecap);
 // Per-device kernel attributes (large dynamic LDS); call once per context.
 hipError_t configure_kernels();
 hipError_t launch_init(const StepPlan& p, hipStream_t s);
+// k_init over an explicit block table (epochs joining a running set).
+hipError_t launch_init_table(const StepPlan& p, const int2* tab, int blocks, hipStream_t s);
+// Copy every epoch's EpochState into out[count].
+hipError_t launch_gather_states(const StepPlan& p, EpochState* out, hipStream_t s);
 hipError_t launch_step(const StepPlan& p, hipStream_t s);
 // Same launches with an event recorded after each kernel (ev[0] before the first).
 hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev /*[kNumStepKernels+1]*/);

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
     __syncthreads();
     if (!(sE[0] > 3 && sE[1] > 3)) return;
     __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
-    const int g = wl.groups > 1 ? br.x % kXcds : 0;
+    const int g = wl.groups > 1 ? ed.xcd : 0;
     if (tid < 2) {
         int m[kNumLists];
         list_sizes(ed, sE[tid], m);
@@ -1160,6 +1160,13 @@ __global__ void k_stop_reset(const EpochDesc* __restrict__ eps, int count, WorkL
 
 // ------------------------------------------------------------ misc kernels
 
+// Snapshot of every epoch's state into one array (one copy to the host).
+__global__ void k_gather_states(const EpochDesc* __restrict__ eps, int count, EpochState* __restrict__ out) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) out[e] = *eps[e].st;
+}
+
+
 __global__ void k_check_coords(const int32_t* __restrict__ x, int T, int* flag) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < T && x[t] != t) atomicOr(flag, 1);
@@ -1222,6 +1229,18 @@ long list_capacity(int list, const EpochDesc& e) {
 
 hipError_t launch_init(const StepPlan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_init, dim3(p.blocks[kMapInit]), dim3(256), 0, s, p.eps, p.tab[kMapInit], p.wl);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_table(const StepPlan& p, const int2* tab, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, s, p.eps, tab, p.wl);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_states(const StepPlan& p, EpochState* out, hipStream_t s) {
+    if (p.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_states, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count, out);
     return hipGetLastError();
 }
 

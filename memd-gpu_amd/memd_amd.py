@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libmemd_amd.so")
 MEMD_OK = 0
 STATUS_NAMES = {0: "MEMD_OK", 1: "MEMD_ERR_ARG", 2: "MEMD_ERR_HIP", 3: "MEMD_ERR_NOMEM",
                 4: "MEMD_ERR_COORDS", 5: "MEMD_ERR_UNSUPPORTED", 6: "MEMD_ERR_ADMISSION",
-                7: "MEMD_ERR_NOT_FOUND"}
+                7: "MEMD_ERR_NOT_FOUND", 8: "MEMD_ERR_IO"}
 MODE_FIX, MODE_STOP = 0, 1
 WINDOW_REFWARP32, WINDOW_FULL = 0, 1
 
@@ -68,7 +68,13 @@ class MemdError(RuntimeError):
 EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_last_error",
            "memd_opts_default", "memd_version", "memd_directions_f64", "memd_run_f64", "memd_run_f32",
            "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
-           "memd_workspace_bytes")
+           "memd_workspace_bytes", "memd_server_create", "memd_server_destroy", "memd_server_submit",
+           "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
+           "memd_server_pool_used", "memd_bin_length", "memd_read_bin_f32", "memd_write_bin_f32")
+#: exports that do not return a memd_status
+_NOT_STATUS = ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes",
+               "memd_server_pool_used")
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
 _lib = None
 
@@ -81,6 +87,14 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: build it with `make -C memd-gpu_amd` "
                           "(or __graft_entry__.build()); there is no fallback path")
+    # torch ships its own HIP runtime (file libamdhip64.so, soname libamdhip64.so.7)
+    # and loads it by file name.  Importing torch first makes the library's
+    # libamdhip64.so.7 dependency resolve to that same runtime; loading the
+    # library first would put a second HIP runtime into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     L.memd_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.memd_ctx_destroy.argtypes = [_vp]
@@ -102,8 +116,20 @@ def load_library(path=LIB_PATH):
                                      ctypes.POINTER(ctypes.c_double)]
     L.memd_profile_batch_f64.argtypes = [_vp, ctypes.POINTER(MemdEpoch), _sz, ctypes.POINTER(MemdOpts),
                                          ctypes.POINTER(MemdKernelProfile)]
+    L.memd_server_create.argtypes = [_vp, _sz, _i32, ctypes.POINTER(MemdOpts), ctypes.POINTER(_vp)]
+    L.memd_server_destroy.argtypes = [_vp]
+    L.memd_server_submit.argtypes = [_vp, ctypes.POINTER(MemdEpoch), ctypes.POINTER(ctypes.c_int64)]
+    L.memd_server_step.argtypes = [_vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]
+    L.memd_server_completed.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), _i32, ctypes.POINTER(_i32)]
+    L.memd_server_poll.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(_i32)]
+    L.memd_server_release.argtypes = [_vp, ctypes.c_int64]
+    L.memd_server_pool_used.argtypes = [_vp]
+    L.memd_server_pool_used.restype = _sz
+    L.memd_bin_length.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]
+    L.memd_read_bin_f32.argtypes = [ctypes.c_char_p, _sz, _sz, _sz, _vp]
+    L.memd_write_bin_f32.argtypes = [ctypes.c_char_p, _vp, _sz]
     for name in EXPORTS:
-        if name not in ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes"):
+        if name not in _NOT_STATUS:
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -138,7 +164,7 @@ class MemdContext:
         h = ctypes.c_void_p()
         rc = self.lib.memd_ctx_create(device, ctypes.byref(h))
         if rc != MEMD_OK:
-            raise MemdError(rc, "memd_ctx_create failed (no HIP device?)")
+            raise MemdError(rc, "memd_ctx_create failed: " + self.lib.memd_last_error(None).decode())
         self.h = h
         self.device = device
 
@@ -267,6 +293,114 @@ class MemdContext:
                 "steps": pr.steps, "knots": pr.knots, "knot_channels": pr.knot_channels,
                 "eval_points": pr.eval_points, "samples": pr.samples}
         return outs, info
+
+
+class MemdServer:
+    """Served continuous batching over one context (no reference counterpart).
+
+    submit() queues an epoch and returns its id; step() admits what fits the
+    HBM pool, advances the active set and retires finished epochs; completed()
+    returns the ids finished since the last call; result() gives the IMFs and
+    sift counts of a finished request."""
+
+    def __init__(self, ctx, pool_bytes, max_active=0, mode=MODE_FIX, window=WINDOW_REFWARP32,
+                 sd=0.075, sd2=0.75, tol=0.075):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        o = default_opts(mode, window, sd, sd2, tol)
+        h = ctypes.c_void_p()
+        ctx._check(self.lib.memd_server_create(ctx.h, int(pool_bytes), int(max_active), ctypes.byref(o),
+                                               ctypes.byref(h)))
+        self.h = h
+        self._req = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.memd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, y, V, n_imf, S, out=None):
+        import torch
+        _require_device(y, torch.float64, "d_y")
+        _require_device(V, torch.float64, "d_V")
+        n, T = y.shape
+        if out is None:
+            out = torch.empty((n_imf, n, T), dtype=torch.float64, device=y.device)
+        _require_device(out, torch.float64, "d_IMFs")
+        sc = (ctypes.c_int32 * n_imf)()
+        no = ctypes.c_int32(0)
+        ep = MemdEpoch(n, T, V.shape[0], n_imf, S, y.data_ptr(), V.data_ptr(), out.data_ptr(),
+                       ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32)), ctypes.pointer(no))
+        rid = ctypes.c_int64(0)
+        self.ctx._check(self.lib.memd_server_submit(self.h, ctypes.byref(ep), ctypes.byref(rid)))
+        # keep the buffers alive until the request is released
+        self._req[rid.value] = (y, V, out, sc, no)
+        return rid.value
+
+    def step(self, steps=1):
+        a, q = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.ctx._check(self.lib.memd_server_step(self.h, int(steps), ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def completed(self, cap=4096):
+        ids = (ctypes.c_int64 * cap)()
+        n = ctypes.c_int32(0)
+        self.ctx._check(self.lib.memd_server_completed(self.h, ids, cap, ctypes.byref(n)))
+        return list(ids[:n.value])
+
+    def poll(self, rid):
+        st = ctypes.c_int32(0)
+        self.ctx._check(self.lib.memd_server_poll(self.h, int(rid), ctypes.byref(st)))
+        return st.value
+
+    def result(self, rid):
+        if self.poll(rid) != REQ_DONE:
+            raise MemdError(1, f"request {rid} is not finished")
+        _, _, out, sc, no = self._req[rid]
+        return out, {"sift_counts": np.array(sc[:], np.int32), "n_imf_out": no.value}
+
+    def release(self, rid):
+        self.ctx._check(self.lib.memd_server_release(self.h, int(rid)))
+        self._req.pop(rid, None)
+
+    def pool_used(self):
+        return int(self.lib.memd_server_pool_used(self.h))
+
+
+# ---- raw sample files (memReducedEpochs_dev.cu:896-932, used at :1484-1501) ----
+
+def _io_check(rc):
+    if rc != MEMD_OK:
+        raise MemdError(rc, load_library().memd_last_error(None).decode())
+
+
+def bin_length(path, n):
+    """Samples per channel of a float32 [n][T] file: bytes / 4 / n, floored (:1486)."""
+    T = _sz(0)
+    _io_check(load_library().memd_bin_length(os.fsencode(path), int(n), ctypes.byref(T)))
+    return T.value
+
+
+def read_bin(path, n, T=None):
+    """float32 array [n][T]: the first T samples of every channel (default: all)."""
+    T_file = bin_length(path, n)
+    T = T_file if T is None else int(T)
+    out = np.empty((n, T), np.float32)
+    _io_check(load_library().memd_read_bin_f32(os.fsencode(path), int(n), T_file, T,
+                                              out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def write_bin(path, arr):
+    """Write a float32 array as raw bytes in C order (writeBin)."""
+    a = np.ascontiguousarray(arr, dtype=np.float32)
+    _io_check(load_library().memd_write_bin_f32(os.fsencode(path), a.ctypes.data_as(ctypes.c_void_p), a.size))
 
 
 # ---- module-level functions with the reference's names -----------------------

@@ -1,0 +1,185 @@
+// memd_cli -- command-line driver over the C ABI, with the behaviour of the
+// reference's host programs:
+//
+//   default   main() (memReducedEpochs_dev.cu:1466-1529): read a float32
+//             [n][T] file (T = bytes / 4 / n), decompose it with n = 32, K = 64,
+//             9 output slots, 10 sifts, write mode_matrix.bin [N_imf][n][T]
+//             float32.
+//   --bench   runBenchmark() (:1370-1464): for each length L of the list
+//             2049 + 2048 i (i = 0..117) that the file holds, decompose the
+//             first L samples of every channel (8 output slots unless --imfs)
+//             and print "Channel: n, Length: L, DirVec: K, Time: seconds".
+//
+// The engine computes in FP64: the float32 samples are widened on the host,
+// directions are the FP64 set of GenerateDirectionVector (:1029-1072), and the
+// IMFs are rounded to float32 for the output file.  Failures print the
+// library's message and exit with the status code.
+#include "../include/memd_amd.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Args {
+    std::string input = "./EEGsample_32CH_30504.bin";
+    std::string output = "./mode_matrix.bin";
+    size_t n = 32, K = 64, imfs = 0, sifts = 10, length = 0;
+    int mode = MEMD_MODE_FIX, window = MEMD_WINDOW_REFWARP32;
+    bool bench = false;
+    std::vector<size_t> lengths;
+};
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: memd_cli [--input FILE] [--output FILE] [--channels n] [--directions K]\n"
+                 "                [--imfs N] [--sifts S] [--length T] [--mode fix|stop]\n"
+                 "                [--window refwarp32|full] [--bench [--lengths L1,L2,...]]\n");
+}
+
+bool parse(int argc, char** argv, Args& a) {
+    for (int i = 1; i < argc; ++i) {
+        std::string k = argv[i];
+        auto val = [&](const char** out) {
+            if (i + 1 >= argc) return false;
+            *out = argv[++i];
+            return true;
+        };
+        const char* v = nullptr;
+        if (k == "--bench") { a.bench = true; continue; }
+        if (!val(&v)) return false;
+        if (k == "--input") a.input = v;
+        else if (k == "--output") a.output = v;
+        else if (k == "--channels") a.n = std::strtoull(v, nullptr, 10);
+        else if (k == "--directions") a.K = std::strtoull(v, nullptr, 10);
+        else if (k == "--imfs") a.imfs = std::strtoull(v, nullptr, 10);
+        else if (k == "--sifts") a.sifts = std::strtoull(v, nullptr, 10);
+        else if (k == "--length") a.length = std::strtoull(v, nullptr, 10);
+        else if (k == "--mode") {
+            if (!std::strcmp(v, "fix")) a.mode = MEMD_MODE_FIX;
+            else if (!std::strcmp(v, "stop")) a.mode = MEMD_MODE_STOP;
+            else return false;
+        } else if (k == "--window") {
+            if (!std::strcmp(v, "refwarp32")) a.window = MEMD_WINDOW_REFWARP32;
+            else if (!std::strcmp(v, "full")) a.window = MEMD_WINDOW_FULL;
+            else return false;
+        } else if (k == "--lengths") {
+            std::string s = v;
+            size_t pos = 0;
+            while (pos < s.size()) {
+                size_t c = s.find(',', pos);
+                if (c == std::string::npos) c = s.size();
+                a.lengths.push_back(std::strtoull(s.substr(pos, c - pos).c_str(), nullptr, 10));
+                pos = c + 1;
+            }
+        } else {
+            return false;
+        }
+    }
+    return true;
+}
+
+int die(memd_ctx* ctx, int rc, const char* what) {
+    std::fprintf(stderr, "memd_cli: %s failed (status %d): %s\n", what, rc, memd_last_error(ctx));
+    return rc;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t count) { return hipMalloc((void**)&p, count * sizeof(T)) == hipSuccess; }
+};
+
+// One decomposition of h_y [n][T]; writes the float32 IMF stack to `output`.
+int run_one(memd_ctx* ctx, const Args& a, size_t T, size_t N_imf, const std::vector<float>& h_y, double* secs) {
+    const size_t n = a.n, K = a.K;
+    std::vector<double> y64(h_y.begin(), h_y.end());
+    std::vector<int32_t> h_x(T);
+    for (size_t i = 0; i < T; ++i) h_x[i] = (int32_t)i;   // sample index 0..T-1 (:1504-1506)
+    DevBuf<double> d_y, d_V, d_IMFs;
+    DevBuf<int32_t> d_x;
+    if (!d_y.alloc(n * T) || !d_V.alloc(K * n) || !d_IMFs.alloc(N_imf * n * T) || !d_x.alloc(T)) {
+        std::fprintf(stderr, "memd_cli: device allocation failed\n");
+        return MEMD_ERR_NOMEM;
+    }
+    if (hipMemcpy(d_y.p, y64.data(), n * T * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_x.p, h_x.data(), T * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        std::fprintf(stderr, "memd_cli: upload failed\n");
+        return MEMD_ERR_HIP;
+    }
+    int rc = memd_directions_f64(ctx, n, K, nullptr, d_V.p);
+    if (rc != MEMD_OK) return die(ctx, rc, "memd_directions_f64");
+    memd_opts o;
+    memd_opts_default(&o);
+    o.mode = a.mode;
+    o.window = a.window;
+    rc = memd_run_f64(ctx, n, T, K, N_imf, a.sifts, d_x.p, d_y.p, d_V.p, d_IMFs.p, &o, secs);
+    if (rc != MEMD_OK) return die(ctx, rc, "memd_run_f64");
+    std::vector<double> imfs(N_imf * n * T);
+    if (hipMemcpy(imfs.data(), d_IMFs.p, imfs.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        std::fprintf(stderr, "memd_cli: download failed\n");
+        return MEMD_ERR_HIP;
+    }
+    std::vector<float> out(imfs.begin(), imfs.end());
+    rc = memd_write_bin_f32(a.output.c_str(), out.data(), out.size());
+    if (rc != MEMD_OK) return die(nullptr, rc, "memd_write_bin_f32");
+    return MEMD_OK;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args a;
+    if (!parse(argc, argv, a) || a.n == 0 || a.K == 0 || a.sifts == 0) {
+        usage();
+        return MEMD_ERR_ARG;
+    }
+    size_t T_file = 0;
+    int rc = memd_bin_length(a.input.c_str(), a.n, &T_file);
+    if (rc != MEMD_OK) return die(nullptr, rc, "memd_bin_length");
+    memd_ctx* ctx = nullptr;
+    rc = memd_ctx_create(0, &ctx);
+    if (rc != MEMD_OK) return die(nullptr, rc, "memd_ctx_create");
+
+    std::vector<size_t> lengths;
+    size_t N_imf = a.imfs;
+    if (a.bench) {
+        if (!N_imf) N_imf = 8;   // runBenchmark's num_IMFs (:1372)
+        if (!a.lengths.empty()) lengths = a.lengths;
+        else
+            for (size_t i = 0; i < 118; ++i) lengths.push_back(2049 + 2048 * i);   // :1383-1395
+    } else {
+        if (!N_imf) N_imf = 9;   // main()'s num_IMFs (:1475)
+        lengths.push_back(a.length ? a.length : T_file);
+    }
+    for (size_t L : lengths) {
+        if (L > T_file) {
+            if (a.bench) continue;   // the list runs past the file: those lengths are skipped
+            std::fprintf(stderr, "memd_cli: --length %zu exceeds the file's %zu samples per channel\n", L, T_file);
+            memd_ctx_destroy(ctx);
+            return MEMD_ERR_ARG;
+        }
+        std::vector<float> h_y(a.n * L);
+        rc = memd_read_bin_f32(a.input.c_str(), a.n, T_file, L, h_y.data());
+        if (rc != MEMD_OK) {
+            memd_ctx_destroy(ctx);
+            return die(nullptr, rc, "memd_read_bin_f32");
+        }
+        double secs = 0.0;
+        rc = run_one(ctx, a, L, N_imf, h_y, &secs);
+        if (rc != MEMD_OK) {
+            memd_ctx_destroy(ctx);
+            return rc;
+        }
+        std::printf("Channel: %zu, Length: %zu, DirVec: %zu, Time: %f \n", a.n, L, a.K, secs);
+        std::fflush(stdout);
+    }
+    memd_ctx_destroy(ctx);
+    return 0;
+}

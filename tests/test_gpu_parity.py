@@ -208,3 +208,20 @@ def test_batch_bit_identical_to_alone(ctx, oracle):
     y0 = eps[0]["y"].cpu().numpy()
     imfs_o, _ = oracle.memd(y0, eps[0]["V"].cpu().numpy(), 4, 3)
     assert np.array_equal(outs[0].cpu().numpy(), imfs_o)
+
+
+def test_batch_many_epochs_xcd_groups(ctx, oracle):
+    # more than 8 epochs: blocks and work items are pinned to XCD groups
+    import synth
+    rng = np.random.default_rng(11)
+    eps = []
+    for i in range(11):
+        n = int(rng.choice([2, 4, 8]))
+        T = int(rng.integers(600, 1800))
+        y = synth.sines_epoch(n, T, 400 + i)
+        eps.append({"y": _dev(y), "V": _dev(oracle.directions(n, 2 * n)), "n_imf": int(rng.integers(3, 6)),
+                    "S": int(rng.integers(2, 4))})
+    outs, info = ctx.run_batch(eps)
+    for ep, out in zip(eps, outs):
+        alone, _ = ctx.run(ep["y"], ep["V"], ep["n_imf"], ep["S"])
+        assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
