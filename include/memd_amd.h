@@ -38,6 +38,15 @@ enum {
     MEMD_MODE_FIX = 0,  /* exactly S sifts per IMF (memReducedEpochs_dev.cu:1176-1179) */
     MEMD_MODE_STOP = 1  /* memd.m stop criterion [sd, sd2, tol], at most S sifts per IMF */
 };
+/* Spline end conditions. */
+enum {
+    MEMD_SPLINE_NATURAL = 0,   /* the reference's (memReducedEpochs_dev.cu:704-736) */
+    MEMD_SPLINE_NOT_A_KNOT = 1 /* third-derivative continuity at the second and second-to-last
+                                  knots: the reference's unused tridiagonal_setup_nak (:738-773)
+                                  restated with the end unknowns eliminated, so the system is
+                                  diagonally dominant (the reference's form has a zero pivot
+                                  whenever the first two knot gaps are equal) */
+};
 /* Extrema window. */
 enum {
     MEMD_WINDOW_REFWARP32 = 0, /* the reference's coverage: interior t <= 240*ceil(T/242)
@@ -55,6 +64,7 @@ typedef struct memd_opts {
     int32_t max_passes;
     int32_t* h_sift_counts;  /* host [N_imf]: sifts per IMF */
     int32_t* h_n_imf_out;    /* host: output slots used (IMFs + residual) */
+    int32_t spline;          /* MEMD_SPLINE_* (default NATURAL) */
 } memd_opts;
 
 typedef struct memd_ctx memd_ctx;
@@ -82,6 +92,10 @@ const char* memd_version(void);
  */
 memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes,
                                 double* d_V);
+/* The same FP64 set rounded to float, for the GenerateDirectionVector<float>
+ * instantiation the reference's main() uses (memReducedEpochs_dev.cu:1513-1515). */
+memd_status memd_directions_f32(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes,
+                                float* d_V);
 
 /*
  * One epoch, all pointers on the device.

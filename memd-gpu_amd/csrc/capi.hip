@@ -73,7 +73,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, mean, tstat, red, sift, st, total;
+    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, mend, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -94,6 +94,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.ysol = take((size_t)K * 2 * n * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
     L.sep = take((size_t)K * 2 * n * nb * 8);
+    L.mend = take((size_t)K * 2 * n * 2 * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
     L.red = take(64);
@@ -161,6 +162,7 @@ EpochDesc desc_geometry(const EpochIn& I, const memd_opts& o, bool records) {
     d.sd = o.sd; d.sd2 = o.sd2; d.tol = o.tol;
     d.y = I.y; d.V = I.V; d.imfs = I.imfs;
     d.coef_records = records ? 1 : 0;
+    d.spline = o.spline;
     return d;
 }
 
@@ -177,6 +179,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.ysol = (double*)(b + L.ysol);
     d.coef = (double*)(b + L.coef);
     d.sep = (double*)(b + L.sep);
+    d.mend = (double*)(b + L.mend);
     d.mean = (double*)(b + L.mean);
     d.tstat = (double*)(b + L.tstat);
     d.stopred = (int*)(b + L.red);
@@ -318,6 +321,8 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
     if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
         return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
+    if (o.spline != MEMD_SPLINE_NATURAL && o.spline != MEMD_SPLINE_NOT_A_KNOT)
+        return fail(ctx, MEMD_ERR_ARG, "unknown spline end condition %d", o.spline);
     const int count = (int)in.size();
     if (count == 0) return MEMD_OK;
     if (o.d_knot_counts && count != 1)
@@ -668,6 +673,23 @@ memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t
     return MEMD_OK;
 }
 
+memd_status memd_directions_f32(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes, float* d_V) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    if (n < 1 || K < 1 || !d_V) return fail(ctx, MEMD_ERR_ARG, "invalid direction arguments n=%zu K=%zu", n, K);
+    if (n > (size_t)kMaxChannels || K > 65536) return fail(ctx, MEMD_ERR_UNSUPPORTED, "n or K too large");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> P(n);
+    if (h_primes) std::memcpy(P.data(), h_primes, n * sizeof(int32_t));
+    else reference_primes((int)n, (int)K, P.data());
+    if (!primes_valid((int)n, P.data())) return fail(ctx, MEMD_ERR_ARG, "prime table needs P[0] != 0 and P[j>0] >= 2");
+    const double* dV = nullptr;
+    memd_status rs = cached_directions(ctx, (int)n, (int)K, P.data(), &dV);
+    if (rs != MEMD_OK) return rs;
+    HIP_TRY(ctx, launch_f64_to_f32(dV, d_V, n * K, work_stream(ctx)));
+    HIP_TRY(ctx, hipStreamSynchronize(work_stream(ctx)));
+    return MEMD_OK;
+}
+
 memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S, const int32_t* d_x,
                          const double* d_y, const double* d_V, double* d_IMFs, const memd_opts* opts, double* elapsed_s) {
     if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
@@ -989,6 +1011,8 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP) return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
     if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
         return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
+    if (o.spline != MEMD_SPLINE_NATURAL && o.spline != MEMD_SPLINE_NOT_A_KNOT)
+        return fail(ctx, MEMD_ERR_ARG, "unknown spline end condition %d", o.spline);
     if (o.d_knot_counts || o.h_sift_counts || o.h_n_imf_out)
         return fail(ctx, MEMD_ERR_ARG, "per-call diagnostics are not available on a server (use the request's outputs)");
     if (hbm_pool_bytes < kAlign || max_active < 0) return fail(ctx, MEMD_ERR_ARG, "invalid pool size or active cap");

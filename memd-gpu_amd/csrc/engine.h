@@ -91,6 +91,8 @@ struct EpochDesc {
     int* ndone;                // shared by the batch: epochs finished (stop mode)
     int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
+    int spline;                // 0 natural ends, 1 not-a-knot
+    double* mend;              // [K][2][n][2] not-a-knot: M_0 and M_{E-1} per channel
 };
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its

@@ -100,6 +100,30 @@ __device__ __forceinline__ BlockGeom block_geom(int E, int b) {
     return g;
 }
 
+// Row u (1..N, N = E-2) of a knot system.  Natural ends: the reference's rows
+// (memReducedEpochs_dev.cu:704-736).  Not-a-knot: rows 1 and N with M_0 and
+// M_{E-1} eliminated (integer products, exact in FP64).
+__device__ __forceinline__ void sys_row(const int* X, int u, int E, int spline, double& lo, double& di,
+                                        double& up) {
+    lo = (double)(X[u] - X[u - 1]);
+    up = (double)(X[u + 1] - X[u]);
+    di = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
+    if (spline && E >= 4) {
+        const int N = E - 2;
+        if (u == 1) {
+            long long h0 = X[1] - X[0], h1 = X[2] - X[1];
+            lo = 0.0;
+            di = (double)((h0 + h1) * (h0 + 2 * h1));
+            up = (double)(h1 * h1 - h0 * h0);
+        } else if (u == N) {
+            long long a = X[N] - X[N - 1], b = X[N + 1] - X[N];
+            lo = (double)(a * a - b * b);
+            di = (double)((a + b) * (2 * a + b));
+            up = 0.0;
+        }
+    }
+}
+
 __device__ __forceinline__ double* fac_row(const EpochDesc& ed, int ke, int field) {
     return ed.fac + ((size_t)ke * 6 + field) * ed.Ecap;
 }
@@ -274,7 +298,7 @@ __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     m[kListFactor] = 1;
     m[kListKnots] = (E + kKnotChunk - 1) / kKnotChunk;
     m[kListBlock] = (J + 1 + bpw - 1) / bpw;
-    m[kListReduced] = J > 0 ? 1 : 0;
+    m[kListReduced] = (J > 0 || ed.spline) ? 1 : 0;   // not-a-knot: end values
     m[kListCoef] = (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems));
 }
 
@@ -389,8 +413,8 @@ __global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ ep
             }
             double cprev = 0.0;
             for (int u = p; u <= q; ++u) {
-                double lo = (double)(X[u] - X[u - 1]), up = (double)(X[u + 1] - X[u]);
-                double di = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
+                double lo, di, up;
+                sys_row(X, u, E, ed.spline, lo, di, up);
                 double den = (u == p) ? di : fma(-lo, cprev, di);
                 double r = 1.0 / den;
                 cprev = up * r;
@@ -401,7 +425,8 @@ __global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ ep
             if (b >= 1) {   // left spike A_b^{-1}(lo_p e_first): forward sweep into al, back substitution
                 double z = 0.0;
                 for (int u = p; u <= q; ++u) {
-                    double lo = (double)(X[u] - X[u - 1]);
+                    double lo, di, up;
+                    sys_row(X, u, E, ed.spline, lo, di, up);
                     double gv = (u == p) ? lo : 0.0;
                     z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
                     al[u] = z;
@@ -415,8 +440,9 @@ __global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ ep
             if (b < J) {    // right spike A_b^{-1}(up_q e_last)
                 double z = 0.0;
                 for (int u = p; u <= q; ++u) {
-                    double lo = (double)(X[u] - X[u - 1]);
-                    double gv = (u == q) ? (double)(X[u + 1] - X[u]) : 0.0;
+                    double lo, di, up;
+                    sys_row(X, u, E, ed.spline, lo, di, up);
+                    double gv = (u == q) ? up : 0.0;
                     z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
                     be[u] = z;
                 }
@@ -487,6 +513,12 @@ __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restric
             if (i < E) {
                 double ihi = (i <= E - 2) ? ihv[i] : 0.0;
                 double ihm = (i >= 1) ? ihv[i - 1] : 0.0;
+                // not-a-knot: rows 1 and N = E-2 are scaled by h_1 and h_{N-1}
+                double rscale = 0.0;
+                if (ed.spline && E >= 4) {
+                    if (i == 1) rscale = (double)(X[2] - X[1]);
+                    else if (i == E - 2) rscale = (double)(X[E - 2] - X[E - 3]);
+                }
 #pragma unroll
                 for (int cc = 0; cc < CH; ++cc) {
                     int c = c0 + cc;
@@ -496,7 +528,9 @@ __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restric
                     if (i >= 1 && i <= E - 2) {
                         double sl = (Ys[cc][l + 2] - Ys[cc][l + 1]) * ihi;
                         double slm = (Ys[cc][l + 1] - Ys[cc][l]) * ihm;
-                        ed.ysol[at] = 3.0 * (sl - slm);
+                        double r = 3.0 * (sl - slm);
+                        if (rscale != 0.0) r = r * rscale;
+                        ed.ysol[at] = r;
                     }
                 }
             }
@@ -535,7 +569,9 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
                 int u = min(g.p + i, g.q);
                 s_rd[sb][i] = rdv[u];
                 s_cp[sb][i] = cpv[u];
-                s_lo[sb][i] = (double)(X[u] - X[u - 1]);
+                double lo, di, up;
+                sys_row(X, u, E, ed.spline, lo, di, up);
+                s_lo[sb][i] = lo;
             }
             __syncthreads();
             int slot = threadIdx.x / n;
@@ -564,6 +600,22 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         }
         __syncthreads();
     }
+}
+
+// Second derivative M_u of knot row u of one channel after phase 2: phase-3
+// combination inside blocks, the separator value at separator rows, and the end
+// values (0 for natural ends, me[0] / me[1] for not-a-knot).
+__device__ __forceinline__ double knot_M(const double* row, const double* sv, const double* al,
+                                         const double* be, int u, int E, int J, const double* me) {
+    constexpr int L = kSolverBlock;
+    if (u == 0) return me ? me[0] : 0.0;
+    if (u == E - 1) return me ? me[1] : 0.0;
+    if (u % L == 0 && u / L <= J) return sv[u / L];
+    int b = (u - 1) / L;
+    double m = row[u];
+    if (b >= 1) m = fma(-sv[b], al[u], m);
+    if (b < J) m = fma(-sv[b + 1], be[u], m);
+    return m;
 }
 
 // Phase 2: separator values, one lane per channel (no division: the reduced
@@ -598,28 +650,32 @@ __global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ e
                 x = fma(-Rcp[j], x, sv[j]);
                 sv[j] = x;
             }
+            if (ed.spline) {
+                // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2), read
+                // before k_coef overwrites the rows with M
+                const double* al = fac_row(ed, ke, kFacAl);
+                const double* be = fac_row(ed, ke, kFacBe);
+                const double* ihv = fac_row(ed, ke, kFacIh);
+                double M1 = knot_M(row, sv, al, be, 1, E, J, nullptr);
+                double M2 = knot_M(row, sv, al, be, 2, E, J, nullptr);
+                double MN1 = knot_M(row, sv, al, be, E - 3, E, J, nullptr);
+                double MN = knot_M(row, sv, al, be, E - 2, E, J, nullptr);
+                double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
+                double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
+                double* me = ed.mend + ((size_t)ke * n + c) * 2;
+                me[0] = ((h0 + h1) * M1 - h0 * M2) * ihv[1];
+                me[1] = ((a + b) * MN - b * MN1) * ihv[E - 3];
+            }
         }
     }
 }
 
 // Phase 3 + coefficients: for every interval j of every channel, M_j and
-// M_{j+1} (x_u = fma(-X_{b+1}, be_u, fma(-X_b, al_u, y_u)) inside blocks, the
-// separator values at separator rows, 0 at the two end knots) and the spline
+// M_{j+1} (knot_M: phase 3 inside blocks, separator values, end values) and the spline
 // record (a, b, c, d) of memReducedEpochs_dev.cu:775-793 with the shared
 // reciprocals.  Lanes run over consecutive intervals of one channel, so the
 // record rows are written coalesced; each item is 256*kCoefItems (channel,
 // interval) pairs, four per thread at a time with their loads issued together.
-__device__ __forceinline__ double knot_M(const double* row, const double* sv, const double* al,
-                                         const double* be, int u, int E, int J) {
-    constexpr int L = kSolverBlock;
-    if (u == 0 || u == E - 1) return 0.0;
-    if (u % L == 0 && u / L <= J) return sv[u / L];
-    int b = (u - 1) / L;
-    double m = row[u];
-    if (b >= 1) m = fma(-sv[b], al[u], m);
-    if (b < J) m = fma(-sv[b + 1], be[u], m);
-    return m;
-}
 
 __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
@@ -650,7 +706,8 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                     us[a] = id2 - cs[a] * E;
                     const double* row = ed.ysol + ((size_t)ke * n + cs[a]) * ed.Ecap;
                     const double* sv = ed.sep + ((size_t)ke * n + cs[a]) * ed.nbmax;
-                    m[a] = knot_M(row, sv, al, be, us[a], E, J);
+                    const double* me = ed.spline ? ed.mend + ((size_t)ke * n + cs[a]) * 2 : nullptr;
+                    m[a] = knot_M(row, sv, al, be, us[a], E, J, me);
                 }
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
@@ -680,10 +737,11 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
                 const double* yk = ed.yknot + ((size_t)ke * n + c) * ed.Ecap;
                 const double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
+                const double* me = ed.spline ? ed.mend + ((size_t)ke * n + c) * 2 : nullptr;
                 y0[a] = yk[j];
                 y1[a] = yk[j + 1];
-                m0[a] = knot_M(row, sv, al, be, j, E, J);
-                m1[a] = knot_M(row, sv, al, be, j + 1, E, J);
+                m0[a] = knot_M(row, sv, al, be, j, E, J, me);
+                m1[a] = knot_M(row, sv, al, be, j + 1, E, J, me);
                 h[a] = (double)(X[j + 1] - X[j]);
                 ih[a] = ihv[j];
                 i3h[a] = i3hv[j];

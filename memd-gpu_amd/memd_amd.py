@@ -28,6 +28,7 @@ STATUS_NAMES = {0: "MEMD_OK", 1: "MEMD_ERR_ARG", 2: "MEMD_ERR_HIP", 3: "MEMD_ERR
                 7: "MEMD_ERR_NOT_FOUND", 8: "MEMD_ERR_IO"}
 MODE_FIX, MODE_STOP = 0, 1
 WINDOW_REFWARP32, WINDOW_FULL = 0, 1
+SPLINE_NATURAL, SPLINE_NOT_A_KNOT = 0, 1
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -38,7 +39,7 @@ class MemdOpts(ctypes.Structure):
     _fields_ = [("mode", _i32), ("window", _i32), ("sd", ctypes.c_double), ("sd2", ctypes.c_double),
                 ("tol", ctypes.c_double), ("use_graph", _i32), ("d_knot_counts", _vp),
                 ("max_passes", _i32), ("h_sift_counts", ctypes.POINTER(_i32)),
-                ("h_n_imf_out", ctypes.POINTER(_i32))]
+                ("h_n_imf_out", ctypes.POINTER(_i32)), ("spline", _i32)]
 
 
 class MemdEpoch(ctypes.Structure):
@@ -66,7 +67,8 @@ class MemdError(RuntimeError):
 
 #: every symbol include/memd_amd.h declares (checked by the CPU test-suite)
 EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_last_error",
-           "memd_opts_default", "memd_version", "memd_directions_f64", "memd_run_f64", "memd_run_f32",
+           "memd_opts_default", "memd_version", "memd_directions_f64", "memd_directions_f32", "memd_run_f64",
+           "memd_run_f32",
            "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
            "memd_workspace_bytes", "memd_server_create", "memd_server_destroy", "memd_server_submit",
            "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
@@ -107,6 +109,7 @@ def load_library(path=LIB_PATH):
     L.memd_workspace_bytes.argtypes = [_sz, _sz, _sz]
     L.memd_workspace_bytes.restype = _sz
     L.memd_directions_f64.argtypes = [_vp, _sz, _sz, ctypes.POINTER(_i32), _vp]
+    L.memd_directions_f32.argtypes = L.memd_directions_f64.argtypes
     L.memd_run_f64.argtypes = [_vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, _vp, _vp,
                                ctypes.POINTER(MemdOpts), ctypes.POINTER(ctypes.c_double)]
     L.memd_run_f32.argtypes = L.memd_run_f64.argtypes
@@ -135,10 +138,12 @@ def load_library(path=LIB_PATH):
     return L
 
 
-def default_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, use_graph=True):
+def default_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, use_graph=True,
+                 spline=SPLINE_NATURAL):
     o = MemdOpts()
     load_library().memd_opts_default(ctypes.byref(o))
     o.mode, o.window, o.sd, o.sd2, o.tol, o.use_graph = mode, window, sd, sd2, tol, int(bool(use_graph))
+    o.spline = spline
     return o
 
 
@@ -192,17 +197,19 @@ class MemdContext:
         import torch
         if out is None:
             out = torch.empty((K, n), dtype=torch.float64, device=f"cuda:{self.device}")
-        _require_device(out, torch.float64, "d_DirectionVector")
+        f32 = out.dtype == torch.float32
+        _require_device(out, torch.float32 if f32 else torch.float64, "d_DirectionVector")
         p = None
         if primes is not None:
             arr = (ctypes.c_int32 * n)(*[int(v) for v in primes])
             p = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int32))
-        self._check(self.lib.memd_directions_f64(self.h, n, K, p, _ptr(out)))
+        fn = self.lib.memd_directions_f32 if f32 else self.lib.memd_directions_f64
+        self._check(fn(self.h, n, K, p, _ptr(out)))
         return out
 
     # ---- memd (memReducedEpochs_dev.cu:1074-1368)
     def run(self, y, V, n_imf, S, x=None, out=None, mode=MODE_FIX, window=WINDOW_REFWARP32,
-            sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0):
+            sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0, spline=SPLINE_NATURAL):
         """Decompose one epoch y [n][T] (float64 or float32 device tensor).
 
         Returns (imfs [n_imf][n][T], info) with info = elapsed_s, sift_counts,
@@ -219,7 +226,7 @@ class MemdContext:
         _require_device(out, dt, "d_IMFs")
         if x is not None:
             _require_device(x, torch.int32, "d_x")
-        o = default_opts(mode, window, sd, sd2, tol, use_graph)
+        o = default_opts(mode, window, sd, sd2, tol, use_graph, spline)
         sc = (ctypes.c_int32 * n_imf)()
         nout = ctypes.c_int32(0)
         o.h_sift_counts = ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32))
@@ -254,7 +261,8 @@ class MemdContext:
         return outputData
 
     # ---- batched epochs (no reference counterpart)
-    def run_batch(self, epochs, mode=MODE_FIX, window=WINDOW_REFWARP32, use_graph=True, profile=False):
+    def run_batch(self, epochs, mode=MODE_FIX, window=WINDOW_REFWARP32, use_graph=True, profile=False,
+                  spline=SPLINE_NATURAL):
         """epochs: list of dicts with y, V, n_imf, S (and optional out).
         Returns (list of imfs tensors, info); with profile=True info["profile"]
         holds per-kernel average launch times (ms) and the work counters."""
@@ -277,7 +285,7 @@ class MemdContext:
             outs.append(out)
             scs.append(sc)
             nouts.append(no)
-        o = default_opts(mode, window, use_graph=use_graph)
+        o = default_opts(mode, window, use_graph=use_graph, spline=spline)
         el = ctypes.c_double(0.0)
         if profile:
             pr = MemdKernelProfile()
@@ -304,10 +312,10 @@ class MemdServer:
     sift counts of a finished request."""
 
     def __init__(self, ctx, pool_bytes, max_active=0, mode=MODE_FIX, window=WINDOW_REFWARP32,
-                 sd=0.075, sd2=0.75, tol=0.075):
+                 sd=0.075, sd2=0.75, tol=0.075, spline=SPLINE_NATURAL):
         self.ctx = ctx
         self.lib = ctx.lib
-        o = default_opts(mode, window, sd, sd2, tol)
+        o = default_opts(mode, window, sd, sd2, tol, spline=spline)
         h = ctypes.c_void_p()
         ctx._check(self.lib.memd_server_create(ctx.h, int(pool_bytes), int(max_active), ctypes.byref(o),
                                                ctypes.byref(h)))

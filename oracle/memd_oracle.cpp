@@ -34,6 +34,15 @@
  *     (the reference divides; the results differ by at most an ulp per operation).
  *   - mean over directions: sum = sum + (e_max + e_min) * 0.5 in ascending k over
  *     valid directions, then cur - sum / nvalid (0/0 -> NaN as the reference).
+ *   - not-a-knot ends (option; the reference's tridiagonal_setup_nak, :738-773, is
+ *     dead code): its row-0 form has diagonal h0^2 - h1^2, zero whenever the first
+ *     two gaps are equal, so a pivot-free solve of it divides by zero.  Here M_0
+ *     and M_{E-1} are eliminated instead (de Boor's form): row 1 becomes
+ *     [(h0+h1)(h0+2h1), h1^2-h0^2 | h1 r_1], row N = E-2 becomes
+ *     [a^2-b^2, (a+b)(2a+b) | a r_N] with a = h_{N-1}, b = h_N (strictly
+ *     diagonally dominant), and after the solve M_0 = ((h0+h1) M_1 - h0 M_2) * ih_1,
+ *     M_{E-1} = ((a+b) M_{N} - b M_{N-1}) * ih_{N-1}.  The integer-valued row
+ *     coefficients are exact in FP64.
  *
  * Stop mode (not in the reference, which only has fix mode) follows the public
  * memd.m criterion: sx(t) = |mean(t)|_2 / amp(t), stop when
@@ -188,10 +197,12 @@ constexpr double kThird = 1.0 / 3.0;
 
 struct KnotSystem {
     int E = 0;
+    bool nak = false;
     std::vector<double> h, ih, i3h, lo, di, up;
     Blocked solver;
-    void build(int E_, const int32_t* X) {
+    void build(int E_, const int32_t* X, int spline = ORACLE_SPLINE_NATURAL) {
         E = E_;
+        nak = (spline == ORACLE_SPLINE_NOT_A_KNOT) && E >= 4;
         h.assign(std::max(E - 1, 1), 0.0);
         ih.assign(std::max(E - 1, 1), 0.0);
         i3h.assign(std::max(E - 1, 1), 0.0);
@@ -207,6 +218,16 @@ struct KnotSystem {
             di[u] = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
             up[u] = h[u];
         }
+        if (nak) {
+            long long h0 = X[1] - X[0], h1 = X[2] - X[1];
+            lo[1] = 0.0;
+            di[1] = (double)((h0 + h1) * (h0 + 2 * h1));
+            up[1] = (double)(h1 * h1 - h0 * h0);
+            long long a = X[N] - X[N - 1], b = X[N + 1] - X[N];
+            lo[N] = (double)(a * a - b * b);
+            di[N] = (double)((a + b) * (2 * a + b));
+            up[N] = 0.0;
+        }
         solver.factor(N, lo.data(), di.data(), up.data());
     }
     /* one channel: Y (clamped, E values) -> coefficients [E-1][4] */
@@ -218,7 +239,15 @@ struct KnotSystem {
         M.assign(E + 1, 0.0);
         for (int i = 0; i + 1 < E; ++i) sl[i] = (Y[i + 1] - Y[i]) * ih[i];
         for (int u = 1; u <= N; ++u) r[u] = 3.0 * (sl[u] - sl[u - 1]);
+        if (nak) {
+            r[1] = r[1] * h[1];
+            r[N] = r[N] * h[N - 1];
+        }
         solver.solve(r.data(), M.data());   // M[0] = M[E-1] = 0
+        if (nak) {
+            M[0] = ((h[0] + h[1]) * M[1] - h[0] * M[2]) * ih[1];
+            M[E - 1] = ((h[N - 1] + h[N]) * M[N] - h[N] * M[N - 1]) * ih[N - 1];
+        }
         for (int j = 0; j + 1 < E; ++j) {
             coef[4 * j + 0] = Y[j];
             coef[4 * j + 1] = sl[j] - (h[j] * (2.0 * M[j] + M[j + 1])) * kThird;
@@ -277,7 +306,7 @@ struct Workspace {
 
 /* Envelopes of cur for all directions; accumulate mean into sum (ascending k),
  * optional memd.m amplitude.  nthreads>1 parallelises over channels / samples. */
-void envelope_pass(int n, int T, int K, const double* cur, const double* V, int window,
+void envelope_pass(int n, int T, int K, const double* cur, const double* V, int window, int spline,
                    bool want_amp, int nthreads, double* sum, double* ampsum, double* nvalid_out,
                    int32_t* counts) {
     std::vector<double> Pk(T);
@@ -304,7 +333,7 @@ void envelope_pass(int n, int T, int K, const double* cur, const double* V, int 
         bool valid = (E[0] > 3) && (E[1] > 3);   // memReducedEpochs_dev.cu:840,857
         if (!valid) continue;
         nvalid = nvalid + 1.0;
-        for (int e = 0; e < 2; ++e) sys[e].build(E[e], X[e].data());
+        for (int e = 0; e < 2; ++e) sys[e].build(E[e], X[e].data(), spline);
 #pragma omp parallel num_threads(nthreads) if (nthreads > 1)
         {
             std::vector<double> Y[2], coef[2], sl, r, M, env[2];
@@ -434,8 +463,12 @@ int oracle_extrema(int T, const double* Pk, int env, int window, int32_t* X) {
 void oracle_clamp(int E, const int32_t* X, double* Y, int env) { clamp_ends(E, X, Y, env); }
 
 void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, double* M) {
+    oracle_spline_solve_bc(E, X, nrhs, Y, M, ORACLE_SPLINE_NATURAL);
+}
+
+void oracle_spline_solve_bc(int E, const int32_t* X, int nrhs, const double* Y, double* M, int spline) {
     KnotSystem sys;
-    sys.build(E, X);
+    sys.build(E, X, spline);
     std::vector<double> sl, r, Mv, coef(4 * (size_t)std::max(E - 1, 1));
     for (int c = 0; c < nrhs; ++c) {
         sys.channel(Y + (size_t)c * E, coef.data(), sl, r, Mv);
@@ -473,10 +506,11 @@ void oracle_tridiag_blocked(int N, const double* lo, const double* di, const dou
 void oracle_envelope_mean(int n, int T, int K, const double* cur, const double* V,
                           const oracle_opts* o, double* mean, double* amp, int32_t* counts) {
     int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
+    int spline = o ? o->spline : ORACLE_SPLINE_NATURAL;
     int nth = (o && o->threads > 1) ? o->threads : 1;
     std::vector<double> ampsum(T);
     double nv = 0.0;
-    envelope_pass(n, T, K, cur, V, window, amp != nullptr, nth, mean, ampsum.data(), &nv, counts);
+    envelope_pass(n, T, K, cur, V, window, spline, amp != nullptr, nth, mean, ampsum.data(), &nv, counts);
     for (size_t i = 0; i < (size_t)n * T; ++i) mean[i] = mean[i] / nv;
     if (amp) for (int t = 0; t < T; ++t) amp[t] = ampsum[t] / nv;
 }
@@ -486,6 +520,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
     if (n < 1 || T < 4 || K < 1 || N_imf < 1 || S < 1) return -1;
     int mode = o ? o->mode : ORACLE_MODE_FIX;
     int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
+    int spline = o ? o->spline : ORACLE_SPLINE_NATURAL;
     int nth = (o && o->threads > 1) ? o->threads : 1;
     double sd = o ? o->sd : 0.075, sd2 = o ? o->sd2 : 0.75, tol = o ? o->tol : 0.075;
     size_t nT = (size_t)n * T;
@@ -506,7 +541,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
             std::memcpy(cur, running.data(), sizeof(double) * nT);
             for (int s = 0; s < S; ++s) {
                 double nv = 0.0;
-                envelope_pass(n, T, K, cur, V, window, false, nth, sum.data(), nullptr, &nv, counts.data());
+                envelope_pass(n, T, K, cur, V, window, spline, false, nth, sum.data(), nullptr, &nv, counts.data());
                 record();
                 for (size_t i = 0; i < nT; ++i) cur[i] = cur[i] - sum[i] / nv;
             }
@@ -521,7 +556,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
             int nsift = 0;
             for (;;) {
                 double nv = 0.0;
-                envelope_pass(n, T, K, cur, V, window, true, nth, sum.data(), ampsum.data(), &nv,
+                envelope_pass(n, T, K, cur, V, window, spline, true, nth, sum.data(), ampsum.data(), &nv,
                               counts.data());
                 record();
                 if (nsift == 0) {   // memd.m stop_emd: every direction < 3 interior extrema

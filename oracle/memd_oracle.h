@@ -27,11 +27,17 @@ enum { ORACLE_MODE_FIX = 0, ORACLE_MODE_STOP = 1 };
  * interior sample. */
 enum { ORACLE_WINDOW_REFWARP32 = 0, ORACLE_WINDOW_FULL = 1 };
 
+/* Spline end conditions. NATURAL = the reference's (memReducedEpochs_dev.cu:704-736);
+ * NOT_A_KNOT = third-derivative continuity at the second and second-to-last knots
+ * (the reference's dead tridiagonal_setup_nak, :738-773, in a pivot-free form). */
+enum { ORACLE_SPLINE_NATURAL = 0, ORACLE_SPLINE_NOT_A_KNOT = 1 };
+
 typedef struct oracle_opts {
     int mode;        /* ORACLE_MODE_* */
     int window;      /* ORACLE_WINDOW_* */
     double sd, sd2, tol; /* stop thresholds (stop mode only) */
     int threads;     /* OpenMP threads; <=1 runs the serial code path */
+    int spline;      /* ORACLE_SPLINE_* */
 } oracle_opts;
 
 typedef struct oracle_trace {
@@ -61,6 +67,8 @@ void oracle_clamp(int E, const int32_t* X, double* Y, int env);
 /* natural-spline second-derivative-type solve for nrhs channels:
  * Y is [nrhs][E] (already clamped), M is [nrhs][E] output. */
 void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, double* M);
+/* the same with a choice of end conditions (ORACLE_SPLINE_*); not-a-knot needs E >= 4 */
+void oracle_spline_solve_bc(int E, const int32_t* X, int nrhs, const double* Y, double* M, int spline);
 /* coefficients [E-1][4] = (a,b,c,d) per interval of one channel */
 void oracle_spline_coefs(int E, const int32_t* X, const double* Y, const double* M, double* coef);
 /* envelope of one channel at t = 0..T-1 */

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmemd_oracle.so")
 
 MODE_FIX, MODE_STOP = 0, 1
 WINDOW_REFWARP32, WINDOW_FULL = 0, 1
+SPLINE_NATURAL, SPLINE_NOT_A_KNOT = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -22,7 +23,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 class Opts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("window", ctypes.c_int),
                 ("sd", ctypes.c_double), ("sd2", ctypes.c_double), ("tol", ctypes.c_double),
-                ("threads", ctypes.c_int)]
+                ("threads", ctypes.c_int), ("spline", ctypes.c_int)]
 
 
 class Trace(ctypes.Structure):
@@ -52,6 +53,7 @@ def lib():
         L.oracle_extrema.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _ip]
         L.oracle_clamp.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int]
         L.oracle_spline_solve.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp]
+        L.oracle_spline_solve_bc.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.oracle_spline_coefs.argtypes = [ctypes.c_int, _ip, _dp, _dp, _dp]
         L.oracle_envelope.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
         L.oracle_tridiag_blocked.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -70,8 +72,9 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
-def make_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, threads=1):
-    return Opts(mode, window, sd, sd2, tol, threads)
+def make_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, threads=1,
+              spline=SPLINE_NATURAL):
+    return Opts(mode, window, sd, sd2, tol, threads, spline)
 
 
 def solver_block():
@@ -128,12 +131,15 @@ def clamp(X, Y, env):
     return Y
 
 
-def spline_solve(X, Y):
-    """Y: [nrhs][E] clamped knot values -> M: [nrhs][E] (M[:,0] = M[:,-1] = 0)."""
+def spline_solve(X, Y, spline=SPLINE_NATURAL):
+    """Y: [nrhs][E] clamped knot values -> M: [nrhs][E] (natural: M[:,0] = M[:,-1] = 0)."""
     X = np.ascontiguousarray(X, np.int32)
     Y = np.ascontiguousarray(np.atleast_2d(Y), np.float64)
     M = np.zeros_like(Y)
-    lib().oracle_spline_solve(len(X), _i(X), Y.shape[0], _d(Y), _d(M))
+    if spline == SPLINE_NATURAL:
+        lib().oracle_spline_solve(len(X), _i(X), Y.shape[0], _d(Y), _d(M))
+    else:
+        lib().oracle_spline_solve_bc(len(X), _i(X), Y.shape[0], _d(Y), _d(M), spline)
     return M
 
 
@@ -163,7 +169,7 @@ def tridiag_blocked(lo, di, up, r):
     return x
 
 
-def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1):
+def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1, spline=SPLINE_NATURAL):
     cur = np.ascontiguousarray(cur, np.float64)
     V = np.ascontiguousarray(V, np.float64)
     n, T = cur.shape
@@ -171,14 +177,14 @@ def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1):
     mean = np.zeros((n, T), np.float64)
     amp = np.zeros(T, np.float64) if want_amp else None
     counts = np.zeros((K, 2), np.int32)
-    o = make_opts(window=window, threads=threads)
+    o = make_opts(window=window, threads=threads, spline=spline)
     lib().oracle_envelope_mean(n, T, K, _d(cur), _d(V), ctypes.byref(o), _d(mean),
                                _d(amp) if want_amp else None, _i(counts))
     return mean, amp, counts
 
 
 def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075,
-         threads=1, trace=True):
+         threads=1, trace=True, spline=SPLINE_NATURAL):
     """Whole epoch. Returns (imfs [N_imf][n][T], info dict)."""
     y = np.ascontiguousarray(y, np.float64)
     V = np.ascontiguousarray(V, np.float64)
@@ -191,7 +197,7 @@ def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0
     nout = np.zeros(1, np.int32)
     npass = np.zeros(1, np.int32)
     tr = Trace(_i(kc) if trace else None, max_passes, _i(sc), _i(nout), _i(npass))
-    o = make_opts(mode, window, sd, sd2, tol, threads)
+    o = make_opts(mode, window, sd, sd2, tol, threads, spline)
     rc = lib().oracle_memd(n, T, K, n_imf, S, _d(y), _d(V), _d(imfs), ctypes.byref(o), ctypes.byref(tr))
     if rc != 0:
         raise ValueError("oracle_memd rejected its arguments")

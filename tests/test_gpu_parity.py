@@ -225,3 +225,44 @@ def test_batch_many_epochs_xcd_groups(ctx, oracle):
     for ep, out in zip(eps, outs):
         alone, _ = ctx.run(ep["y"], ep["V"], ep["n_imf"], ep["S"])
         assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
+
+
+def test_directions_f32_is_rounded_f64(ctx, oracle):
+    # GenerateDirectionVector<float>, as the reference's main() instantiates it
+    import torch
+    out = torch.empty((64, 32), dtype=torch.float32, device="cuda:0")
+    ctx.directions(32, 64, out=out)
+    assert np.array_equal(out.cpu().numpy(), oracle.directions(32, 64).astype(np.float32))
+
+
+@pytest.mark.parametrize("gen,n,T,K,n_imf,S,seed,mode", [
+    ("sines", 8, 1000, 16, 5, 4, 30, 0),
+    ("eeg", 4, 5000, 8, 3, 3, 31, 0),      # many solver blocks per direction
+    ("sines", 1, 600, 4, 4, 3, 32, 0),
+    ("eeg", 32, 4096, 64, 3, 2, 33, 0),
+    ("eeg", 8, 1500, 16, 8, 12, 34, 1),    # stop mode reads the record array
+])
+def test_not_a_knot_matches_oracle(ctx, oracle, gen, n, T, K, n_imf, S, seed, mode):
+    import synth
+    y = synth.sines_epoch(n, T, seed) if gen == "sines" else synth.eeg_epoch(n, T, seed)
+    V = oracle.directions(n, K)
+    imfs_o, info_o = oracle.memd(y, V, n_imf, S, mode=mode, spline=oracle.SPLINE_NOT_A_KNOT)
+    imfs_g, info_g = ctx.run(_dev(y), _dev(V), n_imf, S, mode=mode, spline=1)
+    used = info_o["n_imf_out"]
+    assert info_g["n_imf_out"] == used
+    assert np.array_equal(info_g["sift_counts"], info_o["sift_counts"])
+    got = imfs_g.cpu().numpy()[:used]
+    mism = np.argwhere(got != imfs_o[:used])
+    assert mism.size == 0, f"{len(mism)} samples differ, first at {mism[0].tolist()}"
+
+
+def test_not_a_knot_batch_matches_alone(ctx, oracle):
+    import synth
+    eps = []
+    for i, (n, T, K) in enumerate([(8, 1200, 16), (4, 2600, 8), (16, 900, 32)]):
+        eps.append({"y": _dev(synth.eeg_epoch(n, T, 40 + i)), "V": _dev(oracle.directions(n, K)),
+                    "n_imf": 4, "S": 3})
+    outs, _ = ctx.run_batch(eps, spline=1)
+    for ep, out in zip(eps, outs):
+        alone, _ = ctx.run(ep["y"], ep["V"], 4, 3, spline=1)
+        assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())

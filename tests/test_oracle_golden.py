@@ -144,3 +144,55 @@ def test_envelope_mean_pass_matches_independent_pipeline(oracle, gold):
     assert np.array_equal(counts, gold["pass_counts"])
     ref = gold["pass_mean"]
     np.testing.assert_allclose(mean, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_not_a_knot_matches_scipy(oracle, seed):
+    # optional end condition (the reference's dead tridiagonal_setup_nak, restated
+    # pivot-free): the spline must be scipy's not-a-knot interpolant, including equal
+    # end gaps (where the reference's row-0 form has a zero diagonal) and knot counts
+    # that span several solver blocks
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(100 + seed)
+    for E in (4, 5, 7, 33, 34, 65, 200):
+        gaps = rng.integers(1, 15, size=E - 1)
+        if seed % 2 == 0:
+            gaps[:3] = 4
+            gaps[-3:] = 9
+        X = np.concatenate([[0], np.cumsum(gaps)]).astype(np.int32)
+        Y = rng.standard_normal((2, E)) * 50
+        M = oracle.spline_solve(X, Y, spline=oracle.SPLINE_NOT_A_KNOT)
+        T = int(X[-1]) + 1
+        for c in range(2):
+            coef = oracle.spline_coefs(X, Y[c], M[c])
+            env = oracle.envelope(T, X, coef)
+            ref = CubicSpline(X.astype(float), Y[c], bc_type="not-a-knot")(np.arange(T, dtype=float))
+            assert np.abs(env - ref).max() <= 1e-11 * np.abs(ref).max()
+            # not-a-knot: the cubic term is shared by the first two and the last two intervals
+            d = coef[:, 3]
+            assert abs(d[0] - d[1]) <= 1e-9 * max(abs(d).max(), 1e-300)
+            assert abs(d[-1] - d[-2]) <= 1e-9 * max(abs(d).max(), 1e-300)
+            # and it interpolates
+            assert np.allclose(env[X], Y[c], rtol=0, atol=1e-12 * np.abs(Y[c]).max())
+
+
+def test_not_a_knot_four_knots_is_one_cubic(oracle):
+    # E = 4: both end conditions leave a single cubic through the four knots
+    X = np.array([0, 3, 4, 9], np.int32)
+    Y = np.array([[1.0, -2.0, 0.5, 3.0]])
+    M = oracle.spline_solve(X, Y, spline=oracle.SPLINE_NOT_A_KNOT)
+    coef = oracle.spline_coefs(X, Y[0], M[0])
+    env = oracle.envelope(10, X, coef)
+    p = np.polyfit(X.astype(float), Y[0], 3)
+    assert np.allclose(env, np.polyval(p, np.arange(10.0)), rtol=0, atol=1e-12)
+
+
+def test_not_a_knot_memd_reconstructs(oracle):
+    import synth
+    y = synth.sines_epoch(4, 900, 3)
+    V = oracle.directions(4, 8)
+    imfs, info = oracle.memd(y, V, 4, 3, spline=oracle.SPLINE_NOT_A_KNOT)
+    assert info["n_imf_out"] == 4
+    np.testing.assert_allclose(imfs.sum(0), y, rtol=0, atol=1e-12 * np.abs(y).max())
+    nat, _ = oracle.memd(y, V, 4, 3)
+    assert not np.array_equal(nat, imfs)   # the option changes the envelopes
