@@ -137,6 +137,8 @@ memd_status validate_shape(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N
         return fail(ctx, MEMD_ERR_UNSUPPORTED, "n=%zu channels exceeds the engine limit of %d", n, kMaxChannels);
     if (T > (size_t)(1u << 30) || K > 65536 || N_imf > 4096 || S > (1u << 24))
         return fail(ctx, MEMD_ERR_UNSUPPORTED, "shape too large (T=%zu K=%zu N_imf=%zu S=%zu)", T, K, N_imf, S);
+    if (n * T > ((size_t)1 << 31))   // knot rows are addressed with 32-bit offsets
+        return fail(ctx, MEMD_ERR_UNSUPPORTED, "n*T=%zu exceeds the engine limit of 2^31 samples per epoch", n * T);
     return MEMD_OK;
 }
 
@@ -298,8 +300,10 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
             plan.wl.cap[l][x] = (int)P.gcap[l][x];
             at += P.gcap[l][x];
         }
-        long g = P.lcap[l] < kListGrid ? P.lcap[l] : kListGrid;
-        if (groups > 1) g = kListGrid;   // a multiple of kXcds, every XCD gets its share
+        // one-wave workgroups for the factor list: a larger persistent grid
+        const long gmax = (l == kListFactor) ? 4L * kListGrid : (long)kListGrid;
+        long g = P.lcap[l] < gmax ? P.lcap[l] : gmax;
+        if (groups > 1) g = gmax;   // a multiple of kXcds, every XCD gets its share
         plan.grid[l] = (int)(g > 0 ? g : 1);
     }
     int nmax = 0;

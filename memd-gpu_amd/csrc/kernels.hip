@@ -100,6 +100,22 @@ __device__ __forceinline__ BlockGeom block_geom(int E, int b) {
     return g;
 }
 
+// Pointers in the global address space: loads through them compile to
+// global_load with a scalar base and a 32-bit per-lane offset instead of flat
+// loads with 64-bit address arithmetic.
+template <class T>
+using GPtr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ GPtr<T> gptr(const T* p) {
+    return (GPtr<T>)p;
+}
+template <class T>
+using GMut = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ GMut<T> gmut(T* p) {
+    return (GMut<T>)p;
+}
+
 // Row u (1..N, N = E-2) of a knot system.  Natural ends: the reference's rows
 // (memReducedEpochs_dev.cu:704-736).  Not-a-knot: rows 1 and N with M_0 and
 // M_{E-1} eliminated (integer products, exact in FP64).
@@ -117,6 +133,27 @@ __device__ __forceinline__ void sys_row(const int* X, int u, int E, int spline, 
             up = (double)(h1 * h1 - h0 * h0);
         } else if (u == N) {
             long long a = X[N] - X[N - 1], b = X[N + 1] - X[N];
+            lo = (double)(a * a - b * b);
+            di = (double)((a + b) * (2 * a + b));
+            up = 0.0;
+        }
+    }
+}
+
+// The same row from the three knot positions X[u-1], X[u], X[u+1].
+__device__ __forceinline__ void sys_row3(int xm, int x0, int xp, int u, int E, int spline, double& lo, double& di,
+                                         double& up) {
+    lo = (double)(x0 - xm);
+    up = (double)(xp - x0);
+    di = (double)(2 * ((x0 - xm) + (xp - x0)));
+    if (spline && E >= 4) {
+        if (u == 1) {
+            long long h0 = x0 - xm, h1 = xp - x0;
+            lo = 0.0;
+            di = (double)((h0 + h1) * (h0 + 2 * h1));
+            up = (double)(h1 * h1 - h0 * h0);
+        } else if (u == E - 2) {
+            long long a = x0 - xm, b = xp - x0;
             lo = (double)(a * a - b * b);
             di = (double)((a + b) * (2 * a + b));
             up = 0.0;
@@ -387,94 +424,117 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
 // Shared factors of one (direction, envelope): per block the forward factors,
 // the reciprocals of the intervals it owns and its two spike vectors; then the
 // reduced system's factorisation (one lane, J steps).
-__global__ __launch_bounds__(256) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
+__global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
+    __shared__ double s_rlo[64], s_rdi[64], s_rup[64];
     const ListCursor cu = list_cursor(wl, kListFactor);
+    const int lane = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
         const int E = ed.E[ke];
-        const int* X = ed.X + (size_t)ke * ed.Ecap;
-        double* rd = fac_row(ed, ke, kFacRd);
-        double* cp = fac_row(ed, ke, kFacCp);
-        double* al = fac_row(ed, ke, kFacAl);
-        double* be = fac_row(ed, ke, kFacBe);
-        double* ih = fac_row(ed, ke, kFacIh);
-        double* i3h = fac_row(ed, ke, kFacI3h);
+        const int spline = ed.spline;
+        const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
+        const GMut<double> rd = gmut(fac_row(ed, ke, kFacRd));
+        const GMut<double> cp = gmut(fac_row(ed, ke, kFacCp));
+        const GMut<double> al = gmut(fac_row(ed, ke, kFacAl));
+        const GMut<double> be = gmut(fac_row(ed, ke, kFacBe));
+        const GMut<double> ih = gmut(fac_row(ed, ke, kFacIh));
+        const GMut<double> i3h = gmut(fac_row(ed, ke, kFacI3h));
         const int J = block_geom(E, 0).J;
-        for (int b = threadIdx.x; b <= J; b += blockDim.x) {
-            BlockGeom g = block_geom(E, b);
-            const int p = g.p, q = g.q;
-            for (int j = p - 1; j <= q; ++j) {   // intervals owned by this block
-                double h = (double)(X[j + 1] - X[j]);
-                ih[j] = 1.0 / h;
-                i3h[j] = 1.0 / (3.0 * h);
+        // one lane per block: the block's rows live in registers, memory sees stores only
+        for (int b = lane; b <= J; b += 64) {
+            const BlockGeom g = block_geom(E, b);
+            const int p = g.p, len = g.q - g.p + 1;
+            int xs[L + 2];   // X[p-1 .. q+1]
+#pragma unroll
+            for (int i = 0; i < L + 2; ++i) xs[i] = X[p - 1 + min(i, len + 1)];
+#pragma unroll 1
+            for (int i = 0; i <= len; ++i) {   // intervals p-1 .. q
+                double h = (double)(X[p + i] - X[p - 1 + i]);
+                ih[p - 1 + i] = 1.0 / h;
+                i3h[p - 1 + i] = 1.0 / (3.0 * h);
             }
-            double cprev = 0.0;
-            for (int u = p; u <= q; ++u) {
-                double lo, di, up;
-                sys_row(X, u, E, ed.spline, lo, di, up);
-                double den = (u == p) ? di : fma(-lo, cprev, di);
-                double r = 1.0 / den;
-                cprev = up * r;
-                rd[u] = r;
-                cp[u] = cprev;
-            }
-            if (J == 0) continue;
-            if (b >= 1) {   // left spike A_b^{-1}(lo_p e_first): forward sweep into al, back substitution
-                double z = 0.0;
-                for (int u = p; u <= q; ++u) {
+            const bool left = b >= 1, right = b < J;
+            double cpr[L], zl[L];
+            double cprev = 0.0, z = 0.0, zr = 0.0;
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                if (i < len) {
                     double lo, di, up;
-                    sys_row(X, u, E, ed.spline, lo, di, up);
-                    double gv = (u == p) ? lo : 0.0;
-                    z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
-                    al[u] = z;
-                }
-                double x = al[q];
-                for (int u = q - 1; u >= p; --u) {
-                    x = fma(-cp[u], x, al[u]);
-                    al[u] = x;
+                    sys_row3(xs[i], xs[i + 1], xs[i + 2], p + i, E, spline, lo, di, up);
+                    double den = (i == 0) ? di : fma(-lo, cprev, di);
+                    double r = 1.0 / den;
+                    cprev = up * r;
+                    rd[p + i] = r;
+                    cp[p + i] = cprev;
+                    cpr[i] = cprev;
+                    // left spike A_b^{-1}(lo_p e_first), forward sweep
+                    z = (i == 0) ? lo * r : fma(-lo, z, 0.0) * r;
+                    zl[i] = z;
+                    // right spike A_b^{-1}(up_q e_last): the forward sweep is +0 above the
+                    // last row (every pivot is positive) and up_q * rd_q there
+                    if (i == len - 1) zr = up * r;
                 }
             }
-            if (b < J) {    // right spike A_b^{-1}(up_q e_last)
-                double z = 0.0;
-                for (int u = p; u <= q; ++u) {
-                    double lo, di, up;
-                    sys_row(X, u, E, ed.spline, lo, di, up);
-                    double gv = (u == q) ? up : 0.0;
-                    z = (u == p) ? gv * rd[u] : fma(-lo, z, gv) * rd[u];
-                    be[u] = z;
+            if (left) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = L - 1; i >= 0; --i) {
+                    if (i < len) {
+                        x = (i == len - 1) ? zl[i] : fma(-cpr[i], x, zl[i]);
+                        al[p + i] = x;
+                    }
                 }
-                double x = be[q];
-                for (int u = q - 1; u >= p; --u) {
-                    x = fma(-cp[u], x, be[u]);
-                    be[u] = x;
+            }
+            if (right) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = L - 1; i >= 0; --i) {
+                    if (i < len) {
+                        x = (i == len - 1) ? zr : fma(-cpr[i], x, 0.0);
+                        be[p + i] = x;
+                    }
                 }
             }
         }
         __syncthreads();
-        if (J > 0 && threadIdx.x == 0) {
+        if (J > 0) {
+            // reduced system: coefficients of every separator in parallel (LDS), then one
+            // lane runs the factorisation chain
             double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
             double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
             double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
             double cprev = 0.0;
-            for (int j = 1; j <= J; ++j) {
-                int s = j * L;
-                double lo = (double)(X[s] - X[s - 1]), up = (double)(X[s + 1] - X[s]);
-                double di = (double)(2 * ((X[s] - X[s - 1]) + (X[s + 1] - X[s])));
-                double rlo = (j >= 2) ? -(lo * al[s - 1]) : 0.0;
-                double rdi = fma(-up, al[s + 1], fma(-lo, be[s - 1], di));
-                double rup = (j <= J - 1) ? -(up * be[s + 1]) : 0.0;
-                double den = (j == 1) ? rdi : fma(-rlo, cprev, rdi);
-                double r = 1.0 / den;
-                cprev = rup * r;
-                Rlo[j] = rlo;
-                Rrd[j] = r;
-                Rcp[j] = cprev;
+            for (int j0 = 1; j0 <= J; j0 += 64) {
+                const int j = j0 + lane;
+                if (j <= J) {
+                    const int sp = j * L;
+                    int xm = X[sp - 1], x0 = X[sp], xp = X[sp + 1];
+                    double lo = (double)(x0 - xm), up = (double)(xp - x0);
+                    double di = (double)(2 * ((x0 - xm) + (xp - x0)));
+                    s_rlo[lane] = (j >= 2) ? -(lo * al[sp - 1]) : 0.0;
+                    s_rdi[lane] = fma(-up, al[sp + 1], fma(-lo, be[sp - 1], di));
+                    s_rup[lane] = (j <= J - 1) ? -(up * be[sp + 1]) : 0.0;
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    const int jn = min(64, J - j0 + 1);
+                    for (int jj = 0; jj < jn; ++jj) {
+                        const int jg = j0 + jj;
+                        double rlo = s_rlo[jj];
+                        double den = (jg == 1) ? s_rdi[jj] : fma(-rlo, cprev, s_rdi[jj]);
+                        double r = 1.0 / den;
+                        cprev = s_rup[jj] * r;
+                        Rlo[jg] = rlo;
+                        Rrd[jg] = r;
+                        Rcp[jg] = cprev;
+                    }
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
 }
 
@@ -870,7 +930,7 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, 
 __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, KC = 64;
-    constexpr int MI = (2 * NI * CG + 255) / 256;   // phase-A items per thread
+    constexpr int MI = 3;   // phase-A items per thread per round (rounds repeat while items remain)
     __shared__ double4 rec[2][CG][NI];
     __shared__ int xk[2][NI + 1];
     __shared__ int sE[KC][2], sPre[KC][2];
@@ -908,61 +968,73 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
             const int k = kc0 + kk;
             if (!(sE[kk][0] > 3 && sE[kk][1] > 3)) continue;
             nv = nv + 1.0;
-            int jlo[2], nint[2];
+            int jlo[2], nint[2], magic[2];
 #pragma unroll
             for (int env = 0; env < 2; ++env) {
-                int E = sE[kk][env];
-                int pr = sPre[kk][env];
-                unsigned long long m = sMask[kk][env];
-                int lo = min(max(pr - 1, 0), E - 2);
-                int hi = min(max(pr + __popcll(m & ((1ull << (tlast - t0)) - 1ull)) - 1, 0), E - 2);
+                const int E = sE[kk][env];
+                const int pr = sPre[kk][env];
+                const unsigned long long m = sMask[kk][env];
+                const int inside = (int)__popcll(m & ((1ull << (tlast - t0)) - 1ull));
+                const int lo = min(max(pr - 1, 0), E - 2);
+                const int hi = min(max(pr + inside - 1, 0), E - 2);
                 jlo[env] = lo;
                 nint[env] = hi - lo + 1;
+                // rem / nint == (rem * magic) >> 16 exactly for rem < 2^16 / nint (rem < 1152, nint <= 37)
+                magic[env] = (65536 + nint[env] - 1) / nint[env];
             }
-            // phase A: loads of every item first, then the records
+            // phase A: every item's loads first, then the records.  Element offsets
+            // are 32-bit (n * T <= 2^30) from per-direction bases.
             const int per0 = nint[0] * CG;
             const int total = per0 + nint[1] * CG;
+            const int nitems = (total + 255) >> 8;   // uniform: unused item slots issue no loads
+            const size_t rowbase = (size_t)(2 * k) * n * ed.Ecap;
+            const GPtr<double> Mb[2] = {gptr(ed.ysol + rowbase), gptr(ed.ysol + rowbase + (size_t)n * ed.Ecap)};
+            const GPtr<double> Yb[2] = {gptr(ed.yknot + rowbase), gptr(ed.yknot + rowbase + (size_t)n * ed.Ecap)};
+            const GPtr<int> Xb[2] = {gptr(ed.X + (size_t)(2 * k) * ed.Ecap), gptr(ed.X + (size_t)(2 * k + 1) * ed.Ecap)};
+            const GPtr<double> Hb[2] = {gptr(fac_row(ed, 2 * k, kFacIh)), gptr(fac_row(ed, 2 * k + 1, kFacIh))};
+            const GPtr<double> H3b[2] = {gptr(fac_row(ed, 2 * k, kFacI3h)), gptr(fac_row(ed, 2 * k + 1, kFacI3h))};
+            for (int a0 = 0; a0 < nitems; a0 += MI) {
             double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
-            int x0[MI], x1[MI], slot[MI];
+            int x0[MI], x1[MI], lds[MI];
 #pragma unroll
             for (int a = 0; a < MI; ++a) {
-                int idx = tid + 256 * a;
-                bool ok = idx < total;
-                int env = (idx < per0) ? 0 : 1;
-                int rem = env == 0 ? idx : idx - per0;
-                int ni = max(nint[env], 1);
-                int cl2 = rem / ni;
-                int i = rem - cl2 * ni;
-                int c2 = min(cg * CG + cl2, n - 1);
-                int ke = 2 * k + env;
-                int j = ok ? jlo[env] + i : 0;
-                const int* X = ed.X + (size_t)ke * ed.Ecap;
-                const double* M = ed.ysol + ((size_t)ke * n + c2) * ed.Ecap;
-                const double* yk = ed.yknot + ((size_t)ke * n + c2) * ed.Ecap;
-                x0[a] = X[j];
-                x1[a] = X[j + 1];
-                v0[a] = yk[j];
-                v1[a] = yk[j + 1];
-                m0[a] = M[j];
-                m1[a] = M[j + 1];
-                ih[a] = fac_row(ed, ke, kFacIh)[j];
-                i3h[a] = fac_row(ed, ke, kFacI3h)[j];
-                slot[a] = ok ? ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) : -1;
+                lds[a] = -1;
+                if (a0 + a >= nitems) continue;
+                const int idx = tid + 256 * (a0 + a);
+                const int env = (idx < per0) ? 0 : 1;
+                const int rem = (env == 0) ? idx : idx - per0;
+                const int cl2 = (rem * magic[env]) >> 16;
+                const int i = rem - cl2 * nint[env];
+                const bool ok = idx < total;
+                const int c2 = min(cg * CG + cl2, n - 1);
+                const int j = ok ? jlo[env] + i : 0;
+                const unsigned off = (unsigned)c2 * (unsigned)ed.Ecap + (unsigned)j;
+                x0[a] = Xb[env][j];
+                x1[a] = Xb[env][j + 1];
+                v0[a] = Yb[env][off];
+                v1[a] = Yb[env][off + 1];
+                m0[a] = Mb[env][off];
+                m1[a] = Mb[env][off + 1];
+                ih[a] = Hb[env][j];
+                i3h[a] = H3b[env][j];
+                // LDS slot of the record; bit 30: channel beyond n (ragged group), bit 29: first channel
+                if (ok)
+                    lds[a] = ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) |
+                             (cl2 == 0 ? 0x20000000 : 0);
             }
 #pragma unroll
             for (int a = 0; a < MI; ++a) {
-                int sl0 = slot[a];
+                if (a0 + a >= nitems) continue;
+                const int sl0 = lds[a];
                 if (sl0 < 0) continue;
-                int sv = sl0 & 0x3fffffff;
-                int env = sv / (CG * NI);
-                int rem = sv - env * CG * NI;
-                int cl2 = rem / NI;
-                int i = rem - cl2 * NI;
-                if (cl2 == 0) {
+                const int sv = sl0 & 0x1fffffff;
+                if (sl0 & 0x20000000) {   // first channel of the group: knot positions
+                    const int env = sv >= CG * NI ? 1 : 0;
+                    const int i = sv - env * CG * NI;
                     xk[env][i] = x0[a];
                     if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
                 }
-                if (sl0 & 0x40000000) continue;   // channel beyond n (ragged group)
+                if (sl0 & 0x40000000) continue;
                 double h = (double)(x1[a] - x0[a]);
                 double slp = (v1[a] - v0[a]) * ih[a];
                 double4 r;
@@ -970,7 +1042,8 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
                 r.y = slp - (h * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
                 r.z = m0[a];
                 r.w = (m1[a] - m0[a]) * i3h[a];
-                rec[env][cl2][i] = r;
+                (&rec[0][0][0])[sv] = r;
+            }
             }
             __syncthreads();
             // phase B
@@ -1325,7 +1398,7 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     case 1:
         hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(256), 0, s, p.eps, p.tab[kMapCompact], p.wl);
         break;
-    case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 256); break;
+    case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
     case 3: MEMD_LIST_LAUNCH(k_knots, kListKnots, kKnotChunk); break;
     case 4: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
     case 5: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
