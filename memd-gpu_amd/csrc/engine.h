@@ -29,9 +29,9 @@ constexpr int kEvalDefault = kEvalFused;
 constexpr int kFlagLdsMaxCh = 64;
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
 constexpr int kKnotChunk = 256;
-constexpr int kKnotCh = 4;
 // Coefficient kernel: items (channel, interval) per thread.
-constexpr int kCoefItems = 16;
+constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
+constexpr int kKnotCB = 8;         // channels whose gathers k_knots issues together
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;
 enum : int { kFacRd = 0, kFacCp = 1, kFacAl = 2, kFacBe = 3, kFacIh = 4, kFacI3h = 5 };

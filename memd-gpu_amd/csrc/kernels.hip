@@ -538,12 +538,29 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
     }
 }
 
+// End-knot clamp of memReducedEpochs_dev.cu:451-485 (max) / :549-583 (min),
+// from the raw values at the end knot (ye), its neighbour (y1) and the next one
+// (y2), and the positions x_e, x_1, x_2 (same operations as KnotRow::Y).
+__device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int xe, int x1, int x2, int env,
+                                            bool left) {
+    double tv;
+    if (left) {
+        double sl = (y2 - y1) / (double)(x2 - x1);
+        tv = y1 - sl * (double)(x1 - xe);
+    } else {
+        double sl = (y1 - y2) / (double)(x1 - x2);
+        tv = y1 + sl * (double)(xe - x1);
+    }
+    bool take = (env == 0) ? (tv > ye) : (tv < ye);
+    return take ? tv : ye;
+}
+
 // Knot values and right-hand sides, per (direction, envelope) and 256-knot
-// chunk: Y_i (end knots clamped) and r_i = 3 (sl_i - sl_{i-1}) with
-// sl_i = (Y_{i+1} - Y_i) * ih_i, for every channel; lanes run over knots.
+// chunk; lanes run over knots.  Each lane loads its knot's position and its two
+// neighbours' once, then issues the gathers of kKnotCB channels together:
+// Y_i (end knots clamped) and r_i = 3 (sl_i - sl_{i-1}) with sl_i = (Y_{i+1} - Y_i) * ih_i.
 __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restrict__ eps, WorkLists wl) {
-    constexpr int CH = kKnotCh;
-    __shared__ double Ys[CH][kKnotChunk + 2];
+    constexpr int CB = kKnotCB;
     const ListCursor cu = list_cursor(wl, kListKnots);
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
@@ -551,51 +568,70 @@ __global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restric
         const EpochState* st = ed.st;
         const int ke = it.y, env = ke & 1;
         const int E = ed.E[ke];
-        const int i0 = it.z * kKnotChunk;
-        const int n = ed.n;
-        const int* X = ed.X + (size_t)ke * ed.Ecap;
-        const double* ihv = fac_row(ed, ke, kFacIh);
-        const double* cur = cur_slot(ed, st->imf);
-        const int l = threadIdx.x;
-        const int i = i0 + l;
-        for (int c0 = 0; c0 < n; c0 += CH) {
-            __syncthreads();
+        const int i = it.z * kKnotChunk + (int)threadIdx.x;
+        if (i >= E) continue;   // no barriers in this kernel
+        const int n = ed.n, T = ed.T;
+        const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
+        const GPtr<double> ihv = gptr(fac_row(ed, ke, kFacIh));
+        const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
+        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
+        const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
+        const int im = max(i - 1, 0), ip = min(i + 1, E - 1);
+        const int xm = X[im], x0 = X[i], xp = X[ip];
+        const bool interior = i >= 1 && i <= E - 2;
+        const double ihi = interior ? (double)ihv[i] : 0.0;
+        const double ihm = interior ? (double)ihv[i - 1] : 0.0;
+        // not-a-knot: rows 1 and N = E-2 are scaled by h_1 and h_{N-1}
+        double rscale = 0.0;
+        if (ed.spline && E >= 4) {
+            if (i == 1) rscale = (double)(xp - x0);
+            else if (i == E - 2) rscale = (double)(x0 - xm);
+        }
+        // lanes whose values involve a clamped end knot (E >= 6): i in {0, 1, E-2, E-1}
+        const bool lend = E >= 6 && i <= 1, rend = E >= 6 && i >= E - 2;
+        int xa = 0, xb = 0, xc = 0;   // X[0..2] or X[E-1], X[E-2], X[E-3]
+        if (lend) { xa = X[0]; xb = X[1]; xc = X[2]; }
+        if (rend) { xa = X[E - 1]; xb = X[E - 2]; xc = X[E - 3]; }
+        for (int c0 = 0; c0 < n; c0 += CB) {
+            double ym[CB], y0[CB], yp[CB], y3[CB];
 #pragma unroll
-            for (int cc = 0; cc < CH; ++cc) {
-                int c = c0 + cc;
-                if (c >= n) break;
-                KnotRow kr{X, cur + (size_t)c * ed.T, E, env};
-                if (i < E) Ys[cc][l + 1] = kr.Y(i);
-                if (l == 0 && i0 >= 1) Ys[cc][0] = kr.Y(i0 - 1);
-                if (l == kKnotChunk - 1 && i0 + kKnotChunk < E) Ys[cc][kKnotChunk + 1] = kr.Y(i0 + kKnotChunk);
+            for (int cc = 0; cc < CB; ++cc) {
+                const unsigned base = (unsigned)min(c0 + cc, n - 1) * (unsigned)T;
+                ym[cc] = cur[base + xm];
+                y0[cc] = cur[base + x0];
+                yp[cc] = cur[base + xp];
+                y3[cc] = 0.0;
+                if (lend || rend) y3[cc] = cur[base + xc];   // raw value two knots in from the end
             }
-            __syncthreads();
-            if (i < E) {
-                double ihi = (i <= E - 2) ? ihv[i] : 0.0;
-                double ihm = (i >= 1) ? ihv[i - 1] : 0.0;
-                // not-a-knot: rows 1 and N = E-2 are scaled by h_1 and h_{N-1}
-                double rscale = 0.0;
-                if (ed.spline && E >= 4) {
-                    if (i == 1) rscale = (double)(X[2] - X[1]);
-                    else if (i == E - 2) rscale = (double)(X[E - 2] - X[E - 3]);
-                }
 #pragma unroll
-                for (int cc = 0; cc < CH; ++cc) {
-                    int c = c0 + cc;
-                    if (c >= n) break;
-                    size_t at = ((size_t)ke * n + c) * ed.Ecap + i;
-                    ed.yknot[at] = Ys[cc][l + 1];
-                    if (i >= 1 && i <= E - 2) {
-                        double sl = (Ys[cc][l + 2] - Ys[cc][l + 1]) * ihi;
-                        double slm = (Ys[cc][l + 1] - Ys[cc][l]) * ihm;
-                        double r = 3.0 * (sl - slm);
-                        if (rscale != 0.0) r = r * rscale;
-                        ed.ysol[at] = r;
-                    }
+            for (int cc = 0; cc < CB; ++cc) {
+                const int c = c0 + cc;
+                if (c >= n) break;
+                double a = ym[cc], b = y0[cc], d = yp[cc];
+                if (lend) {
+                    // raw(0), raw(1), raw(2) are (b, d, y3) for i = 0 and (a, b, d) for i = 1
+                    double r0 = (i == 0) ? b : a, r1 = (i == 0) ? d : b, r2 = (i == 0) ? y3[cc] : d;
+                    double yc = clamp_end(r0, r1, r2, xa, xb, xc, env, true);
+                    if (i == 0) b = yc; else a = yc;
+                }
+                if (rend) {
+                    // raw(E-1), raw(E-2), raw(E-3) are (b, a, y3) for i = E-1 and (d, b, a) for i = E-2
+                    const bool last = (i == E - 1);
+                    double r0 = last ? b : d, r1 = last ? a : b, r2 = last ? y3[cc] : a;
+                    double yc = clamp_end(r0, r1, r2, xa, xb, xc, env, false);
+                    if (last) b = yc; else d = yc;
+                }
+                const unsigned at = (unsigned)c * (unsigned)ed.Ecap + (unsigned)i;
+                yk[at] = b;
+                if (interior) {
+                    double sl = (d - b) * ihi;
+                    double slm = (b - a) * ihm;
+                    double r = 3.0 * (sl - slm);
+                    if (rscale != 0.0) r = r * rscale;
+                    ys[at] = r;
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -730,15 +766,49 @@ __global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ e
     }
 }
 
-// Phase 3 + coefficients: for every interval j of every channel, M_j and
-// M_{j+1} (knot_M: phase 3 inside blocks, separator values, end values) and the spline
-// record (a, b, c, d) of memReducedEpochs_dev.cu:775-793 with the shared
-// reciprocals.  Lanes run over consecutive intervals of one channel, so the
-// record rows are written coalesced; each item is 256*kCoefItems (channel,
-// interval) pairs, four per thread at a time with their loads issued together.
+// Phase 3 + coefficients: for every knot of every channel, M (phase-3
+// combination inside blocks, separator values, end values), written in place of
+// y; or, where the evaluation reads records, the spline record (a, b, c, d) of
+// memReducedEpochs_dev.cu:775-793 with the shared reciprocals.  A work item is
+// 256 * kCoefItems (channel, knot) pairs of one (direction, envelope); lanes run
+// over consecutive knots, and the loads of G pairs are issued together.
 
+// M of knot u of one channel row after phase 2, from values loaded up front:
+// knot_M's operations, selected without branches.
+struct KnotMIn {
+    double y, al, be, s0, s1, ss, me;
+};
+
+__device__ __forceinline__ KnotMIn knot_m_load(const double* row, const GPtr<double>& al, const GPtr<double>& be,
+                                                const double* sv, const double* me, int u, int E, int J) {
+    const int b = max(u - 1, 0) >> 5;
+    KnotMIn v;
+    v.y = row[u];
+    v.al = al[u];
+    v.be = be[u];
+    v.s0 = sv[b];
+    v.s1 = sv[min(b + 1, J)];
+    v.ss = sv[min(u >> 5, J)];
+    v.me = me ? me[u == 0 ? 0 : 1] : 0.0;
+    return v;
+}
+
+__device__ __forceinline__ double knot_m_select(const KnotMIn& v, int u, int E, int J) {
+    constexpr int L = kSolverBlock;
+    const int b = max(u - 1, 0) >> 5;
+    double x = v.y;
+    x = (b >= 1) ? fma(-v.s0, v.al, x) : x;
+    x = (b < J) ? fma(-v.s1, v.be, x) : x;
+    const bool sep = u >= L && (u & (L - 1)) == 0 && (u >> 5) <= J;
+    x = sep ? v.ss : x;
+    x = (u == 0 || u == E - 1) ? v.me : x;
+    return x;
+}
+
+template <bool RECORDS>
 __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
+    constexpr int G = 4;   // pairs whose loads are issued together
     const ListCursor cu = list_cursor(wl, kListCoef);
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
@@ -748,74 +818,79 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const int E = ed.E[ke];
         const int J = (E - 3) / L;
         const int base = it.z * 256 * kCoefItems;
-        const double* al = fac_row(ed, ke, kFacAl);
-        const double* be = fac_row(ed, ke, kFacBe);
-        if (!ed.coef_records) {
-            // M-only: the second derivative of every knot row, in place of y
+        const GPtr<double> al = gptr(fac_row(ed, ke, kFacAl));
+        const GPtr<double> be = gptr(fac_row(ed, ke, kFacBe));
+        const size_t kerow = (size_t)ke * n;
+        if (!RECORDS) {
+            // M-only: the second derivative of every knot row, in place of y.  Each pair is
+            // read and written by one lane only, so the in-place update has no hazard.
             const int items = n * E;
-            for (int r0 = 0; r0 < kCoefItems; r0 += 4) {
-                double m[4];
-                int cs[4], us[4];
-                bool ok[4];
+            for (int r0 = 0; r0 < kCoefItems; r0 += G) {
+                KnotMIn v[G];
+                int cs[G], us[G];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    int idx = base + (r0 + a) * 256 + threadIdx.x;
-                    ok[a] = idx < items;
-                    int id2 = ok[a] ? idx : 0;
-                    cs[a] = id2 / E;
-                    us[a] = id2 - cs[a] * E;
-                    const double* row = ed.ysol + ((size_t)ke * n + cs[a]) * ed.Ecap;
-                    const double* sv = ed.sep + ((size_t)ke * n + cs[a]) * ed.nbmax;
-                    const double* me = ed.spline ? ed.mend + ((size_t)ke * n + cs[a]) * 2 : nullptr;
-                    m[a] = knot_M(row, sv, al, be, us[a], E, J, me);
+                for (int a = 0; a < G; ++a) {
+                    const int idx = min(base + (r0 + a) * 256 + (int)threadIdx.x, items - 1);
+                    cs[a] = idx / E;
+                    us[a] = idx - cs[a] * E;
+                    const size_t rr = kerow + cs[a];
+                    v[a] = knot_m_load(ed.ysol + rr * ed.Ecap, al, be, ed.sep + rr * ed.nbmax,
+                                       ed.spline ? ed.mend + rr * 2 : nullptr, us[a], E, J);
                 }
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
-                    if (ok[a]) ed.ysol[((size_t)ke * n + cs[a]) * ed.Ecap + us[a]] = m[a];
+                for (int a = 0; a < G; ++a) {
+                    const int idx = base + (r0 + a) * 256 + (int)threadIdx.x;
+                    if (idx < items) ed.ysol[(kerow + cs[a]) * ed.Ecap + us[a]] = knot_m_select(v[a], us[a], E, J);
+                }
             }
             continue;
         }
+        // records (a, b, c, d) per interval (memReducedEpochs_dev.cu:775-793) with the
+        // shared reciprocals
         const int nI = E - 1;
         const int items = n * nI;
-        const int* X = ed.X + (size_t)ke * ed.Ecap;
-        const double* ihv = fac_row(ed, ke, kFacIh);
-        const double* i3hv = fac_row(ed, ke, kFacI3h);
+        const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
+        const GPtr<double> ihv = gptr(fac_row(ed, ke, kFacIh));
+        const GPtr<double> i3hv = gptr(fac_row(ed, ke, kFacI3h));
         double4* out = reinterpret_cast<double4*>(ed.coef);
-        for (int r0 = 0; r0 < kCoefItems; r0 += 4) {
-            double y0[4], y1[4], m0[4], m1[4], h[4], ih[4], i3h[4];
-            int cs[4], js[4];
-            bool ok[4];
+        for (int r0 = 0; r0 < kCoefItems; r0 += G) {
+            KnotMIn v0[G], v1[G];
+            double y0[G], y1[G], ih[G], i3h[G];
+            int cs[G], js[G], x0[G], x1[G];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                int idx = base + (r0 + a) * 256 + threadIdx.x;
-                ok[a] = idx < items;
-                int id2 = ok[a] ? idx : 0;
-                int c = id2 / nI;
-                int j = id2 - c * nI;
+            for (int a = 0; a < G; ++a) {
+                const int idx = min(base + (r0 + a) * 256 + (int)threadIdx.x, items - 1);
+                const int c = idx / nI;
+                const int j = idx - c * nI;
                 cs[a] = c;
                 js[a] = j;
-                const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
-                const double* yk = ed.yknot + ((size_t)ke * n + c) * ed.Ecap;
-                const double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
-                const double* me = ed.spline ? ed.mend + ((size_t)ke * n + c) * 2 : nullptr;
-                y0[a] = yk[j];
-                y1[a] = yk[j + 1];
-                m0[a] = knot_M(row, sv, al, be, j, E, J, me);
-                m1[a] = knot_M(row, sv, al, be, j + 1, E, J, me);
-                h[a] = (double)(X[j + 1] - X[j]);
+                const size_t rr = kerow + c;
+                const double* row = ed.ysol + rr * ed.Ecap;
+                const double* sv = ed.sep + rr * ed.nbmax;
+                const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
+                v0[a] = knot_m_load(row, al, be, sv, me, j, E, J);
+                v1[a] = knot_m_load(row, al, be, sv, me, j + 1, E, J);
+                y0[a] = ed.yknot[rr * ed.Ecap + j];
+                y1[a] = ed.yknot[rr * ed.Ecap + j + 1];
+                x0[a] = X[j];
+                x1[a] = X[j + 1];
                 ih[a] = ihv[j];
                 i3h[a] = i3hv[j];
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                if (!ok[a]) continue;
-                double sl = (y1[a] - y0[a]) * ih[a];
+            for (int a = 0; a < G; ++a) {
+                const int idx = base + (r0 + a) * 256 + (int)threadIdx.x;
+                if (idx >= items) continue;
+                const double m0 = knot_m_select(v0[a], js[a], E, J);
+                const double m1 = knot_m_select(v1[a], js[a] + 1, E, J);
+                const double h = (double)(x1[a] - x0[a]);
+                const double sl = (y1[a] - y0[a]) * ih[a];
                 double4 r;
                 r.x = y0[a];
-                r.y = sl - (h[a] * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
-                r.z = m0[a];
-                r.w = (m1[a] - m0[a]) * i3h[a];
-                out[((size_t)ke * n + cs[a]) * ed.Ecap + js[a]] = r;
+                r.y = sl - (h * (2.0 * m0 + m1)) * (1.0 / 3.0);
+                r.z = m0;
+                r.w = (m1 - m0) * i3h[a];
+                out[(kerow + cs[a]) * ed.Ecap + js[a]] = r;
             }
         }
     }
@@ -1402,7 +1477,10 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     case 3: MEMD_LIST_LAUNCH(k_knots, kListKnots, kKnotChunk); break;
     case 4: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
     case 5: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
-    case 6: MEMD_LIST_LAUNCH(k_coef, kListCoef, 256); break;
+    case 6:
+        if (p.coef_records) MEMD_LIST_LAUNCH(k_coef<true>, kListCoef, 256);
+        else MEMD_LIST_LAUNCH(k_coef<false>, kListCoef, 256);
+        break;
     case 7:
         if (p.mode == kModeStop)
             MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
