@@ -156,11 +156,11 @@ memd_status memd_run_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t c
  * launches and HIP events between kernels, and returns the average device time
  * per launch of every kernel plus the algorithmic work counters the roofline
  * figures are computed from.  Results are the same as memd_run_batch_f64's.
- * Kernel slots: 0 flags, 1 compact, 2 factor, 3 knots, 4 block, 5 reduced,
- * 6 coef, 7 eval (stop mode: eval_stop), 8 advance (stop mode: stop_decide),
- * 9 stop_apply (stop mode only), 10 init.
+ * Kernel slots: 0 flags, 1 compact, 2 factor, 3 block (knot values + block
+ * sweeps), 4 reduced, 5 coef, 6 eval (stop mode: eval_stop), 7 advance (stop
+ * mode: stop_decide), 8 stop_apply (stop mode only), 9 init.
  */
-#define MEMD_NUM_KERNELS 11
+#define MEMD_NUM_KERNELS 10
 typedef struct memd_kernel_profile {
     double ms_per_launch[MEMD_NUM_KERNELS];
     int64_t launches[MEMD_NUM_KERNELS];

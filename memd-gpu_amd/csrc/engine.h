@@ -28,10 +28,13 @@ constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
 constexpr int kFlagLdsMaxCh = 64;
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
-constexpr int kKnotChunk = 256;
 // Coefficient kernel: items (channel, interval) per thread.
+// Solver blocks per k_block work item: at most 8 (255 rows), and at most 256
+// (block, channel) lanes.  The LDS tile holds rows p0-1 .. qlast+2 of every
+// channel with an odd row stride: n * ((R + 3) | 1) <= kBlockTile doubles.
+__host__ __device__ inline int blocks_per_item(int n) { return (256 / n) < 8 ? (256 / n) : 8; }
+constexpr int kBlockTile = 256 * 35;
 constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
-constexpr int kKnotCB = 8;         // channels whose gathers k_knots issues together
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;
 enum : int { kFacRd = 0, kFacCp = 1, kFacAl = 2, kFacBe = 3, kFacIh = 4, kFacI3h = 5 };
@@ -101,7 +104,7 @@ enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kNumMaps };
 
 // List-driven launches: k_compact appends the exact work items (epoch, ke, sub)
 // of every valid direction; persistent grids drain the lists.
-enum : int { kListFactor = 0, kListKnots, kListBlock, kListReduced, kListCoef, kNumLists };
+enum : int { kListFactor = 0, kListBlock, kListReduced, kListCoef, kNumLists };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup id mod 8).
 // With >= kXcds epochs in a batch, every epoch is pinned to XCD (epoch mod 8):
@@ -135,10 +138,9 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant);
 // Worst-case work items of one epoch in a list.
 long list_capacity(int list, const EpochDesc& e);
 // Kernel slots of one sift step (profiling order): flags, compact, factor,
-// knots, block, reduced, coef, eval | eval_stop, advance | stop_decide,
-// stop_apply (stop mode only); slot kNumStepKernels is the one-off init.
-constexpr int kNumStepKernels = 10;
-int solver_blocks_per_wg(int n);
+// block, reduced, coef, eval | eval_stop, advance | stop_decide, stop_apply
+// (stop mode only); slot kNumStepKernels is the one-off init.
+constexpr int kNumStepKernels = 9;
 int knot_capacity(int T);
 int solver_block_capacity(int Ecap);
 

@@ -331,9 +331,8 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
 // Work items one direction-envelope with E knots contributes to each solver list.
 __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     const int J = (E - 3) / kSolverBlock;
-    const int bpw = 256 / ed.n;
+    const int bpw = blocks_per_item(ed.n);
     m[kListFactor] = 1;
-    m[kListKnots] = (E + kKnotChunk - 1) / kKnotChunk;
     m[kListBlock] = (J + 1 + bpw - 1) / bpw;
     m[kListReduced] = (J > 0 || ed.spline) ? 1 : 0;   // not-a-knot: end values
     m[kListCoef] = (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems));
@@ -555,146 +554,146 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
     return take ? tv : ye;
 }
 
-// Knot values and right-hand sides, per (direction, envelope) and 256-knot
-// chunk; lanes run over knots.  Each lane loads its knot's position and its two
-// neighbours' once, then issues the gathers of kKnotCB channels together:
-// Y_i (end knots clamped) and r_i = 3 (sl_i - sl_{i-1}) with sl_i = (Y_{i+1} - Y_i) * ih_i.
-__global__ __launch_bounds__(kKnotChunk) void k_knots(const EpochDesc* __restrict__ eps, WorkLists wl) {
-    constexpr int CB = kKnotCB;
-    const ListCursor cu = list_cursor(wl, kListKnots);
+// Knot values, right-hand sides and phase 1, per (direction, envelope) and group
+// of blocks_per_item(n) solver blocks (rows p0..qlast, at most 256).
+//   1. Lanes over rows stage the group's shared factors and reciprocals, and
+//      gather the knot values Y of rows p0-1 .. qlast+2 of every channel into an
+//      LDS tile (end knots clamped, memReducedEpochs_dev.cu:447-486, 545-584);
+//      Y of rows p0 .. qlast+1 is written to yknot (coalesced).
+//   2. Lanes (block, channel) form r_u = 3 (sl_u - sl_{u-1}), sl_u = (Y_{u+1} -
+//      Y_u) * ih_u from the tile and run the block's Thomas sweeps in registers
+//      (y = A_b^{-1} r), in place in the tile; the separator row after a block
+//      keeps r, stored directly for the reduced system.
+//   3. Lanes over rows store the block rows' y (coalesced).
+// The right-hand side of the separator after a block reads the next block's
+// first row, so every lane finishes reading (2a) before any lane writes (2b).
+__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
+    constexpr int L = kSolverBlock;
+    constexpr int CB = 8;   // channels whose gathers are issued together
+    __shared__ double sY[kBlockTile];
+    __shared__ double s_rd[256], s_cp[256], s_lo[256], s_ih[257];
+    const ListCursor cu = list_cursor(wl, kListBlock);
+    const int tid = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const EpochState* st = ed.st;
         const int ke = it.y, env = ke & 1;
-        const int E = ed.E[ke];
-        const int i = it.z * kKnotChunk + (int)threadIdx.x;
-        if (i >= E) continue;   // no barriers in this kernel
         const int n = ed.n, T = ed.T;
+        const int E = ed.E[ke];
+        const int J = block_geom(E, 0).J;
+        const int bpw = blocks_per_item(n);
+        const int b0 = it.z * bpw;
+        const int nb = min(bpw, J - b0 + 1);
+        const int p0 = b0 * L + 1;
+        const int qlast = block_geom(E, b0 + nb - 1).q;
+        const int R = qlast - p0 + 1;             // rows of the group, <= 256
+        const int RS = (R + 3) | 1;               // LDS row stride (odd: conflict-free columns)
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
-        const GPtr<double> ihv = gptr(fac_row(ed, ke, kFacIh));
         const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
-        const int im = max(i - 1, 0), ip = min(i + 1, E - 1);
-        const int xm = X[im], x0 = X[i], xp = X[ip];
-        const bool interior = i >= 1 && i <= E - 2;
-        const double ihi = interior ? (double)ihv[i] : 0.0;
-        const double ihm = interior ? (double)ihv[i - 1] : 0.0;
-        // not-a-knot: rows 1 and N = E-2 are scaled by h_1 and h_{N-1}
-        double rscale = 0.0;
-        if (ed.spline && E >= 4) {
-            if (i == 1) rscale = (double)(xp - x0);
-            else if (i == E - 2) rscale = (double)(x0 - xm);
+        __syncthreads();   // the previous item's tile is no longer read
+        // 1a. factors of rows p0..qlast, reciprocals of intervals p0-1..qlast
+        if (tid < R) {   // R <= 256
+            const int u = p0 + tid;
+            const double rd = gptr(fac_row(ed, ke, kFacRd))[u], cpv = gptr(fac_row(ed, ke, kFacCp))[u];
+            const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
+            double lo, di, up;
+            sys_row3(xm, x0, xp, u, E, ed.spline, lo, di, up);
+            s_rd[tid] = rd;
+            s_cp[tid] = cpv;
+            s_lo[tid] = lo;
         }
-        // lanes whose values involve a clamped end knot (E >= 6): i in {0, 1, E-2, E-1}
-        const bool lend = E >= 6 && i <= 1, rend = E >= 6 && i >= E - 2;
-        int xa = 0, xb = 0, xc = 0;   // X[0..2] or X[E-1], X[E-2], X[E-3]
-        if (lend) { xa = X[0]; xb = X[1]; xc = X[2]; }
-        if (rend) { xa = X[E - 1]; xb = X[E - 2]; xc = X[E - 3]; }
-        for (int c0 = 0; c0 < n; c0 += CB) {
-            double ym[CB], y0[CB], yp[CB], y3[CB];
+        for (int r = tid; r <= R; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + r];   // R + 1 <= 257
+        // 1b. knot values of rows p0-1 .. qlast+2 (tile row r = u - p0 + 1)
+        for (int r = tid; r < R + 3; r += 256) {
+            const int u = min(p0 - 1 + r, E - 1);
+            const int xu = X[u];
+            const bool lclamp = E >= 6 && u == 0, rclamp = E >= 6 && u == E - 1;
+            int xb = 0, xc = 0;
+            if (lclamp) { xb = X[1]; xc = X[2]; }
+            if (rclamp) { xb = X[E - 2]; xc = X[E - 3]; }
+            const bool write_yk = (r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0);
+            for (int c0 = 0; c0 < n; c0 += CB) {
+                double v[CB], v1[CB], v2[CB];
 #pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const unsigned base = (unsigned)min(c0 + cc, n - 1) * (unsigned)T;
-                ym[cc] = cur[base + xm];
-                y0[cc] = cur[base + x0];
-                yp[cc] = cur[base + xp];
-                y3[cc] = 0.0;
-                if (lend || rend) y3[cc] = cur[base + xc];   // raw value two knots in from the end
-            }
-#pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const int c = c0 + cc;
-                if (c >= n) break;
-                double a = ym[cc], b = y0[cc], d = yp[cc];
-                if (lend) {
-                    // raw(0), raw(1), raw(2) are (b, d, y3) for i = 0 and (a, b, d) for i = 1
-                    double r0 = (i == 0) ? b : a, r1 = (i == 0) ? d : b, r2 = (i == 0) ? y3[cc] : d;
-                    double yc = clamp_end(r0, r1, r2, xa, xb, xc, env, true);
-                    if (i == 0) b = yc; else a = yc;
-                }
-                if (rend) {
-                    // raw(E-1), raw(E-2), raw(E-3) are (b, a, y3) for i = E-1 and (d, b, a) for i = E-2
-                    const bool last = (i == E - 1);
-                    double r0 = last ? b : d, r1 = last ? a : b, r2 = last ? y3[cc] : a;
-                    double yc = clamp_end(r0, r1, r2, xa, xb, xc, env, false);
-                    if (last) b = yc; else d = yc;
-                }
-                const unsigned at = (unsigned)c * (unsigned)ed.Ecap + (unsigned)i;
-                yk[at] = b;
-                if (interior) {
-                    double sl = (d - b) * ihi;
-                    double slm = (b - a) * ihm;
-                    double r = 3.0 * (sl - slm);
-                    if (rscale != 0.0) r = r * rscale;
-                    ys[at] = r;
-                }
-            }
-        }
-    }
-}
-
-// Phase 1: y = A_b^{-1} r per (block, channel), fma chains from registers.
-// The block factors (rd, cp, lo) are shared by the block's channels and staged
-// in LDS; each lane loads its block's right-hand sides up front.  Rows of
-// separators keep their r for the reduced system.
-__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
-    constexpr int L = kSolverBlock;
-    __shared__ double s_rd[8][L], s_cp[8][L], s_lo[8][L];
-    const ListCursor cu = list_cursor(wl, kListBlock);
-    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
-        const int4 it = cu.base[w];
-        const EpochDesc& ed = eps[it.x];
-        const int ke = it.y;
-        const int n = ed.n;
-        const int bpw = 256 / n;
-        const int E = ed.E[ke];
-        const int J = block_geom(E, 0).J;
-        const int b0 = it.z * bpw;
-        const int nb = min(bpw, J - b0 + 1);
-        const int* X = ed.X + (size_t)ke * ed.Ecap;
-        const double* rdv = fac_row(ed, ke, kFacRd);
-        const double* cpv = fac_row(ed, ke, kFacCp);
-        for (int pass0 = 0; pass0 < nb; pass0 += 8) {
-            int npass = min(8, nb - pass0);
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < npass * L; idx += 256) {
-                int sb = idx / L, i = idx - sb * L;
-                BlockGeom g = block_geom(E, b0 + pass0 + sb);
-                int u = min(g.p + i, g.q);
-                s_rd[sb][i] = rdv[u];
-                s_cp[sb][i] = cpv[u];
-                double lo, di, up;
-                sys_row(X, u, E, ed.spline, lo, di, up);
-                s_lo[sb][i] = lo;
-            }
-            __syncthreads();
-            int slot = threadIdx.x / n;
-            int c = threadIdx.x - slot * n;
-            int sb = slot - pass0;
-            if (slot < bpw && sb >= 0 && sb < npass) {
-                BlockGeom g = block_geom(E, b0 + slot);
-                const int p = g.p, q = g.q;
-                double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
-                double z[L];
-#pragma unroll
-                for (int i = 0; i < L; ++i) z[i] = row[min(p + i, q)];
-#pragma unroll
-                for (int i = 0; i < L; ++i)
-                    z[i] = (i == 0) ? z[0] * s_rd[sb][0] : fma(-s_lo[sb][i], z[i > 0 ? i - 1 : 0], z[i]) * s_rd[sb][i];
-                double x = 0.0;
-#pragma unroll
-                for (int i = L - 1; i >= 0; --i) {
-                    int u = p + i;
-                    if (u <= q) {
-                        x = (u == q) ? z[i] : fma(-s_cp[sb][i], x, z[i]);
-                        row[u] = x;
+                for (int cc = 0; cc < CB; ++cc) {
+                    const unsigned base = (unsigned)min(c0 + cc, n - 1) * (unsigned)T;
+                    v[cc] = cur[base + xu];
+                    v1[cc] = 0.0;
+                    v2[cc] = 0.0;
+                    if (lclamp || rclamp) {
+                        v1[cc] = cur[base + xb];
+                        v2[cc] = cur[base + xc];
                     }
+                }
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const int c = c0 + cc;
+                    if (c >= n) break;
+                    double y = v[cc];
+                    if (lclamp || rclamp) y = clamp_end(y, v1[cc], v2[cc], xu, xb, xc, env, lclamp);
+                    sY[c * RS + r] = y;
+                    if (write_yk) yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
                 }
             }
         }
         __syncthreads();
+        // 2a. per (block, channel): right-hand sides and the forward sweep (reads only;
+        //     the separator after a block reads the next block's first row)
+        const int slot = tid / n;
+        const int c = tid - slot * n;
+        const bool active = slot < nb;
+        const BlockGeom g = block_geom(E, b0 + (active ? slot : 0));
+        const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
+        double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
+        double z[L];
+        double rsep = 0.0;
+        const bool has_sep = active && b0 + slot < J;
+        if (active) {
+            double zp = 0.0;
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                if (i < len) {
+                    const int u = p + i;
+                    const double sl = (Yc[i + 2] - Yc[i + 1]) * s_ih[off + i + 1];
+                    const double slm = (Yc[i + 1] - Yc[i]) * s_ih[off + i];
+                    double r = 3.0 * (sl - slm);
+                    if (ed.spline && E >= 4) {   // not-a-knot rows 1 and N = E-2
+                        if (u == 1) r = r * (double)(X[2] - X[1]);
+                        else if (u == E - 2) r = r * (double)(X[E - 2] - X[E - 3]);
+                    }
+                    zp = (i == 0) ? r * s_rd[off] : fma(-s_lo[off + i], zp, r) * s_rd[off + i];
+                    z[i] = zp;
+                }
+            }
+            if (has_sep) {   // separator row q+1 keeps its right-hand side
+                const double sl = (Yc[len + 2] - Yc[len + 1]) * (double)gptr(fac_row(ed, ke, kFacIh))[q + 1];
+                const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
+                rsep = 3.0 * (sl - slm);
+            }
+        }
+        __syncthreads();
+        // 2b. backward sweep, y in place of the block's rows in the tile
+        if (active) {
+            if (has_sep) ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = rsep;
+            double x = 0.0;
+#pragma unroll
+            for (int i = L - 1; i >= 0; --i) {
+                if (i < len) {
+                    x = (i == len - 1) ? z[i] : fma(-s_cp[off + i], x, z[i]);
+                    Yc[i + 1] = x;
+                }
+            }
+        }
+        __syncthreads();
+        // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped)
+        for (int r = tid; r < R; r += 256) {
+            const int u = p0 + r;
+            if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;   // separator (row N = 32(J+1) is not one)
+            for (int c2 = 0; c2 < n; ++c2) ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = sY[c2 * RS + r + 1];
+        }
     }
 }
 
@@ -733,18 +732,55 @@ __global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ e
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
             const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
             double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
+            const GPtr<double> rowg = gptr(row);
+            const GMut<double> svg = gmut(sv);
+            const GPtr<double> Rlog = gptr(Rlo), Rrdg = gptr(Rrd), Rcpg = gptr(Rcp);
+            const GPtr<int> Xg = gptr(X);
+            // forward sweep over the separators, inputs of RC separators loaded together
+            constexpr int RC = 8;
             double z = 0.0;
-            for (int j = 1; j <= J; ++j) {
-                int s = j * L;
-                double lo = (double)(X[s] - X[s - 1]), up = (double)(X[s + 1] - X[s]);
-                double D = fma(-up, row[s + 1], fma(-lo, row[s - 1], row[s]));
-                z = (j == 1) ? D * Rrd[j] : fma(-Rlo[j], z, D) * Rrd[j];
-                sv[j] = z;
+            for (int j0 = 1; j0 <= J; j0 += RC) {
+                double ym[RC], yc[RC], yp[RC], rl[RC], rr[RC];
+                int xm[RC], x0[RC], xp[RC];
+#pragma unroll
+                for (int a = 0; a < RC; ++a) {
+                    const int j = min(j0 + a, J), s0 = j * L;
+                    ym[a] = rowg[s0 - 1];
+                    yc[a] = rowg[s0];
+                    yp[a] = rowg[s0 + 1];
+                    rl[a] = Rlog[j];
+                    rr[a] = Rrdg[j];
+                    xm[a] = Xg[s0 - 1];
+                    x0[a] = Xg[s0];
+                    xp[a] = Xg[s0 + 1];
+                }
+#pragma unroll
+                for (int a = 0; a < RC; ++a) {
+                    const int j = j0 + a;
+                    if (j > J) break;
+                    const double lo = (double)(x0[a] - xm[a]), up = (double)(xp[a] - x0[a]);
+                    const double D = fma(-up, yp[a], fma(-lo, ym[a], yc[a]));
+                    z = (j == 1) ? D * rr[a] : fma(-rl[a], z, D) * rr[a];
+                    svg[j] = z;
+                }
             }
+            // back substitution, RC separators at a time
             double x = z;
-            for (int j = J - 1; j >= 1; --j) {
-                x = fma(-Rcp[j], x, sv[j]);
-                sv[j] = x;
+            for (int j1 = J - 1; j1 >= 1; j1 -= RC) {
+                double cpj[RC], zj[RC];
+#pragma unroll
+                for (int a = 0; a < RC; ++a) {
+                    const int j = max(j1 - a, 1);
+                    cpj[a] = Rcpg[j];
+                    zj[a] = svg[j];
+                }
+#pragma unroll
+                for (int a = 0; a < RC; ++a) {
+                    const int j = j1 - a;
+                    if (j < 1) break;
+                    x = fma(-cpj[a], x, zj[a]);
+                    svg[j] = x;
+                }
             }
             if (ed.spline) {
                 // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2), read
@@ -1397,7 +1433,6 @@ int solver_block_capacity(int Ecap) {
     int N = Ecap - 2;
     return (N >= 1) ? (N - 1) / kSolverBlock + 1 : 1;
 }
-int solver_blocks_per_wg(int n) { return 256 / n; }
 
 int blocks_for(int map, const EpochDesc& e, int eval_variant) {
     switch (map) {
@@ -1417,10 +1452,9 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
 long list_capacity(int list, const EpochDesc& e) {
     const long K2 = 2L * e.K;
     const long J = (e.Ecap - 3) / kSolverBlock;
-    const long bpw = solver_blocks_per_wg(e.n);
+    const long bpw = blocks_per_item(e.n);
     switch (list) {
     case kListFactor: return K2;
-    case kListKnots: return K2 * ((e.Ecap + kKnotChunk - 1) / kKnotChunk);
     case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
     case kListReduced: return K2;
     case kListCoef: return K2 * (((long)e.n * e.Ecap + 256L * kCoefItems - 1) / (256L * kCoefItems));
@@ -1474,20 +1508,19 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(256), 0, s, p.eps, p.tab[kMapCompact], p.wl);
         break;
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
-    case 3: MEMD_LIST_LAUNCH(k_knots, kListKnots, kKnotChunk); break;
-    case 4: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
-    case 5: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
-    case 6:
+    case 3: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
+    case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
+    case 5:
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef<true>, kListCoef, 256);
         else MEMD_LIST_LAUNCH(k_coef<false>, kListCoef, 256);
         break;
-    case 7:
+    case 6:
         if (p.mode == kModeStop)
             MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
         else
             launch_eval(p, s);
         break;
-    case 8:
+    case 7:
         if (p.mode == kModeStop)
             hipLaunchKernelGGL(k_stop_decide, dim3(p.count), dim3(256), 0, s, p.eps, p.count);
         else

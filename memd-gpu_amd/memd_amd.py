@@ -49,13 +49,13 @@ class MemdEpoch(ctypes.Structure):
 
 
 class MemdKernelProfile(ctypes.Structure):
-    _fields_ = [("ms_per_launch", ctypes.c_double * 11), ("launches", ctypes.c_int64 * 11),
+    _fields_ = [("ms_per_launch", ctypes.c_double * 10), ("launches", ctypes.c_int64 * 10),
                 ("steps", ctypes.c_int64), ("knots", ctypes.c_int64), ("knot_channels", ctypes.c_int64),
                 ("eval_points", ctypes.c_int64), ("samples", ctypes.c_int64)]
 
 
 #: kernel order of MemdKernelProfile.ms_per_launch
-KERNEL_NAMES = ("k_flags", "k_compact", "k_factor", "k_knots", "k_block", "k_reduced", "k_coef", "k_eval", "k_advance",
+KERNEL_NAMES = ("k_flags", "k_compact", "k_factor", "k_block", "k_reduced", "k_coef", "k_eval", "k_advance",
                 "k_stop_apply", "k_init")
 
 
