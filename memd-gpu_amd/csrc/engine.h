@@ -27,6 +27,8 @@ constexpr int kEvalMaxInt = 36;   // >= intervals meeting a 64-sample tile (<= 3
 constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
 constexpr int kFlagLdsMaxCh = 64;
+constexpr int kFlagDirs = 8;       // directions per k_flags pass
+constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
 // Coefficient kernel: items (channel, interval) per thread.
 // Solver blocks per k_block work item: at most 8 (255 rows), and at most 256

@@ -109,6 +109,14 @@ template <class T>
 __device__ __forceinline__ GPtr<T> gptr(const T* p) {
     return (GPtr<T>)p;
 }
+// Read-only data every lane of a wave reads at the same address (direction
+// vectors): the constant address space lets uniform loads use the scalar cache.
+template <class T>
+using CPtr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ CPtr<T> cptr(const T* p) {
+    return (CPtr<T>)p;
+}
 template <class T>
 using GMut = __attribute__((address_space(1))) T*;
 template <class T>
@@ -272,57 +280,106 @@ __global__ __launch_bounds__(kFlagTile) void k_flags_global(const EpochDesc* __r
 // every direction's projection, extremum tests and ballots run from LDS.
 __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict__ eps,
                                                      const int2* __restrict__ tab) {
-    extern __shared__ double curT[];   // [n][kFlagTile + 2]
-    __shared__ double Ps[kFlagTile + 2];
+    constexpr int TW = kFlagTile + 2;
+    constexpr int KD = kFlagDirs;   // directions per pass: independent fma chains per thread
+    constexpr int NW = kFlagTile / 64;
+    extern __shared__ double curT[];   // [n][TW]: samples t0-1 .. t0+kFlagTile
+    __shared__ double Pe[KD][NW][2];      // each wave's first and last projection
+    __shared__ double Ph[kFlagHalo][2];   // projections at t0-1 and t0+kFlagTile
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
     const int n = ed.n, T = ed.T, K = ed.K;
-    constexpr int TW = kFlagTile + 2;
     const int tile = br.y;
     const int t0 = tile * kFlagTile;
-    const double* cur = cur_slot(ed, st->imf);
+    const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
+    const CPtr<double> V = cptr(ed.V);
     const int tid = threadIdx.x;
     for (int idx = tid; idx < n * TW; idx += kFlagTile) {
-        int c = idx / TW, i = idx - c * TW;
-        int t = t0 - 1 + i;
-        curT[idx] = (t >= 0 && t < T) ? cur[(size_t)c * T + t] : 0.0;
+        const int c = idx / TW, i = idx - c * TW;
+        const int t = t0 - 1 + i;
+        curT[idx] = (t >= 0 && t < T) ? (double)cur[(unsigned)c * (unsigned)T + (unsigned)t] : 0.0;
     }
-    __syncthreads();
     const int lim = window_limit(T, ed.window);
     const int t = t0 + tid;
     const int lane = tid & 63;
     const int w = (t0 >> 6) + (tid >> 6);
-    for (int k = 0; k < K; ++k) {
-        const double* Vk = ed.V + (size_t)k * n;
-        // positions t0-1+i for i = tid and (tid < 2) i = 256 + tid
-        double acc = 0.0;
-        for (int c = 0; c < n; ++c) acc = fma(Vk[c], curT[c * TW + tid], acc);
-        Ps[tid] = acc;
-        if (tid < 2) {
-            double acc2 = 0.0;
-            for (int c = 0; c < n; ++c) acc2 = fma(Vk[c], curT[c * TW + kFlagTile + tid], acc2);
-            Ps[kFlagTile + tid] = acc2;
-        }
+    for (int kh = 0; kh < K; kh += kFlagHalo) {
+        const int khn = min(kFlagHalo, K - kh);
         __syncthreads();
-        bool fmax = false, fmin = false;
-        if (t < T) {
-            if (t == 0 || t == T - 1) {
-                fmax = fmin = true;
-            } else if (t <= lim) {
-                double p = Ps[tid + 1], a = Ps[tid], b = Ps[tid + 2];
-                fmax = (p > a) && (p > b);
-                fmin = (p < a) && (p < b);
+        // halo projections of the pass's directions, one chain per (direction, side)
+        if (tid < 2 * khn) {
+            const int kk = tid >> 1, side = tid & 1;
+            const int col = side ? TW - 1 : 0;
+            const unsigned vb = (unsigned)(kh + kk) * (unsigned)n;
+            double acc = 0.0;
+            for (int c = 0; c < n; ++c) acc = fma((double)V[vb + c], curT[c * TW + col], acc);
+            Ph[kk][side] = acc;
+        }
+        for (int k0 = kh; k0 < kh + khn; k0 += KD) {
+            const int kd = min(KD, kh + khn - k0);
+            // the tile's projections for kd directions: every curT read feeds kd chains,
+            // each chain in ascending channel order
+            double acc[KD];
+            unsigned vrow[KD];   // chains past the last direction repeat it and are ignored
+#pragma unroll
+            for (int d = 0; d < KD; ++d) {
+                acc[d] = 0.0;
+                vrow[d] = (unsigned)min(k0 + d, K - 1) * (unsigned)n;
+            }
+            int c = 0;
+            for (; c + 4 <= n; c += 4) {
+                const double x0 = curT[(c + 0) * TW + tid + 1], x1 = curT[(c + 1) * TW + tid + 1];
+                const double x2 = curT[(c + 2) * TW + tid + 1], x3 = curT[(c + 3) * TW + tid + 1];
+#pragma unroll
+                for (int d = 0; d < KD; ++d) {
+                    const CPtr<double> v = V + vrow[d] + c;
+                    acc[d] = fma((double)v[0], x0, acc[d]);
+                    acc[d] = fma((double)v[1], x1, acc[d]);
+                    acc[d] = fma((double)v[2], x2, acc[d]);
+                    acc[d] = fma((double)v[3], x3, acc[d]);
+                }
+            }
+            for (; c < n; ++c) {
+                const double x = curT[c * TW + tid + 1];
+#pragma unroll
+                for (int d = 0; d < KD; ++d) acc[d] = fma((double)V[vrow[d] + c], x, acc[d]);
+            }
+            const int wv = tid >> 6;
+            __syncthreads();   // Pe of the previous pass has been read; Ph is complete
+            if (lane == 0 || lane == 63) {
+#pragma unroll
+                for (int d = 0; d < KD; ++d) Pe[d][wv][lane == 63 ? 1 : 0] = acc[d];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int d = 0; d < KD; ++d) {
+                if (d >= kd) break;
+                const int k = k0 + d;
+                // neighbours: adjacent lanes; across waves and tiles through LDS
+                const double p = acc[d];
+                double a = __shfl_up(p, 1);
+                double b = __shfl_down(p, 1);
+                if (lane == 0) a = wv > 0 ? Pe[d][wv - 1][1] : Ph[k - kh][0];
+                if (lane == 63) b = wv < NW - 1 ? Pe[d][wv + 1][0] : Ph[k - kh][1];
+                bool fmax = false, fmin = false;
+                if (t < T) {
+                    if (t == 0 || t == T - 1) {
+                        fmax = fmin = true;
+                    } else if (t <= lim) {
+                        fmax = (p > a) && (p > b);
+                        fmin = (p < a) && (p < b);
+                    }
+                }
+                const unsigned long long mmax = __ballot(fmax);
+                const unsigned long long mmin = __ballot(fmin);
+                if (lane == 0 && w < ed.W) {
+                    ed.mask[((size_t)k * 2 + 0) * ed.W + w] = mmax;
+                    ed.mask[((size_t)k * 2 + 1) * ed.W + w] = mmin;
+                }
             }
         }
-        unsigned long long mmax = __ballot(fmax);
-        unsigned long long mmin = __ballot(fmin);
-        if (lane == 0 && w < ed.W) {
-            ed.mask[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-            ed.mask[((size_t)k * 2 + 1) * ed.W + w] = mmin;
-        }
-        __syncthreads();
     }
 }
 
@@ -396,12 +453,17 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
     if (!(sE[0] > 3 && sE[1] > 3)) return;
     __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
     const int g = wl.groups > 1 ? ed.xcd : 0;
-    if (tid < 2) {
-        int m[kNumLists];
-        list_sizes(ed, sE[tid], m);
+    if (tid == 0) {   // one counter update per list for both envelopes
+        int m0[kNumLists], m1[kNumLists];
+        list_sizes(ed, sE[0], m0);
+        list_sizes(ed, sE[1], m1);
         for (int l = 0; l < kNumLists; ++l) {
-            s_m[tid][l] = m[l];
-            s_base[tid][l] = m[l] ? atomicAdd(&wl.count[l * kXcds + g], m[l]) : 0;
+            const int tot = m0[l] + m1[l];
+            const int base = tot ? atomicAdd(&wl.count[l * kXcds + g], tot) : 0;
+            s_m[0][l] = m0[l];
+            s_m[1][l] = m1[l];
+            s_base[0][l] = base;
+            s_base[1][l] = base + m0[l];
         }
     }
     __syncthreads();
