@@ -23,7 +23,10 @@ constexpr int kNumEvalVariants = 7;
 // (no record array in HBM); tiles of 64 samples x kEvalFusedCG channels.
 constexpr int kEvalFused = 6;
 constexpr int kEvalFusedCG = 16;
-constexpr int kEvalMaxInt = 36;   // >= intervals meeting a 64-sample tile (<= 34)
+// Intervals meeting a 64-sample tile: at most 32 knots of one kind lie inside it
+// (strict extrema are never adjacent), so at most 33.  34 also makes the record
+// row stride (34 x 32 B) put a wave's 4 channels on disjoint LDS banks.
+constexpr int kEvalMaxInt = 34;
 constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
 constexpr int kFlagLdsMaxCh = 64;

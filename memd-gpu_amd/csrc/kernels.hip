@@ -1090,24 +1090,63 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, 
     }
 }
 
+// rem / nint == (rem * kDivMagic[nint]) >> 16 exactly for 1 <= nint <= kEvalMaxInt and
+// rem < 2^16 / nint (phase-A items: rem < kEvalFusedCG * kEvalMaxInt = 544).
+struct DivMagicTable {
+    int v[kEvalMaxInt + 1];
+    constexpr DivMagicTable() : v() {
+        for (int i = 1; i <= kEvalMaxInt; ++i) v[i] = (65536 + i - 1) / i;
+    }
+};
+__constant__ DivMagicTable kDivMagic = DivMagicTable();
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Phase B of one envelope for a lane's 4 consecutive samples (interval i of the
+// first): each sample uses the record of X_j < t <= X_{j+1}.  ENV 0 stores the
+// upper envelope, ENV 1 adds the mean of both to the accumulator.
+template <bool FULL, int ENV>
+__device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, const int* __restrict__ xkrow, int i,
+                                          int ts, int tlast, double (&ep)[4], double (&acc)[4]) {
+    int xj = xkrow[i], xn = xkrow[i + 1];
+    double4 r = recrow[i];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int t = ts + s;
+        if (FULL || t <= tlast) {
+            if (t > xn) {
+                ++i;
+                xj = xn;
+                xn = xkrow[i + 1];
+                r = recrow[i];
+            }
+            const double v = horner4(r, (double)(t - xj));
+            if (ENV == 0) ep[s] = v;
+            else acc[s] = acc[s] + (ep[s] + v) * 0.5;
+        }
+    }
+}
+
 // Fused records + evaluation (fix mode).  One workgroup = one 64-sample tile
 // (one flag word) x kEvalFusedCG channels.  For each chunk of 64 directions a
 // prologue stages, per (direction, envelope), the knot count, the tile's flag
-// word and its knot prefix in LDS.  Per valid direction, phase A builds in LDS
-// the spline record (a,b,c,d) of every interval meeting the tile, for both
-// envelopes, from M (written in place by k_coef), the knot values and the
-// shared reciprocals -- the same operations as the record path, so the records
-// are bit-identical; each thread issues the loads of all its items before
-// using any.  Phase B evaluates both envelopes (each lane walks 4 consecutive
-// samples of one channel) and adds the mean in ascending direction order.
+// word and knot prefix, and the tile's interval window [jlo, jlo + nint).  Per
+// valid direction, phase A builds in LDS the spline record (a,b,c,d) of every
+// interval meeting the tile, for both envelopes, from M (written in place by
+// k_coef), the knot values and the shared reciprocals -- the same operations as
+// the record path, so the records are bit-identical -- and the first interval
+// of every 4-sample quad; waves without items skip it.  Phase B evaluates both
+// envelopes (each lane 4 consecutive samples of one channel) and adds the mean
+// in ascending direction order.
 __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, KC = 64;
     constexpr int MI = 3;   // phase-A items per thread per round (rounds repeat while items remain)
-    __shared__ double4 rec[2][CG][NI];
+    __shared__ double4 rec[2][CG][NI];   // row stride NI double4: the wave's 4 channels hit disjoint banks
     __shared__ int xk[2][NI + 1];
-    __shared__ int sE[KC][2], sPre[KC][2];
+    __shared__ int sE[KC][2], sPre[KC][2], sJlo[KC][2], sNint[KC][2];
     __shared__ unsigned long long sMask[KC][2];
+    __shared__ int sQ[2][16];   // first interval (local) of every 4-sample quad
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
@@ -1119,6 +1158,7 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
     const int t0 = tile * TL;
     const int tlast = min(t0 + TL - 1, T - 1);
     const int tid = threadIdx.x;
+    const int wbase = tid & ~63;     // first thread of this wave
     const int q = tid & 15;          // sample quad
     const int cl = tid >> 4;         // channel in group
     const int c = cg * CG + cl;
@@ -1130,36 +1170,39 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
         const int kcn = min(KC, K - kc0);
         __syncthreads();
         if (tid < 2 * kcn) {
-            int kk = tid >> 1, env = tid & 1;
-            int ke = 2 * (kc0 + kk) + env;
-            sE[kk][env] = ed.E[ke];
-            sMask[kk][env] = ed.mask[(size_t)ke * ed.W + tile];
-            sPre[kk][env] = ed.pre[(size_t)ke * ed.W + tile];
+            const int kk = tid >> 1, env = tid & 1;
+            const int ke = 2 * (kc0 + kk) + env;
+            const int E = ed.E[ke];
+            const unsigned long long m = ed.mask[(size_t)ke * ed.W + tile];
+            const int pr = ed.pre[(size_t)ke * ed.W + tile];
+            const int inside = (int)__popcll(m & ((1ull << (tlast - t0)) - 1ull));
+            const int lo = min(max(pr - 1, 0), E - 2);
+            const int hi = min(max(pr + inside - 1, 0), E - 2);
+            sE[kk][env] = E;
+            sMask[kk][env] = m;
+            sPre[kk][env] = pr;
+            sJlo[kk][env] = lo;
+            sNint[kk][env] = hi - lo + 1;
         }
         __syncthreads();
         for (int kk = 0; kk < kcn; ++kk) {
             const int k = kc0 + kk;
-            if (!(sE[kk][0] > 3 && sE[kk][1] > 3)) continue;
+            if (!(uniform(sE[kk][0]) > 3 && uniform(sE[kk][1]) > 3)) continue;
             nv = nv + 1.0;
-            int jlo[2], nint[2], magic[2];
-#pragma unroll
-            for (int env = 0; env < 2; ++env) {
-                const int E = sE[kk][env];
-                const int pr = sPre[kk][env];
-                const unsigned long long m = sMask[kk][env];
-                const int inside = (int)__popcll(m & ((1ull << (tlast - t0)) - 1ull));
-                const int lo = min(max(pr - 1, 0), E - 2);
-                const int hi = min(max(pr + inside - 1, 0), E - 2);
-                jlo[env] = lo;
-                nint[env] = hi - lo + 1;
-                // rem / nint == (rem * magic) >> 16 exactly for rem < 2^16 / nint (rem < 1152, nint <= 37)
-                magic[env] = (65536 + nint[env] - 1) / nint[env];
+            const int jlo[2] = {uniform(sJlo[kk][0]), uniform(sJlo[kk][1])};
+            const int nint[2] = {uniform(sNint[kk][0]), uniform(sNint[kk][1])};
+            const int magic[2] = {kDivMagic.v[nint[0]], kDivMagic.v[nint[1]]};
+            // first interval of every quad (shared by the group's channels)
+            if (tid < 32) {
+                const int env = tid >> 4, qq = tid & 15;
+                const int jj = sPre[kk][env] + (int)__popcll(sMask[kk][env] & ((1ull << (4 * qq)) - 1ull)) - 1;
+                sQ[env][qq] = min(max(jj, 0), sE[kk][env] - 2) - jlo[env];
             }
             // phase A: every item's loads first, then the records.  Element offsets
-            // are 32-bit (n * T <= 2^30) from per-direction bases.
+            // are 32-bit (n * T <= 2^31) from per-direction bases.
             const int per0 = nint[0] * CG;
             const int total = per0 + nint[1] * CG;
-            const int nitems = (total + 255) >> 8;   // uniform: unused item slots issue no loads
+            const int nitems = (total + 255) >> 8;
             const size_t rowbase = (size_t)(2 * k) * n * ed.Ecap;
             const GPtr<double> Mb[2] = {gptr(ed.ysol + rowbase), gptr(ed.ysol + rowbase + (size_t)n * ed.Ecap)};
             const GPtr<double> Yb[2] = {gptr(ed.yknot + rowbase), gptr(ed.yknot + rowbase + (size_t)n * ed.Ecap)};
@@ -1167,85 +1210,61 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
             const GPtr<double> Hb[2] = {gptr(fac_row(ed, 2 * k, kFacIh)), gptr(fac_row(ed, 2 * k + 1, kFacIh))};
             const GPtr<double> H3b[2] = {gptr(fac_row(ed, 2 * k, kFacI3h)), gptr(fac_row(ed, 2 * k + 1, kFacI3h))};
             for (int a0 = 0; a0 < nitems; a0 += MI) {
-            double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
-            int x0[MI], x1[MI], lds[MI];
+                double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
+                int x0[MI], x1[MI], lds[MI];
 #pragma unroll
-            for (int a = 0; a < MI; ++a) {
-                lds[a] = -1;
-                if (a0 + a >= nitems) continue;
-                const int idx = tid + 256 * (a0 + a);
-                const int env = (idx < per0) ? 0 : 1;
-                const int rem = (env == 0) ? idx : idx - per0;
-                const int cl2 = (rem * magic[env]) >> 16;
-                const int i = rem - cl2 * nint[env];
-                const bool ok = idx < total;
-                const int c2 = min(cg * CG + cl2, n - 1);
-                const int j = ok ? jlo[env] + i : 0;
-                const unsigned off = (unsigned)c2 * (unsigned)ed.Ecap + (unsigned)j;
-                x0[a] = Xb[env][j];
-                x1[a] = Xb[env][j + 1];
-                v0[a] = Yb[env][off];
-                v1[a] = Yb[env][off + 1];
-                m0[a] = Mb[env][off];
-                m1[a] = Mb[env][off + 1];
-                ih[a] = Hb[env][j];
-                i3h[a] = H3b[env][j];
-                // LDS slot of the record; bit 30: channel beyond n (ragged group), bit 29: first channel
-                if (ok)
-                    lds[a] = ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) |
-                             (cl2 == 0 ? 0x20000000 : 0);
-            }
-#pragma unroll
-            for (int a = 0; a < MI; ++a) {
-                if (a0 + a >= nitems) continue;
-                const int sl0 = lds[a];
-                if (sl0 < 0) continue;
-                const int sv = sl0 & 0x1fffffff;
-                if (sl0 & 0x20000000) {   // first channel of the group: knot positions
-                    const int env = sv >= CG * NI ? 1 : 0;
-                    const int i = sv - env * CG * NI;
-                    xk[env][i] = x0[a];
-                    if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
+                for (int a = 0; a < MI; ++a) {
+                    lds[a] = -1;
+                    if (a0 + a >= nitems || wbase + 256 * (a0 + a) >= total) continue;   // wave-uniform
+                    const int idx = tid + 256 * (a0 + a);
+                    const int env = (idx < per0) ? 0 : 1;
+                    const int rem = (env == 0) ? idx : idx - per0;
+                    const int cl2 = (rem * magic[env]) >> 16;
+                    const int i = rem - cl2 * nint[env];
+                    const bool ok = idx < total;
+                    const int c2 = min(cg * CG + cl2, n - 1);
+                    const int j = ok ? jlo[env] + i : 0;
+                    const unsigned off = (unsigned)c2 * (unsigned)ed.Ecap + (unsigned)j;
+                    x0[a] = Xb[env][j];
+                    x1[a] = Xb[env][j + 1];
+                    v0[a] = Yb[env][off];
+                    v1[a] = Yb[env][off + 1];
+                    m0[a] = Mb[env][off];
+                    m1[a] = Mb[env][off + 1];
+                    ih[a] = Hb[env][j];
+                    i3h[a] = H3b[env][j];
+                    // LDS slot of the record; bit 30: channel beyond n (ragged group), bit 29: first channel
+                    if (ok)
+                        lds[a] = ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) |
+                                 (cl2 == 0 ? 0x20000000 : 0);
                 }
-                if (sl0 & 0x40000000) continue;
-                double h = (double)(x1[a] - x0[a]);
-                double slp = (v1[a] - v0[a]) * ih[a];
-                double4 r;
-                r.x = v0[a];
-                r.y = slp - (h * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
-                r.z = m0[a];
-                r.w = (m1[a] - m0[a]) * i3h[a];
-                (&rec[0][0][0])[sv] = r;
-            }
+#pragma unroll
+                for (int a = 0; a < MI; ++a) {
+                    const int sl0 = lds[a];
+                    if (sl0 < 0) continue;
+                    const int sv = sl0 & 0x1fffffff;
+                    if (sl0 & 0x20000000) {   // first channel of the group: knot positions
+                        const int env = sv >= CG * NI ? 1 : 0;
+                        const int i = sv - env * CG * NI;
+                        xk[env][i] = x0[a];
+                        if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
+                    }
+                    if (sl0 & 0x40000000) continue;
+                    double h = (double)(x1[a] - x0[a]);
+                    double slp = (v1[a] - v0[a]) * ih[a];
+                    double4 r;
+                    r.x = v0[a];
+                    r.y = slp - (h * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
+                    r.z = m0[a];
+                    r.w = (m1[a] - m0[a]) * i3h[a];
+                    (&rec[0][0][0])[sv] = r;
+                }
             }
             __syncthreads();
             // phase B
             if (c < n && ts <= tlast) {
-#pragma unroll
-                for (int env = 0; env < 2; ++env) {
-                    int E = sE[kk][env];
-                    unsigned long long m = sMask[kk][env];
-                    int jj = sPre[kk][env] + __popcll(m & ((1ull << (ts - t0)) - 1ull)) - 1;
-                    jj = min(max(jj, 0), E - 2);
-                    int i = jj - jlo[env];
-                    int xj = xk[env][i], xn = xk[env][i + 1];
-                    double4 r = rec[env][cl][i];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        int t = ts + s;
-                        if (t <= tlast) {
-                            if (t > xn) {
-                                ++i;
-                                xj = xn;
-                                xn = xk[env][i + 1];
-                                r = rec[env][cl][i];
-                            }
-                            double v = horner4(r, (double)(t - xj));
-                            if (env == 0) ep[s] = v;
-                            else acc[s] = acc[s] + (ep[s] + v) * 0.5;
-                        }
-                    }
-                }
+                eval_quad<false, 0>(rec[0][cl], xk[0], sQ[0][q], ts, tlast, ep, acc);
+                eval_quad<false, 1>(rec[1][cl], xk[1], sQ[1][q], ts, tlast, ep, acc);
             }
             __syncthreads();
         }
