@@ -392,7 +392,7 @@ __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     m[kListFactor] = 1;
     m[kListBlock] = (J + 1 + bpw - 1) / bpw;
     m[kListReduced] = (J > 0 || ed.spline) ? 1 : 0;   // not-a-knot: end values
-    m[kListCoef] = (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems));
+    m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
 }
 
 // One workgroup per (epoch, direction): both envelopes' knot lists, then the
@@ -864,12 +864,13 @@ __global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ e
     }
 }
 
-// Phase 3 + coefficients: for every knot of every channel, M (phase-3
-// combination inside blocks, separator values, end values), written in place of
-// y; or, where the evaluation reads records, the spline record (a, b, c, d) of
-// memReducedEpochs_dev.cu:775-793 with the shared reciprocals.  A work item is
-// 256 * kCoefItems (channel, knot) pairs of one (direction, envelope); lanes run
-// over consecutive knots, and the loads of G pairs are issued together.
+// Phase 3 + coefficients for the record readers (stop mode and the record
+// evaluation variants): the spline record (a, b, c, d) of every interval of every
+// channel (memReducedEpochs_dev.cu:775-793) with the shared reciprocals, from M of
+// its two knots (knot_m_select).  A work item is 256 * kCoefItems (channel,
+// interval) pairs of one (direction, envelope); lanes run over consecutive
+// intervals, and the loads of G pairs are issued together.  The fused fix-mode
+// evaluation forms M the same way inside its record build.
 
 // M of knot u of one channel row after phase 2, from values loaded up front:
 // knot_M's operations, selected without branches.
@@ -903,7 +904,6 @@ __device__ __forceinline__ double knot_m_select(const KnotMIn& v, int u, int E, 
     return x;
 }
 
-template <bool RECORDS>
 __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
     constexpr int G = 4;   // pairs whose loads are issued together
@@ -919,30 +919,14 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const GPtr<double> al = gptr(fac_row(ed, ke, kFacAl));
         const GPtr<double> be = gptr(fac_row(ed, ke, kFacBe));
         const size_t kerow = (size_t)ke * n;
-        if (!RECORDS) {
-            // M-only: the second derivative of every knot row, in place of y.  Each pair is
-            // read and written by one lane only, so the in-place update has no hazard.
-            const int items = n * E;
-            for (int r0 = 0; r0 < kCoefItems; r0 += G) {
-                KnotMIn v[G];
-                int cs[G], us[G];
-#pragma unroll
-                for (int a = 0; a < G; ++a) {
-                    const int idx = min(base + (r0 + a) * 256 + (int)threadIdx.x, items - 1);
-                    cs[a] = idx / E;
-                    us[a] = idx - cs[a] * E;
-                    const size_t rr = kerow + cs[a];
-                    v[a] = knot_m_load(ed.ysol + rr * ed.Ecap, al, be, ed.sep + rr * ed.nbmax,
-                                       ed.spline ? ed.mend + rr * 2 : nullptr, us[a], E, J);
-                }
-#pragma unroll
-                for (int a = 0; a < G; ++a) {
-                    const int idx = base + (r0 + a) * 256 + (int)threadIdx.x;
-                    if (idx < items) ed.ysol[(kerow + cs[a]) * ed.Ecap + us[a]] = knot_m_select(v[a], us[a], E, J);
-                }
-            }
-            continue;
-        }
+        // idx / D as (idx * ceil(2^40 / D)) >> 40: exact for idx < n * D when n * D^2 < 2^40
+        // (the error term idx * (magic * D - 2^40) stays below 2^40); otherwise divide
+        const int D = E - 1;
+        const unsigned long long dmagic = ((1ull << 40) + (unsigned long long)D - 1) / (unsigned long long)D;
+        const bool use_magic = (unsigned long long)n * (unsigned long long)D * (unsigned long long)D < (1ull << 40);
+        auto divD = [&](int idx) -> int {
+            return use_magic ? (int)(((unsigned long long)idx * dmagic) >> 40) : idx / D;
+        };
         // records (a, b, c, d) per interval (memReducedEpochs_dev.cu:775-793) with the
         // shared reciprocals
         const int nI = E - 1;
@@ -958,7 +942,7 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
 #pragma unroll
             for (int a = 0; a < G; ++a) {
                 const int idx = min(base + (r0 + a) * 256 + (int)threadIdx.x, items - 1);
-                const int c = idx / nI;
+                const int c = divD(idx);
                 const int j = idx - c * nI;
                 cs[a] = c;
                 js[a] = j;
@@ -1138,10 +1122,10 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
 // of every 4-sample quad; waves without items skip it.  Phase B evaluates both
 // envelopes (each lane 4 consecutive samples of one channel) and adds the mean
 // in ascending direction order.
-__global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict__ eps,
+__global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, KC = 64;
-    constexpr int MI = 3;   // phase-A items per thread per round (rounds repeat while items remain)
+    constexpr int MI = 2;   // phase-A items per thread per round (rounds repeat while items remain)
     __shared__ double4 rec[2][CG][NI];   // row stride NI double4: the wave's 4 channels hit disjoint banks
     __shared__ int xk[2][NI + 1];
     __shared__ int sE[KC][2], sPre[KC][2], sJlo[KC][2], sNint[KC][2];
@@ -1203,15 +1187,22 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
             const int per0 = nint[0] * CG;
             const int total = per0 + nint[1] * CG;
             const int nitems = (total + 255) >> 8;
+            const int Jv[2] = {(uniform(sE[kk][0]) - 3) / kSolverBlock, (uniform(sE[kk][1]) - 3) / kSolverBlock};
+            const int Ev[2] = {uniform(sE[kk][0]), uniform(sE[kk][1])};
             const size_t rowbase = (size_t)(2 * k) * n * ed.Ecap;
+            const size_t sepbase = (size_t)(2 * k) * n * ed.nbmax;
             const GPtr<double> Mb[2] = {gptr(ed.ysol + rowbase), gptr(ed.ysol + rowbase + (size_t)n * ed.Ecap)};
+            const GPtr<double> Sb[2] = {gptr(ed.sep + sepbase), gptr(ed.sep + sepbase + (size_t)n * ed.nbmax)};
+            const GPtr<double> Ab[2] = {gptr(fac_row(ed, 2 * k, kFacAl)), gptr(fac_row(ed, 2 * k + 1, kFacAl))};
+            const GPtr<double> Bb[2] = {gptr(fac_row(ed, 2 * k, kFacBe)), gptr(fac_row(ed, 2 * k + 1, kFacBe))};
             const GPtr<double> Yb[2] = {gptr(ed.yknot + rowbase), gptr(ed.yknot + rowbase + (size_t)n * ed.Ecap)};
             const GPtr<int> Xb[2] = {gptr(ed.X + (size_t)(2 * k) * ed.Ecap), gptr(ed.X + (size_t)(2 * k + 1) * ed.Ecap)};
             const GPtr<double> Hb[2] = {gptr(fac_row(ed, 2 * k, kFacIh)), gptr(fac_row(ed, 2 * k + 1, kFacIh))};
             const GPtr<double> H3b[2] = {gptr(fac_row(ed, 2 * k, kFacI3h)), gptr(fac_row(ed, 2 * k + 1, kFacI3h))};
             for (int a0 = 0; a0 < nitems; a0 += MI) {
                 double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
-                int x0[MI], x1[MI], lds[MI];
+                double a0v[MI], a1v[MI], b0v[MI], b1v[MI], s0v[MI], s1v[MI], s2v[MI], me0[MI], me1[MI];
+                int x0[MI], x1[MI], lds[MI], jv[MI], ev[MI];
 #pragma unroll
                 for (int a = 0; a < MI; ++a) {
                     lds[a] = -1;
@@ -1225,14 +1216,34 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
                     const int c2 = min(cg * CG + cl2, n - 1);
                     const int j = ok ? jlo[env] + i : 0;
                     const unsigned off = (unsigned)c2 * (unsigned)ed.Ecap + (unsigned)j;
+                    // M at knots j and j+1 from the block solutions y, the spikes and the
+                    // separator values of blocks b, b+1, b+2 (b = block of j)
+                    const int bj = max(j - 1, 0) >> 5;
+                    const unsigned soff = (unsigned)c2 * (unsigned)ed.nbmax;
                     x0[a] = Xb[env][j];
                     x1[a] = Xb[env][j + 1];
                     v0[a] = Yb[env][off];
                     v1[a] = Yb[env][off + 1];
                     m0[a] = Mb[env][off];
                     m1[a] = Mb[env][off + 1];
+                    a0v[a] = Ab[env][j];
+                    a1v[a] = Ab[env][j + 1];
+                    b0v[a] = Bb[env][j];
+                    b1v[a] = Bb[env][j + 1];
+                    s0v[a] = Sb[env][soff + bj];
+                    s1v[a] = Sb[env][soff + min(bj + 1, Jv[env])];
+                    s2v[a] = Sb[env][soff + min(bj + 2, Jv[env])];
+                    me0[a] = 0.0;
+                    me1[a] = 0.0;
+                    if (ed.spline) {
+                        const size_t mo = ((size_t)(2 * k + env) * n + c2) * 2;
+                        me0[a] = ed.mend[mo];
+                        me1[a] = ed.mend[mo + 1];
+                    }
                     ih[a] = Hb[env][j];
                     i3h[a] = H3b[env][j];
+                    jv[a] = j;
+                    ev[a] = env;
                     // LDS slot of the record; bit 30: channel beyond n (ragged group), bit 29: first channel
                     if (ok)
                         lds[a] = ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) |
@@ -1250,6 +1261,16 @@ __global__ __launch_bounds__(256) void k_eval_fused(const EpochDesc* __restrict_
                         if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
                     }
                     if (sl0 & 0x40000000) continue;
+                    {   // M_j and M_{j+1}: knot_m_select's operations
+                        const int j = jv[a], E = Ev[ev[a]], J = Jv[ev[a]];
+                        const int bj = max(j - 1, 0) >> 5, bn = j >> 5;   // blocks of j and j+1
+                        KnotMIn in0{m0[a], a0v[a], b0v[a], s0v[a], s1v[a], s1v[a], me0[a]};
+                        KnotMIn in1{m1[a], a1v[a], b1v[a], bn == bj ? s0v[a] : s1v[a],
+                                    bn == bj ? s1v[a] : s2v[a], bn == bj ? s1v[a] : s2v[a], me1[a]};
+                        if (j == 0) in0.me = me0[a];
+                        m0[a] = knot_m_select(in0, j, E, J);
+                        m1[a] = knot_m_select(in1, j + 1, E, J);
+                    }
                     double h = (double)(x1[a] - x0[a]);
                     double slp = (v1[a] - v0[a]) * ih[a];
                     double4 r;
@@ -1538,7 +1559,7 @@ long list_capacity(int list, const EpochDesc& e) {
     case kListFactor: return K2;
     case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
     case kListReduced: return K2;
-    case kListCoef: return K2 * (((long)e.n * e.Ecap + 256L * kCoefItems - 1) / (256L * kCoefItems));
+    case kListCoef: return e.coef_records ? K2 * (((long)e.n * e.Ecap + 256L * kCoefItems - 1) / (256L * kCoefItems)) : 0;
     }
     return 0;
 }
@@ -1591,9 +1612,8 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
     case 3: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
     case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
-    case 5:
-        if (p.coef_records) MEMD_LIST_LAUNCH(k_coef<true>, kListCoef, 256);
-        else MEMD_LIST_LAUNCH(k_coef<false>, kListCoef, 256);
+    case 5:   // the fused evaluation forms M itself; records only for the record readers
+        if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
         break;
     case 6:
         if (p.mode == kModeStop)
