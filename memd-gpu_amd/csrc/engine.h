@@ -34,11 +34,16 @@ constexpr int kFlagDirs = 8;       // directions per k_flags pass
 constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
 // Coefficient kernel: items (channel, interval) per thread.
-// Solver blocks per k_block work item: at most 8 (255 rows), and at most 256
-// (block, channel) lanes.  The LDS tile holds rows p0-1 .. qlast+2 of every
-// channel with an odd row stride: n * ((R + 3) | 1) <= kBlockTile doubles.
-__host__ __device__ inline int blocks_per_item(int n) { return (256 / n) < 8 ? (256 / n) : 8; }
-constexpr int kBlockTile = 256 * 35;
+// Solver blocks per k_block work item: at most 4 (<= 128 rows) and n * blocks <=
+// 128, so the LDS tile (rows p0-1 .. qlast+2 of every channel, odd row stride)
+// stays near 35 KB for n <= 128 and four workgroups fit a CU.
+__host__ __device__ inline int blocks_per_item(int n) {
+    int b = 128 / n;
+    return b < 1 ? 1 : (b > 4 ? 4 : b);
+}
+constexpr int kBlockRowsMax = 4 * kSolverBlock;
+// Dynamic LDS bytes of k_block's tile for a batch whose widest epoch has nmax channels.
+size_t block_tile_bytes(int nmax);
 constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;

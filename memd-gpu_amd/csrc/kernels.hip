@@ -617,23 +617,24 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 }
 
 // Knot values, right-hand sides and phase 1, per (direction, envelope) and group
-// of blocks_per_item(n) solver blocks (rows p0..qlast, at most 256).
-//   1. Lanes over rows stage the group's shared factors and reciprocals, and
-//      gather the knot values Y of rows p0-1 .. qlast+2 of every channel into an
-//      LDS tile (end knots clamped, memReducedEpochs_dev.cu:447-486, 545-584);
-//      Y of rows p0 .. qlast+1 is written to yknot (coalesced).
-//   2. Lanes (block, channel) form r_u = 3 (sl_u - sl_{u-1}), sl_u = (Y_{u+1} -
-//      Y_u) * ih_u from the tile and run the block's Thomas sweeps in registers
-//      (y = A_b^{-1} r), in place in the tile; the separator row after a block
-//      keeps r, stored directly for the reduced system.
-//   3. Lanes over rows store the block rows' y (coalesced).
-// The right-hand side of the separator after a block reads the next block's
-// first row, so every lane finishes reading (2a) before any lane writes (2b).
+// of blocks_per_item(n) solver blocks (rows p0..qlast, at most 128).
+//   1. The group's knot positions, shared factors and reciprocals are staged in
+//      LDS; lanes over (channel, row) pairs gather the knot values Y of rows
+//      p0-1 .. qlast+2 of every channel into an LDS tile (end knots clamped,
+//      memReducedEpochs_dev.cu:447-486, 545-584) and write Y of rows p0 .. qlast+1
+//      to yknot (coalesced).
+//   2. Lanes (block, channel): the right-hand side of the separator after the
+//      block (it reads the next block's first row) is formed first; after a
+//      barrier, r_u = 3 (sl_u - sl_{u-1}) with sl_u = (Y_{u+1} - Y_u) * ih_u and the
+//      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
+//      window of Y values in registers keeping every read ahead of the writes.
+//   3. Lanes over (channel, row) pairs store the block rows' y (coalesced).
 __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    constexpr int CB = 8;   // channels whose gathers are issued together
-    __shared__ double sY[kBlockTile];
-    __shared__ double s_rd[256], s_cp[256], s_lo[256], s_ih[257];
+    constexpr int RM = kBlockRowsMax;
+    extern __shared__ double sY[];   // [n][RS] tile, RS = (R + 3) | 1
+    __shared__ double s_rd[RM], s_cp[RM], s_lo[RM], s_ih[RM + 1];
+    __shared__ int sX[RM + 3];
     const ListCursor cu = list_cursor(wl, kListBlock);
     const int tid = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
@@ -649,15 +650,21 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const int nb = min(bpw, J - b0 + 1);
         const int p0 = b0 * L + 1;
         const int qlast = block_geom(E, b0 + nb - 1).q;
-        const int R = qlast - p0 + 1;             // rows of the group, <= 256
-        const int RS = (R + 3) | 1;               // LDS row stride (odd: conflict-free columns)
+        const int R = qlast - p0 + 1;             // rows of the group, <= RM
+        const int RW = R + 3;                     // tile rows p0-1 .. qlast+2
+        const int RS = RW | 1;                    // LDS row stride (odd: conflict-free columns)
+        // idx / D as (idx * ceil(2^32 / D)) >> 32 in 64 bits: exact for idx * (D - 1) < 2^32
+        // (idx < n * RW <= 2^14 here); the constant is 2^32 itself when D = 1
+        const unsigned long long rwmagic = (0x100000000ull + (unsigned)RW - 1) / (unsigned)RW;
+        const unsigned long long rmagic = (0x100000000ull + (unsigned)R - 1) / (unsigned)R;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
         __syncthreads();   // the previous item's tile is no longer read
-        // 1a. factors of rows p0..qlast, reciprocals of intervals p0-1..qlast
-        if (tid < R) {   // R <= 256
+        // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast
+        if (tid < RW) sX[tid] = X[min(p0 - 1 + tid, E - 1)];
+        if (tid < R) {
             const int u = p0 + tid;
             const double rd = gptr(fac_row(ed, ke, kFacRd))[u], cpv = gptr(fac_row(ed, ke, kFacCp))[u];
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
@@ -667,94 +674,78 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
             s_cp[tid] = cpv;
             s_lo[tid] = lo;
         }
-        for (int r = tid; r <= R; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + r];   // R + 1 <= 257
-        // 1b. knot values of rows p0-1 .. qlast+2 (tile row r = u - p0 + 1)
-        for (int r = tid; r < R + 3; r += 256) {
+        if (tid <= R) s_ih[tid] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + tid];
+        __syncthreads();
+        // 1b. knot values: lanes over (channel, tile row) pairs
+        const int X0 = sX[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
+        const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
+        for (int idx = tid; idx < n * RW; idx += 256) {
+            const int c = (int)(((unsigned long long)idx * rwmagic) >> 32);
+            const int r = idx - c * RW;
             const int u = min(p0 - 1 + r, E - 1);
-            const int xu = X[u];
-            const bool lclamp = E >= 6 && u == 0, rclamp = E >= 6 && u == E - 1;
-            int xb = 0, xc = 0;
-            if (lclamp) { xb = X[1]; xc = X[2]; }
-            if (rclamp) { xb = X[E - 2]; xc = X[E - 3]; }
-            const bool write_yk = (r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0);
-            for (int c0 = 0; c0 < n; c0 += CB) {
-                double v[CB], v1[CB], v2[CB];
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const unsigned base = (unsigned)min(c0 + cc, n - 1) * (unsigned)T;
-                    v[cc] = cur[base + xu];
-                    v1[cc] = 0.0;
-                    v2[cc] = 0.0;
-                    if (lclamp || rclamp) {
-                        v1[cc] = cur[base + xb];
-                        v2[cc] = cur[base + xc];
-                    }
-                }
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const int c = c0 + cc;
-                    if (c >= n) break;
-                    double y = v[cc];
-                    if (lclamp || rclamp) y = clamp_end(y, v1[cc], v2[cc], xu, xb, xc, env, lclamp);
-                    sY[c * RS + r] = y;
-                    if (write_yk) yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
-                }
+            const int xu = sX[r];
+            const unsigned base = (unsigned)c * (unsigned)T;
+            double y = cur[base + xu];
+            if (E >= 6 && (u == 0 || u == E - 1)) {
+                const bool left = u == 0;
+                const int xb = left ? X1 : XE2, xc = left ? X2 : XE3;
+                y = clamp_end(y, cur[base + xb], cur[base + xc], left ? X0 : XE1, xb, xc, env, left);
             }
+            sY[c * RS + r] = y;
+            if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
+                yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
         }
         __syncthreads();
-        // 2a. per (block, channel): right-hand sides and the forward sweep (reads only;
-        //     the separator after a block reads the next block's first row)
+        // 2. per (block, channel)
         const int slot = tid / n;
         const int c = tid - slot * n;
         const bool active = slot < nb;
         const BlockGeom g = block_geom(E, b0 + (active ? slot : 0));
         const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
         double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
-        double z[L];
-        double rsep = 0.0;
-        const bool has_sep = active && b0 + slot < J;
-        if (active) {
-            double zp = 0.0;
-#pragma unroll
-            for (int i = 0; i < L; ++i) {
-                if (i < len) {
-                    const int u = p + i;
-                    const double sl = (Yc[i + 2] - Yc[i + 1]) * s_ih[off + i + 1];
-                    const double slm = (Yc[i + 1] - Yc[i]) * s_ih[off + i];
-                    double r = 3.0 * (sl - slm);
-                    if (ed.spline && E >= 4) {   // not-a-knot rows 1 and N = E-2
-                        if (u == 1) r = r * (double)(X[2] - X[1]);
-                        else if (u == E - 2) r = r * (double)(X[E - 2] - X[E - 3]);
-                    }
-                    zp = (i == 0) ? r * s_rd[off] : fma(-s_lo[off + i], zp, r) * s_rd[off + i];
-                    z[i] = zp;
-                }
-            }
-            if (has_sep) {   // separator row q+1 keeps its right-hand side
-                const double sl = (Yc[len + 2] - Yc[len + 1]) * (double)gptr(fac_row(ed, ke, kFacIh))[q + 1];
-                const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
-                rsep = 3.0 * (sl - slm);
-            }
+        if (active && b0 + slot < J) {   // separator row q+1 keeps its right-hand side
+            const double sl = (Yc[len + 2] - Yc[len + 1]) * (double)gptr(fac_row(ed, ke, kFacIh))[q + 1];
+            const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
+            ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = 3.0 * (sl - slm);
         }
         __syncthreads();
-        // 2b. backward sweep, y in place of the block's rows in the tile
         if (active) {
-            if (has_sep) ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = rsep;
+            // forward sweep, z_i written over Y(p+i) once no read needs it
+            double ym = Yc[0], y0 = Yc[1], zp = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < len; ++i) {
+                const int u = p + i;
+                const double yp = Yc[i + 2];
+                const double sl = (yp - y0) * s_ih[off + i + 1];
+                const double slm = (y0 - ym) * s_ih[off + i];
+                double r = 3.0 * (sl - slm);
+                if (ed.spline && E >= 4) {   // not-a-knot rows 1 and N = E-2
+                    if (u == 1) r = r * (double)(X2 - X1);                 // h_1
+                    else if (u == E - 2) r = r * (double)(XE2 - XE3);      // h_{N-1}
+                }
+                zp = (i == 0) ? r * s_rd[off] : fma(-s_lo[off + i], zp, r) * s_rd[off + i];
+                Yc[i + 1] = zp;
+                ym = y0;
+                y0 = yp;
+            }
+            // backward sweep, y in place
             double x = 0.0;
-#pragma unroll
-            for (int i = L - 1; i >= 0; --i) {
-                if (i < len) {
-                    x = (i == len - 1) ? z[i] : fma(-s_cp[off + i], x, z[i]);
-                    Yc[i + 1] = x;
-                }
+#pragma unroll 4
+            for (int i = len - 1; i >= 0; --i) {
+                const double z = Yc[i + 1];
+                x = (i == len - 1) ? z : fma(-s_cp[off + i], x, z);
+                Yc[i + 1] = x;
             }
         }
         __syncthreads();
-        // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped)
-        for (int r = tid; r < R; r += 256) {
+        // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped;
+        //    row N = 32(J+1) is a block row)
+        for (int idx = tid; idx < n * R; idx += 256) {
+            const int c2 = (int)(((unsigned long long)idx * rmagic) >> 32);
+            const int r = idx - c2 * R;
             const int u = p0 + r;
-            if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;   // separator (row N = 32(J+1) is not one)
-            for (int c2 = 0; c2 < n; ++c2) ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = sY[c2 * RS + r + 1];
+            if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
+            ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = sY[c2 * RS + r + 1];
         }
     }
 }
@@ -1551,6 +1542,16 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
     return 0;
 }
 
+size_t block_tile_bytes(int nmax) {
+    // largest n * ((32 * blocks_per_item(n) + 3) | 1) over the batch's channel counts
+    size_t best = 0;
+    for (int n = 1; n <= nmax; ++n) {
+        size_t v = (size_t)n * (size_t)((kSolverBlock * blocks_per_item(n) + 3) | 1);
+        best = v > best ? v : best;
+    }
+    return best * sizeof(double);
+}
+
 long list_capacity(int list, const EpochDesc& e) {
     const long K2 = 2L * e.K;
     const long J = (e.Ecap - 3) / kSolverBlock;
@@ -1610,7 +1611,9 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(256), 0, s, p.eps, p.tab[kMapCompact], p.wl);
         break;
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
-    case 3: MEMD_LIST_LAUNCH(k_block, kListBlock, 256); break;
+    case 3:
+        hipLaunchKernelGGL(k_block, dim3(p.grid[kListBlock]), dim3(256), block_tile_bytes(p.nmax), s, p.eps, p.wl);
+        break;
     case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
     case 5:   // the fused evaluation forms M itself; records only for the record readers
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
@@ -1653,6 +1656,9 @@ hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
 
 hipError_t configure_kernels() {
     // the LDS-staged projection kernel holds up to 64 x 258 doubles (132 KB)
+    hipError_t e0 = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)block_tile_bytes(kMaxChannels));
+    if (e0 != hipSuccess) return e0;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flags),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        kFlagLdsMaxCh * (kFlagTile + 2) * (int)sizeof(double));
