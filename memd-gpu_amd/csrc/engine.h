@@ -90,7 +90,7 @@ struct EpochDesc {
                            //   rd, cp (block factors), al, be (spikes), ih = 1/h, i3h = 1/(3h)
     double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
     double* yknot;         // [K][2][n][Ecap] knot values Y (end knots clamped)
-    double* ysol;          // [K][2][n][Ecap] per channel: rhs r; block rows then hold y
+    double* ysol;          // [K][2][n][Ecap] per channel: block rows hold y, separator rows r
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][n][nbmax] separator values (z, then X_j)
     double* mean;          // [n][T] stop mode: envelope mean

@@ -1107,10 +1107,11 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
 // prologue stages, per (direction, envelope), the knot count, the tile's flag
 // word and knot prefix, and the tile's interval window [jlo, jlo + nint).  Per
 // valid direction, phase A builds in LDS the spline record (a,b,c,d) of every
-// interval meeting the tile, for both envelopes, from M (written in place by
-// k_coef), the knot values and the shared reciprocals -- the same operations as
-// the record path, so the records are bit-identical -- and the first interval
-// of every 4-sample quad; waves without items skip it.  Phase B evaluates both
+// interval meeting the tile, for both envelopes, from M at its two knots (formed
+// from the block solutions, spikes and separator values as knot_m_select does),
+// the knot values and the shared reciprocals -- the same operations as the
+// record path, so the records are bit-identical -- and the first interval of
+// every 4-sample quad; waves without items skip it.  Phase B evaluates both
 // envelopes (each lane 4 consecutive samples of one channel) and adds the mean
 // in ascending direction order.
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
