@@ -174,6 +174,34 @@ typedef struct memd_kernel_profile {
 memd_status memd_profile_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t count,
                                    const memd_opts* opts, memd_kernel_profile* out);
 
+/*
+ * Tile shapes of the engine's kernels for epochs of n channels.  No counterpart
+ * in the reference (its kernels have fixed launch shapes); the choice never
+ * changes results, only speed.
+ *   block_groups     solver blocks (32 knot rows) per work item of the block
+ *                    solve, 1..8, limited by 160 KB of LDS for n channels;
+ *   eval_tile_words  evaluation tile of 64 * w samples x 16 / w channels,
+ *                    w in {1, 2, 4}.
+ * memd_autotune times the candidates on a synthetic batch of (n, T, K) epochs
+ * on this context's device and installs the fastest for n; memd_ctx_set_tuning
+ * installs a choice (e.g. one saved from an earlier autotune);
+ * memd_ctx_get_tuning reports the choice used for n (the built-in policy when
+ * none is installed).  block_ms / eval_ms: the chosen shapes' kernel times per
+ * sift step in memd_autotune (0 otherwise).
+ */
+typedef struct memd_tuning {
+    int32_t n;
+    int32_t block_groups;
+    int32_t eval_tile_words;
+    int32_t reserved;
+    double block_ms;
+    double eval_ms;
+} memd_tuning;
+
+memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuning* out);
+memd_status memd_ctx_set_tuning(memd_ctx* ctx, const memd_tuning* t);
+memd_status memd_ctx_get_tuning(memd_ctx* ctx, int32_t n, memd_tuning* out);
+
 /* Device bytes the engine needs for one epoch of this shape (workspace only). */
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K);
 

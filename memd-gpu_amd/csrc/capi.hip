@@ -12,10 +12,12 @@
 #include "directions.h"
 #include "engine.h"
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdlib>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <string>
@@ -37,6 +39,7 @@ struct memd_ctx {
     std::string graph_key;
     int* flag = nullptr;            // device word for the coordinate check
     hipEvent_t pev[kNumStepKernels + 1] = {};   // per-kernel timing events (profiling entry point)
+    std::map<int, memd_tuning> tuning;          // tile shapes per channel count (memd_autotune / set_tuning)
 };
 
 namespace {
@@ -154,8 +157,25 @@ int eval_variant_choice() {
     return kEvalDefault;
 }
 
+// Built-in tile shapes for n channels: k_block groups of blocks_per_item(n)
+// solver blocks; evaluation tiles of 16 / TW channels close to n.
+memd_tuning default_tuning(int n) {
+    memd_tuning t{};
+    t.n = n;
+    t.block_groups = block_bpw_fit(n, blocks_per_item(n));
+    t.eval_tile_words = n <= 4 ? 4 : (n <= 8 ? 2 : 1);
+    return t;
+}
+
+// Tile shapes used for n channels: the context's installed choice, else the
+// built-in one.
+memd_tuning tuning_for(const memd_ctx* ctx, int n) {
+    auto it = ctx->tuning.find(n);
+    return it != ctx->tuning.end() ? it->second : default_tuning(n);
+}
+
 // Descriptor fields that depend only on the epoch's shape and the options.
-EpochDesc desc_geometry(const EpochIn& I, const memd_opts& o, bool records) {
+EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, bool records) {
     EpochDesc d{};
     d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
     d.Ecap = knot_capacity(I.T);
@@ -165,6 +185,9 @@ EpochDesc desc_geometry(const EpochIn& I, const memd_opts& o, bool records) {
     d.y = I.y; d.V = I.V; d.imfs = I.imfs;
     d.coef_records = records ? 1 : 0;
     d.spline = o.spline;
+    const memd_tuning tu = tuning_for(ctx, I.n);
+    d.bpw = tu.block_groups;
+    d.etw = tu.eval_tile_words;
     return d;
 }
 
@@ -307,13 +330,45 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
         plan.grid[l] = (int)(g > 0 ? g : 1);
     }
     int nmax = 0;
-    for (const EpochDesc& d : eds) nmax = d.n > nmax ? d.n : nmax;
+    std::vector<int> ns(count), bpws(count);
+    for (int e = 0; e < count; ++e) {
+        nmax = eds[e].n > nmax ? eds[e].n : nmax;
+        ns[e] = eds[e].n;
+        bpws[e] = eds[e].bpw;
+    }
+    const BlockLds bl = block_lds(ns.data(), bpws.data(), count);
+    plan.block_rmax = bl.rmax;
+    plan.block_tile_off = bl.tile_off;
+    plan.block_lds = bl.bytes;
     plan.mode = mode;
     plan.nmax = nmax;
     plan.eval_variant = variant;
     plan.coef_records = records ? 1 : 0;
     *out = plan;
     return MEMD_OK;
+}
+
+// Packing of a batch onto the XCDs: every sift step runs all epochs, so a step
+// lasts as long as the most loaded XCD.  Longest-processing-time first: epochs
+// by decreasing n * T * K (the work of a step; ties keep batch order) go to the
+// least loaded XCD.  Packing never changes an epoch's arithmetic.
+void xcd_pack(const std::vector<EpochIn>& in, std::vector<int>& grp) {
+    const int count = (int)in.size();
+    std::vector<int> order(count);
+    std::vector<long> cost(count);
+    for (int e = 0; e < count; ++e) {
+        order[e] = e;
+        cost[e] = (long)in[e].n * (long)in[e].T * (long)in[e].K;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    long load[kXcds] = {};
+    for (int e : order) {
+        int g = 0;
+        for (int x = 1; x < kXcds; ++x)
+            if (load[x] < load[g]) g = x;
+        grp[e] = g;
+        load[g] += cost[e];
+    }
 }
 
 // Core: run a set of epochs (all already FP64 and on the device) to completion.
@@ -347,12 +402,12 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, records);
         base[e] = total;
         total += lay[e].total;
-        eds[e] = desc_geometry(in[e], o, records);
+        eds[e] = desc_geometry(ctx, in[e], o, records);
     }
     // XCD affinity when the batch has enough epochs to keep every XCD busy
     const int groups = count >= kXcds ? kXcds : 1;
-    std::vector<int> grp(count);
-    for (int e = 0; e < count; ++e) grp[e] = groups > 1 ? e % kXcds : 0;
+    std::vector<int> grp(count, 0);
+    if (groups > 1) xcd_pack(in, grp);
     PlanLayout P;
     memd_status rs = plan_layout(ctx, eds, grp, groups, variant, total, &P);
     if (rs != MEMD_OK) return rs;
@@ -433,7 +488,8 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         // key is the arena address plus the launch geometry
         std::string key = std::to_string((uintptr_t)ctx->arena) + "/" + std::to_string(count) + "/" +
                           std::to_string(plan.mode) + "/" + std::to_string(plan.nmax) + "/" +
-                          std::to_string(plan.eval_variant);
+                          std::to_string(plan.eval_variant) + "/b" + std::to_string(plan.block_lds) + ":" +
+                          std::to_string(plan.block_rmax);
         for (int f = 0; f < kNumMaps; ++f) key += "/" + std::to_string(plan.blocks[f]);
         key += "/g" + std::to_string(plan.wl.groups);
         for (int l = 0; l < kNumLists; ++l) {
@@ -799,6 +855,94 @@ static memd_status batch_common(memd_ctx* ctx, const memd_epoch* epochs, size_t 
     return run_epochs(ctx, in, opts, elapsed_s, prof);
 }
 
+memd_status memd_ctx_set_tuning(memd_ctx* ctx, const memd_tuning* t) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    if (!t) return fail(ctx, MEMD_ERR_ARG, "null tuning");
+    if (t->n < 1 || t->n > kMaxChannels) return fail(ctx, MEMD_ERR_ARG, "tuning: n=%d out of range", t->n);
+    if (t->block_groups < 1 || t->block_groups > kBlockBpwMax || block_bpw_fit(t->n, t->block_groups) != t->block_groups)
+        return fail(ctx, MEMD_ERR_ARG, "tuning: block_groups=%d does not fit %d channels (1..%d)", t->block_groups, t->n,
+                    block_bpw_fit(t->n, kBlockBpwMax));
+    if (t->eval_tile_words != 1 && t->eval_tile_words != 2 && t->eval_tile_words != 4)
+        return fail(ctx, MEMD_ERR_ARG, "tuning: eval_tile_words=%d (1, 2 or 4)", t->eval_tile_words);
+    ctx->tuning[t->n] = *t;
+    return MEMD_OK;
+}
+
+memd_status memd_ctx_get_tuning(memd_ctx* ctx, int32_t n, memd_tuning* out) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    if (!out) return fail(ctx, MEMD_ERR_ARG, "null output");
+    if (n < 1 || n > kMaxChannels) return fail(ctx, MEMD_ERR_ARG, "n=%d out of range", n);
+    *out = tuning_for(ctx, n);
+    return MEMD_OK;
+}
+
+// Times every candidate shape on a synthetic batch of (n, T, K) epochs, sized to
+// fill the device (about 5.4e8 n*T*K per step, at most 256 epochs), with one IMF
+// of 4 sifts under the profiling path.  The block solve's time depends only on
+// block_groups and the evaluation's only on eval_tile_words, so the candidates
+// are run in pairs and each knob keeps its fastest value.
+memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuning* out) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    memd_status rs = validate_shape(ctx, n, T, K, 2, 4);
+    if (rs != MEMD_OK) return rs;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t work = n * T * K;
+    size_t count = (size_t)5.4e8 / (work ? work : 1);
+    count = count < 1 ? 1 : (count > 256 ? 256 : count);
+    // synthetic channels: three sinusoids per channel plus uniform noise (fixed LCG)
+    std::vector<double> hy(n * T);
+    uint64_t lcg = 0x9e3779b97f4a7c15ull;
+    for (size_t c = 0; c < n; ++c)
+        for (size_t t = 0; t < T; ++t) {
+            lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+            const double u = (double)(lcg >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+            hy[c * T + t] = std::sin(0.031 * (double)t + 0.7 * (double)c) + 0.5 * std::sin(0.173 * (double)t + (double)c) +
+                            0.25 * std::sin(0.61 * (double)t) + 0.2 * u;
+        }
+    Scratch y, V, imfs;
+    hipError_t e = hipMalloc((void**)&y.p, n * T * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&V.p, K * n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&imfs.p, count * 2 * n * T * sizeof(double));
+    if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "autotune buffers: %s", hipGetErrorString(e));
+    HIP_TRY(ctx, hipMemcpy(y.p, hy.data(), n * T * sizeof(double), hipMemcpyHostToDevice));
+    rs = memd_directions_f64(ctx, n, K, nullptr, V.p);
+    if (rs != MEMD_OK) return rs;
+    std::vector<EpochIn> in(count);
+    for (size_t i = 0; i < count; ++i)
+        in[i] = {(int)n, (int)T, (int)K, 2, 4, y.p, V.p, imfs.p + i * 2 * n * T, nullptr, nullptr};
+    std::vector<int> bg, tw = {1, 2, 4};
+    for (int b : {1, 2, 4, 8})
+        if (block_bpw_fit((int)n, b) == b) bg.push_back(b);
+    const bool had = ctx->tuning.count((int)n) != 0;
+    const memd_tuning saved = tuning_for(ctx, (int)n);
+    memd_tuning best = default_tuning((int)n);
+    best.block_ms = best.eval_ms = 1e300;
+    const size_t rounds = bg.size() > tw.size() ? bg.size() : tw.size();
+    memd_opts o;
+    memd_opts_default(&o);
+    for (size_t r = 0; r <= rounds; ++r) {   // round 0 warms up (plan region, caches)
+        const size_t i = r == 0 ? 0 : r - 1;
+        memd_tuning cand = default_tuning((int)n);
+        cand.block_groups = bg[i < bg.size() ? i : bg.size() - 1];
+        cand.eval_tile_words = tw[i < tw.size() ? i : tw.size() - 1];
+        ctx->tuning[(int)n] = cand;
+        memd_kernel_profile prof;
+        rs = run_epochs(ctx, in, &o, nullptr, &prof);
+        if (rs != MEMD_OK) break;
+        if (r == 0) continue;
+        const double bms = prof.ms_per_launch[3], ems = prof.ms_per_launch[6];
+        if (bms < best.block_ms) { best.block_ms = bms; best.block_groups = cand.block_groups; }
+        if (ems < best.eval_ms) { best.eval_ms = ems; best.eval_tile_words = cand.eval_tile_words; }
+    }
+    if (had) ctx->tuning[(int)n] = saved;
+    else ctx->tuning.erase((int)n);
+    if (rs != MEMD_OK) return rs;
+    best.n = (int)n;
+    ctx->tuning[(int)n] = best;
+    if (out) *out = best;
+    return MEMD_OK;
+}
+
 memd_status memd_run_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size_t count, const memd_opts* opts,
                                double* elapsed_s) {
     return batch_common(ctx, epochs, count, opts, elapsed_s, nullptr);
@@ -928,7 +1072,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     std::map<int64_t, int> index;
     for (int e = 0; e < count; ++e) {
         memd_server::Req& r = S->reqs[S->active[e]];
-        S->eds[e] = desc_geometry(r.in, S->opts, S->records);
+        S->eds[e] = desc_geometry(ctx, r.in, S->opts, S->records);
         bind_workspace(S->eds[e], S->pool + r.off, r.lay);
         S->grp[e] = groups > 1 ? r.grp : 0;
         index[S->active[e]] = e;

@@ -19,14 +19,19 @@ constexpr int kFlagTile = 256;
 struct EvalVariant { int cpl, unroll; };
 constexpr EvalVariant kEvalVariants[] = {{4, 1}, {4, 2}, {2, 4}, {8, 1}, {2, 2}, {1, 4}, {0, 0}};
 constexpr int kNumEvalVariants = 7;
-// Variant kEvalFused builds the spline records of each 64-sample tile in LDS
-// (no record array in HBM); tiles of 64 samples x kEvalFusedCG channels.
+// Variant kEvalFused builds the spline records of each tile in LDS (no record
+// array in HBM); tiles of 64 * TW samples x 16 / TW channels, TW = EpochDesc::etw
+// in {1, 2, 4}.
 constexpr int kEvalFused = 6;
-constexpr int kEvalFusedCG = 16;
-// Intervals meeting a 64-sample tile: at most 32 knots of one kind lie inside it
-// (strict extrema are never adjacent), so at most 33.  34 also makes the record
-// row stride (34 x 32 B) put a wave's 4 channels on disjoint LDS banks.
-constexpr int kEvalMaxInt = 34;
+// Intervals meeting a tile of 64 * TW samples: at most 32 * TW knots of one kind
+// lie inside it (strict extrema are never adjacent), so at most 32 * TW + 1.
+__host__ __device__ inline int eval_max_int(int tw) { return 32 * tw + 2; }
+// LDS budget of one k_eval_fused workgroup.  kEvalCap records hold one
+// direction's worst case at every tile shape: 2 envelopes x channels x odd row
+// stride, at most 2 x 16 x 35 (TW = 1), 2 x 8 x 67, 2 x 4 x 131.
+constexpr int kEvalCap = 1120;
+constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
+constexpr int kEvalBatch = 8;      // directions per batch
 constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
 constexpr int kFlagLdsMaxCh = 64;
@@ -34,16 +39,26 @@ constexpr int kFlagDirs = 8;       // directions per k_flags pass
 constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
 // Coefficient kernel: items (channel, interval) per thread.
-// Solver blocks per k_block work item: at most 4 (<= 128 rows) and n * blocks <=
-// 128, so the LDS tile (rows p0-1 .. qlast+2 of every channel, odd row stride)
-// stays near 35 KB for n <= 128 and four workgroups fit a CU.
+// Solver blocks per k_block work item (EpochDesc::bpw).  Default: at most 4 (<= 128
+// rows) and n * blocks <= 128, so the LDS tile (rows p0-1 .. qlast+2 of every
+// channel, odd row stride) stays near 35 KB for n <= 128 and four workgroups fit
+// a CU; tuning may choose up to kBlockBpwMax.
 __host__ __device__ inline int blocks_per_item(int n) {
     int b = 128 / n;
     return b < 1 ? 1 : (b > 4 ? 4 : b);
 }
-constexpr int kBlockRowsMax = 4 * kSolverBlock;
-// Dynamic LDS bytes of k_block's tile for a batch whose widest epoch has nmax channels.
-size_t block_tile_bytes(int nmax);
+constexpr int kBlockBpwMax = 8;
+// k_block's dynamic LDS: per-row factors for rmax rows, knot positions and the
+// tile of the widest epoch.  rmax = 32 * max bpw over the batch.
+struct BlockLds {
+    int rmax;
+    size_t tile_off;   // bytes: start of the tile
+    size_t bytes;
+};
+BlockLds block_lds(const int* n, const int* bpw, int count);
+constexpr int kBlockLdsMax = 160 * 1024;   // gfx950 LDS per workgroup
+// Largest bpw <= want (>= 1) whose k_block LDS for n channels fits kBlockLdsMax.
+int block_bpw_fit(int n, int want);
 constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;
@@ -104,6 +119,8 @@ struct EpochDesc {
     int* ndone;                // shared by the batch: epochs finished (stop mode)
     int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
+    int bpw;                   // solver blocks per k_block work item
+    int etw;                   // k_eval_fused tile: 64 * etw samples x 16 / etw channels (etw in {1, 2, 4})
     int spline;                // 0 natural ends, 1 not-a-knot
     double* mend;              // [K][2][n][2] not-a-knot: M_0 and M_{E-1} per channel
 };
@@ -141,6 +158,9 @@ struct StepPlan {
     int nmax;               // largest channel count in the batch
     int eval_variant;       // index into kEvalVariants
     int coef_records;       // launch k_coef (the evaluation reads the record array)
+    int block_rmax;         // k_block: rows of the longest work item
+    size_t block_tile_off;  // k_block: LDS byte offset of the tile
+    size_t block_lds;       // k_block: dynamic LDS bytes
 };
 
 // Host-side block counts for one epoch (must match the kernels' decoding).

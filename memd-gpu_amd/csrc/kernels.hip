@@ -124,31 +124,10 @@ __device__ __forceinline__ GMut<T> gmut(T* p) {
     return (GMut<T>)p;
 }
 
-// Row u (1..N, N = E-2) of a knot system.  Natural ends: the reference's rows
+// Row u (1..N, N = E-2) of a knot system from the three knot positions X[u-1],
+// X[u], X[u+1].  Natural ends: the reference's rows
 // (memReducedEpochs_dev.cu:704-736).  Not-a-knot: rows 1 and N with M_0 and
 // M_{E-1} eliminated (integer products, exact in FP64).
-__device__ __forceinline__ void sys_row(const int* X, int u, int E, int spline, double& lo, double& di,
-                                        double& up) {
-    lo = (double)(X[u] - X[u - 1]);
-    up = (double)(X[u + 1] - X[u]);
-    di = (double)(2 * ((X[u] - X[u - 1]) + (X[u + 1] - X[u])));
-    if (spline && E >= 4) {
-        const int N = E - 2;
-        if (u == 1) {
-            long long h0 = X[1] - X[0], h1 = X[2] - X[1];
-            lo = 0.0;
-            di = (double)((h0 + h1) * (h0 + 2 * h1));
-            up = (double)(h1 * h1 - h0 * h0);
-        } else if (u == N) {
-            long long a = X[N] - X[N - 1], b = X[N + 1] - X[N];
-            lo = (double)(a * a - b * b);
-            di = (double)((a + b) * (2 * a + b));
-            up = 0.0;
-        }
-    }
-}
-
-// The same row from the three knot positions X[u-1], X[u], X[u+1].
 __device__ __forceinline__ void sys_row3(int xm, int x0, int xp, int u, int E, int spline, double& lo, double& di,
                                          double& up) {
     lo = (double)(x0 - xm);
@@ -388,7 +367,7 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
 // Work items one direction-envelope with E knots contributes to each solver list.
 __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     const int J = (E - 3) / kSolverBlock;
-    const int bpw = blocks_per_item(ed.n);
+    const int bpw = ed.bpw;
     m[kListFactor] = 1;
     m[kListBlock] = (J + 1 + bpw - 1) / bpw;
     m[kListReduced] = (J > 0 || ed.spline) ? 1 : 0;   // not-a-knot: end values
@@ -617,7 +596,7 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 }
 
 // Knot values, right-hand sides and phase 1, per (direction, envelope) and group
-// of blocks_per_item(n) solver blocks (rows p0..qlast, at most 128).
+// of EpochDesc::bpw solver blocks (rows p0..qlast, at most 32 * bpw).
 //   1. The group's knot positions, shared factors and reciprocals are staged in
 //      LDS; lanes over (channel, row) pairs gather the knot values Y of rows
 //      p0-1 .. qlast+2 of every channel into an LDS tile (end knots clamped,
@@ -629,12 +608,16 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 //      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
 //      window of Y values in registers keeping every read ahead of the writes.
 //   3. Lanes over (channel, row) pairs store the block rows' y (coalesced).
-__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl) {
+__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl, int rmax,
+                                               int tile_off) {
     constexpr int L = kSolverBlock;
-    constexpr int RM = kBlockRowsMax;
-    extern __shared__ double sY[];   // [n][RS] tile, RS = (R + 3) | 1
-    __shared__ double s_rd[RM], s_cp[RM], s_lo[RM], s_ih[RM + 1];
-    __shared__ int sX[RM + 3];
+    extern __shared__ double lds[];
+    double* s_rd = lds;                   // [rmax]
+    double* s_cp = lds + rmax;            // [rmax]
+    double* s_lo = lds + 2 * rmax;        // [rmax]
+    double* s_ih = lds + 3 * rmax;        // [rmax + 1]
+    int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 1);   // [rmax + 3]
+    double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);   // [n][RS], RS = (R + 3) | 1
     const ListCursor cu = list_cursor(wl, kListBlock);
     const int tid = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
@@ -645,16 +628,17 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const int n = ed.n, T = ed.T;
         const int E = ed.E[ke];
         const int J = block_geom(E, 0).J;
-        const int bpw = blocks_per_item(n);
+        const int bpw = ed.bpw;
         const int b0 = it.z * bpw;
         const int nb = min(bpw, J - b0 + 1);
         const int p0 = b0 * L + 1;
         const int qlast = block_geom(E, b0 + nb - 1).q;
-        const int R = qlast - p0 + 1;             // rows of the group, <= RM
+        const int R = qlast - p0 + 1;             // rows of the group, <= rmax
         const int RW = R + 3;                     // tile rows p0-1 .. qlast+2
         const int RS = RW | 1;                    // LDS row stride (odd: conflict-free columns)
         // idx / D as (idx * ceil(2^32 / D)) >> 32 in 64 bits: exact for idx * (D - 1) < 2^32
-        // (idx < n * RW <= 2^14 here); the constant is 2^32 itself when D = 1
+        // (the tile bounds idx < n * RW by 160 KB / 8 and D <= 259 here); the
+        // constant is 2^32 itself when D = 1
         const unsigned long long rwmagic = (0x100000000ull + (unsigned)RW - 1) / (unsigned)RW;
         const unsigned long long rmagic = (0x100000000ull + (unsigned)R - 1) / (unsigned)R;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
@@ -663,18 +647,18 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast
-        if (tid < RW) sX[tid] = X[min(p0 - 1 + tid, E - 1)];
-        if (tid < R) {
-            const int u = p0 + tid;
+        for (int r = tid; r < RW; r += 256) sX[r] = X[min(p0 - 1 + r, E - 1)];
+        for (int r = tid; r < R; r += 256) {
+            const int u = p0 + r;
             const double rd = gptr(fac_row(ed, ke, kFacRd))[u], cpv = gptr(fac_row(ed, ke, kFacCp))[u];
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
             double lo, di, up;
             sys_row3(xm, x0, xp, u, E, ed.spline, lo, di, up);
-            s_rd[tid] = rd;
-            s_cp[tid] = cpv;
-            s_lo[tid] = lo;
+            s_rd[r] = rd;
+            s_cp[r] = cpv;
+            s_lo[r] = lo;
         }
-        if (tid <= R) s_ih[tid] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + tid];
+        for (int r = tid; r <= R; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + r];
         __syncthreads();
         // 1b. knot values: lanes over (channel, tile row) pairs
         const int X0 = sX[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
@@ -696,20 +680,23 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
                 yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
         }
         __syncthreads();
-        // 2. per (block, channel)
-        const int slot = tid / n;
-        const int c = tid - slot * n;
-        const bool active = slot < nb;
-        const BlockGeom g = block_geom(E, b0 + (active ? slot : 0));
-        const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
-        double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
-        if (active && b0 + slot < J) {   // separator row q+1 keeps its right-hand side
+        // 2. per (block, channel) lane; n * nb may exceed the workgroup
+        for (int lane = tid; lane < n * nb; lane += 256) {   // separator rows q+1 keep their right-hand side
+            const int slot = lane / n, c = lane - slot * n;
+            if (b0 + slot >= J) continue;
+            const BlockGeom g = block_geom(E, b0 + slot);
+            const int q = g.q, off = g.p - p0, len = q - g.p + 1;
+            const double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
             const double sl = (Yc[len + 2] - Yc[len + 1]) * (double)gptr(fac_row(ed, ke, kFacIh))[q + 1];
             const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
             ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = 3.0 * (sl - slm);
         }
         __syncthreads();
-        if (active) {
+        for (int lane = tid; lane < n * nb; lane += 256) {
+            const int slot = lane / n, c = lane - slot * n;
+            const BlockGeom g = block_geom(E, b0 + slot);
+            const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
+            double* Yc = sY + c * RS + off;
             // forward sweep, z_i written over Y(p+i) once no read needs it
             double ym = Yc[0], y0 = Yc[1], zp = 0.0;
 #pragma unroll 4
@@ -1065,16 +1052,6 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, 
     }
 }
 
-// rem / nint == (rem * kDivMagic[nint]) >> 16 exactly for 1 <= nint <= kEvalMaxInt and
-// rem < 2^16 / nint (phase-A items: rem < kEvalFusedCG * kEvalMaxInt = 544).
-struct DivMagicTable {
-    int v[kEvalMaxInt + 1];
-    constexpr DivMagicTable() : v() {
-        for (int i = 1; i <= kEvalMaxInt; ++i) v[i] = (65536 + i - 1) / i;
-    }
-};
-__constant__ DivMagicTable kDivMagic = DivMagicTable();
-
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Phase B of one envelope for a lane's 4 consecutive samples (interval i of the
@@ -1102,191 +1079,246 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
     }
 }
 
-// Fused records + evaluation (fix mode).  One workgroup = one 64-sample tile
-// (one flag word) x kEvalFusedCG channels.  For each chunk of 64 directions a
-// prologue stages, per (direction, envelope), the knot count, the tile's flag
-// word and knot prefix, and the tile's interval window [jlo, jlo + nint).  Per
-// valid direction, phase A builds in LDS the spline record (a,b,c,d) of every
-// interval meeting the tile, for both envelopes, from M at its two knots (formed
-// from the block solutions, spikes and separator values as knot_m_select does),
-// the knot values and the shared reciprocals -- the same operations as the
-// record path, so the records are bit-identical -- and the first interval of
-// every 4-sample quad; waves without items skip it.  Phase B evaluates both
-// envelopes (each lane 4 consecutive samples of one channel) and adds the mean
-// in ascending direction order.
+// rem / d == (rem * ceil(2^16 / d)) >> 16 exactly for 1 <= d <= 32 * TW + 2 and
+// 0 <= rem < (16 / TW) * d, TW in {1, 2, 4} (checked exhaustively): the split of
+// a phase-A item index into (channel, interval).
+__device__ __forceinline__ int div_magic16(int d) { return (65536 + d - 1) / d; }
+
+// Fused records + evaluation (fix mode).  One workgroup = one tile of 64 * TW
+// samples (TW = EpochDesc::etw flag words) x 16 / TW channels; lane = (channel,
+// quad of 4 consecutive samples).  Directions are taken in batches of at most
+// kEvalBatch whose records fit the LDS budget:
+//   formation (wave 0): per (direction, envelope) the knot count, the tile's flag
+//     words, the knots before each word and the tile's interval window
+//     [jlo, jlo + nint); prefix sums over the directions' record and knot counts
+//     close the batch;
+//   phase A: every (direction, envelope, channel, interval) item of the batch
+//     builds the spline record (a,b,c,d) of its interval in LDS from M at its two
+//     knots (formed from the block solutions, spikes and separator values as
+//     knot_m_select does), the knot values and the shared reciprocals -- the same
+//     operations as the record path, so the records are bit-identical;
+//   phase B: each lane evaluates both envelopes of every valid direction of the
+//     batch at its 4 samples and adds their mean in ascending direction order.
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
-    constexpr int TL = 64, CG = kEvalFusedCG, NI = kEvalMaxInt, KC = 64;
-    constexpr int MI = 2;   // phase-A items per thread per round (rounds repeat while items remain)
-    __shared__ double4 rec[2][CG][NI];   // row stride NI double4: the wave's 4 channels hit disjoint banks
-    __shared__ int xk[2][NI + 1];
-    __shared__ int sE[KC][2], sPre[KC][2], sJlo[KC][2], sNint[KC][2];
-    __shared__ unsigned long long sMask[KC][2];
-    __shared__ int sQ[2][16];   // first interval (local) of every 4-sample quad
+    constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
+    constexpr int MI = 2;                // phase-A items per thread per round
+    __shared__ double4 rec[kEvalCap];    // segment s: [channel][stride] from its record base
+    __shared__ int xk[kEvalXkCap];       // segment s: knot positions jlo .. jlo + nint
+    __shared__ int4 sSegA[NS];           // item prefix, division magic, nint, jlo
+    __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
+    __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
+    __shared__ int sCum[NS][4];          // knots before each word of the tile
+    __shared__ unsigned long long sMsk[NS][4];
+    __shared__ int sNb;
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
-    const int n = ed.n, T = ed.T, K = ed.K;
+    const int n = ed.n, T = ed.T, K = ed.K, W = ed.W;
+    const int TW = ed.etw;
+    const int lgq = 4 + (TW >> 1);       // log2 quads per channel: 4, 5, 6 for TW = 1, 2, 4
+    const int CG = 16 / TW;
     const int groups = (n + CG - 1) / CG;
     const int tile = br.y / groups;
     const int cg = br.y - tile * groups;
-    const int t0 = tile * TL;
-    const int tlast = min(t0 + TL - 1, T - 1);
+    const int cgn = min(CG, n - cg * CG);   // channels of this group
+    const int w0 = tile * TW;
+    const int t0 = 64 * w0;
+    const int tlast = min(t0 + 64 * TW - 1, T - 1);
     const int tid = threadIdx.x;
-    const int wbase = tid & ~63;     // first thread of this wave
-    const int q = tid & 15;          // sample quad
-    const int cl = tid >> 4;         // channel in group
+    const int wbase = tid & ~63;
+    const int cl = tid >> lgq;
+    const int q = tid & ((1 << lgq) - 1);
     const int c = cg * CG + cl;
     const int ts = t0 + 4 * q;
+    const int wi = q >> 4;               // tile word of the quad
+    const unsigned long long below = (1ull << (4 * (q & 15))) - 1ull;
+    const bool lane_on = cl < cgn && ts <= tlast;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
-    for (int kc0 = 0; kc0 < K; kc0 += KC) {
-        const int kcn = min(KC, K - kc0);
-        __syncthreads();
-        if (tid < 2 * kcn) {
-            const int kk = tid >> 1, env = tid & 1;
-            const int ke = 2 * (kc0 + kk) + env;
-            const int E = ed.E[ke];
-            const unsigned long long m = ed.mask[(size_t)ke * ed.W + tile];
-            const int pr = ed.pre[(size_t)ke * ed.W + tile];
-            const int inside = (int)__popcll(m & ((1ull << (tlast - t0)) - 1ull));
+    for (int kb = 0; kb < K;) {
+        if (tid < 64) {   // formation
+            const int k = kb + (tid >> 1), env = tid & 1;
+            const bool have = tid < NS && k < K;
+            int E = 0, pr = 0, inside = 0;
+            int cum[4];
+            unsigned long long msk[4];
+            if (have) {
+                E = ed.E[2 * k + env];
+                pr = ed.pre[(size_t)(2 * k + env) * W + w0];
+            }
+            int run = pr;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                cum[w] = run;
+                msk[w] = 0;
+                if (have && w < TW && w0 + w < W) {
+                    const unsigned long long m = ed.mask[(size_t)(2 * k + env) * W + w0 + w];
+                    const int lim = tlast - 64 * (w0 + w);   // bits below tlast in this word
+                    msk[w] = m;
+                    inside += lim >= 64 ? (int)__popcll(m) : (lim > 0 ? (int)__popcll(m & ((1ull << lim) - 1ull)) : 0);
+                    run += (int)__popcll(m);
+                }
+            }
             const int lo = min(max(pr - 1, 0), E - 2);
             const int hi = min(max(pr + inside - 1, 0), E - 2);
-            sE[kk][env] = E;
-            sMask[kk][env] = m;
-            sPre[kk][env] = pr;
-            sJlo[kk][env] = lo;
-            sNint[kk][env] = hi - lo + 1;
+            const int ni = hi - lo + 1;
+            const int Ep = __shfl_xor(E, 1);
+            const bool valid = have && E > 3 && Ep > 3;
+            const int stride = ni | 1;   // odd: a wave's channels read disjoint banks
+            const int need = valid ? cgn * stride : 0;
+            const int nxk = valid ? ni + 1 : 0;
+            const int nit = valid ? cgn * ni : 0;
+            int pneed = need, pxk = nxk, pit = nit;
+#pragma unroll
+            for (int o = 1; o < NS; o <<= 1) {
+                const int a = __shfl_up(pneed, o), b = __shfl_up(pxk, o), d = __shfl_up(pit, o);
+                if (tid >= o) {
+                    pneed += a;
+                    pxk += b;
+                    pit += d;
+                }
+            }
+            // leading directions whose records and knot positions fit (the first always does)
+            const unsigned long long fit =
+                __ballot(have && (tid & 1) && pneed <= kEvalCap && pxk <= kEvalXkCap);
+            const int nb = max((int)__popcll(fit), 1);
+            const int total = __shfl(pit, 2 * nb - 1);
+            if (tid < 2 * nb) {
+                sSegA[tid] = make_int4(pit - nit, div_magic16(max(ni, 1)), ni, lo);
+                sSegB[tid] = make_int4(pneed - need, stride, E, pxk - nxk);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    sCum[tid][w] = cum[w];
+                    sMsk[tid][w] = msk[w];
+                }
+            }
+            if (tid <= NS) sIp[tid] = tid < 2 * nb ? pit - nit : total;
+            if (tid == 0) sNb = nb;
         }
         __syncthreads();
-        for (int kk = 0; kk < kcn; ++kk) {
-            const int k = kc0 + kk;
-            if (!(uniform(sE[kk][0]) > 3 && uniform(sE[kk][1]) > 3)) continue;
-            nv = nv + 1.0;
-            const int jlo[2] = {uniform(sJlo[kk][0]), uniform(sJlo[kk][1])};
-            const int nint[2] = {uniform(sNint[kk][0]), uniform(sNint[kk][1])};
-            const int magic[2] = {kDivMagic.v[nint[0]], kDivMagic.v[nint[1]]};
-            // first interval of every quad (shared by the group's channels)
-            if (tid < 32) {
-                const int env = tid >> 4, qq = tid & 15;
-                const int jj = sPre[kk][env] + (int)__popcll(sMask[kk][env] & ((1ull << (4 * qq)) - 1ull)) - 1;
-                sQ[env][qq] = min(max(jj, 0), sE[kk][env] - 2) - jlo[env];
-            }
-            // phase A: every item's loads first, then the records.  Element offsets
-            // are 32-bit (n * T <= 2^31) from per-direction bases.
-            const int per0 = nint[0] * CG;
-            const int total = per0 + nint[1] * CG;
-            const int nitems = (total + 255) >> 8;
-            const int Jv[2] = {(uniform(sE[kk][0]) - 3) / kSolverBlock, (uniform(sE[kk][1]) - 3) / kSolverBlock};
-            const int Ev[2] = {uniform(sE[kk][0]), uniform(sE[kk][1])};
-            const size_t rowbase = (size_t)(2 * k) * n * ed.Ecap;
-            const size_t sepbase = (size_t)(2 * k) * n * ed.nbmax;
-            const GPtr<double> Mb[2] = {gptr(ed.ysol + rowbase), gptr(ed.ysol + rowbase + (size_t)n * ed.Ecap)};
-            const GPtr<double> Sb[2] = {gptr(ed.sep + sepbase), gptr(ed.sep + sepbase + (size_t)n * ed.nbmax)};
-            const GPtr<double> Ab[2] = {gptr(fac_row(ed, 2 * k, kFacAl)), gptr(fac_row(ed, 2 * k + 1, kFacAl))};
-            const GPtr<double> Bb[2] = {gptr(fac_row(ed, 2 * k, kFacBe)), gptr(fac_row(ed, 2 * k + 1, kFacBe))};
-            const GPtr<double> Yb[2] = {gptr(ed.yknot + rowbase), gptr(ed.yknot + rowbase + (size_t)n * ed.Ecap)};
-            const GPtr<int> Xb[2] = {gptr(ed.X + (size_t)(2 * k) * ed.Ecap), gptr(ed.X + (size_t)(2 * k + 1) * ed.Ecap)};
-            const GPtr<double> Hb[2] = {gptr(fac_row(ed, 2 * k, kFacIh)), gptr(fac_row(ed, 2 * k + 1, kFacIh))};
-            const GPtr<double> H3b[2] = {gptr(fac_row(ed, 2 * k, kFacI3h)), gptr(fac_row(ed, 2 * k + 1, kFacI3h))};
-            for (int a0 = 0; a0 < nitems; a0 += MI) {
-                double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
-                double a0v[MI], a1v[MI], b0v[MI], b1v[MI], s0v[MI], s1v[MI], s2v[MI], me0[MI], me1[MI];
-                int x0[MI], x1[MI], lds[MI], jv[MI], ev[MI];
+        const int nb = uniform(sNb);
+        int ip[NS + 1];
 #pragma unroll
-                for (int a = 0; a < MI; ++a) {
-                    lds[a] = -1;
-                    if (a0 + a >= nitems || wbase + 256 * (a0 + a) >= total) continue;   // wave-uniform
-                    const int idx = tid + 256 * (a0 + a);
-                    const int env = (idx < per0) ? 0 : 1;
-                    const int rem = (env == 0) ? idx : idx - per0;
-                    const int cl2 = (rem * magic[env]) >> 16;
-                    const int i = rem - cl2 * nint[env];
-                    const bool ok = idx < total;
-                    const int c2 = min(cg * CG + cl2, n - 1);
-                    const int j = ok ? jlo[env] + i : 0;
-                    const unsigned off = (unsigned)c2 * (unsigned)ed.Ecap + (unsigned)j;
-                    // M at knots j and j+1 from the block solutions y, the spikes and the
-                    // separator values of blocks b, b+1, b+2 (b = block of j)
-                    const int bj = max(j - 1, 0) >> 5;
-                    const unsigned soff = (unsigned)c2 * (unsigned)ed.nbmax;
-                    x0[a] = Xb[env][j];
-                    x1[a] = Xb[env][j + 1];
-                    v0[a] = Yb[env][off];
-                    v1[a] = Yb[env][off + 1];
-                    m0[a] = Mb[env][off];
-                    m1[a] = Mb[env][off + 1];
-                    a0v[a] = Ab[env][j];
-                    a1v[a] = Ab[env][j + 1];
-                    b0v[a] = Bb[env][j];
-                    b1v[a] = Bb[env][j + 1];
-                    s0v[a] = Sb[env][soff + bj];
-                    s1v[a] = Sb[env][soff + min(bj + 1, Jv[env])];
-                    s2v[a] = Sb[env][soff + min(bj + 2, Jv[env])];
-                    me0[a] = 0.0;
-                    me1[a] = 0.0;
-                    if (ed.spline) {
-                        const size_t mo = ((size_t)(2 * k + env) * n + c2) * 2;
-                        me0[a] = ed.mend[mo];
-                        me1[a] = ed.mend[mo + 1];
-                    }
-                    ih[a] = Hb[env][j];
-                    i3h[a] = H3b[env][j];
-                    jv[a] = j;
-                    ev[a] = env;
-                    // LDS slot of the record; bit 30: channel beyond n (ragged group), bit 29: first channel
-                    if (ok)
-                        lds[a] = ((env * CG + cl2) * NI + i) | ((cg * CG + cl2 < n) ? 0 : 0x40000000) |
-                                 (cl2 == 0 ? 0x20000000 : 0);
-                }
+        for (int s = 0; s <= NS; ++s) ip[s] = uniform(sIp[s]);
+        const int total = ip[NS];
+        // phase A
+        for (int a0 = 0; a0 * 256 < total; a0 += MI) {
+            double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
+            double a0v[MI], a1v[MI], b0v[MI], b1v[MI], s0v[MI], s1v[MI], s2v[MI], me0[MI], me1[MI];
+            int x0[MI], x1[MI], fv[MI], jv[MI];
 #pragma unroll
-                for (int a = 0; a < MI; ++a) {
-                    const int sl0 = lds[a];
-                    if (sl0 < 0) continue;
-                    const int sv = sl0 & 0x1fffffff;
-                    if (sl0 & 0x20000000) {   // first channel of the group: knot positions
-                        const int env = sv >= CG * NI ? 1 : 0;
-                        const int i = sv - env * CG * NI;
-                        xk[env][i] = x0[a];
-                        if (i == nint[env] - 1) xk[env][nint[env]] = x1[a];
-                    }
-                    if (sl0 & 0x40000000) continue;
-                    {   // M_j and M_{j+1}: knot_m_select's operations
-                        const int j = jv[a], E = Ev[ev[a]], J = Jv[ev[a]];
-                        const int bj = max(j - 1, 0) >> 5, bn = j >> 5;   // blocks of j and j+1
-                        KnotMIn in0{m0[a], a0v[a], b0v[a], s0v[a], s1v[a], s1v[a], me0[a]};
-                        KnotMIn in1{m1[a], a1v[a], b1v[a], bn == bj ? s0v[a] : s1v[a],
-                                    bn == bj ? s1v[a] : s2v[a], bn == bj ? s1v[a] : s2v[a], me1[a]};
-                        if (j == 0) in0.me = me0[a];
-                        m0[a] = knot_m_select(in0, j, E, J);
-                        m1[a] = knot_m_select(in1, j + 1, E, J);
-                    }
-                    double h = (double)(x1[a] - x0[a]);
-                    double slp = (v1[a] - v0[a]) * ih[a];
-                    double4 r;
-                    r.x = v0[a];
-                    r.y = slp - (h * (2.0 * m0[a] + m1[a])) * (1.0 / 3.0);
-                    r.z = m0[a];
-                    r.w = (m1[a] - m0[a]) * i3h[a];
-                    (&rec[0][0][0])[sv] = r;
+            for (int a = 0; a < MI; ++a) {
+                fv[a] = -1;
+                if (wbase + 256 * (a0 + a) >= total) continue;   // wave-uniform
+                const int f = tid + 256 * (a0 + a);
+                if (f >= total) continue;
+                int seg = 0;
+#pragma unroll
+                for (int s = 1; s < NS; ++s) seg += (f >= ip[s]) ? 1 : 0;
+                const int4 A = sSegA[seg];
+                const int E = sSegB[seg].z;
+                const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
+                const int rem = f - A.x;
+                const int cl2 = (rem * A.y) >> 16;
+                const int j = A.w + rem - cl2 * A.z;
+                const int c2 = cg * CG + cl2;
+                const int J = (E - 3) / kSolverBlock;
+                const int bj = max(j - 1, 0) >> 5;
+                const size_t rowoff = ((size_t)ke * n + c2) * ed.Ecap;
+                const GPtr<double> Yr = gptr(ed.yknot + rowoff), Mr = gptr(ed.ysol + rowoff);
+                const GPtr<double> Sr = gptr(ed.sep + ((size_t)ke * n + c2) * ed.nbmax);
+                const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
+                const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
+                x0[a] = Xr[j];
+                x1[a] = Xr[j + 1];
+                v0[a] = Yr[j];
+                v1[a] = Yr[j + 1];
+                m0[a] = Mr[j];
+                m1[a] = Mr[j + 1];
+                a0v[a] = F[kFacAl * ed.Ecap + j];
+                a1v[a] = F[kFacAl * ed.Ecap + j + 1];
+                b0v[a] = F[kFacBe * ed.Ecap + j];
+                b1v[a] = F[kFacBe * ed.Ecap + j + 1];
+                ih[a] = F[kFacIh * ed.Ecap + j];
+                i3h[a] = F[kFacI3h * ed.Ecap + j];
+                s0v[a] = Sr[bj];
+                s1v[a] = Sr[min(bj + 1, J)];
+                s2v[a] = Sr[min(bj + 2, J)];
+                me0[a] = 0.0;
+                me1[a] = 0.0;
+                if (ed.spline) {
+                    const GPtr<double> Me = gptr(ed.mend + ((size_t)ke * n + c2) * 2);
+                    me0[a] = Me[0];
+                    me1[a] = Me[1];
                 }
+                fv[a] = seg;
+                jv[a] = j;
             }
-            __syncthreads();
-            // phase B
-            if (c < n && ts <= tlast) {
-                eval_quad<false, 0>(rec[0][cl], xk[0], sQ[0][q], ts, tlast, ep, acc);
-                eval_quad<false, 1>(rec[1][cl], xk[1], sQ[1][q], ts, tlast, ep, acc);
+#pragma unroll
+            for (int a = 0; a < MI; ++a) {
+                const int seg = fv[a];
+                if (seg < 0) continue;
+                const int4 A = sSegA[seg];
+                const int4 B = sSegB[seg];
+                const int j = jv[a], E = B.z, J = (E - 3) / kSolverBlock;
+                const int i = j - A.w;
+                const int rem = tid + 256 * (a0 + a) - A.x;
+                const int cl2 = (rem * A.y) >> 16;
+                if (cl2 == 0) {   // first channel of the group: knot positions
+                    xk[B.w + i] = x0[a];
+                    if (i == A.z - 1) xk[B.w + A.z] = x1[a];
+                }
+                // M_j and M_{j+1}: knot_m_select's operations
+                const int bj = max(j - 1, 0) >> 5, bn = j >> 5;   // blocks of j and j+1
+                const KnotMIn in0{m0[a], a0v[a], b0v[a], s0v[a], s1v[a], s1v[a], me0[a]};
+                const KnotMIn in1{m1[a], a1v[a], b1v[a], bn == bj ? s0v[a] : s1v[a],
+                                  bn == bj ? s1v[a] : s2v[a], bn == bj ? s1v[a] : s2v[a], me1[a]};
+                const double mj = knot_m_select(in0, j, E, J);
+                const double mk = knot_m_select(in1, j + 1, E, J);
+                const double h = (double)(x1[a] - x0[a]);
+                const double slp = (v1[a] - v0[a]) * ih[a];
+                double4 r;
+                r.x = v0[a];
+                r.y = slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0);
+                r.z = mj;
+                r.w = (mk - mj) * i3h[a];
+                rec[B.x + cl2 * B.y + i] = r;
             }
-            __syncthreads();
         }
+        __syncthreads();
+        // phase B
+        for (int d = 0; d < nb; ++d) {
+            const int E0 = uniform(sSegB[2 * d].z), E1 = uniform(sSegB[2 * d + 1].z);
+            if (!(E0 > 3 && E1 > 3)) continue;
+            nv = nv + 1.0;
+            if (lane_on) {
+                {
+                    const int4 A = sSegA[2 * d];
+                    const int4 B = sSegB[2 * d];
+                    const int jj = sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1;
+                    eval_quad<false, 0>(rec + B.x + cl * B.y, xk + B.w, min(max(jj, 0), E0 - 2) - A.w, ts, tlast,
+                                        ep, acc);
+                }
+                {
+                    const int4 A = sSegA[2 * d + 1];
+                    const int4 B = sSegB[2 * d + 1];
+                    const int jj = sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1;
+                    eval_quad<false, 1>(rec + B.x + cl * B.y, xk + B.w, min(max(jj, 0), E1 - 2) - A.w, ts, tlast,
+                                        ep, acc);
+                }
+            }
+        }
+        __syncthreads();
+        kb += nb;
     }
     if (ed.stats && br.y == 0 && tid == 0) {
         atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
         atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
     }
-    if (c >= n) return;
+    if (!(cl < cgn)) return;
     double* cur = cur_slot(ed, st->imf);
     const bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
     double* nxt = cur_slot(ed, st->imf + 1);
@@ -1537,26 +1569,43 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
     case kMapCompact: return e.K;
     case kMapEval:
         if (e.mode == kModeStop) return (e.T + 63) / 64;
-        if (eval_variant == kEvalFused) return ((e.T + 63) / 64) * ((e.n + kEvalFusedCG - 1) / kEvalFusedCG);
+        if (eval_variant == kEvalFused)
+            return ((e.W + e.etw - 1) / e.etw) * ((e.n + 16 / e.etw - 1) / (16 / e.etw));
         return ((e.T + 63) / 64) * ((e.n + kEvalVariants[eval_variant].cpl - 1) / kEvalVariants[eval_variant].cpl);
     }
     return 0;
 }
 
-size_t block_tile_bytes(int nmax) {
-    // largest n * ((32 * blocks_per_item(n) + 3) | 1) over the batch's channel counts
-    size_t best = 0;
-    for (int n = 1; n <= nmax; ++n) {
-        size_t v = (size_t)n * (size_t)((kSolverBlock * blocks_per_item(n) + 3) | 1);
-        best = v > best ? v : best;
+int block_bpw_fit(int n, int want) {
+    int b = want < 1 ? 1 : (want > kBlockBpwMax ? kBlockBpwMax : want);
+    for (; b > 1; --b) {
+        const BlockLds L = block_lds(&n, &b, 1);
+        if (L.bytes <= (size_t)kBlockLdsMax) break;
     }
-    return best * sizeof(double);
+    return b;
+}
+
+BlockLds block_lds(const int* n, const int* bpw, int count) {
+    int rmax = kSolverBlock;
+    size_t tile = 0;
+    for (int e = 0; e < count; ++e) {
+        const int r = kSolverBlock * bpw[e];
+        rmax = r > rmax ? r : rmax;
+        const size_t v = (size_t)n[e] * (size_t)((r + 3) | 1);
+        tile = v > tile ? v : tile;
+    }
+    BlockLds L;
+    L.rmax = rmax;
+    // rd, cp, lo: rmax doubles each; ih: rmax + 1 doubles; X: rmax + 3 ints (8-byte aligned)
+    L.tile_off = sizeof(double) * (4 * (size_t)rmax + 1) + (((size_t)rmax + 3) * sizeof(int) + 7) / 8 * 8;
+    L.bytes = L.tile_off + tile * sizeof(double);
+    return L;
 }
 
 long list_capacity(int list, const EpochDesc& e) {
     const long K2 = 2L * e.K;
     const long J = (e.Ecap - 3) / kSolverBlock;
-    const long bpw = blocks_per_item(e.n);
+    const long bpw = e.bpw;
     switch (list) {
     case kListFactor: return K2;
     case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
@@ -1613,7 +1662,8 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         break;
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
     case 3:
-        hipLaunchKernelGGL(k_block, dim3(p.grid[kListBlock]), dim3(256), block_tile_bytes(p.nmax), s, p.eps, p.wl);
+        hipLaunchKernelGGL(k_block, dim3(p.grid[kListBlock]), dim3(256), p.block_lds, s, p.eps, p.wl, p.block_rmax,
+                           (int)p.block_tile_off);
         break;
     case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
     case 5:   // the fused evaluation forms M itself; records only for the record readers
@@ -1658,7 +1708,7 @@ hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
 hipError_t configure_kernels() {
     // the LDS-staged projection kernel holds up to 64 x 258 doubles (132 KB)
     hipError_t e0 = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)block_tile_bytes(kMaxChannels));
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
     if (e0 != hipSuccess) return e0;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flags),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,

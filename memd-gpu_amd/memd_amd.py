@@ -54,6 +54,15 @@ class MemdKernelProfile(ctypes.Structure):
                 ("eval_points", ctypes.c_int64), ("samples", ctypes.c_int64)]
 
 
+class MemdTuning(ctypes.Structure):
+    _fields_ = [("n", _i32), ("block_groups", _i32), ("eval_tile_words", _i32), ("reserved", _i32),
+                ("block_ms", ctypes.c_double), ("eval_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"n": self.n, "block_groups": self.block_groups, "eval_tile_words": self.eval_tile_words,
+                "block_ms": self.block_ms, "eval_ms": self.eval_ms}
+
+
 #: kernel order of MemdKernelProfile.ms_per_launch
 KERNEL_NAMES = ("k_flags", "k_compact", "k_factor", "k_block", "k_reduced", "k_coef", "k_eval", "k_advance",
                 "k_stop_apply", "k_init")
@@ -72,7 +81,8 @@ EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_l
            "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
            "memd_workspace_bytes", "memd_server_create", "memd_server_destroy", "memd_server_submit",
            "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
-           "memd_server_pool_used", "memd_bin_length", "memd_read_bin_f32", "memd_write_bin_f32")
+           "memd_server_pool_used", "memd_bin_length", "memd_read_bin_f32", "memd_write_bin_f32",
+           "memd_autotune", "memd_ctx_set_tuning", "memd_ctx_get_tuning")
 #: exports that do not return a memd_status
 _NOT_STATUS = ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes",
                "memd_server_pool_used")
@@ -131,6 +141,9 @@ def load_library(path=LIB_PATH):
     L.memd_bin_length.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]
     L.memd_read_bin_f32.argtypes = [ctypes.c_char_p, _sz, _sz, _sz, _vp]
     L.memd_write_bin_f32.argtypes = [ctypes.c_char_p, _vp, _sz]
+    L.memd_autotune.argtypes = [_vp, _sz, _sz, _sz, ctypes.POINTER(MemdTuning)]
+    L.memd_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(MemdTuning)]
+    L.memd_ctx_get_tuning.argtypes = [_vp, _i32, ctypes.POINTER(MemdTuning)]
     for name in EXPORTS:
         if name not in _NOT_STATUS:
             getattr(L, name).restype = ctypes.c_int
@@ -191,6 +204,23 @@ class MemdContext:
     def set_stream(self, stream):
         """Bind a torch.cuda.Stream (or None for the legacy default stream)."""
         self._check(self.lib.memd_ctx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
+
+    # ---- tile shapes (no reference counterpart; never change results)
+    def autotune(self, n, T, K):
+        """Time the candidate tile shapes for (n, T, K) epochs on this device and
+        install the fastest for n; returns the choice as a dict."""
+        t = MemdTuning()
+        self._check(self.lib.memd_autotune(self.h, n, T, K, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, n, block_groups, eval_tile_words):
+        t = MemdTuning(n=n, block_groups=block_groups, eval_tile_words=eval_tile_words)
+        self._check(self.lib.memd_ctx_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self, n):
+        t = MemdTuning()
+        self._check(self.lib.memd_ctx_get_tuning(self.h, n, ctypes.byref(t)))
+        return t.as_dict()
 
     # ---- GenerateDirectionVector (memReducedEpochs_dev.cu:1029-1072)
     def directions(self, n, K, primes=None, out=None):
