@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--workload", choices=("m", "c1", "c3", "c4"), default="m",
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in tile shapes (default: memd_autotune before the untimed warmup)")
+    ap.add_argument("--workload", choices=("m", "c1", "c2", "c3", "c4"), default="m",
                     help="m: 32ch x 4096 K=64 batched (headline); c1: 32ch x 30504 K=64 single epochs "
                          "vs the CPU oracle; c3: 512 uneven epochs batched vs plain; c4: served "
                          "continuous batching under Poisson arrivals")
@@ -136,6 +138,57 @@ def bench_c1(a):
         "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
 
 
+def bench_c2(a):
+    """128 ch x 16384 samples (EEG-like generator + 40 Hz on a quarter of the
+    channels), K = 256, 9 slots x 10 sifts: plain per-epoch GPU path vs the FP64
+    CPU oracle on this host."""
+    import torch
+    import memd_amd
+    import synth
+    from oracle import memd_oracle
+    n, T, K, N_imf, S = 128, 16384, 256, 9, 10
+    ctx = memd_amd.MemdContext(0)
+    tune_s, tuning = 0.0, None
+    if not a.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(n, T, K)
+        tune_s = time.perf_counter() - t0
+    V = ctx.directions(n, K)
+    ys = [torch.from_numpy(synth.eeg_epoch(n, T, 4000 + i, gamma40=True)).to("cuda:0")
+          for i in range(max(a.steps, 1))]
+    out = torch.empty((N_imf, n, T), dtype=torch.float64, device="cuda:0")
+    for _ in range(a.warmup):
+        ctx.run(ys[0], V, N_imf, S, out=out)
+    torch.cuda.synchronize()
+    lat = []
+    for y in ys:
+        s0 = time.perf_counter()
+        ctx.run(y, V, N_imf, S, out=out)
+        lat.append(time.perf_counter() - s0)
+    gpu_eps = len(lat) / sum(lat)
+    ctx.close()
+    # CPU: the oracle for 1 sift of the first IMF, scaled to the 80-sift epoch
+    threads = max(1, min(16, os.cpu_count() or 1))
+    y0 = synth.eeg_epoch(n, T, 4000, gamma40=True)
+    Vh = memd_oracle.directions(n, K)
+    t0 = time.perf_counter()
+    memd_oracle.memd(y0, Vh, 2, 1, threads=threads, trace=False)
+    per_sift = time.perf_counter() - t0
+    cpu_eps = 1.0 / (per_sift * (N_imf - 1) * S)
+    print(json.dumps({
+        "metric": "MEMD epochs/s, 128ch x 16384, K=256 (plain GPU vs FP64 CPU)", "value": round(gpu_eps, 3),
+        "unit": "epochs/s", "n_gpus": 1, "steps": len(lat), "warmup": a.warmup,
+        "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "memd_128ch_x16384_K256_fix9x10_plain",
+                   "generator": "eeg pink+alpha, fs=512, 40 Hz on a quarter of the channels"},
+        "latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
+        "tuning": tuning, "autotune_s": round(tune_s, 3),
+        "cpu_baseline": {"value": round(cpu_eps, 6), "unit": "epochs/s", "cores": threads, "kind": "port",
+                         "sample": f"1 sift of IMF 1 of one epoch ({per_sift:.1f} s) scaled to 80 sifts"},
+        "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
+
+
 def bench_c3(a):
     """512 independent epochs, n in {8,16,32,64}, T in [1000,8000], K = 2n (EEG-like
     generator, per-epoch seeds), 9 slots x 10 sifts: one batched call vs one call
@@ -146,7 +199,16 @@ def bench_c3(a):
     count = 512
     ctx = memd_amd.MemdContext(0)
     eps, Vs = [], {}
-    for i, (n, T, K) in enumerate(synth.batch_shapes(count, 77)):
+    shapes = synth.batch_shapes(count, 77)
+    tunings, tune_s = {}, 0.0
+    if not a.no_autotune:
+        # one tuning per channel-count class, at the class's median length
+        t0 = time.perf_counter()
+        for n in sorted({s[0] for s in shapes}):
+            Tm = int(np.median([T for (nn, T, _) in shapes if nn == n]))
+            tunings[n] = ctx.autotune(n, Tm, 2 * n)
+        tune_s = time.perf_counter() - t0
+    for i, (n, T, K) in enumerate(shapes):
         if (n, K) not in Vs:
             Vs[(n, K)] = ctx.directions(n, K)
         y = torch.from_numpy(synth.eeg_epoch(n, T, 78 + i)).to("cuda:0")
@@ -161,7 +223,7 @@ def bench_c3(a):
         ctx.run_batch(eps)
         lat.append(time.perf_counter() - s0)
     batched = count * len(lat) / sum(lat)
-    plain_n = min(32, count)
+    plain_n = count   # every epoch once, one call each
     ctx.run(eps[0]["y"], eps[0]["V"], 9, 10, out=eps[0]["out"])
     torch.cuda.synchronize()
     s0 = time.perf_counter()
@@ -179,6 +241,7 @@ def bench_c3(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "memd_512_uneven_fix9x10", "n": [8, 16, 32, 64], "T": [1000, 8000], "K": "2n"},
         "plain_gpu_epochs_per_s": round(plain, 2), "batched_over_plain": round(batched / plain, 2),
+        "tuning": {str(k): v for k, v in tunings.items()}, "autotune_s": round(tune_s, 3),
         "latency_ms": {"batch_p50": round(pct(lat, 50) * 1e3, 1), "plain_epoch_p50": round(pct(plat, 50) * 1e3, 3),
                        "plain_epoch_p99": round(pct(plat, 99) * 1e3, 3)}}), flush=True)
 
@@ -287,10 +350,17 @@ def main():
         return bench_c4(a)
     if a.workload == "c3":
         return bench_c3(a)
+    if a.workload == "c2":
+        return bench_c2(a)
 
     n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
     dev = "cuda:0"
     ctx = memd_amd.MemdContext(0)
+    tuning, tune_s = None, 0.0
+    if not a.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(n, T, K)   # once per device and shape class; not timed
+        tune_s = time.perf_counter() - t0
     V = ctx.directions(n, K)
     ys = [torch.from_numpy(synth.sines_epoch(n, T, 1000 + i)).to(dev) for i in range(B)]
     outs = [torch.empty((N_imf, n, T), dtype=torch.float64, device=dev) for _ in range(B)]
@@ -367,6 +437,7 @@ def main():
         "batched_over_plain": round(value / plain_eps, 2) if plain_eps else None,
         "roofline": roofline,
         "kernel_us_per_launch": kernels,
+        "tuning": tuning, "autotune_s": round(tune_s, 3),
         "cpu_baseline": cpu,
         "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

@@ -615,8 +615,8 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
     double* s_rd = lds;                   // [rmax]
     double* s_cp = lds + rmax;            // [rmax]
     double* s_lo = lds + 2 * rmax;        // [rmax]
-    double* s_ih = lds + 3 * rmax;        // [rmax + 1]
-    int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 1);   // [rmax + 3]
+    double* s_ih = lds + 3 * rmax;        // [rmax + 2]
+    int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 2);   // [rmax + 3]
     double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);   // [n][RS], RS = (R + 3) | 1
     const ListCursor cu = list_cursor(wl, kListBlock);
     const int tid = threadIdx.x;
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
         __syncthreads();   // the previous item's tile is no longer read
-        // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast
+        // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
         for (int r = tid; r < RW; r += 256) sX[r] = X[min(p0 - 1 + r, E - 1)];
         for (int r = tid; r < R; r += 256) {
             const int u = p0 + r;
@@ -658,26 +658,58 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
             s_cp[r] = cpv;
             s_lo[r] = lo;
         }
-        for (int r = tid; r <= R; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[p0 - 1 + r];
+        // reciprocals of intervals p0-1 .. qlast+1 (the last one: the next separator's
+        // right-hand side; clamped where the group ends the system)
+        for (int r = tid; r <= R + 1; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[min(p0 - 1 + r, E - 2)];
         __syncthreads();
         // 1b. knot values: lanes over (channel, tile row) pairs
         const int X0 = sX[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
         const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
-        for (int idx = tid; idx < n * RW; idx += 256) {
-            const int c = (int)(((unsigned long long)idx * rwmagic) >> 32);
-            const int r = idx - c * RW;
-            const int u = min(p0 - 1 + r, E - 1);
-            const int xu = sX[r];
-            const unsigned base = (unsigned)c * (unsigned)T;
-            double y = cur[base + xu];
-            if (E >= 6 && (u == 0 || u == E - 1)) {
-                const bool left = u == 0;
-                const int xb = left ? X1 : XE2, xc = left ? X2 : XE3;
-                y = clamp_end(y, cur[base + xb], cur[base + xc], left ? X0 : XE1, xb, xc, env, left);
+        // GU gathers per lane are issued before any is used (their latencies overlap)
+        constexpr int GU = 8;
+        for (int i0 = tid; i0 < n * RW; i0 += 256 * GU) {
+            double yv[GU];
+            int cv[GU], rv[GU];
+#pragma unroll
+            for (int g = 0; g < GU; ++g) {
+                const int idx = min(i0 + 256 * g, n * RW - 1);
+                const int c = (int)(((unsigned long long)idx * rwmagic) >> 32);
+                const int r = idx - c * RW;
+                cv[g] = c;
+                rv[g] = r;
+                yv[g] = cur[(unsigned)c * (unsigned)T + (unsigned)sX[r]];
             }
-            sY[c * RS + r] = y;
-            if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
-                yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
+#pragma unroll
+            for (int g = 0; g < GU; ++g) {
+                if (i0 + 256 * g >= n * RW) break;
+                const int c = cv[g], r = rv[g];
+                const int u = min(p0 - 1 + r, E - 1);
+                if (E >= 6 && (u == 0 || u == E - 1)) continue;   // clamped end knots: below
+                const double y = yv[g];
+                sY[c * RS + r] = y;
+                if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
+                    yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
+            }
+        }
+        // end knots of E >= 6 clamped (tile row 0 is knot 0 in the first group; rows
+        // rE.. hold knot E-1 in the last)
+        const int rE = E - 1 - (p0 - 1);
+        const bool has_left = p0 == 1, has_right = rE <= RW - 1;
+        if (E >= 6 && (has_left || has_right)) {
+            for (int c = tid; c < n; c += 256) {
+                const unsigned base = (unsigned)c * (unsigned)T;
+                if (has_left) {
+                    const double y = clamp_end(cur[base + X0], cur[base + X1], cur[base + X2], X0, X1, X2, env, true);
+                    sY[c * RS] = y;
+                    yk[(unsigned)c * (unsigned)ed.Ecap] = y;
+                }
+                if (has_right) {
+                    const double y =
+                        clamp_end(cur[base + XE1], cur[base + XE2], cur[base + XE3], XE1, XE2, XE3, env, false);
+                    for (int r = rE; r < RW; ++r) sY[c * RS + r] = y;
+                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(E - 1)] = y;
+                }
+            }
         }
         __syncthreads();
         // 2. per (block, channel) lane; n * nb may exceed the workgroup
@@ -687,7 +719,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
             const BlockGeom g = block_geom(E, b0 + slot);
             const int q = g.q, off = g.p - p0, len = q - g.p + 1;
             const double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
-            const double sl = (Yc[len + 2] - Yc[len + 1]) * (double)gptr(fac_row(ed, ke, kFacIh))[q + 1];
+            const double sl = (Yc[len + 2] - Yc[len + 1]) * s_ih[off + len + 1];
             const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
             ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = 3.0 * (sl - slm);
         }
@@ -1596,8 +1628,8 @@ BlockLds block_lds(const int* n, const int* bpw, int count) {
     }
     BlockLds L;
     L.rmax = rmax;
-    // rd, cp, lo: rmax doubles each; ih: rmax + 1 doubles; X: rmax + 3 ints (8-byte aligned)
-    L.tile_off = sizeof(double) * (4 * (size_t)rmax + 1) + (((size_t)rmax + 3) * sizeof(int) + 7) / 8 * 8;
+    // rd, cp, lo: rmax doubles each; ih: rmax + 2 doubles; X: rmax + 3 ints (8-byte aligned)
+    L.tile_off = sizeof(double) * (4 * (size_t)rmax + 2) + (((size_t)rmax + 3) * sizeof(int) + 7) / 8 * 8;
     L.bytes = L.tile_off + tile * sizeof(double);
     return L;
 }

@@ -81,12 +81,24 @@ def test_headline_shape_full_schedule(ctx, oracle):
     np.testing.assert_allclose(got.sum(0), y, rtol=0, atol=1e-12 * np.abs(y).max())
 
 
+@pytest.mark.parametrize("n,T,K,n_imf,S,seed", [
+    (128, 2048, 256, 3, 2, 60),   # the 128 x 16384, K=256 configuration's channels and directions
+    (200, 700, 16, 3, 2, 61),     # above 128 channels, ragged evaluation groups
+    (65, 1500, 130, 3, 2, 62),    # just past the LDS-staged projection kernel's 64 channels
+])
+def test_high_density_matches_oracle(ctx, oracle, n, T, K, n_imf, S, seed):
+    # more than 64 channels: the projection/extrema pass runs per direction from HBM
+    import synth
+    _check_epoch(ctx, oracle, synth.eeg_epoch(n, T, seed, gamma40=True), K, n_imf, S)
+
+
 @pytest.mark.parametrize("gen,n,T,K,n_imf,S,seed", [
     ("sines", 8, 1000, 16, 12, 50, 20),   # the 8 x 1000, K=16 host case with [0.075, 0.75, 0.075]
     ("sines", 1, 700, 4, 10, 30, 21),     # univariate
     ("eeg", 4, 2000, 8, 12, 20, 22),
     ("sines", 3, 64, 6, 8, 10, 23),       # short: the decomposition ends early (too few extrema)
     ("eeg", 32, 4096, 64, 12, 10, 24),    # headline shape
+    ("eeg", 128, 1024, 256, 4, 6, 25),    # stop mode at its 128-channel limit
 ])
 def test_stop_mode_matches_oracle(ctx, oracle, gen, n, T, K, n_imf, S, seed):
     import synth
