@@ -256,7 +256,7 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
         total += align_up(sizeof(int4) * (size_t)P->lcap[l]);
     }
     P->cnt_at = total;
-    total += align_up(sizeof(int) * kNumLists * kXcds);
+    total += align_up(kCountBytes);
     P->stats_at = total;
     total += align_up(sizeof(unsigned long long) * 4);
     P->ndone_at = total;
@@ -1099,7 +1099,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->opts.mode, S->variant, S->records, S->tabs, s, &S->plan);
     if (rs != MEMD_OK) return rs;
     // counters are zero between steps; a new region starts from zero too
-    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, sizeof(int) * kNumLists * kXcds, s));
+    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, kCountBytes, s));
     HIP_TRY(ctx, hipMemsetAsync(ndone, 0, sizeof(int), s));
     S->fresh.clear();
     for (int64_t id : joined) {

@@ -138,10 +138,15 @@ enum : int { kListFactor = 0, kListBlock, kListReduced, kListCoef, kNumLists };
 // its table blocks sit at ids = XCD (mod 8) and its work items go to that
 // XCD's sub-list, so its signal and knot data stay in one XCD's L2.
 constexpr int kXcds = 8;
+// Work-list counters sit kCountStride ints (one 128-byte line) apart, so that the
+// appends of different lists and XCD groups do not contend for one line.
+constexpr int kCountStride = 32;
+__host__ __device__ inline int count_slot(int list, int g) { return (list * kXcds + g) * kCountStride; }
+constexpr size_t kCountBytes = sizeof(int) * kCountStride * kNumLists * kXcds;
 
 struct WorkLists {
     int4* item[kNumLists];
-    int* count;                     // [kNumLists][kXcds], cleared at the end of every step
+    int* count;                     // [kNumLists][kXcds] slots of kCountStride ints, cleared every step
     int off[kNumLists][kXcds];      // sub-list start in item[list]
     int cap[kNumLists][kXcds];
     int groups;                     // kXcds (XCD affinity) or 1

@@ -153,7 +153,8 @@ __device__ __forceinline__ double* fac_row(const EpochDesc& ed, int ke, int fiel
 }
 
 __device__ __forceinline__ void zero_counters(const WorkLists& wl) {
-    for (int i = 0; i < kNumLists * kXcds; ++i) wl.count[i] = 0;
+    for (int l = 0; l < kNumLists; ++l)
+        for (int g = 0; g < kXcds; ++g) wl.count[count_slot(l, g)] = 0;
 }
 
 // Persistent-grid iteration over one list: with XCD affinity, workgroup w
@@ -169,7 +170,7 @@ __device__ __forceinline__ ListCursor list_cursor(const WorkLists& wl, int list)
     cu.base = wl.item[list] + wl.off[list][g];
     cu.start = wl.groups > 1 ? (int)(blockIdx.x / kXcds) : (int)blockIdx.x;
     cu.stride = wl.groups > 1 ? (int)(gridDim.x / kXcds) : (int)gridDim.x;
-    cu.cnt = min(wl.count[list * kXcds + g], wl.cap[list][g]);
+    cu.cnt = min(wl.count[count_slot(list, g)], wl.cap[list][g]);
     return cu;
 }
 
@@ -374,9 +375,10 @@ __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
 }
 
-// One workgroup per (epoch, direction): both envelopes' knot lists, then the
-// work items of a valid direction appended to the solver lists.
-__global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab,
+// One workgroup per (epoch, direction), one wave per envelope: the knot list
+// (exclusive prefix of the flag words' popcounts, a wave scan), then the work
+// items of a valid direction appended to the solver lists.
+__global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab,
                                                  WorkLists wl) {
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
@@ -385,39 +387,36 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
     const int k = br.y;
     const int W = ed.W;
     const int tid = threadIdx.x;
-    __shared__ int sc[256];
+    const int env = tid >> 6, lane = tid & 63;   // wave e compacts envelope e
     __shared__ int sE[2];
-    for (int env = 0; env < 2; ++env) {
+    {
         const int ke = 2 * k + env;
         const unsigned long long* m = ed.mask + (size_t)ke * W;
         int* pre = ed.pre + (size_t)ke * W;
         int* X = ed.X + (size_t)ke * ed.Ecap;
-        int per = (W + 255) / 256;
-        int w0 = tid * per;
-        int w1 = min(w0 + per, W);
+        const int per = (W + 63) / 64;   // consecutive words per lane
+        const int w0 = lane * per;
+        const int w1 = min(w0 + per, W);
         int mine = 0;
         for (int w = w0; w < w1; ++w) mine += __popcll(m[w]);
-        __syncthreads();
-        sc[tid] = mine;
-        __syncthreads();
-        for (int d = 1; d < 256; d <<= 1) {   // inclusive Hillis-Steele scan of 256 partials
-            int v = (tid >= d) ? sc[tid - d] : 0;
-            __syncthreads();
-            sc[tid] += v;
-            __syncthreads();
+        int incl = mine;   // inclusive scan over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
         }
-        int base = sc[tid] - mine;
+        int base = incl - mine;
         for (int w = w0; w < w1; ++w) {
             unsigned long long bits = m[w];
             pre[w] = base;
             while (bits) {
-                int b = __ffsll((long long)bits) - 1;
+                const int b = __ffsll((long long)bits) - 1;
                 X[base++] = (w << 6) + b;
                 bits &= bits - 1;
             }
         }
-        if (tid == 255) {
-            int Etot = sc[255];
+        if (lane == 63) {
+            const int Etot = incl;
             sE[env] = Etot;
             ed.E[ke] = Etot;
             if (ed.d_trace && st->pass < ed.max_passes)
@@ -438,7 +437,7 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
         list_sizes(ed, sE[1], m1);
         for (int l = 0; l < kNumLists; ++l) {
             const int tot = m0[l] + m1[l];
-            const int base = tot ? atomicAdd(&wl.count[l * kXcds + g], tot) : 0;
+            const int base = tot ? atomicAdd(&wl.count[count_slot(l, g)], tot) : 0;
             s_m[0][l] = m0[l];
             s_m[1][l] = m1[l];
             s_base[0][l] = base;
@@ -446,11 +445,11 @@ __global__ __launch_bounds__(256) void k_compact(const EpochDesc* __restrict__ e
         }
     }
     __syncthreads();
-    for (int env = 0; env < 2; ++env)
+    for (int e2 = 0; e2 < 2; ++e2)
         for (int l = 0; l < kNumLists; ++l)
-            for (int i = tid; i < s_m[env][l]; i += 256) {
-                int at = s_base[env][l] + i;
-                if (at < wl.cap[l][g]) wl.item[l][wl.off[l][g] + at] = make_int4(br.x, 2 * k + env, i, 0);
+            for (int i = tid; i < s_m[e2][l]; i += 128) {
+                int at = s_base[e2][l] + i;
+                if (at < wl.cap[l][g]) wl.item[l][wl.off[l][g] + at] = make_int4(br.x, 2 * k + e2, i, 0);
             }
 }
 
@@ -1690,7 +1689,7 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
             MEMD_TABLE_LAUNCH(k_flags_global, kMapFlags, kFlagTile, 0);
         break;
     case 1:
-        hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(256), 0, s, p.eps, p.tab[kMapCompact], p.wl);
+        hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(128), 0, s, p.eps, p.tab[kMapCompact], p.wl);
         break;
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
     case 3:
