@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in tile shapes (default: memd_autotune before the untimed warmup)")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="BLOCK_GROUPS,EVAL_TILE_WORDS: install these shapes for --n instead of autotuning "
+                         "(profiling runs repeat a bench run's choice without the tuner's own kernels)")
     ap.add_argument("--workload", choices=("m", "c1", "c2", "c3", "c4"), default="m",
                     help="m: 32ch x 4096 K=64 batched (headline); c1: 32ch x 30504 K=64 single epochs "
                          "vs the CPU oracle; c3: 512 uneven epochs batched vs plain; c4: served "
@@ -258,6 +261,8 @@ def bench_c4(a):
     ctx = memd_amd.MemdContext(0)
     # distinct inputs resident in HBM, reused cyclically (inputs are read-only)
     short = [(int(n), int(T), 2 * int(n)) for n, T, _ in synth.batch_shapes(48, 91)]
+    # built-in tile shapes: the autotuner times full-device batches, while a server at
+    # moderate load advances a few epochs per step
     pool_in = {"short": [], "long": []}
     Vs = {}
     for i, (n, T, K) in enumerate(short + [(32, 60000, 64)] * 4):
@@ -357,7 +362,11 @@ def main():
     dev = "cuda:0"
     ctx = memd_amd.MemdContext(0)
     tuning, tune_s = None, 0.0
-    if not a.no_autotune:
+    if a.tuning:
+        bg, tw = (int(v) for v in a.tuning.split(","))
+        ctx.set_tuning(n, bg, tw)
+        tuning = ctx.get_tuning(n)
+    elif not a.no_autotune:
         t0 = time.perf_counter()
         tuning = ctx.autotune(n, T, K)   # once per device and shape class; not timed
         tune_s = time.perf_counter() - t0

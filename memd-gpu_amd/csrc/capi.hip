@@ -915,29 +915,43 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
         if (block_bpw_fit((int)n, b) == b) bg.push_back(b);
     const bool had = ctx->tuning.count((int)n) != 0;
     const memd_tuning saved = tuning_for(ctx, (int)n);
-    memd_tuning best = default_tuning((int)n);
-    best.block_ms = best.eval_ms = 1e300;
+    const memd_tuning def = default_tuning((int)n);
+    std::vector<double> bms(bg.size(), 1e300), ems(tw.size(), 1e300);
     const size_t rounds = bg.size() > tw.size() ? bg.size() : tw.size();
     memd_opts o;
     memd_opts_default(&o);
     for (size_t r = 0; r <= rounds; ++r) {   // round 0 warms up (plan region, caches)
         const size_t i = r == 0 ? 0 : r - 1;
-        memd_tuning cand = default_tuning((int)n);
-        cand.block_groups = bg[i < bg.size() ? i : bg.size() - 1];
-        cand.eval_tile_words = tw[i < tw.size() ? i : tw.size() - 1];
+        const size_t ib = i < bg.size() ? i : bg.size() - 1, it = i < tw.size() ? i : tw.size() - 1;
+        memd_tuning cand = def;
+        cand.block_groups = bg[ib];
+        cand.eval_tile_words = tw[it];
         ctx->tuning[(int)n] = cand;
         memd_kernel_profile prof;
         rs = run_epochs(ctx, in, &o, nullptr, &prof);
         if (rs != MEMD_OK) break;
         if (r == 0) continue;
-        const double bms = prof.ms_per_launch[3], ems = prof.ms_per_launch[6];
-        if (bms < best.block_ms) { best.block_ms = bms; best.block_groups = cand.block_groups; }
-        if (ems < best.eval_ms) { best.eval_ms = ems; best.eval_tile_words = cand.eval_tile_words; }
+        bms[ib] = std::min(bms[ib], prof.ms_per_launch[3]);
+        ems[it] = std::min(ems[it], prof.ms_per_launch[6]);
     }
     if (had) ctx->tuning[(int)n] = saved;
     else ctx->tuning.erase((int)n);
     if (rs != MEMD_OK) return rs;
-    best.n = (int)n;
+    // fastest value of each knob; the built-in one unless another is faster by
+    // more than 3% (timing noise does not flip the choice)
+    auto pick = [](const std::vector<int>& vals, const std::vector<double>& t, int dflt, double* tbest) {
+        size_t b = 0, d = 0;
+        for (size_t i = 0; i < vals.size(); ++i) {
+            if (t[i] < t[b]) b = i;
+            if (vals[i] == dflt) d = i;
+        }
+        if (vals[d] == dflt && t[b] > 0.97 * t[d]) b = d;
+        *tbest = t[b];
+        return vals[b];
+    };
+    memd_tuning best = def;
+    best.block_groups = pick(bg, bms, def.block_groups, &best.block_ms);
+    best.eval_tile_words = pick(tw, ems, def.eval_tile_words, &best.eval_ms);
     ctx->tuning[(int)n] = best;
     if (out) *out = best;
     return MEMD_OK;
