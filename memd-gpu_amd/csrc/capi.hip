@@ -329,10 +329,11 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
         if (groups > 1) g = gmax;   // a multiple of kXcds, every XCD gets its share
         plan.grid[l] = (int)(g > 0 ? g : 1);
     }
-    int nmax = 0;
+    int nmax = 0, nlds = 0;
     std::vector<int> ns(count), bpws(count);
     for (int e = 0; e < count; ++e) {
         nmax = eds[e].n > nmax ? eds[e].n : nmax;
+        if (eds[e].n <= kFlagLdsMaxCh && eds[e].n > nlds) nlds = eds[e].n;
         ns[e] = eds[e].n;
         bpws[e] = eds[e].bpw;
     }
@@ -342,6 +343,7 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     plan.block_lds = bl.bytes;
     plan.mode = mode;
     plan.nmax = nmax;
+    plan.nlds = nlds;
     plan.eval_variant = variant;
     plan.coef_records = records ? 1 : 0;
     *out = plan;
@@ -488,6 +490,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         // key is the arena address plus the launch geometry
         std::string key = std::to_string((uintptr_t)ctx->arena) + "/" + std::to_string(count) + "/" +
                           std::to_string(plan.mode) + "/" + std::to_string(plan.nmax) + "/" +
+                          std::to_string(plan.nlds) + "/" +
                           std::to_string(plan.eval_variant) + "/b" + std::to_string(plan.block_lds) + ":" +
                           std::to_string(plan.block_rmax);
         for (int f = 0; f < kNumMaps; ++f) key += "/" + std::to_string(plan.blocks[f]);

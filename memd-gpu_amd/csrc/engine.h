@@ -161,6 +161,7 @@ struct StepPlan {
     int grid[kNumLists];    // persistent grid size per list
     int mode;               // kModeFix or kModeStop (all epochs of one call)
     int nmax;               // largest channel count in the batch
+    int nlds;               // largest channel count <= kFlagLdsMaxCh (k_flags' LDS tile), 0 if none
     int eval_variant;       // index into kEvalVariants
     int coef_records;       // launch k_coef (the evaluation reads the record array)
     int block_rmax;         // k_block: rows of the longest work item

@@ -211,18 +211,13 @@ __device__ __forceinline__ double project_at(const double* __restrict__ Vk, cons
     return acc;
 }
 
-// Wide-channel fallback (n > kFlagLdsMaxCh): one workgroup per (direction, tile),
-// signal read through the caches.
-__global__ __launch_bounds__(kFlagTile) void k_flags_global(const EpochDesc* __restrict__ eps,
-                                                     const int2* __restrict__ tab) {
-    const int2 br = tab[blockIdx.x];
-    const EpochDesc& ed = eps[br.x];
-    const EpochState* st = ed.st;
-    if (st->done) return;
+// Wide-channel path of k_flags (n > kFlagLdsMaxCh): one workgroup per (direction,
+// tile), signal read through the caches.
+__device__ void flags_wide(const EpochDesc& ed, const EpochState* st, int by) {
     const int n = ed.n, T = ed.T;
     int tiles = (T + kFlagTile - 1) / kFlagTile;
-    int k = br.y / tiles;
-    int tile = br.y - k * tiles;
+    int k = by / tiles;
+    int tile = by - k * tiles;
     const double* cur = cur_slot(ed, st->imf);
     const double* Vk = ed.V + (size_t)k * n;
     __shared__ double Ps[kFlagTile + 2];
@@ -270,6 +265,10 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
+    if (ed.n > kFlagLdsMaxCh) {   // its blocks are (direction, tile) pairs (blocks_for)
+        flags_wide(ed, st, br.y);
+        return;
+    }
     const int n = ed.n, T = ed.T, K = ed.K;
     const int tile = br.y;
     const int t0 = tile * kFlagTile;
@@ -1682,11 +1681,8 @@ static void launch_eval(const StepPlan& p, hipStream_t s) {
 
 static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     switch (which) {
-    case 0:
-        if (p.nmax <= kFlagLdsMaxCh)
-            MEMD_TABLE_LAUNCH(k_flags, kMapFlags, kFlagTile, (size_t)p.nmax * (kFlagTile + 2) * sizeof(double));
-        else
-            MEMD_TABLE_LAUNCH(k_flags_global, kMapFlags, kFlagTile, 0);
+    case 0:   // per epoch: LDS-staged tiles (n <= kFlagLdsMaxCh) or (direction, tile) blocks
+        MEMD_TABLE_LAUNCH(k_flags, kMapFlags, kFlagTile, (size_t)p.nlds * (kFlagTile + 2) * sizeof(double));
         break;
     case 1:
         hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(128), 0, s, p.eps, p.tab[kMapCompact], p.wl);

@@ -239,6 +239,25 @@ def test_batch_many_epochs_xcd_groups(ctx, oracle):
         assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_mixing_wide_and_narrow_epochs(ctx, oracle, mode):
+    # epochs above and below the 64-channel LDS projection limit in one batch: each
+    # takes its own projection path (the batch's widest epoch must not decide it)
+    import synth
+    eps = []
+    for i, (n, T, K) in enumerate([(96, 700, 12), (16, 1300, 32), (64, 900, 16), (8, 1100, 16), (72, 600, 8)]):
+        eps.append({"y": _dev(synth.eeg_epoch(n, T, 520 + i)), "V": _dev(oracle.directions(n, K)),
+                    "n_imf": 4, "S": 3})
+    outs, _ = ctx.run_batch(eps, mode=mode)
+    for ep, out in zip(eps, outs):
+        alone, _ = ctx.run(ep["y"], ep["V"], 4, 3, mode=mode)
+        assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
+    y1, V1 = eps[1]["y"].cpu().numpy(), eps[1]["V"].cpu().numpy()
+    imfs_o, info_o = oracle.memd(y1, V1, 4, 3, mode=mode)
+    used = info_o["n_imf_out"]
+    assert np.array_equal(outs[1].cpu().numpy()[:used], imfs_o[:used])
+
+
 def test_directions_f32_is_rounded_f64(ctx, oracle):
     # GenerateDirectionVector<float>, as the reference's main() instantiates it
     import torch
