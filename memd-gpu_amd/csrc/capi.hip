@@ -188,6 +188,11 @@ EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& 
     const memd_tuning tu = tuning_for(ctx, I.n);
     d.bpw = tu.block_groups;
     d.etw = tu.eval_tile_words;
+    // k_block items: at most 2K * groups; workgroups per epoch: the item count at a
+    // typical knot density (one knot per 16 samples), 1..256
+    d.bcap = 2 * I.K * ((d.nbmax + d.bpw - 1) / d.bpw);
+    const long typical = 2L * I.K * (((long)I.T / 512 + 1 + d.bpw - 1) / d.bpw);
+    d.bblocks = (int)std::min(256L, std::max(1L, typical));
     return d;
 }
 
@@ -219,6 +224,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
 struct PlanLayout {
     size_t desc_at = 0, tab_at[kNumMaps] = {}, list_at[kNumLists] = {};
     size_t cnt_at = 0, stats_at = 0, ndone_at = 0, fresh_at = 0, states_at = 0, total = 0;
+    size_t bitem_at = 0, bcnt_at = 0;   // per-epoch k_block items (int2) and counters (one line each)
     int nblk[kNumMaps] = {};
     long lcap[kNumLists] = {};
     long gcap[kNumLists][kXcds] = {};
@@ -257,6 +263,12 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
     }
     P->cnt_at = total;
     total += align_up(kCountBytes);
+    size_t bitems = 0;
+    for (int e = 0; e < count; ++e) bitems += (size_t)eds[e].bcap;
+    P->bitem_at = total;
+    total += align_up(sizeof(int2) * bitems);
+    P->bcnt_at = total;
+    total += align_up(kBcountStride * (size_t)count);
     P->stats_at = total;
     total += align_up(sizeof(unsigned long long) * 4);
     P->ndone_at = total;
@@ -279,7 +291,13 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     const int count = (int)eds.size();
     const int groups = P.groups;
     // the sub-list an epoch appends to is the one its capacity was planned in
-    for (int e = 0; e < count; ++e) eds[e].xcd = grp[e];
+    size_t bpos = 0;
+    for (int e = 0; e < count; ++e) {
+        eds[e].xcd = grp[e];
+        eds[e].bitem = (int2*)(region + P.bitem_at) + bpos;
+        eds[e].bcount = (int*)(region + P.bcnt_at + kBcountStride * (size_t)e);
+        bpos += (size_t)eds[e].bcap;
+    }
     HIP_TRY(ctx, hipMemcpyAsync(region + P.desc_at, eds.data(), sizeof(EpochDesc) * count, hipMemcpyHostToDevice, s));
     tabs.clear();
     for (int f = 0; f < kNumMaps; ++f) {
@@ -1117,6 +1135,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     if (rs != MEMD_OK) return rs;
     // counters are zero between steps; a new region starts from zero too
     HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, kCountBytes, s));
+    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.bcnt_at, 0, kBcountStride * S->eds.size(), s));
     HIP_TRY(ctx, hipMemsetAsync(ndone, 0, sizeof(int), s));
     S->fresh.clear();
     for (int64_t id : joined) {

@@ -48,6 +48,7 @@ __host__ __device__ inline int blocks_per_item(int n) {
     return b < 1 ? 1 : (b > 4 ? 4 : b);
 }
 constexpr int kBlockBpwMax = 8;
+constexpr int kBlockThreads = 256;   // k_block workgroup size
 // k_block's dynamic LDS: per-row factors for rmax rows, knot positions and the
 // tile of the widest epoch.  rmax = 32 * max bpw over the batch.
 struct BlockLds {
@@ -120,6 +121,13 @@ struct EpochDesc {
     int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
     int bpw;                   // solver blocks per k_block work item
+    // k_block work items (ke, group) of this epoch, appended by k_compact and
+    // drained by the epoch's bblocks workgroups (an XCD works through one epoch's
+    // items at a time, so the epoch's signal stays in its L2)
+    int2* bitem;
+    int* bcount;
+    int bcap;
+    int bblocks;
     int etw;                   // k_eval_fused tile: 64 * etw samples x 16 / etw channels (etw in {1, 2, 4})
     int spline;                // 0 natural ends, 1 not-a-knot
     double* mend;              // [K][2][n][2] not-a-knot: M_0 and M_{E-1} per channel
@@ -127,11 +135,11 @@ struct EpochDesc {
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its
 // local block tab[b].y.
-enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kNumMaps };
+enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kMapBlock, kNumMaps };
 
 // List-driven launches: k_compact appends the exact work items (epoch, ke, sub)
 // of every valid direction; persistent grids drain the lists.
-enum : int { kListFactor = 0, kListBlock, kListReduced, kListCoef, kNumLists };
+enum : int { kListFactor = 0, kListReduced, kListCoef, kNumLists };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup id mod 8).
 // With >= kXcds epochs in a batch, every epoch is pinned to XCD (epoch mod 8):
@@ -143,6 +151,7 @@ constexpr int kXcds = 8;
 constexpr int kCountStride = 32;
 __host__ __device__ inline int count_slot(int list, int g) { return (list * kXcds + g) * kCountStride; }
 constexpr size_t kCountBytes = sizeof(int) * kCountStride * kNumLists * kXcds;
+constexpr size_t kBcountStride = 128;   // bytes between the per-epoch k_block item counters
 
 struct WorkLists {
     int4* item[kNumLists];

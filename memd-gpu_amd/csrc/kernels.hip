@@ -198,6 +198,7 @@ __global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps,
         st->n_imf_out = (ed.N_imf <= 1) ? 1 : 0;
         if (ed.N_imf <= 1) atomicAdd(ed.ndone, 1);
         for (int i2 = 0; i2 < ed.N_imf; ++i2) ed.sift_counts[i2] = 0;
+        *ed.bcount = 0;
         if (br.x == 0) zero_counters(wl);
     }
 }
@@ -364,12 +365,17 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
 
 // ------------------------------------------------------------ compaction
 
+// k_block work items of one direction-envelope with E knots (groups of bpw
+// solver blocks).
+__host__ __device__ inline int block_items(int E, int bpw) {
+    const int J = (E - 3) / kSolverBlock;
+    return (J + 1 + bpw - 1) / bpw;
+}
+
 // Work items one direction-envelope with E knots contributes to each solver list.
 __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     const int J = (E - 3) / kSolverBlock;
-    const int bpw = ed.bpw;
     m[kListFactor] = 1;
-    m[kListBlock] = (J + 1 + bpw - 1) / bpw;
     m[kListReduced] = (J > 0 || ed.spline) ? 1 : 0;   // not-a-knot: end values
     m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
 }
@@ -443,13 +449,25 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
             s_base[1][l] = base + m0[l];
         }
     }
+    __shared__ int s_bbase, s_bm[2];
+    if (tid == 64) {   // the epoch's own k_block items, both envelopes together
+        const int b0 = block_items(sE[0], ed.bpw), b1 = block_items(sE[1], ed.bpw);
+        s_bm[0] = b0;
+        s_bm[1] = b1;
+        s_bbase = atomicAdd(ed.bcount, b0 + b1);
+    }
     __syncthreads();
-    for (int e2 = 0; e2 < 2; ++e2)
+    for (int e2 = 0; e2 < 2; ++e2) {
         for (int l = 0; l < kNumLists; ++l)
             for (int i = tid; i < s_m[e2][l]; i += 128) {
                 int at = s_base[e2][l] + i;
                 if (at < wl.cap[l][g]) wl.item[l][wl.off[l][g] + at] = make_int4(br.x, 2 * k + e2, i, 0);
             }
+        for (int i = tid; i < s_bm[e2]; i += 128) {
+            const int at = s_bbase + (e2 ? s_bm[0] : 0) + i;
+            if (at < ed.bcap) ed.bitem[at] = make_int2(2 * k + e2, i);
+        }
+    }
 }
 
 // ------------------------------------------------- blocked tridiagonal solver
@@ -606,8 +624,8 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 //      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
 //      window of Y values in registers keeping every read ahead of the writes.
 //   3. Lanes over (channel, row) pairs store the block rows' y (coalesced).
-__global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps, WorkLists wl, int rmax,
-                                               int tile_off) {
+__global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __restrict__ eps,
+                                                         const int2* __restrict__ tab, int rmax, int tile_off) {
     constexpr int L = kSolverBlock;
     extern __shared__ double lds[];
     double* s_rd = lds;                   // [rmax]
@@ -616,18 +634,19 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
     double* s_ih = lds + 3 * rmax;        // [rmax + 2]
     int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 2);   // [rmax + 3]
     double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);   // [n][RS], RS = (R + 3) | 1
-    const ListCursor cu = list_cursor(wl, kListBlock);
+    const int2 br = tab[blockIdx.x];   // (epoch, workgroup of the epoch)
+    const EpochDesc& ed = eps[br.x];
     const int tid = threadIdx.x;
-    for (int w = cu.start; w < cu.cnt; w += cu.stride) {
-        const int4 it = cu.base[w];
-        const EpochDesc& ed = eps[it.x];
+    const int cnt = min(*ed.bcount, ed.bcap);
+    for (int w = br.y; w < cnt; w += ed.bblocks) {
+        const int2 it = ed.bitem[w];
         const EpochState* st = ed.st;
-        const int ke = it.y, env = ke & 1;
+        const int ke = it.x, env = ke & 1;
         const int n = ed.n, T = ed.T;
         const int E = ed.E[ke];
         const int J = block_geom(E, 0).J;
         const int bpw = ed.bpw;
-        const int b0 = it.z * bpw;
+        const int b0 = it.y * bpw;
         const int nb = min(bpw, J - b0 + 1);
         const int p0 = b0 * L + 1;
         const int qlast = block_geom(E, b0 + nb - 1).q;
@@ -645,8 +664,8 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
-        for (int r = tid; r < RW; r += 256) sX[r] = X[min(p0 - 1 + r, E - 1)];
-        for (int r = tid; r < R; r += 256) {
+        for (int r = tid; r < RW; r += kBlockThreads) sX[r] = X[min(p0 - 1 + r, E - 1)];
+        for (int r = tid; r < R; r += kBlockThreads) {
             const int u = p0 + r;
             const double rd = gptr(fac_row(ed, ke, kFacRd))[u], cpv = gptr(fac_row(ed, ke, kFacCp))[u];
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
@@ -658,19 +677,19 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         }
         // reciprocals of intervals p0-1 .. qlast+1 (the last one: the next separator's
         // right-hand side; clamped where the group ends the system)
-        for (int r = tid; r <= R + 1; r += 256) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[min(p0 - 1 + r, E - 2)];
+        for (int r = tid; r <= R + 1; r += kBlockThreads) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[min(p0 - 1 + r, E - 2)];
         __syncthreads();
         // 1b. knot values: lanes over (channel, tile row) pairs
         const int X0 = sX[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
         const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
         // GU gathers per lane are issued before any is used (their latencies overlap)
         constexpr int GU = 8;
-        for (int i0 = tid; i0 < n * RW; i0 += 256 * GU) {
+        for (int i0 = tid; i0 < n * RW; i0 += kBlockThreads * GU) {
             double yv[GU];
             int cv[GU], rv[GU];
 #pragma unroll
             for (int g = 0; g < GU; ++g) {
-                const int idx = min(i0 + 256 * g, n * RW - 1);
+                const int idx = min(i0 + kBlockThreads * g, n * RW - 1);
                 const int c = (int)(((unsigned long long)idx * rwmagic) >> 32);
                 const int r = idx - c * RW;
                 cv[g] = c;
@@ -679,7 +698,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
             }
 #pragma unroll
             for (int g = 0; g < GU; ++g) {
-                if (i0 + 256 * g >= n * RW) break;
+                if (i0 + kBlockThreads * g >= n * RW) break;
                 const int c = cv[g], r = rv[g];
                 const int u = min(p0 - 1 + r, E - 1);
                 if (E >= 6 && (u == 0 || u == E - 1)) continue;   // clamped end knots: below
@@ -694,7 +713,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         const int rE = E - 1 - (p0 - 1);
         const bool has_left = p0 == 1, has_right = rE <= RW - 1;
         if (E >= 6 && (has_left || has_right)) {
-            for (int c = tid; c < n; c += 256) {
+            for (int c = tid; c < n; c += kBlockThreads) {
                 const unsigned base = (unsigned)c * (unsigned)T;
                 if (has_left) {
                     const double y = clamp_end(cur[base + X0], cur[base + X1], cur[base + X2], X0, X1, X2, env, true);
@@ -711,7 +730,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         }
         __syncthreads();
         // 2. per (block, channel) lane; n * nb may exceed the workgroup
-        for (int lane = tid; lane < n * nb; lane += 256) {   // separator rows q+1 keep their right-hand side
+        for (int lane = tid; lane < n * nb; lane += kBlockThreads) {   // separator rows q+1 keep their right-hand side
             const int slot = lane / n, c = lane - slot * n;
             if (b0 + slot >= J) continue;
             const BlockGeom g = block_geom(E, b0 + slot);
@@ -722,7 +741,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
             ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = 3.0 * (sl - slm);
         }
         __syncthreads();
-        for (int lane = tid; lane < n * nb; lane += 256) {
+        for (int lane = tid; lane < n * nb; lane += kBlockThreads) {
             const int slot = lane / n, c = lane - slot * n;
             const BlockGeom g = block_geom(E, b0 + slot);
             const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
@@ -757,7 +776,7 @@ __global__ __launch_bounds__(256) void k_block(const EpochDesc* __restrict__ eps
         __syncthreads();
         // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped;
         //    row N = 32(J+1) is a block row)
-        for (int idx = tid; idx < n * R; idx += 256) {
+        for (int idx = tid; idx < n * R; idx += kBlockThreads) {
             const int c2 = (int)(((unsigned long long)idx * rmagic) >> 32);
             const int r = idx - c2 * R;
             const int u = p0 + r;
@@ -1375,6 +1394,7 @@ __global__ void k_advance(const EpochDesc* __restrict__ eps, int count, WorkList
     if (e == 0) zero_counters(wl);
     if (e >= count) return;
     const EpochDesc& ed = eps[e];
+    *ed.bcount = 0;
     EpochState* st = ed.st;
     if (st->done) return;
     st->pass += 1;
@@ -1553,6 +1573,7 @@ __global__ __launch_bounds__(256) void k_stop_apply(const EpochDesc* __restrict_
 __global__ void k_stop_reset(const EpochDesc* __restrict__ eps, int count, WorkLists wl) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e == 0) zero_counters(wl);
+    if (e < count) *eps[e].bcount = 0;
     if (e < count && eps[e].st->done) eps[e].st->action = kActNone;
 }
 
@@ -1597,6 +1618,7 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
         if (e.n <= kFlagLdsMaxCh) return (e.T + kFlagTile - 1) / kFlagTile;
         return e.K * ((e.T + kFlagTile - 1) / kFlagTile);
     case kMapCompact: return e.K;
+    case kMapBlock: return e.bblocks;
     case kMapEval:
         if (e.mode == kModeStop) return (e.T + 63) / 64;
         if (eval_variant == kEvalFused)
@@ -1634,11 +1656,8 @@ BlockLds block_lds(const int* n, const int* bpw, int count) {
 
 long list_capacity(int list, const EpochDesc& e) {
     const long K2 = 2L * e.K;
-    const long J = (e.Ecap - 3) / kSolverBlock;
-    const long bpw = e.bpw;
     switch (list) {
     case kListFactor: return K2;
-    case kListBlock: return K2 * ((J + 1 + bpw - 1) / bpw);
     case kListReduced: return K2;
     case kListCoef: return e.coef_records ? K2 * (((long)e.n * e.Ecap + 256L * kCoefItems - 1) / (256L * kCoefItems)) : 0;
     }
@@ -1689,8 +1708,8 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         break;
     case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
     case 3:
-        hipLaunchKernelGGL(k_block, dim3(p.grid[kListBlock]), dim3(256), p.block_lds, s, p.eps, p.wl, p.block_rmax,
-                           (int)p.block_tile_off);
+        hipLaunchKernelGGL(k_block, dim3(p.blocks[kMapBlock]), dim3(kBlockThreads), p.block_lds, s, p.eps,
+                           p.tab[kMapBlock], p.block_rmax, (int)p.block_tile_off);
         break;
     case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
     case 5:   // the fused evaluation forms M itself; records only for the record readers
