@@ -341,8 +341,8 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
             plan.wl.cap[l][x] = (int)P.gcap[l][x];
             at += P.gcap[l][x];
         }
-        // one-wave workgroups for the factor list: a larger persistent grid
-        const long gmax = (l == kListFactor) ? 4L * kListGrid : (long)kListGrid;
+        // one-wave workgroups for the factor and reduced lists: a larger persistent grid
+        const long gmax = (l == kListFactor || l == kListReduced) ? 4L * kListGrid : (long)kListGrid;
         long g = P.lcap[l] < gmax ? P.lcap[l] : gmax;
         if (groups > 1) g = gmax;   // a multiple of kXcds, every XCD gets its share
         plan.grid[l] = (int)(g > 0 ? g : 1);

@@ -802,9 +802,9 @@ __device__ __forceinline__ double knot_M(const double* row, const double* sv, co
     return m;
 }
 
-// Phase 2: separator values, one lane per channel (no division: the reduced
-// factorisation comes from k_factor).
-__global__ __launch_bounds__(256) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
+// Phase 2: separator values, one lane per channel, one wave per (direction,
+// envelope) (no division: the reduced factorisation comes from k_factor).
+__global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
     const ListCursor cu = list_cursor(wl, kListReduced);
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
@@ -1711,7 +1711,7 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         hipLaunchKernelGGL(k_block, dim3(p.blocks[kMapBlock]), dim3(kBlockThreads), p.block_lds, s, p.eps,
                            p.tab[kMapBlock], p.block_rmax, (int)p.block_tile_off);
         break;
-    case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 256); break;
+    case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 64); break;
     case 5:   // the fused evaluation forms M itself; records only for the record readers
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
         break;
