@@ -24,14 +24,17 @@ constexpr int kNumEvalVariants = 7;
 // in {1, 2, 4}.
 constexpr int kEvalFused = 6;
 // Intervals meeting a tile of 64 * TW samples: at most 32 * TW knots of one kind
-// lie inside it (strict extrema are never adjacent), so at most 32 * TW + 1.
-__host__ __device__ inline int eval_max_int(int tw) { return 32 * tw + 2; }
+// lie in its first 64 * TW - 1 samples (strict extrema are never adjacent; the
+// forced knot at t = 0 next to an extremum at t = 1 still leaves at most 32 in
+// samples 0..62), so at most 32 * TW + 1 intervals.
+__host__ __device__ inline int eval_max_int(int tw) { return 32 * tw + 1; }
 // LDS budget of one k_eval_fused workgroup.  kEvalCap records hold one
 // direction's worst case at every tile shape: 2 envelopes x channels x odd row
-// stride, at most 2 x 16 x 35 (TW = 1), 2 x 8 x 67, 2 x 4 x 131.
-constexpr int kEvalCap = 1120;
+// stride, at most 2 x 16 x 33 (TW = 1), 2 x 8 x 65, 2 x 4 x 129.
+constexpr int kEvalCap = 1056;
 constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
 constexpr int kEvalBatch = 8;      // directions per batch
+constexpr int kEvalChunk = 32;     // directions whose formation inputs are staged together
 constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
 constexpr int kFlagLdsMaxCh = 64;

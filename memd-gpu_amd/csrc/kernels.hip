@@ -1160,6 +1160,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     __shared__ int sCum[NS][4];          // knots before each word of the tile
     __shared__ unsigned long long sMsk[NS][4];
     __shared__ int sNb;
+    __shared__ int cE[2 * kEvalChunk], cPr[2 * kEvalChunk];   // staged formation inputs
+    __shared__ unsigned long long cM[2 * kEvalChunk][4];
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
@@ -1187,16 +1189,36 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
+    int kc1 = 0;   // end of the staged chunk of formation inputs
     for (int kb = 0; kb < K;) {
         if (tid < 64) {   // formation
+            if (kb >= kc1) {   // stage the next kEvalChunk directions: lane (direction, envelope)
+                const int kk = kb + (tid >> 1), ke = 2 * kk + (tid & 1);
+                int E = 0, pr = 0;
+                unsigned long long m[4] = {0, 0, 0, 0};
+                if (kk < K) {
+                    E = ed.E[ke];
+                    pr = ed.pre[(size_t)ke * W + w0];
+#pragma unroll
+                    for (int w = 0; w < 4; ++w)
+                        if (w < TW && w0 + w < W) m[w] = ed.mask[(size_t)ke * W + w0 + w];
+                }
+                cE[tid] = E;
+                cPr[tid] = pr;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) cM[tid][w] = m[w];
+                kc1 = kb + kEvalChunk;
+            }
             const int k = kb + (tid >> 1), env = tid & 1;
-            const bool have = tid < NS && k < K;
+            // a batch stays inside the staged chunk
+            const bool have = tid < NS && k < K && k < kc1;
+            const int ci = 2 * (k - (kc1 - kEvalChunk)) + env;   // chunk entry
             int E = 0, pr = 0, inside = 0;
             int cum[4];
             unsigned long long msk[4];
             if (have) {
-                E = ed.E[2 * k + env];
-                pr = ed.pre[(size_t)(2 * k + env) * W + w0];
+                E = cE[ci];
+                pr = cPr[ci];
             }
             int run = pr;
 #pragma unroll
@@ -1204,7 +1226,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 cum[w] = run;
                 msk[w] = 0;
                 if (have && w < TW && w0 + w < W) {
-                    const unsigned long long m = ed.mask[(size_t)(2 * k + env) * W + w0 + w];
+                    const unsigned long long m = cM[ci][w];
                     const int lim = tlast - 64 * (w0 + w);   // bits below tlast in this word
                     msk[w] = m;
                     inside += lim >= 64 ? (int)__popcll(m) : (lim > 0 ? (int)__popcll(m & ((1ull << lim) - 1ull)) : 0);
