@@ -76,7 +76,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, mend, mean, tstat, red, sift, st, total;
+    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -97,6 +97,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.ysol = take((size_t)K * 2 * n * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
     L.sep = take((size_t)K * 2 * n * nb * 8);
+    L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
     L.mend = take((size_t)K * 2 * n * 2 * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
@@ -209,6 +210,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.ysol = (double*)(b + L.ysol);
     d.coef = (double*)(b + L.coef);
     d.sep = (double*)(b + L.sep);
+    d.sepin = (double*)(b + L.sepin);
     d.mend = (double*)(b + L.mend);
     d.mean = (double*)(b + L.mean);
     d.tstat = (double*)(b + L.tstat);

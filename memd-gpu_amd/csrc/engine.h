@@ -111,7 +111,8 @@ struct EpochDesc {
     double* yknot;         // [K][2][n][Ecap] knot values Y (end knots clamped)
     double* ysol;          // [K][2][n][Ecap] per channel: block rows hold y, separator rows r
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
-    double* sep;           // [K][2][n][nbmax] separator values (z, then X_j)
+    double* sep;           // [K][2][nbmax][n] separator values (z, then X_j), separator-major
+    double* sepin;         // [K][2][nbmax][3][n] y before, right-hand side, y after each separator
     double* mean;          // [n][T] stop mode: envelope mean
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
     int* stopred;          // [4] stop mode: reduction scratch

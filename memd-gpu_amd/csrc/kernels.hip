@@ -662,6 +662,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
+        const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
         for (int r = tid; r < RW; r += kBlockThreads) sX[r] = X[min(p0 - 1 + r, E - 1)];
@@ -738,7 +739,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             const double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
             const double sl = (Yc[len + 2] - Yc[len + 1]) * s_ih[off + len + 1];
             const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
-            ys[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(q + 1)] = 3.0 * (sl - slm);
+            sip[((size_t)(b0 + slot + 1) * 3 + 1) * n + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
         }
         __syncthreads();
         for (int lane = tid; lane < n * nb; lane += kBlockThreads) {
@@ -781,7 +782,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             const int r = idx - c2 * R;
             const int u = p0 + r;
             if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
-            ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = sY[c2 * RS + r + 1];
+            const double yv = sY[c2 * RS + r + 1];
+            ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = yv;
+            // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
+            if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * n + c2] = yv;
+            if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * n + c2] = yv;
         }
     }
 }
@@ -789,16 +794,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
 // Second derivative M_u of knot row u of one channel after phase 2: phase-3
 // combination inside blocks, the separator value at separator rows, and the end
 // values (0 for natural ends, me[0] / me[1] for not-a-knot).
-__device__ __forceinline__ double knot_M(const double* row, const double* sv, const double* al,
+// sv[j * sst]: separator value j of the channel.
+__device__ __forceinline__ double knot_M(const double* row, const double* sv, int sst, const double* al,
                                          const double* be, int u, int E, int J, const double* me) {
     constexpr int L = kSolverBlock;
     if (u == 0) return me ? me[0] : 0.0;
     if (u == E - 1) return me ? me[1] : 0.0;
-    if (u % L == 0 && u / L <= J) return sv[u / L];
+    if (u % L == 0 && u / L <= J) return sv[(u / L) * sst];
     int b = (u - 1) / L;
     double m = row[u];
-    if (b >= 1) m = fma(-sv[b], al[u], m);
-    if (b < J) m = fma(-sv[b + 1], be[u], m);
+    if (b >= 1) m = fma(-sv[b * sst], al[u], m);
+    if (b < J) m = fma(-sv[(b + 1) * sst], be[u], m);
     return m;
 }
 
@@ -820,8 +826,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
         const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
             const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
-            double* sv = ed.sep + ((size_t)ke * n + c) * ed.nbmax;
-            const GPtr<double> rowg = gptr(row);
+            // separator-major arrays: lanes (channels) read and write consecutive words
+            double* sv = ed.sep + (size_t)ke * ed.nbmax * n + c;   // sv[j * n]
+            const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * n + c;   // sip[(3j + v) * n]
+            const GPtr<double> sing = gptr(sip);
             const GMut<double> svg = gmut(sv);
             const GPtr<double> Rlog = gptr(Rlo), Rrdg = gptr(Rrd), Rcpg = gptr(Rcp);
             const GPtr<int> Xg = gptr(X);
@@ -834,9 +842,9 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
 #pragma unroll
                 for (int a = 0; a < RC; ++a) {
                     const int j = min(j0 + a, J), s0 = j * L;
-                    ym[a] = rowg[s0 - 1];
-                    yc[a] = rowg[s0];
-                    yp[a] = rowg[s0 + 1];
+                    ym[a] = sing[(3 * j + 0) * n];   // y of row s0-1, r of row s0, y of row s0+1
+                    yc[a] = sing[(3 * j + 1) * n];
+                    yp[a] = sing[(3 * j + 2) * n];
                     rl[a] = Rlog[j];
                     rr[a] = Rrdg[j];
                     xm[a] = Xg[s0 - 1];
@@ -850,7 +858,7 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                     const double lo = (double)(x0[a] - xm[a]), up = (double)(xp[a] - x0[a]);
                     const double D = fma(-up, yp[a], fma(-lo, ym[a], yc[a]));
                     z = (j == 1) ? D * rr[a] : fma(-rl[a], z, D) * rr[a];
-                    svg[j] = z;
+                    svg[j * n] = z;
                 }
             }
             // back substitution, RC separators at a time
@@ -861,14 +869,14 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                 for (int a = 0; a < RC; ++a) {
                     const int j = max(j1 - a, 1);
                     cpj[a] = Rcpg[j];
-                    zj[a] = svg[j];
+                    zj[a] = svg[j * n];
                 }
 #pragma unroll
                 for (int a = 0; a < RC; ++a) {
                     const int j = j1 - a;
                     if (j < 1) break;
                     x = fma(-cpj[a], x, zj[a]);
-                    svg[j] = x;
+                    svg[j * n] = x;
                 }
             }
             if (ed.spline) {
@@ -877,10 +885,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                 const double* al = fac_row(ed, ke, kFacAl);
                 const double* be = fac_row(ed, ke, kFacBe);
                 const double* ihv = fac_row(ed, ke, kFacIh);
-                double M1 = knot_M(row, sv, al, be, 1, E, J, nullptr);
-                double M2 = knot_M(row, sv, al, be, 2, E, J, nullptr);
-                double MN1 = knot_M(row, sv, al, be, E - 3, E, J, nullptr);
-                double MN = knot_M(row, sv, al, be, E - 2, E, J, nullptr);
+                double M1 = knot_M(row, sv, n, al, be, 1, E, J, nullptr);
+                double M2 = knot_M(row, sv, n, al, be, 2, E, J, nullptr);
+                double MN1 = knot_M(row, sv, n, al, be, E - 3, E, J, nullptr);
+                double MN = knot_M(row, sv, n, al, be, E - 2, E, J, nullptr);
                 double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
                 double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
                 double* me = ed.mend + ((size_t)ke * n + c) * 2;
@@ -906,15 +914,15 @@ struct KnotMIn {
 };
 
 __device__ __forceinline__ KnotMIn knot_m_load(const double* row, const GPtr<double>& al, const GPtr<double>& be,
-                                                const double* sv, const double* me, int u, int E, int J) {
+                                                const double* sv, int sst, const double* me, int u, int E, int J) {
     const int b = max(u - 1, 0) >> 5;
     KnotMIn v;
     v.y = row[u];
     v.al = al[u];
     v.be = be[u];
-    v.s0 = sv[b];
-    v.s1 = sv[min(b + 1, J)];
-    v.ss = sv[min(u >> 5, J)];
+    v.s0 = sv[b * sst];
+    v.s1 = sv[min(b + 1, J) * sst];
+    v.ss = sv[min(u >> 5, J) * sst];
     v.me = me ? me[u == 0 ? 0 : 1] : 0.0;
     return v;
 }
@@ -975,10 +983,10 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 js[a] = j;
                 const size_t rr = kerow + c;
                 const double* row = ed.ysol + rr * ed.Ecap;
-                const double* sv = ed.sep + rr * ed.nbmax;
+                const double* sv = ed.sep + (size_t)ke * ed.nbmax * n + cs[a];   // sv[j * n]
                 const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
-                v0[a] = knot_m_load(row, al, be, sv, me, j, E, J);
-                v1[a] = knot_m_load(row, al, be, sv, me, j + 1, E, J);
+                v0[a] = knot_m_load(row, al, be, sv, n, me, j, E, J);
+                v1[a] = knot_m_load(row, al, be, sv, n, me, j + 1, E, J);
                 y0[a] = ed.yknot[rr * ed.Ecap + j];
                 y1[a] = ed.yknot[rr * ed.Ecap + j + 1];
                 x0[a] = X[j];
@@ -1300,7 +1308,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int bj = max(j - 1, 0) >> 5;
                 const size_t rowoff = ((size_t)ke * n + c2) * ed.Ecap;
                 const GPtr<double> Yr = gptr(ed.yknot + rowoff), Mr = gptr(ed.ysol + rowoff);
-                const GPtr<double> Sr = gptr(ed.sep + ((size_t)ke * n + c2) * ed.nbmax);
+                const GPtr<double> Sr = gptr(ed.sep + (size_t)ke * ed.nbmax * n + c2);   // Sr[j * n]
                 const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
                 x0[a] = Xr[j];
@@ -1315,9 +1323,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 b1v[a] = F[kFacBe * ed.Ecap + j + 1];
                 ih[a] = F[kFacIh * ed.Ecap + j];
                 i3h[a] = F[kFacI3h * ed.Ecap + j];
-                s0v[a] = Sr[bj];
-                s1v[a] = Sr[min(bj + 1, J)];
-                s2v[a] = Sr[min(bj + 2, J)];
+                s0v[a] = Sr[bj * n];
+                s1v[a] = Sr[min(bj + 1, J) * n];
+                s2v[a] = Sr[min(bj + 2, J) * n];
                 me0[a] = 0.0;
                 me1[a] = 0.0;
                 if (ed.spline) {
