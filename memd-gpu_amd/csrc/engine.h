@@ -108,8 +108,8 @@ struct EpochDesc {
     double* fac;           // [K][2][6][Ecap] per knot row, shared by channels:
                            //   rd, cp (block factors), al, be (spikes), ih = 1/h, i3h = 1/(3h)
     double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
-    double* yknot;         // [K][2][n][Ecap] knot values Y (end knots clamped)
-    double* ysol;          // [K][2][n][Ecap] per channel: block rows hold y, separator rows r
+    double* yknot;         // [K][2][Ecap][n] knot values Y (end knots clamped), knot-major
+    double* ysol;          // [K][2][Ecap][n] block solutions y at block rows (separator rows unused)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][nbmax][n] separator values (z, then X_j), separator-major
     double* sepin;         // [K][2][nbmax][3][n] y before, right-hand side, y after each separator

@@ -617,13 +617,13 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 //      LDS; lanes over (channel, row) pairs gather the knot values Y of rows
 //      p0-1 .. qlast+2 of every channel into an LDS tile (end knots clamped,
 //      memReducedEpochs_dev.cu:447-486, 545-584) and write Y of rows p0 .. qlast+1
-//      to yknot (coalesced).
+//      to yknot (knot-major, channel-minor: coalesced over channels).
 //   2. Lanes (block, channel): the right-hand side of the separator after the
 //      block (it reads the next block's first row) is formed first; after a
 //      barrier, r_u = 3 (sl_u - sl_{u-1}) with sl_u = (Y_{u+1} - Y_u) * ih_u and the
 //      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
 //      window of Y values in registers keeping every read ahead of the writes.
-//   3. Lanes over (channel, row) pairs store the block rows' y (coalesced).
+//   3. Lanes over (row, channel) pairs store the block rows' y (coalesced).
 __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __restrict__ eps,
                                                          const int2* __restrict__ tab, int rmax, int tile_off) {
     constexpr int L = kSolverBlock;
@@ -653,15 +653,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         const int R = qlast - p0 + 1;             // rows of the group, <= rmax
         const int RW = R + 3;                     // tile rows p0-1 .. qlast+2
         const int RS = RW | 1;                    // LDS row stride (odd: conflict-free columns)
-        // idx / D as (idx * ceil(2^32 / D)) >> 32 in 64 bits: exact for idx * (D - 1) < 2^32
-        // (the tile bounds idx < n * RW by 160 KB / 8 and D <= 259 here); the
-        // constant is 2^32 itself when D = 1
-        const unsigned long long rwmagic = (0x100000000ull + (unsigned)RW - 1) / (unsigned)RW;
-        const unsigned long long rmagic = (0x100000000ull + (unsigned)R - 1) / (unsigned)R;
+        // idx / n as (idx * ceil(2^32 / n)) >> 32 in 64 bits: exact for idx * (n - 1) < 2^32
+        // (the tile bounds idx < n * RW by 160 KB / 8 and n <= 256 here); the constant
+        // is 2^32 itself when n = 1
+        const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
-        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * n * ed.Ecap);
-        const GMut<double> ys = gmut(ed.ysol + (size_t)ke * n * ed.Ecap);
+        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * n);   // [knot][channel]
+        const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * n);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
@@ -690,9 +689,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             int cv[GU], rv[GU];
 #pragma unroll
             for (int g = 0; g < GU; ++g) {
-                const int idx = min(i0 + kBlockThreads * g, n * RW - 1);
-                const int c = (int)(((unsigned long long)idx * rwmagic) >> 32);
-                const int r = idx - c * RW;
+                const int idx = min(i0 + kBlockThreads * g, n * RW - 1);   // (tile row, channel)
+                const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
+                const int c = idx - r * n;
                 cv[g] = c;
                 rv[g] = r;
                 yv[g] = cur[(unsigned)c * (unsigned)T + (unsigned)sX[r]];
@@ -706,7 +705,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
                 const double y = yv[g];
                 sY[c * RS + r] = y;
                 if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
-                    yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)u] = y;
+                    yk[(unsigned)u * (unsigned)n + (unsigned)c] = y;
             }
         }
         // end knots of E >= 6 clamped (tile row 0 is knot 0 in the first group; rows
@@ -719,13 +718,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
                 if (has_left) {
                     const double y = clamp_end(cur[base + X0], cur[base + X1], cur[base + X2], X0, X1, X2, env, true);
                     sY[c * RS] = y;
-                    yk[(unsigned)c * (unsigned)ed.Ecap] = y;
+                    yk[c] = y;
                 }
                 if (has_right) {
                     const double y =
                         clamp_end(cur[base + XE1], cur[base + XE2], cur[base + XE3], XE1, XE2, XE3, env, false);
                     for (int r = rE; r < RW; ++r) sY[c * RS + r] = y;
-                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)c * (unsigned)ed.Ecap + (unsigned)(E - 1)] = y;
+                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)(E - 1) * (unsigned)n + (unsigned)c] = y;
                 }
             }
         }
@@ -777,13 +776,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         __syncthreads();
         // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped;
         //    row N = 32(J+1) is a block row)
-        for (int idx = tid; idx < n * R; idx += kBlockThreads) {
-            const int c2 = (int)(((unsigned long long)idx * rmagic) >> 32);
-            const int r = idx - c2 * R;
+        for (int idx = tid; idx < n * R; idx += kBlockThreads) {   // (row, channel)
+            const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
+            const int c2 = idx - r * n;
             const int u = p0 + r;
             if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
             const double yv = sY[c2 * RS + r + 1];
-            ys[(unsigned)c2 * (unsigned)ed.Ecap + (unsigned)u] = yv;
+            ys[(unsigned)u * (unsigned)n + (unsigned)c2] = yv;
             // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
             if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * n + c2] = yv;
             if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * n + c2] = yv;
@@ -794,15 +793,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
 // Second derivative M_u of knot row u of one channel after phase 2: phase-3
 // combination inside blocks, the separator value at separator rows, and the end
 // values (0 for natural ends, me[0] / me[1] for not-a-knot).
-// sv[j * sst]: separator value j of the channel.
-__device__ __forceinline__ double knot_M(const double* row, const double* sv, int sst, const double* al,
+// row[u * rst]: the channel's block solution at row u; sv[j * sst]: its separator
+// value j.
+__device__ __forceinline__ double knot_M(const double* row, int rst, const double* sv, int sst, const double* al,
                                          const double* be, int u, int E, int J, const double* me) {
     constexpr int L = kSolverBlock;
     if (u == 0) return me ? me[0] : 0.0;
     if (u == E - 1) return me ? me[1] : 0.0;
     if (u % L == 0 && u / L <= J) return sv[(u / L) * sst];
     int b = (u - 1) / L;
-    double m = row[u];
+    double m = row[(size_t)u * rst];
     if (b >= 1) m = fma(-sv[b * sst], al[u], m);
     if (b < J) m = fma(-sv[(b + 1) * sst], be[u], m);
     return m;
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
         const double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
         const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            const double* row = ed.ysol + ((size_t)ke * n + c) * ed.Ecap;
+            const double* row = ed.ysol + (size_t)ke * ed.Ecap * n + c;   // row[u * n]
             // separator-major arrays: lanes (channels) read and write consecutive words
             double* sv = ed.sep + (size_t)ke * ed.nbmax * n + c;   // sv[j * n]
             const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * n + c;   // sip[(3j + v) * n]
@@ -885,10 +885,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                 const double* al = fac_row(ed, ke, kFacAl);
                 const double* be = fac_row(ed, ke, kFacBe);
                 const double* ihv = fac_row(ed, ke, kFacIh);
-                double M1 = knot_M(row, sv, n, al, be, 1, E, J, nullptr);
-                double M2 = knot_M(row, sv, n, al, be, 2, E, J, nullptr);
-                double MN1 = knot_M(row, sv, n, al, be, E - 3, E, J, nullptr);
-                double MN = knot_M(row, sv, n, al, be, E - 2, E, J, nullptr);
+                double M1 = knot_M(row, n, sv, n, al, be, 1, E, J, nullptr);
+                double M2 = knot_M(row, n, sv, n, al, be, 2, E, J, nullptr);
+                double MN1 = knot_M(row, n, sv, n, al, be, E - 3, E, J, nullptr);
+                double MN = knot_M(row, n, sv, n, al, be, E - 2, E, J, nullptr);
                 double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
                 double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
                 double* me = ed.mend + ((size_t)ke * n + c) * 2;
@@ -913,11 +913,12 @@ struct KnotMIn {
     double y, al, be, s0, s1, ss, me;
 };
 
-__device__ __forceinline__ KnotMIn knot_m_load(const double* row, const GPtr<double>& al, const GPtr<double>& be,
-                                                const double* sv, int sst, const double* me, int u, int E, int J) {
+__device__ __forceinline__ KnotMIn knot_m_load(const double* row, int rst, const GPtr<double>& al,
+                                                const GPtr<double>& be, const double* sv, int sst, const double* me,
+                                                int u, int E, int J) {
     const int b = max(u - 1, 0) >> 5;
     KnotMIn v;
-    v.y = row[u];
+    v.y = row[(size_t)u * rst];
     v.al = al[u];
     v.be = be[u];
     v.s0 = sv[b * sst];
@@ -982,13 +983,13 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 cs[a] = c;
                 js[a] = j;
                 const size_t rr = kerow + c;
-                const double* row = ed.ysol + rr * ed.Ecap;
+                const double* row = ed.ysol + (size_t)ke * ed.Ecap * n + c;   // row[u * n]
                 const double* sv = ed.sep + (size_t)ke * ed.nbmax * n + cs[a];   // sv[j * n]
                 const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
-                v0[a] = knot_m_load(row, al, be, sv, n, me, j, E, J);
-                v1[a] = knot_m_load(row, al, be, sv, n, me, j + 1, E, J);
-                y0[a] = ed.yknot[rr * ed.Ecap + j];
-                y1[a] = ed.yknot[rr * ed.Ecap + j + 1];
+                v0[a] = knot_m_load(row, n, al, be, sv, n, me, j, E, J);
+                v1[a] = knot_m_load(row, n, al, be, sv, n, me, j + 1, E, J);
+                y0[a] = ed.yknot[((size_t)ke * ed.Ecap + j) * n + c];
+                y1[a] = ed.yknot[((size_t)ke * ed.Ecap + j + 1) * n + c];
                 x0[a] = X[j];
                 x1[a] = X[j + 1];
                 ih[a] = ihv[j];
@@ -1136,11 +1137,6 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
     }
 }
 
-// rem / d == (rem * ceil(2^16 / d)) >> 16 exactly for 1 <= d <= 32 * TW + 2 and
-// 0 <= rem < (16 / TW) * d, TW in {1, 2, 4} (checked exhaustively): the split of
-// a phase-A item index into (channel, interval).
-__device__ __forceinline__ int div_magic16(int d) { return (65536 + d - 1) / d; }
-
 // Fused records + evaluation (fix mode).  One workgroup = one tile of 64 * TW
 // samples (TW = EpochDesc::etw flag words) x 16 / TW channels; lane = (channel,
 // quad of 4 consecutive samples).  Directions are taken in batches of at most
@@ -1162,7 +1158,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     constexpr int MI = 2;                // phase-A items per thread per round
     __shared__ double4 rec[kEvalCap];    // segment s: [channel][stride] from its record base
     __shared__ int xk[kEvalXkCap];       // segment s: knot positions jlo .. jlo + nint
-    __shared__ int4 sSegA[NS];           // item prefix, division magic, nint, jlo
+    __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
     __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
     __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
     __shared__ int sCum[NS][4];          // knots before each word of the tile
@@ -1194,6 +1190,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int wi = q >> 4;               // tile word of the quad
     const unsigned long long below = (1ull << (4 * (q & 15))) - 1ull;
     const bool lane_on = cl < cgn && ts <= tlast;
+    const int cmag = (65536 + cgn - 1) / cgn;   // rem / cgn exact for rem < 33 * 16 (checked exhaustively)
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
@@ -1266,7 +1263,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             const int nb = max((int)__popcll(fit), 1);
             const int total = __shfl(pit, 2 * nb - 1);
             if (tid < 2 * nb) {
-                sSegA[tid] = make_int4(pit - nit, div_magic16(max(ni, 1)), ni, lo);
+                sSegA[tid] = make_int4(pit - nit, 0, ni, lo);
                 sSegB[tid] = make_int4(pneed - need, stride, E, pxk - nxk);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
@@ -1300,23 +1297,24 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int4 A = sSegA[seg];
                 const int E = sSegB[seg].z;
                 const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
-                const int rem = f - A.x;
-                const int cl2 = (rem * A.y) >> 16;
-                const int j = A.w + rem - cl2 * A.z;
+                const int rem = f - A.x;   // items (interval, channel), channel fastest
+                const int ii = (rem * cmag) >> 16;
+                const int cl2 = rem - ii * cgn;
+                const int j = A.w + ii;
                 const int c2 = cg * CG + cl2;
                 const int J = (E - 3) / kSolverBlock;
                 const int bj = max(j - 1, 0) >> 5;
-                const size_t rowoff = ((size_t)ke * n + c2) * ed.Ecap;
+                const size_t rowoff = ((size_t)ke * ed.Ecap + j) * n + c2;   // knot-major arrays
                 const GPtr<double> Yr = gptr(ed.yknot + rowoff), Mr = gptr(ed.ysol + rowoff);
                 const GPtr<double> Sr = gptr(ed.sep + (size_t)ke * ed.nbmax * n + c2);   // Sr[j * n]
                 const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
                 x0[a] = Xr[j];
                 x1[a] = Xr[j + 1];
-                v0[a] = Yr[j];
-                v1[a] = Yr[j + 1];
-                m0[a] = Mr[j];
-                m1[a] = Mr[j + 1];
+                v0[a] = Yr[0];
+                v1[a] = Yr[n];
+                m0[a] = Mr[0];
+                m1[a] = Mr[n];
                 a0v[a] = F[kFacAl * ed.Ecap + j];
                 a1v[a] = F[kFacAl * ed.Ecap + j + 1];
                 b0v[a] = F[kFacBe * ed.Ecap + j];
@@ -1344,8 +1342,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int4 B = sSegB[seg];
                 const int j = jv[a], E = B.z, J = (E - 3) / kSolverBlock;
                 const int i = j - A.w;
-                const int rem = tid + 256 * (a0 + a) - A.x;
-                const int cl2 = (rem * A.y) >> 16;
+                const int cl2 = tid + 256 * (a0 + a) - A.x - i * cgn;
                 if (cl2 == 0) {   // first channel of the group: knot positions
                     xk[B.w + i] = x0[a];
                     if (i == A.z - 1) xk[B.w + A.z] = x1[a];
