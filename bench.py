@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="c3: skip the plain per-epoch pass (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in tile shapes (default: memd_autotune before the untimed warmup)")
     ap.add_argument("--tuning", type=str, default=None,
@@ -226,16 +227,17 @@ def bench_c3(a):
         ctx.run_batch(eps)
         lat.append(time.perf_counter() - s0)
     batched = count * len(lat) / sum(lat)
-    plain_n = count   # every epoch once, one call each
-    ctx.run(eps[0]["y"], eps[0]["V"], 9, 10, out=eps[0]["out"])
-    torch.cuda.synchronize()
-    s0 = time.perf_counter()
-    plat = []
-    for ep in eps[:plain_n]:
-        s1 = time.perf_counter()
-        ctx.run(ep["y"], ep["V"], 9, 10, out=ep["out"])
-        plat.append(time.perf_counter() - s1)
-    plain = plain_n / (time.perf_counter() - s0)
+    plain_n = 0 if a.no_plain else count   # every epoch once, one call each
+    plat, plain = [], None
+    if plain_n:
+        ctx.run(eps[0]["y"], eps[0]["V"], 9, 10, out=eps[0]["out"])
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for ep in eps[:plain_n]:
+            s1 = time.perf_counter()
+            ctx.run(ep["y"], ep["V"], 9, 10, out=ep["out"])
+            plat.append(time.perf_counter() - s1)
+        plain = plain_n / (time.perf_counter() - s0)
     ctx.close()
     print(json.dumps({
         "metric": "MEMD epochs/s, 512 uneven epochs (batched vs plain GPU)", "value": round(batched, 2),
@@ -243,10 +245,12 @@ def bench_c3(a):
         "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "memd_512_uneven_fix9x10", "n": [8, 16, 32, 64], "T": [1000, 8000], "K": "2n"},
-        "plain_gpu_epochs_per_s": round(plain, 2), "batched_over_plain": round(batched / plain, 2),
+        "plain_gpu_epochs_per_s": round(plain, 2) if plain else None,
+        "batched_over_plain": round(batched / plain, 2) if plain else None,
         "tuning": {str(k): v for k, v in tunings.items()}, "autotune_s": round(tune_s, 3),
-        "latency_ms": {"batch_p50": round(pct(lat, 50) * 1e3, 1), "plain_epoch_p50": round(pct(plat, 50) * 1e3, 3),
-                       "plain_epoch_p99": round(pct(plat, 99) * 1e3, 3)}}), flush=True)
+        "latency_ms": {"batch_p50": round(pct(lat, 50) * 1e3, 1),
+                       "plain_epoch_p50": round(pct(plat, 50) * 1e3, 3) if plat else None,
+                       "plain_epoch_p99": round(pct(plat, 99) * 1e3, 3) if plat else None}}), flush=True)
 
 
 def bench_c4(a):
