@@ -1145,7 +1145,7 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
 //     words, the knots before each word and the tile's interval window
 //     [jlo, jlo + nint); prefix sums over the directions' record and knot counts
 //     close the batch;
-//   phase A: every (direction, envelope, channel, interval) item of the batch
+//   phase A: every (direction, envelope, interval, quad of <= 4 channels) item of the batch
 //     builds the spline record (a,b,c,d) of its interval in LDS from M at its two
 //     knots (formed from the block solutions, spikes and separator values as
 //     knot_m_select does), the knot values and the shared reciprocals -- the same
@@ -1155,7 +1155,6 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
-    constexpr int MI = 2;                // phase-A items per thread per round
     __shared__ double4 rec[kEvalCap];    // segment s: [channel][stride] from its record base
     __shared__ int xk[kEvalXkCap];       // segment s: knot positions jlo .. jlo + nint
     __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
@@ -1190,7 +1189,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int wi = q >> 4;               // tile word of the quad
     const unsigned long long below = (1ull << (4 * (q & 15))) - 1ull;
     const bool lane_on = cl < cgn && ts <= tlast;
-    const int cmag = (65536 + cgn - 1) / cgn;   // rem / cgn exact for rem < 33 * 16 (checked exhaustively)
+    const int nq = (cgn + 3) >> 2;              // channel quads of the group
+    const int qmag = (65536 + nq - 1) / nq;     // rem / nq as (rem * qmag) >> 16: exact for rem < 2^15
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             const int stride = ni | 1;   // odd: a wave's channels read disjoint banks
             const int need = valid ? cgn * stride : 0;
             const int nxk = valid ? ni + 1 : 0;
-            const int nit = valid ? cgn * ni : 0;
+            const int nit = valid ? nq * ni : 0;
             int pneed = need, pxk = nxk, pit = nit;
 #pragma unroll
             for (int o = 1; o < NS; o <<= 1) {
@@ -1280,88 +1280,93 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
 #pragma unroll
         for (int s = 0; s <= NS; ++s) ip[s] = uniform(sIp[s]);
         const int total = ip[NS];
-        // phase A
-        for (int a0 = 0; a0 * 256 < total; a0 += MI) {
-            double v0[MI], v1[MI], m0[MI], m1[MI], ih[MI], i3h[MI];
-            double a0v[MI], a1v[MI], b0v[MI], b1v[MI], s0v[MI], s1v[MI], s2v[MI], me0[MI], me1[MI];
-            int x0[MI], x1[MI], fv[MI], jv[MI];
-#pragma unroll
-            for (int a = 0; a < MI; ++a) {
-                fv[a] = -1;
-                if (wbase + 256 * (a0 + a) >= total) continue;   // wave-uniform
-                const int f = tid + 256 * (a0 + a);
-                if (f >= total) continue;
-                int seg = 0;
+        // phase A: items (interval, quad of <= 4 channels), quad fastest.  The
+        // interval's positions, spikes and reciprocals are loaded once for its
+        // channels; the channel rows of the knot-major arrays are adjacent words.
+        for (int a0 = 0; wbase + 256 * a0 < total; ++a0) {   // wave-uniform bound
+            const int f = tid + 256 * a0;
+            double v0[4], v1[4], m0[4], m1[4], s0v[4], s1v[4], s2v[4];
+            double a0v = 0.0, a1v = 0.0, b0v = 0.0, b1v = 0.0, ih = 0.0, i3h = 0.0;
+            int x0 = 0, x1 = 0, seg = -1, j = 0, qd = 0, nch = 0;
+            if (f < total) {
+                seg = 0;
 #pragma unroll
                 for (int s = 1; s < NS; ++s) seg += (f >= ip[s]) ? 1 : 0;
                 const int4 A = sSegA[seg];
                 const int E = sSegB[seg].z;
                 const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
-                const int rem = f - A.x;   // items (interval, channel), channel fastest
-                const int ii = (rem * cmag) >> 16;
-                const int cl2 = rem - ii * cgn;
-                const int j = A.w + ii;
-                const int c2 = cg * CG + cl2;
+                const int rem = f - A.x;
+                const int ii = (rem * qmag) >> 16;
+                qd = rem - ii * nq;
+                j = A.w + ii;
+                const int c0 = cg * CG + 4 * qd;
+                nch = min(4, cgn - 4 * qd);
                 const int J = (E - 3) / kSolverBlock;
                 const int bj = max(j - 1, 0) >> 5;
-                const size_t rowoff = ((size_t)ke * ed.Ecap + j) * n + c2;   // knot-major arrays
-                const GPtr<double> Yr = gptr(ed.yknot + rowoff), Mr = gptr(ed.ysol + rowoff);
-                const GPtr<double> Sr = gptr(ed.sep + (size_t)ke * ed.nbmax * n + c2);   // Sr[j * n]
+                const GPtr<double> Yr = gptr(ed.yknot + ((size_t)ke * ed.Ecap + j) * n + c0);
+                const GPtr<double> Mr = gptr(ed.ysol + ((size_t)ke * ed.Ecap + j) * n + c0);
+                const GPtr<double> S0 = gptr(ed.sep + ((size_t)ke * ed.nbmax + bj) * n + c0);
+                const GPtr<double> S1 = gptr(ed.sep + ((size_t)ke * ed.nbmax + min(bj + 1, J)) * n + c0);
+                const GPtr<double> S2 = gptr(ed.sep + ((size_t)ke * ed.nbmax + min(bj + 2, J)) * n + c0);
                 const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
-                x0[a] = Xr[j];
-                x1[a] = Xr[j + 1];
-                v0[a] = Yr[0];
-                v1[a] = Yr[n];
-                m0[a] = Mr[0];
-                m1[a] = Mr[n];
-                a0v[a] = F[kFacAl * ed.Ecap + j];
-                a1v[a] = F[kFacAl * ed.Ecap + j + 1];
-                b0v[a] = F[kFacBe * ed.Ecap + j];
-                b1v[a] = F[kFacBe * ed.Ecap + j + 1];
-                ih[a] = F[kFacIh * ed.Ecap + j];
-                i3h[a] = F[kFacI3h * ed.Ecap + j];
-                s0v[a] = Sr[bj * n];
-                s1v[a] = Sr[min(bj + 1, J) * n];
-                s2v[a] = Sr[min(bj + 2, J) * n];
-                me0[a] = 0.0;
-                me1[a] = 0.0;
-                if (ed.spline) {
-                    const GPtr<double> Me = gptr(ed.mend + ((size_t)ke * n + c2) * 2);
-                    me0[a] = Me[0];
-                    me1[a] = Me[1];
-                }
-                fv[a] = seg;
-                jv[a] = j;
-            }
+                x0 = Xr[j];
+                x1 = Xr[j + 1];
+                a0v = F[kFacAl * ed.Ecap + j];
+                a1v = F[kFacAl * ed.Ecap + j + 1];
+                b0v = F[kFacBe * ed.Ecap + j];
+                b1v = F[kFacBe * ed.Ecap + j + 1];
+                ih = F[kFacIh * ed.Ecap + j];
+                i3h = F[kFacI3h * ed.Ecap + j];
 #pragma unroll
-            for (int a = 0; a < MI; ++a) {
-                const int seg = fv[a];
-                if (seg < 0) continue;
+                for (int q = 0; q < 4; ++q) {
+                    if (q < nch) {
+                        v0[q] = Yr[q];
+                        v1[q] = Yr[n + q];
+                        m0[q] = Mr[q];
+                        m1[q] = Mr[n + q];
+                        s0v[q] = S0[q];
+                        s1v[q] = S1[q];
+                        s2v[q] = S2[q];
+                    }
+                }
+            }
+            if (seg >= 0) {
                 const int4 A = sSegA[seg];
                 const int4 B = sSegB[seg];
-                const int j = jv[a], E = B.z, J = (E - 3) / kSolverBlock;
+                const int E = B.z, J = (E - 3) / kSolverBlock;
                 const int i = j - A.w;
-                const int cl2 = tid + 256 * (a0 + a) - A.x - i * cgn;
-                if (cl2 == 0) {   // first channel of the group: knot positions
-                    xk[B.w + i] = x0[a];
-                    if (i == A.z - 1) xk[B.w + A.z] = x1[a];
+                if (qd == 0) {   // first quad of the group: knot positions
+                    xk[B.w + i] = x0;
+                    if (i == A.z - 1) xk[B.w + A.z] = x1;
                 }
                 // M_j and M_{j+1}: knot_m_select's operations
                 const int bj = max(j - 1, 0) >> 5, bn = j >> 5;   // blocks of j and j+1
-                const KnotMIn in0{m0[a], a0v[a], b0v[a], s0v[a], s1v[a], s1v[a], me0[a]};
-                const KnotMIn in1{m1[a], a1v[a], b1v[a], bn == bj ? s0v[a] : s1v[a],
-                                  bn == bj ? s1v[a] : s2v[a], bn == bj ? s1v[a] : s2v[a], me1[a]};
-                const double mj = knot_m_select(in0, j, E, J);
-                const double mk = knot_m_select(in1, j + 1, E, J);
-                const double h = (double)(x1[a] - x0[a]);
-                const double slp = (v1[a] - v0[a]) * ih[a];
-                double4 r;
-                r.x = v0[a];
-                r.y = slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0);
-                r.z = mj;
-                r.w = (mk - mj) * i3h[a];
-                rec[B.x + cl2 * B.y + i] = r;
+                const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
+                const double h = (double)(x1 - x0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (q < nch) {
+                        double me0 = 0.0, me1 = 0.0;
+                        if (ed.spline) {
+                            const GPtr<double> Me = gptr(ed.mend + ((size_t)ke * n + cg * CG + 4 * qd + q) * 2);
+                            me0 = Me[0];
+                            me1 = Me[1];
+                        }
+                        const KnotMIn in0{m0[q], a0v, b0v, s0v[q], s1v[q], s1v[q], me0};
+                        const KnotMIn in1{m1[q], a1v, b1v, bn == bj ? s0v[q] : s1v[q],
+                                          bn == bj ? s1v[q] : s2v[q], bn == bj ? s1v[q] : s2v[q], me1};
+                        const double mj = knot_m_select(in0, j, E, J);
+                        const double mk = knot_m_select(in1, j + 1, E, J);
+                        const double slp = (v1[q] - v0[q]) * ih;
+                        double4 r;
+                        r.x = v0[q];
+                        r.y = slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0);
+                        r.z = mj;
+                        r.w = (mk - mj) * i3h;
+                        rec[B.x + (4 * qd + q) * B.y + i] = r;
+                    }
+                }
             }
         }
         __syncthreads();
