@@ -93,10 +93,11 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.X = take((size_t)K * 2 * Ecap * 4);
     L.fac = take((size_t)K * 2 * 6 * Ecap * 8);
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
-    L.yknot = take((size_t)K * 2 * n * Ecap * 8);
-    L.ysol = take((size_t)K * 2 * n * Ecap * 8);
+    const size_t np = (size_t)channel_stride(n);
+    L.yknot = take((size_t)K * 2 * np * Ecap * 8);
+    L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
-    L.sep = take((size_t)K * 2 * n * nb * 8);
+    L.sep = take((size_t)K * 2 * np * nb * 8);
     L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
     L.mend = take((size_t)K * 2 * n * 2 * 8);
     L.mean = take((size_t)n * T * 8);
@@ -179,6 +180,7 @@ memd_tuning tuning_for(const memd_ctx* ctx, int n) {
 EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, bool records) {
     EpochDesc d{};
     d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
+    d.np = channel_stride(I.n);
     d.Ecap = knot_capacity(I.T);
     d.nbmax = solver_block_capacity(d.Ecap);
     d.N_imf = I.N_imf; d.S = I.S; d.mode = o.mode; d.window = o.window;

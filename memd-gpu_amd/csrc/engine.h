@@ -93,6 +93,7 @@ struct EpochState {
 // Everything a kernel needs to know about one epoch.  Lives in device memory.
 struct EpochDesc {
     int n, T, K, W;        // W = ceil(T/64) bitmask words per projection row
+    int np;                // row stride of yknot, ysol, sep: n rounded up to a multiple of 4
     int Ecap;              // knot capacity per (direction, envelope) = T/2 + 2
     int nbmax;             // solver blocks per (direction, envelope) at Ecap knots
     int N_imf, S, mode, window;
@@ -108,10 +109,13 @@ struct EpochDesc {
     double* fac;           // [K][2][6][Ecap] per knot row, shared by channels:
                            //   rd, cp (block factors), al, be (spikes), ih = 1/h, i3h = 1/(3h)
     double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
-    double* yknot;         // [K][2][Ecap][n] knot values Y (end knots clamped), knot-major
-    double* ysol;          // [K][2][Ecap][n] block solutions y at block rows (separator rows unused)
+    // Knot-major and separator-major arrays: rows of np = n rounded up to a
+    // multiple of 4 words (32-byte aligned 4-channel groups; padding is never read
+    // into results)
+    double* yknot;         // [K][2][Ecap][np] knot values Y (end knots clamped)
+    double* ysol;          // [K][2][Ecap][np] block solutions y at block rows (separator rows unused)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
-    double* sep;           // [K][2][nbmax][n] separator values (z, then X_j), separator-major
+    double* sep;           // [K][2][nbmax][np] separator values (z, then X_j)
     double* sepin;         // [K][2][nbmax][3][n] y before, right-hand side, y after each separator
     double* mean;          // [n][T] stop mode: envelope mean
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
@@ -191,6 +195,7 @@ long list_capacity(int list, const EpochDesc& e);
 // (stop mode only); slot kNumStepKernels is the one-off init.
 constexpr int kNumStepKernels = 9;
 int knot_capacity(int T);
+inline int channel_stride(int n) { return (n + 3) & ~3; }
 int solver_block_capacity(int Ecap);
 
 // Per-device kernel attributes (large dynamic LDS); call once per context.

@@ -119,6 +119,8 @@ __device__ __forceinline__ CPtr<T> cptr(const T* p) {
 }
 template <class T>
 using GMut = __attribute__((address_space(1))) T*;
+// Four doubles loaded together from a 32-byte aligned address.
+typedef double v4d __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ GMut<T> gmut(T* p) {
     return (GMut<T>)p;
@@ -659,8 +661,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
-        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * n);   // [knot][channel]
-        const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * n);
+        const int np = ed.np;
+        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * np);   // [knot][channel], row stride np
+        const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
@@ -705,7 +708,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
                 const double y = yv[g];
                 sY[c * RS + r] = y;
                 if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
-                    yk[(unsigned)u * (unsigned)n + (unsigned)c] = y;
+                    yk[(unsigned)u * (unsigned)np + (unsigned)c] = y;
             }
         }
         // end knots of E >= 6 clamped (tile row 0 is knot 0 in the first group; rows
@@ -724,7 +727,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
                     const double y =
                         clamp_end(cur[base + XE1], cur[base + XE2], cur[base + XE3], XE1, XE2, XE3, env, false);
                     for (int r = rE; r < RW; ++r) sY[c * RS + r] = y;
-                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)(E - 1) * (unsigned)n + (unsigned)c] = y;
+                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)(E - 1) * (unsigned)np + (unsigned)c] = y;
                 }
             }
         }
@@ -782,7 +785,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             const int u = p0 + r;
             if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
             const double yv = sY[c2 * RS + r + 1];
-            ys[(unsigned)u * (unsigned)n + (unsigned)c2] = yv;
+            ys[(unsigned)u * (unsigned)np + (unsigned)c2] = yv;
             // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
             if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * n + c2] = yv;
             if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * n + c2] = yv;
@@ -825,9 +828,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
         const double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
         const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            const double* row = ed.ysol + (size_t)ke * ed.Ecap * n + c;   // row[u * n]
+            const int np = ed.np;
+            const double* row = ed.ysol + (size_t)ke * ed.Ecap * np + c;   // row[u * np]
             // separator-major arrays: lanes (channels) read and write consecutive words
-            double* sv = ed.sep + (size_t)ke * ed.nbmax * n + c;   // sv[j * n]
+            double* sv = ed.sep + (size_t)ke * ed.nbmax * np + c;   // sv[j * np]
             const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * n + c;   // sip[(3j + v) * n]
             const GPtr<double> sing = gptr(sip);
             const GMut<double> svg = gmut(sv);
@@ -858,7 +862,7 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                     const double lo = (double)(x0[a] - xm[a]), up = (double)(xp[a] - x0[a]);
                     const double D = fma(-up, yp[a], fma(-lo, ym[a], yc[a]));
                     z = (j == 1) ? D * rr[a] : fma(-rl[a], z, D) * rr[a];
-                    svg[j * n] = z;
+                    svg[j * np] = z;
                 }
             }
             // back substitution, RC separators at a time
@@ -869,14 +873,14 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                 for (int a = 0; a < RC; ++a) {
                     const int j = max(j1 - a, 1);
                     cpj[a] = Rcpg[j];
-                    zj[a] = svg[j * n];
+                    zj[a] = svg[j * np];
                 }
 #pragma unroll
                 for (int a = 0; a < RC; ++a) {
                     const int j = j1 - a;
                     if (j < 1) break;
                     x = fma(-cpj[a], x, zj[a]);
-                    svg[j * n] = x;
+                    svg[j * np] = x;
                 }
             }
             if (ed.spline) {
@@ -885,10 +889,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                 const double* al = fac_row(ed, ke, kFacAl);
                 const double* be = fac_row(ed, ke, kFacBe);
                 const double* ihv = fac_row(ed, ke, kFacIh);
-                double M1 = knot_M(row, n, sv, n, al, be, 1, E, J, nullptr);
-                double M2 = knot_M(row, n, sv, n, al, be, 2, E, J, nullptr);
-                double MN1 = knot_M(row, n, sv, n, al, be, E - 3, E, J, nullptr);
-                double MN = knot_M(row, n, sv, n, al, be, E - 2, E, J, nullptr);
+                double M1 = knot_M(row, np, sv, np, al, be, 1, E, J, nullptr);
+                double M2 = knot_M(row, np, sv, np, al, be, 2, E, J, nullptr);
+                double MN1 = knot_M(row, np, sv, np, al, be, E - 3, E, J, nullptr);
+                double MN = knot_M(row, np, sv, np, al, be, E - 2, E, J, nullptr);
                 double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
                 double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
                 double* me = ed.mend + ((size_t)ke * n + c) * 2;
@@ -983,13 +987,13 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 cs[a] = c;
                 js[a] = j;
                 const size_t rr = kerow + c;
-                const double* row = ed.ysol + (size_t)ke * ed.Ecap * n + c;   // row[u * n]
-                const double* sv = ed.sep + (size_t)ke * ed.nbmax * n + cs[a];   // sv[j * n]
+                const double* row = ed.ysol + (size_t)ke * ed.Ecap * ed.np + c;   // row[u * np]
+                const double* sv = ed.sep + (size_t)ke * ed.nbmax * ed.np + cs[a];   // sv[j * np]
                 const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
-                v0[a] = knot_m_load(row, n, al, be, sv, n, me, j, E, J);
-                v1[a] = knot_m_load(row, n, al, be, sv, n, me, j + 1, E, J);
-                y0[a] = ed.yknot[((size_t)ke * ed.Ecap + j) * n + c];
-                y1[a] = ed.yknot[((size_t)ke * ed.Ecap + j + 1) * n + c];
+                v0[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j, E, J);
+                v1[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j + 1, E, J);
+                y0[a] = ed.yknot[((size_t)ke * ed.Ecap + j) * ed.np + c];
+                y1[a] = ed.yknot[((size_t)ke * ed.Ecap + j + 1) * ed.np + c];
                 x0[a] = X[j];
                 x1[a] = X[j + 1];
                 ih[a] = ihv[j];
@@ -1303,11 +1307,17 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 nch = min(4, cgn - 4 * qd);
                 const int J = (E - 3) / kSolverBlock;
                 const int bj = max(j - 1, 0) >> 5;
-                const GPtr<double> Yr = gptr(ed.yknot + ((size_t)ke * ed.Ecap + j) * n + c0);
-                const GPtr<double> Mr = gptr(ed.ysol + ((size_t)ke * ed.Ecap + j) * n + c0);
-                const GPtr<double> S0 = gptr(ed.sep + ((size_t)ke * ed.nbmax + bj) * n + c0);
-                const GPtr<double> S1 = gptr(ed.sep + ((size_t)ke * ed.nbmax + min(bj + 1, J)) * n + c0);
-                const GPtr<double> S2 = gptr(ed.sep + ((size_t)ke * ed.nbmax + min(bj + 2, J)) * n + c0);
+                // 4 channels from c0 (a multiple of 4) in rows of np (a multiple of 4)
+                // words: 32-byte aligned; channels past the group are padding or the
+                // next group's and are not used
+                const int np = ed.np;
+                const size_t rb = (size_t)ke * ed.Ecap + j;
+                const GPtr<v4d> Yr = gptr(reinterpret_cast<const v4d*>(ed.yknot + rb * np + c0));
+                const GPtr<v4d> Mr = gptr(reinterpret_cast<const v4d*>(ed.ysol + rb * np + c0));
+                const double* Sb = ed.sep + (size_t)ke * ed.nbmax * np + c0;
+                const GPtr<v4d> S0 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)bj * np));
+                const GPtr<v4d> S1 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 1, J) * np));
+                const GPtr<v4d> S2 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 2, J) * np));
                 const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
                 x0 = Xr[j];
@@ -1318,18 +1328,16 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 b1v = F[kFacBe * ed.Ecap + j + 1];
                 ih = F[kFacIh * ed.Ecap + j];
                 i3h = F[kFacI3h * ed.Ecap + j];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (q < nch) {
-                        v0[q] = Yr[q];
-                        v1[q] = Yr[n + q];
-                        m0[q] = Mr[q];
-                        m1[q] = Mr[n + q];
-                        s0v[q] = S0[q];
-                        s1v[q] = S1[q];
-                        s2v[q] = S2[q];
-                    }
-                }
+                const int nr = np >> 2;   // double4 per row
+                const v4d yv0 = Yr[0], yv1 = Yr[nr], mv0 = Mr[0], mv1 = Mr[nr];
+                const v4d sv0 = S0[0], sv1 = S1[0], sv2 = S2[0];
+                v0[0] = yv0.x; v0[1] = yv0.y; v0[2] = yv0.z; v0[3] = yv0.w;
+                v1[0] = yv1.x; v1[1] = yv1.y; v1[2] = yv1.z; v1[3] = yv1.w;
+                m0[0] = mv0.x; m0[1] = mv0.y; m0[2] = mv0.z; m0[3] = mv0.w;
+                m1[0] = mv1.x; m1[1] = mv1.y; m1[2] = mv1.z; m1[3] = mv1.w;
+                s0v[0] = sv0.x; s0v[1] = sv0.y; s0v[2] = sv0.z; s0v[3] = sv0.w;
+                s1v[0] = sv1.x; s1v[1] = sv1.y; s1v[2] = sv1.z; s1v[3] = sv1.w;
+                s2v[0] = sv2.x; s2v[1] = sv2.y; s2v[2] = sv2.z; s2v[3] = sv2.w;
             }
             if (seg >= 0) {
                 const int4 A = sSegA[seg];
