@@ -1118,16 +1118,23 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 
 // Phase B of one envelope for a lane's 4 consecutive samples (interval i of the
 // first): each sample uses the record of X_j < t <= X_{j+1}.  ENV 0 stores the
-// upper envelope, ENV 1 adds the mean of both to the accumulator.
-template <bool FULL, int ENV>
+// upper envelope, ENV 1 adds the mean of both to the accumulator.  The first
+// sample is in range and lies in interval i (its caller counts the knots below
+// it).
+template <int ENV>
 __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, const int* __restrict__ xkrow, int i,
                                           int ts, int tlast, double (&ep)[4], double (&acc)[4]) {
     int xj = xkrow[i], xn = xkrow[i + 1];
     double4 r = recrow[i];
+    {
+        const double v = horner4(r, (double)(ts - xj));
+        if (ENV == 0) ep[0] = v;
+        else acc[0] = acc[0] + (ep[0] + v) * 0.5;
+    }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 1; s < 4; ++s) {
         const int t = ts + s;
-        if (FULL || t <= tlast) {
+        if (t <= tlast) {
             if (t > xn) {
                 ++i;
                 xj = xn;
@@ -1378,7 +1385,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             }
         }
         __syncthreads();
-        // phase B
+        // phase B.  Samples 0 and T-1 are knots of every direction, so the knots
+        // below a sample t in 1..T-1 number 1..E-1: the interval index needs no
+        // upper clamp (t = 0 gives -1, clamped to interval 0).
         for (int d = 0; d < nb; ++d) {
             const int E0 = uniform(sSegB[2 * d].z), E1 = uniform(sSegB[2 * d + 1].z);
             if (!(E0 > 3 && E1 > 3)) continue;
@@ -1387,16 +1396,16 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 {
                     const int4 A = sSegA[2 * d];
                     const int4 B = sSegB[2 * d];
-                    const int jj = sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1;
-                    eval_quad<false, 0>(rec + B.x + cl * B.y, xk + B.w, min(max(jj, 0), E0 - 2) - A.w, ts, tlast,
-                                        ep, acc);
+                    const int jj = max(sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1, 0);
+                    const double4* rr = rec + B.x + __umul24(cl, B.y);
+                    eval_quad<0>(rr, xk + B.w, jj - A.w, ts, tlast, ep, acc);
                 }
                 {
                     const int4 A = sSegA[2 * d + 1];
                     const int4 B = sSegB[2 * d + 1];
-                    const int jj = sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1;
-                    eval_quad<false, 1>(rec + B.x + cl * B.y, xk + B.w, min(max(jj, 0), E1 - 2) - A.w, ts, tlast,
-                                        ep, acc);
+                    const int jj = max(sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1, 0);
+                    const double4* rr = rec + B.x + __umul24(cl, B.y);
+                    eval_quad<1>(rr, xk + B.w, jj - A.w, ts, tlast, ep, acc);
                 }
             }
         }
