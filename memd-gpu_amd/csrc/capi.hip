@@ -76,7 +76,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -87,6 +87,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t at = off; off += align_up(bytes); return at; };
     L.running = take((size_t)n * T * 8);
+    L.curt = take((size_t)T * channel_stride(n) * 8);
     L.mask = take((size_t)K * 2 * W * 8);
     L.pre = take((size_t)K * 2 * W * 4);
     L.E = take((size_t)K * 2 * 4);
@@ -209,6 +210,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.fac = (double*)(b + L.fac);
     d.red = (double*)(b + L.redsys);
     d.yknot = (double*)(b + L.yknot);
+    d.curt = (double*)(b + L.curt);
     d.ysol = (double*)(b + L.ysol);
     d.coef = (double*)(b + L.coef);
     d.sep = (double*)(b + L.sep);

@@ -102,6 +102,7 @@ struct EpochDesc {
     const double* V;       // [K][n]
     double* imfs;          // [N_imf][n][T]
     double* running;       // [n][T]
+    double* curt;          // [T][np] current signal, sample-major (written by k_flags every step)
     unsigned long long* mask; // [K][2][W] extrema flags (bit t%64 of word t/64)
     int* pre;              // [K][2][W] knots before each word
     int* E;                // [K][2] knot counts

@@ -230,6 +230,11 @@ __device__ void flags_wide(const EpochDesc& ed, const EpochState* st, int by) {
     if (t < T) Ps[tid + 1] = project_at(Vk, cur, n, T, t);
     if (tid == 0 && t0 > 0) Ps[0] = project_at(Vk, cur, n, T, t0 - 1);
     if (tid == kFlagTile - 1 && t0 + kFlagTile < T) Ps[kFlagTile + 1] = project_at(Vk, cur, n, T, t0 + kFlagTile);
+    if (k == 0) {   // sample-major copy of the tile for the knot-value gathers
+        const int tn = min(kFlagTile, T - t0);
+        for (int c = 0; c < n; ++c)
+            if (tid < tn) ed.curt[(size_t)(t0 + tid) * ed.np + c] = cur[(size_t)c * T + t0 + tid];
+    }
     __syncthreads();
     int lim = window_limit(T, ed.window);
     bool fmax = false, fmin = false;
@@ -282,6 +287,18 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
         const int c = idx / TW, i = idx - c * TW;
         const int t = t0 - 1 + i;
         curT[idx] = (t >= 0 && t < T) ? (double)cur[(unsigned)c * (unsigned)T + (unsigned)t] : 0.0;
+    }
+    __syncthreads();
+    {   // sample-major copy of the tile for the knot-value gathers: lanes over
+        // (sample, channel), channel fastest; idx / n as in k_block
+        const int tn = min(kFlagTile, T - t0);
+        const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
+        const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np);
+        for (int idx = tid; idx < n * tn; idx += kFlagTile) {
+            const int tl = (int)(((unsigned long long)idx * nmagic) >> 32);
+            const int c = idx - tl * n;
+            ct[(unsigned)tl * (unsigned)ed.np + (unsigned)c] = curT[c * TW + tl + 1];
+        }
     }
     const int lim = window_limit(T, ed.window);
     const int t = t0 + tid;
@@ -616,8 +633,9 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 // Knot values, right-hand sides and phase 1, per (direction, envelope) and group
 // of EpochDesc::bpw solver blocks (rows p0..qlast, at most 32 * bpw).
 //   1. The group's knot positions, shared factors and reciprocals are staged in
-//      LDS; lanes over (channel, row) pairs gather the knot values Y of rows
-//      p0-1 .. qlast+2 of every channel into an LDS tile (end knots clamped,
+//      LDS; lanes over (row, channel) pairs gather the knot values Y of rows
+//      p0-1 .. qlast+2 of every channel from the sample-major copy of the signal
+//      (one knot's channels are adjacent words) into an LDS tile (end knots clamped,
 //      memReducedEpochs_dev.cu:447-486, 545-584) and write Y of rows p0 .. qlast+1
 //      to yknot (knot-major, channel-minor: coalesced over channels).
 //   2. Lanes (block, channel): the right-hand side of the separator after the
@@ -642,9 +660,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
     const int cnt = min(*ed.bcount, ed.bcap);
     for (int w = br.y; w < cnt; w += ed.bblocks) {
         const int2 it = ed.bitem[w];
-        const EpochState* st = ed.st;
         const int ke = it.x, env = ke & 1;
-        const int n = ed.n, T = ed.T;
+        const int n = ed.n;
         const int E = ed.E[ke];
         const int J = block_geom(E, 0).J;
         const int bpw = ed.bpw;
@@ -660,8 +677,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         // is 2^32 itself when n = 1
         const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
-        const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
         const int np = ed.np;
+        const GPtr<double> curt = gptr(ed.curt);   // [T][np] current signal, sample-major
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * np);   // [knot][channel], row stride np
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
@@ -697,7 +714,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
                 const int c = idx - r * n;
                 cv[g] = c;
                 rv[g] = r;
-                yv[g] = cur[(unsigned)c * (unsigned)T + (unsigned)sX[r]];
+                yv[g] = curt[(unsigned)sX[r] * (unsigned)np + (unsigned)c];
             }
 #pragma unroll
             for (int g = 0; g < GU; ++g) {
@@ -717,15 +734,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         const bool has_left = p0 == 1, has_right = rE <= RW - 1;
         if (E >= 6 && (has_left || has_right)) {
             for (int c = tid; c < n; c += kBlockThreads) {
-                const unsigned base = (unsigned)c * (unsigned)T;
                 if (has_left) {
-                    const double y = clamp_end(cur[base + X0], cur[base + X1], cur[base + X2], X0, X1, X2, env, true);
+                    const double y = clamp_end(curt[(unsigned)X0 * (unsigned)np + (unsigned)c],
+                                               curt[(unsigned)X1 * (unsigned)np + (unsigned)c],
+                                               curt[(unsigned)X2 * (unsigned)np + (unsigned)c], X0, X1, X2, env, true);
                     sY[c * RS] = y;
                     yk[c] = y;
                 }
                 if (has_right) {
                     const double y =
-                        clamp_end(cur[base + XE1], cur[base + XE2], cur[base + XE3], XE1, XE2, XE3, env, false);
+                        clamp_end(curt[(unsigned)XE1 * (unsigned)np + (unsigned)c],
+                                  curt[(unsigned)XE2 * (unsigned)np + (unsigned)c],
+                                  curt[(unsigned)XE3 * (unsigned)np + (unsigned)c], XE1, XE2, XE3, env, false);
                     for (int r = rE; r < RW; ++r) sY[c * RS + r] = y;
                     if (rE >= 1 && rE <= R + 1) yk[(unsigned)(E - 1) * (unsigned)np + (unsigned)c] = y;
                 }
