@@ -230,7 +230,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
 struct PlanLayout {
     size_t desc_at = 0, tab_at[kNumMaps] = {}, list_at[kNumLists] = {};
     size_t cnt_at = 0, stats_at = 0, ndone_at = 0, fresh_at = 0, states_at = 0, total = 0;
-    size_t bitem_at = 0, bcnt_at = 0;   // per-epoch k_block items (int2) and counters (one line each)
+    size_t bitem_at = 0, bcnt_at = 0;   // per-epoch k_block items (int4) and counters (one line each)
     int nblk[kNumMaps] = {};
     long lcap[kNumLists] = {};
     long gcap[kNumLists][kXcds] = {};
@@ -272,7 +272,7 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
     size_t bitems = 0;
     for (int e = 0; e < count; ++e) bitems += (size_t)eds[e].bcap;
     P->bitem_at = total;
-    total += align_up(sizeof(int2) * bitems);
+    total += align_up(sizeof(int4) * bitems);
     P->bcnt_at = total;
     total += align_up(kBcountStride * (size_t)count);
     P->stats_at = total;
@@ -300,7 +300,7 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     size_t bpos = 0;
     for (int e = 0; e < count; ++e) {
         eds[e].xcd = grp[e];
-        eds[e].bitem = (int2*)(region + P.bitem_at) + bpos;
+        eds[e].bitem = (int4*)(region + P.bitem_at) + bpos;
         eds[e].bcount = (int*)(region + P.bcnt_at + kBcountStride * (size_t)e);
         bpos += (size_t)eds[e].bcap;
     }

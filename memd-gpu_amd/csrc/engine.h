@@ -130,10 +130,10 @@ struct EpochDesc {
     int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
     int bpw;                   // solver blocks per k_block work item
-    // k_block work items (ke, group) of this epoch, appended by k_compact and
+    // k_block work items (ke, group, E, 0) of this epoch, appended by k_compact and
     // drained by the epoch's bblocks workgroups (an XCD works through one epoch's
     // items at a time, so the epoch's signal stays in its L2)
-    int2* bitem;
+    int4* bitem;
     int* bcount;
     int bcap;
     int bblocks;

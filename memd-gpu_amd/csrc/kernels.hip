@@ -484,7 +484,7 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
             }
         for (int i = tid; i < s_bm[e2]; i += 128) {
             const int at = s_bbase + (e2 ? s_bm[0] : 0) + i;
-            if (at < ed.bcap) ed.bitem[at] = make_int2(2 * k + e2, i);
+            if (at < ed.bcap) ed.bitem[at] = make_int4(2 * k + e2, i, sE[e2], 0);
         }
     }
 }
@@ -633,18 +633,19 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 // Knot values, right-hand sides and phase 1, per (direction, envelope) and group
 // of EpochDesc::bpw solver blocks (rows p0..qlast, at most 32 * bpw).
 //   1. The group's knot positions, shared factors and reciprocals are staged in
-//      LDS; lanes over (row, channel) pairs gather the knot values Y of rows
-//      p0-1 .. qlast+2 of every channel from the sample-major copy of the signal
-//      (one knot's channels are adjacent words) into an LDS tile (end knots clamped,
-//      memReducedEpochs_dev.cu:447-486, 545-584) and write Y of rows p0 .. qlast+1
-//      to yknot (knot-major, channel-minor: coalesced over channels).
+//      LDS.  The knot values Y of rows p0-1 .. qlast+2 of every channel are then
+//      copied from the sample-major signal (one knot's channels are adjacent words)
+//      into a row-major LDS tile by 16-byte global-to-LDS loads, which hold no
+//      registers, so the whole tile is in flight at once.  The end knots are
+//      clamped in the tile (memReducedEpochs_dev.cu:447-486, 545-584) and Y of rows
+//      p0 .. qlast+1 is stored to yknot (knot-major, channel-minor: coalesced).
 //   2. Lanes (block, channel): the right-hand side of the separator after the
 //      block (it reads the next block's first row) is formed first; after a
 //      barrier, r_u = 3 (sl_u - sl_{u-1}) with sl_u = (Y_{u+1} - Y_u) * ih_u and the
 //      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
 //      window of Y values in registers keeping every read ahead of the writes.
 //   3. Lanes over (row, channel) pairs store the block rows' y (coalesced).
-__global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __restrict__ eps,
+__global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __restrict__ eps,
                                                          const int2* __restrict__ tab, int rmax, int tile_off) {
     constexpr int L = kSolverBlock;
     extern __shared__ double lds[];
@@ -653,16 +654,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
     double* s_lo = lds + 2 * rmax;        // [rmax]
     double* s_ih = lds + 3 * rmax;        // [rmax + 2]
     int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 2);   // [rmax + 3]
-    double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);   // [n][RS], RS = (R + 3) | 1
+    // tile [RW][np]: row r = knot p0 - 1 + r (clamped to E - 1), channels innermost;
+    // 16-byte aligned (tile_off)
+    double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);
     const int2 br = tab[blockIdx.x];   // (epoch, workgroup of the epoch)
     const EpochDesc& ed = eps[br.x];
     const int tid = threadIdx.x;
     const int cnt = min(*ed.bcount, ed.bcap);
+    int4 it = br.y < cnt ? ed.bitem[br.y] : make_int4(0, 0, 0, 0);
     for (int w = br.y; w < cnt; w += ed.bblocks) {
-        const int2 it = ed.bitem[w];
+        // the next item's record is loaded while this one is solved
+        const int4 nit = w + ed.bblocks < cnt ? ed.bitem[w + ed.bblocks] : it;
         const int ke = it.x, env = ke & 1;
-        const int n = ed.n;
-        const int E = ed.E[ke];
+        const int n = ed.n, np = ed.np;
+        const int E = it.z;
         const int J = block_geom(E, 0).J;
         const int bpw = ed.bpw;
         const int b0 = it.y * bpw;
@@ -671,14 +676,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         const int qlast = block_geom(E, b0 + nb - 1).q;
         const int R = qlast - p0 + 1;             // rows of the group, <= rmax
         const int RW = R + 3;                     // tile rows p0-1 .. qlast+2
-        const int RS = RW | 1;                    // LDS row stride (odd: conflict-free columns)
-        // idx / n as (idx * ceil(2^32 / n)) >> 32 in 64 bits: exact for idx * (n - 1) < 2^32
-        // (the tile bounds idx < n * RW by 160 KB / 8 and n <= 256 here); the constant
-        // is 2^32 itself when n = 1
+        // idx / d as (idx * ceil(2^32 / d)) >> 32 in 64 bits: exact for idx * (d - 1) < 2^32
+        // (the tile bounds idx < np * RW by 160 KB / 8 and d <= 256 here); the constant
+        // is 2^32 itself when d = 1
         const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
+        const unsigned long long pmagic = (0x100000000ull + (unsigned)np - 1) / (unsigned)np;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
-        const int np = ed.np;
-        const GPtr<double> curt = gptr(ed.curt);   // [T][np] current signal, sample-major
+        const double* curt = ed.curt;   // [T][np] current signal, sample-major
         const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * np);   // [knot][channel], row stride np
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
@@ -698,69 +702,59 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
         // reciprocals of intervals p0-1 .. qlast+1 (the last one: the next separator's
         // right-hand side; clamped where the group ends the system)
         for (int r = tid; r <= R + 1; r += kBlockThreads) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[min(p0 - 1 + r, E - 2)];
-        __syncthreads();
-        // 1b. knot values: lanes over (channel, tile row) pairs
-        const int X0 = sX[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
+        const int X0 = X[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
         const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
-        // GU gathers per lane are issued before any is used (their latencies overlap)
-        constexpr int GU = 8;
-        for (int i0 = tid; i0 < n * RW; i0 += kBlockThreads * GU) {
-            double yv[GU];
-            int cv[GU], rv[GU];
-#pragma unroll
-            for (int g = 0; g < GU; ++g) {
-                const int idx = min(i0 + kBlockThreads * g, n * RW - 1);   // (tile row, channel)
-                const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
-                const int c = idx - r * n;
-                cv[g] = c;
-                rv[g] = r;
-                yv[g] = curt[(unsigned)sX[r] * (unsigned)np + (unsigned)c];
+        __syncthreads();
+        // 1b. the tile, 16 bytes (two channels of one row) per lane: lane l of a wave
+        //     fills bytes 16 l .. 16 l + 15 from the instruction's LDS base
+        {
+            const int pairs = RW * np / 2;
+            const int lane = tid & 63;
+            for (int q0 = tid & ~63; q0 < pairs; q0 += kBlockThreads) {
+                const int q = q0 + lane;
+                if (q < pairs) {
+                    const int f = 2 * q;
+                    const int r = (int)(((unsigned long long)f * pmagic) >> 32);
+                    const int c = f - r * np;
+                    __builtin_amdgcn_global_load_lds(curt + (size_t)sX[r] * np + c,
+                                                     (__attribute__((address_space(3))) void*)(sY + 2 * q0), 16, 0, 0);
+                }
             }
-#pragma unroll
-            for (int g = 0; g < GU; ++g) {
-                if (i0 + kBlockThreads * g >= n * RW) break;
-                const int c = cv[g], r = rv[g];
-                const int u = min(p0 - 1 + r, E - 1);
-                if (E >= 6 && (u == 0 || u == E - 1)) continue;   // clamped end knots: below
-                const double y = yv[g];
-                sY[c * RS + r] = y;
-                if ((r >= 1 && r <= R + 1 && p0 - 1 + r <= E - 1) || (r == 0 && u == 0))
-                    yk[(unsigned)u * (unsigned)np + (unsigned)c] = y;
-            }
+            __builtin_amdgcn_s_waitcnt(0);   // the tile has landed (vmcnt) before the barrier
         }
-        // end knots of E >= 6 clamped (tile row 0 is knot 0 in the first group; rows
-        // rE.. hold knot E-1 in the last)
+        __syncthreads();
+        // end knots of E >= 6 clamped in the tile (row 0 is knot 0 in the first
+        // group; rows rE - 2 .. rE are knots E-3 .. E-1 in the last, and the rows past
+        // rE repeat knot E-1)
         const int rE = E - 1 - (p0 - 1);
         const bool has_left = p0 == 1, has_right = rE <= RW - 1;
         if (E >= 6 && (has_left || has_right)) {
             for (int c = tid; c < n; c += kBlockThreads) {
-                if (has_left) {
-                    const double y = clamp_end(curt[(unsigned)X0 * (unsigned)np + (unsigned)c],
-                                               curt[(unsigned)X1 * (unsigned)np + (unsigned)c],
-                                               curt[(unsigned)X2 * (unsigned)np + (unsigned)c], X0, X1, X2, env, true);
-                    sY[c * RS] = y;
-                    yk[c] = y;
-                }
+                if (has_left) sY[c] = clamp_end(sY[c], sY[np + c], sY[2 * np + c], X0, X1, X2, env, true);
                 if (has_right) {
-                    const double y =
-                        clamp_end(curt[(unsigned)XE1 * (unsigned)np + (unsigned)c],
-                                  curt[(unsigned)XE2 * (unsigned)np + (unsigned)c],
-                                  curt[(unsigned)XE3 * (unsigned)np + (unsigned)c], XE1, XE2, XE3, env, false);
-                    for (int r = rE; r < RW; ++r) sY[c * RS + r] = y;
-                    if (rE >= 1 && rE <= R + 1) yk[(unsigned)(E - 1) * (unsigned)np + (unsigned)c] = y;
+                    const double y = clamp_end(sY[rE * np + c], sY[(rE - 1) * np + c], sY[(rE - 2) * np + c], XE1,
+                                               XE2, XE3, env, false);
+                    for (int r = rE; r < RW; ++r) sY[r * np + c] = y;
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        // Y of knots p0 .. qlast+1 (and knot 0 in the first group) to yknot
+        for (int idx = tid; idx < n * (R + 2); idx += kBlockThreads) {   // (tile row, channel)
+            const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
+            const int c = idx - r * n;
+            const int u = p0 - 1 + r;
+            if ((r >= 1 && u <= E - 1) || u == 0) yk[(unsigned)u * (unsigned)np + (unsigned)c] = sY[r * np + c];
+        }
         // 2. per (block, channel) lane; n * nb may exceed the workgroup
         for (int lane = tid; lane < n * nb; lane += kBlockThreads) {   // separator rows q+1 keep their right-hand side
             const int slot = lane / n, c = lane - slot * n;
             if (b0 + slot >= J) continue;
             const BlockGeom g = block_geom(E, b0 + slot);
             const int q = g.q, off = g.p - p0, len = q - g.p + 1;
-            const double* Yc = sY + c * RS + off;   // Yc[i] = Y(p - 1 + i)
-            const double sl = (Yc[len + 2] - Yc[len + 1]) * s_ih[off + len + 1];
-            const double slm = (Yc[len + 1] - Yc[len]) * s_ih[off + len];
+            const double* Yc = sY + off * np + c;   // Yc[i * np] = Y(p - 1 + i)
+            const double sl = (Yc[(len + 2) * np] - Yc[(len + 1) * np]) * s_ih[off + len + 1];
+            const double slm = (Yc[(len + 1) * np] - Yc[len * np]) * s_ih[off + len];
             sip[((size_t)(b0 + slot + 1) * 3 + 1) * n + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
         }
         __syncthreads();
@@ -768,22 +762,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             const int slot = lane / n, c = lane - slot * n;
             const BlockGeom g = block_geom(E, b0 + slot);
             const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
-            double* Yc = sY + c * RS + off;
+            double* Yc = sY + off * np + c;   // Yc[i * np] = Y(p - 1 + i)
             // forward sweep, z_i written over Y(p+i) once no read needs it
-            double ym = Yc[0], y0 = Yc[1], zp = 0.0;
+            const bool nak = ed.spline && E >= 4;   // not-a-knot rows 1 and N = E-2
+            double ym = Yc[0], y0 = Yc[np], zp = 0.0;
 #pragma unroll 4
             for (int i = 0; i < len; ++i) {
                 const int u = p + i;
-                const double yp = Yc[i + 2];
+                const double yp = Yc[(i + 2) * np];
                 const double sl = (yp - y0) * s_ih[off + i + 1];
                 const double slm = (y0 - ym) * s_ih[off + i];
                 double r = 3.0 * (sl - slm);
-                if (ed.spline && E >= 4) {   // not-a-knot rows 1 and N = E-2
+                if (nak) {
                     if (u == 1) r = r * (double)(X2 - X1);                 // h_1
                     else if (u == E - 2) r = r * (double)(XE2 - XE3);      // h_{N-1}
                 }
                 zp = (i == 0) ? r * s_rd[off] : fma(-s_lo[off + i], zp, r) * s_rd[off + i];
-                Yc[i + 1] = zp;
+                Yc[(i + 1) * np] = zp;
                 ym = y0;
                 y0 = yp;
             }
@@ -791,9 +786,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             double x = 0.0;
 #pragma unroll 4
             for (int i = len - 1; i >= 0; --i) {
-                const double z = Yc[i + 1];
+                const double z = Yc[(i + 1) * np];
                 x = (i == len - 1) ? z : fma(-s_cp[off + i], x, z);
-                Yc[i + 1] = x;
+                Yc[(i + 1) * np] = x;
             }
         }
         __syncthreads();
@@ -804,12 +799,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_block(const EpochDesc* __rest
             const int c2 = idx - r * n;
             const int u = p0 + r;
             if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
-            const double yv = sY[c2 * RS + r + 1];
+            const double yv = sY[(r + 1) * np + c2];
             ys[(unsigned)u * (unsigned)np + (unsigned)c2] = yv;
             // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
             if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * n + c2] = yv;
             if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * n + c2] = yv;
         }
+        it = nit;
     }
 }
 
@@ -1712,13 +1708,14 @@ BlockLds block_lds(const int* n, const int* bpw, int count) {
     for (int e = 0; e < count; ++e) {
         const int r = kSolverBlock * bpw[e];
         rmax = r > rmax ? r : rmax;
-        const size_t v = (size_t)n[e] * (size_t)((r + 3) | 1);
+        const size_t v = (size_t)channel_stride(n[e]) * (size_t)(r + 3);
         tile = v > tile ? v : tile;
     }
     BlockLds L;
     L.rmax = rmax;
-    // rd, cp, lo: rmax doubles each; ih: rmax + 2 doubles; X: rmax + 3 ints (8-byte aligned)
-    L.tile_off = sizeof(double) * (4 * (size_t)rmax + 2) + (((size_t)rmax + 3) * sizeof(int) + 7) / 8 * 8;
+    // rd, cp, lo: rmax doubles each; ih: rmax + 2 doubles; X: rmax + 3 ints; the tile
+    // [rows][np] starts 16-byte aligned (global-to-LDS loads of 16 bytes per lane)
+    L.tile_off = (sizeof(double) * (4 * (size_t)rmax + 2) + ((size_t)rmax + 3) * sizeof(int) + 15) / 16 * 16;
     L.bytes = L.tile_off + tile * sizeof(double);
     return L;
 }
