@@ -1132,6 +1132,15 @@ __global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, 
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Inclusive prefix sum within each row of 16 lanes (DPP row shifts; lanes whose
+// source would cross the row add 0).
+__device__ __forceinline__ void row_scan16(int& x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);   // row_shr:8
+}
+
 // Phase B of one envelope for a lane's 4 consecutive samples (interval i of the
 // first): each sample uses the record of X_j < t <= X_{j+1}.  ENV 0 stores the
 // upper envelope, ENV 1 adds the mean of both to the accumulator.  The first
@@ -1268,27 +1277,22 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             const int lo = min(max(pr - 1, 0), E - 2);
             const int hi = min(max(pr + inside - 1, 0), E - 2);
             const int ni = hi - lo + 1;
-            const int Ep = __shfl_xor(E, 1);
+            const int Ep = __builtin_amdgcn_mov_dpp(E, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]: the other envelope
             const bool valid = have && E > 3 && Ep > 3;
             const int stride = ni | 1;   // odd: a wave's channels read disjoint banks
             const int need = valid ? cgn * stride : 0;
             const int nxk = valid ? ni + 1 : 0;
             const int nit = valid ? nq * ni : 0;
             int pneed = need, pxk = nxk, pit = nit;
-#pragma unroll
-            for (int o = 1; o < NS; o <<= 1) {
-                const int a = __shfl_up(pneed, o), b = __shfl_up(pxk, o), d = __shfl_up(pit, o);
-                if (tid >= o) {
-                    pneed += a;
-                    pxk += b;
-                    pit += d;
-                }
-            }
+            // inclusive prefix sums over the NS = 16 segments: one DPP row of lanes
+            row_scan16(pneed);
+            row_scan16(pxk);
+            row_scan16(pit);
             // leading directions whose records and knot positions fit (the first always does)
             const unsigned long long fit =
                 __ballot(have && (tid & 1) && pneed <= kEvalCap && pxk <= kEvalXkCap);
             const int nb = max((int)__popcll(fit), 1);
-            const int total = __shfl(pit, 2 * nb - 1);
+            const int total = __builtin_amdgcn_readlane(pit, 2 * nb - 1);
             if (tid < 2 * nb) {
                 sSegA[tid] = make_int4(pit - nit, 0, ni, lo);
                 sSegB[tid] = make_int4(pneed - need, stride, E, pxk - nxk);
