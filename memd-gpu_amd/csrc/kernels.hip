@@ -258,6 +258,19 @@ __device__ void flags_wide(const EpochDesc& ed, const EpochState* st, int by) {
 }
 
 
+// Neighbouring lanes' values by DPP wavefront shifts (no LDS round trip); lane 0
+// (from_prev) and lane 63 (from_next) get no value and are filled by the caller.
+__device__ __forceinline__ double wave_from_prev(double v) {   // lane i <- lane i-1
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, false);   // wave_shr:1
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_from_next(double v) {   // lane i <- lane i+1
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, false);   // wave_shl:1
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // One workgroup per (epoch, 256-sample tile) and all directions: the tile of
 // the current signal (plus one sample each side) is staged in LDS once, then
 // every direction's projection, extremum tests and ballots run from LDS.
@@ -358,8 +371,8 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
                 const int k = k0 + d;
                 // neighbours: adjacent lanes; across waves and tiles through LDS
                 const double p = acc[d];
-                double a = __shfl_up(p, 1);
-                double b = __shfl_down(p, 1);
+                double a = wave_from_prev(p);
+                double b = wave_from_next(p);
                 if (lane == 0) a = wv > 0 ? Pe[d][wv - 1][1] : Ph[k - kh][0];
                 if (lane == 63) b = wv < NW - 1 ? Pe[d][wv + 1][0] : Ph[k - kh][1];
                 bool fmax = false, fmin = false;
