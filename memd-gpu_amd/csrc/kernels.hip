@@ -531,6 +531,12 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
         const GMut<double> ih = gmut(fac_row(ed, ke, kFacIh));
         const GMut<double> i3h = gmut(fac_row(ed, ke, kFacI3h));
         const int J = block_geom(E, 0).J;
+        // reciprocals of every interval 0 .. E-2, all lanes (coalesced)
+        for (int u = lane; u < E - 1; u += 64) {
+            const double h = (double)(X[u + 1] - X[u]);
+            ih[u] = 1.0 / h;
+            i3h[u] = 1.0 / (3.0 * h);
+        }
         // one lane per block: the block's rows live in registers, memory sees stores only
         for (int b = lane; b <= J; b += 64) {
             const BlockGeom g = block_geom(E, b);
@@ -538,12 +544,6 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
             int xs[L + 2];   // X[p-1 .. q+1]
 #pragma unroll
             for (int i = 0; i < L + 2; ++i) xs[i] = X[p - 1 + min(i, len + 1)];
-#pragma unroll 1
-            for (int i = 0; i <= len; ++i) {   // intervals p-1 .. q
-                double h = (double)(X[p + i] - X[p - 1 + i]);
-                ih[p - 1 + i] = 1.0 / h;
-                i3h[p - 1 + i] = 1.0 / (3.0 * h);
-            }
             const bool left = b >= 1, right = b < J;
             double cpr[L], zl[L];
             double cprev = 0.0, z = 0.0, zr = 0.0;
