@@ -76,7 +76,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, X, fac, redsys, yknot, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, X, fac, redsys, yend, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -95,7 +95,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.fac = take((size_t)K * 2 * 6 * Ecap * 8);
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     const size_t np = (size_t)channel_stride(n);
-    L.yknot = take((size_t)K * 2 * np * Ecap * 8);
+    L.yend = take((size_t)K * 2 * 2 * np * 8);
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
     L.sep = take((size_t)K * 2 * np * nb * 8);
@@ -209,7 +209,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.X = (int*)(b + L.X);
     d.fac = (double*)(b + L.fac);
     d.red = (double*)(b + L.redsys);
-    d.yknot = (double*)(b + L.yknot);
+    d.yend = (double*)(b + L.yend);
     d.curt = (double*)(b + L.curt);
     d.ysol = (double*)(b + L.ysol);
     d.coef = (double*)(b + L.coef);

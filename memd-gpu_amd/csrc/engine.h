@@ -93,7 +93,7 @@ struct EpochState {
 // Everything a kernel needs to know about one epoch.  Lives in device memory.
 struct EpochDesc {
     int n, T, K, W;        // W = ceil(T/64) bitmask words per projection row
-    int np;                // row stride of yknot, ysol, sep: n rounded up to a multiple of 4
+    int np;                // row stride of curt, yend, ysol, sep: n rounded up to a multiple of 4
     int Ecap;              // knot capacity per (direction, envelope) = T/2 + 2
     int nbmax;             // solver blocks per (direction, envelope) at Ecap knots
     int N_imf, S, mode, window;
@@ -113,7 +113,8 @@ struct EpochDesc {
     // Knot-major and separator-major arrays: rows of np = n rounded up to a
     // multiple of 4 words (32-byte aligned 4-channel groups; padding is never read
     // into results)
-    double* yknot;         // [K][2][Ecap][np] knot values Y (end knots clamped)
+    double* yend;          // [K][2][2][np] clamped end-knot values Y_0, Y_{E-1} (E >= 6); the other
+                           //   knot values are the signal at the knot (curt)
     double* ysol;          // [K][2][Ecap][np] block solutions y at block rows (separator rows unused)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][nbmax][np] separator values (z, then X_j)

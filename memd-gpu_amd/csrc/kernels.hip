@@ -650,8 +650,9 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
 //      copied from the sample-major signal (one knot's channels are adjacent words)
 //      into a row-major LDS tile by 16-byte global-to-LDS loads, which hold no
 //      registers, so the whole tile is in flight at once.  The end knots are
-//      clamped in the tile (memReducedEpochs_dev.cu:447-486, 545-584) and Y of rows
-//      p0 .. qlast+1 is stored to yknot (knot-major, channel-minor: coalesced).
+//      clamped in the tile (memReducedEpochs_dev.cu:447-486, 545-584); the clamped
+//      end values go to yend (the other knot values are the signal itself, which
+//      the readers take from curt).
 //   2. Lanes (block, channel): the right-hand side of the separator after the
 //      block (it reads the next block's first row) is formed first; after a
 //      barrier, r_u = 3 (sl_u - sl_{u-1}) with sl_u = (Y_{u+1} - Y_u) * ih_u and the
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const unsigned long long pmagic = (0x100000000ull + (unsigned)np - 1) / (unsigned)np;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const double* curt = ed.curt;   // [T][np] current signal, sample-major
-        const GMut<double> yk = gmut(ed.yknot + (size_t)ke * ed.Ecap * np);   // [knot][channel], row stride np
+        const GMut<double> ye = gmut(ed.yend + (size_t)ke * 2 * np);   // [2][np]: Y_0, Y_{E-1} clamped
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
         __syncthreads();   // the previous item's tile is no longer read
@@ -743,21 +744,19 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const bool has_left = p0 == 1, has_right = rE <= RW - 1;
         if (E >= 6 && (has_left || has_right)) {
             for (int c = tid; c < n; c += kBlockThreads) {
-                if (has_left) sY[c] = clamp_end(sY[c], sY[np + c], sY[2 * np + c], X0, X1, X2, env, true);
+                if (has_left) {
+                    const double y = clamp_end(sY[c], sY[np + c], sY[2 * np + c], X0, X1, X2, env, true);
+                    sY[c] = y;
+                    ye[c] = y;
+                }
                 if (has_right) {
                     const double y = clamp_end(sY[rE * np + c], sY[(rE - 1) * np + c], sY[(rE - 2) * np + c], XE1,
                                                XE2, XE3, env, false);
                     for (int r = rE; r < RW; ++r) sY[r * np + c] = y;
+                    ye[np + c] = y;
                 }
             }
             __syncthreads();
-        }
-        // Y of knots p0 .. qlast+1 (and knot 0 in the first group) to yknot
-        for (int idx = tid; idx < n * (R + 2); idx += kBlockThreads) {   // (tile row, channel)
-            const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
-            const int c = idx - r * n;
-            const int u = p0 - 1 + r;
-            if ((r >= 1 && u <= E - 1) || u == 0) yk[(unsigned)u * (unsigned)np + (unsigned)c] = sY[r * np + c];
         }
         // 2. per (block, channel) lane; n * nb may exceed the workgroup
         for (int lane = tid; lane < n * nb; lane += kBlockThreads) {   // separator rows q+1 keep their right-hand side
@@ -1021,8 +1020,11 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
                 v0[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j, E, J);
                 v1[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j + 1, E, J);
-                y0[a] = ed.yknot[((size_t)ke * ed.Ecap + j) * ed.np + c];
-                y1[a] = ed.yknot[((size_t)ke * ed.Ecap + j + 1) * ed.np + c];
+                // knot values: the signal at the knot, or the clamped end values
+                y0[a] = (E >= 6 && j == 0) ? ed.yend[(size_t)ke * 2 * ed.np + c]
+                                           : ed.curt[(size_t)X[j] * ed.np + c];
+                y1[a] = (E >= 6 && j + 1 == E - 1) ? ed.yend[((size_t)ke * 2 + 1) * ed.np + c]
+                                                   : ed.curt[(size_t)X[j + 1] * ed.np + c];
                 x0[a] = X[j];
                 x1[a] = X[j + 1];
                 ih[a] = ihv[j];
@@ -1352,7 +1354,6 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 // next group's and are not used
                 const int np = ed.np;
                 const size_t rb = (size_t)ke * ed.Ecap + j;
-                const GPtr<v4d> Yr = gptr(reinterpret_cast<const v4d*>(ed.yknot + rb * np + c0));
                 const GPtr<v4d> Mr = gptr(reinterpret_cast<const v4d*>(ed.ysol + rb * np + c0));
                 const double* Sb = ed.sep + (size_t)ke * ed.nbmax * np + c0;
                 const GPtr<v4d> S0 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)bj * np));
@@ -1362,6 +1363,11 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
                 x0 = Xr[j];
                 x1 = Xr[j + 1];
+                // knot values: the signal at the two knots (sample-major rows), or the
+                // clamped end values (E >= 6)
+                const double* yb0 = (E >= 6 && j == 0) ? ed.yend + (size_t)ke * 2 * np : ed.curt + (size_t)x0 * np;
+                const double* yb1 =
+                    (E >= 6 && j + 1 == E - 1) ? ed.yend + ((size_t)ke * 2 + 1) * np : ed.curt + (size_t)x1 * np;
                 a0v = F[kFacAl * ed.Ecap + j];
                 a1v = F[kFacAl * ed.Ecap + j + 1];
                 b0v = F[kFacBe * ed.Ecap + j];
@@ -1369,7 +1375,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 ih = F[kFacIh * ed.Ecap + j];
                 i3h = F[kFacI3h * ed.Ecap + j];
                 const int nr = np >> 2;   // double4 per row
-                const v4d yv0 = Yr[0], yv1 = Yr[nr], mv0 = Mr[0], mv1 = Mr[nr];
+                const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(yb0 + c0));
+                const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(yb1 + c0));
+                const v4d mv0 = Mr[0], mv1 = Mr[nr];
                 const v4d sv0 = S0[0], sv1 = S1[0], sv2 = S2[0];
                 v0[0] = yv0.x; v0[1] = yv0.y; v0[2] = yv0.z; v0[3] = yv0.w;
                 v1[0] = yv1.x; v1[1] = yv1.y; v1[2] = yv1.z; v1[3] = yv1.w;
