@@ -76,7 +76,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, X, fac, redsys, yend, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, X, fac, redsys, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -87,7 +87,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t at = off; off += align_up(bytes); return at; };
     L.running = take((size_t)n * T * 8);
-    L.curt = take((size_t)T * channel_stride(n) * 8);
+    L.curt = take(((size_t)T + 4 * (size_t)K) * channel_stride(n) * 8);   // then yend
     L.mask = take((size_t)K * 2 * W * 8);
     L.pre = take((size_t)K * 2 * W * 4);
     L.E = take((size_t)K * 2 * 4);
@@ -95,7 +95,6 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.fac = take((size_t)K * 2 * 6 * Ecap * 8);
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     const size_t np = (size_t)channel_stride(n);
-    L.yend = take((size_t)K * 2 * 2 * np * 8);
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
     L.sep = take((size_t)K * 2 * np * nb * 8);
@@ -209,8 +208,8 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.X = (int*)(b + L.X);
     d.fac = (double*)(b + L.fac);
     d.red = (double*)(b + L.redsys);
-    d.yend = (double*)(b + L.yend);
     d.curt = (double*)(b + L.curt);
+    d.yend = d.curt + (size_t)d.T * d.np;   // the rows after the signal (desc_geometry set T and np)
     d.ysol = (double*)(b + L.ysol);
     d.coef = (double*)(b + L.coef);
     d.sep = (double*)(b + L.sep);

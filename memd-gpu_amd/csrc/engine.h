@@ -102,7 +102,8 @@ struct EpochDesc {
     const double* V;       // [K][n]
     double* imfs;          // [N_imf][n][T]
     double* running;       // [n][T]
-    double* curt;          // [T][np] current signal, sample-major (written by k_flags every step)
+    double* curt;          // [T][np] current signal, sample-major (written by k_flags every step),
+                           //   followed by yend
     unsigned long long* mask; // [K][2][W] extrema flags (bit t%64 of word t/64)
     int* pre;              // [K][2][W] knots before each word
     int* E;                // [K][2] knot counts
@@ -113,8 +114,8 @@ struct EpochDesc {
     // Knot-major and separator-major arrays: rows of np = n rounded up to a
     // multiple of 4 words (32-byte aligned 4-channel groups; padding is never read
     // into results)
-    double* yend;          // [K][2][2][np] clamped end-knot values Y_0, Y_{E-1} (E >= 6); the other
-                           //   knot values are the signal at the knot (curt)
+    double* yend;          // [K][2][2][np] clamped end-knot values Y_0, Y_{E-1} (E >= 6): rows T ..
+                           //   T + 4K - 1 of curt; the other knot values are the signal at the knot
     double* ysol;          // [K][2][Ecap][np] block solutions y at block rows (separator rows unused)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][nbmax][np] separator values (z, then X_j)

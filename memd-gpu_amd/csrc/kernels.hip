@@ -1020,11 +1020,12 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
                 v0[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j, E, J);
                 v1[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j + 1, E, J);
-                // knot values: the signal at the knot, or the clamped end values
-                y0[a] = (E >= 6 && j == 0) ? ed.yend[(size_t)ke * 2 * ed.np + c]
-                                           : ed.curt[(size_t)X[j] * ed.np + c];
-                y1[a] = (E >= 6 && j + 1 == E - 1) ? ed.yend[((size_t)ke * 2 + 1) * ed.np + c]
-                                                   : ed.curt[(size_t)X[j + 1] * ed.np + c];
+                // knot values: the signal at the knot, or the clamped end values (rows
+                // T + 2 ke and T + 2 ke + 1 of the same array)
+                const int r0 = (E >= 6 && j == 0) ? ed.T + 2 * ke : X[j];
+                const int r1 = (E >= 6 && j + 1 == E - 1) ? ed.T + 2 * ke + 1 : X[j + 1];
+                y0[a] = ed.curt[(size_t)r0 * ed.np + c];
+                y1[a] = ed.curt[(size_t)r1 * ed.np + c];
                 x0[a] = X[j];
                 x1[a] = X[j + 1];
                 ih[a] = ihv[j];
@@ -1221,6 +1222,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const EpochState* st = ed.st;
     if (st->done) return;
     const int n = ed.n, T = ed.T, K = ed.K, W = ed.W;
+    const double* curt = ed.curt;   // [T + 4K][np]: the signal sample-major, then the clamped end values
     const int TW = ed.etw;
     const int lgq = 4 + (TW >> 1);       // log2 quads per channel: 4, 5, 6 for TW = 1, 2, 4
     const int CG = 16 / TW;
@@ -1361,13 +1363,10 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const GPtr<v4d> S2 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 2, J) * np));
                 const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
                 const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
+                // the knot positions first: the knot-value loads below depend on them,
+                // and the other loads are issued while they are in flight
                 x0 = Xr[j];
                 x1 = Xr[j + 1];
-                // knot values: the signal at the two knots (sample-major rows), or the
-                // clamped end values (E >= 6)
-                const double* yb0 = (E >= 6 && j == 0) ? ed.yend + (size_t)ke * 2 * np : ed.curt + (size_t)x0 * np;
-                const double* yb1 =
-                    (E >= 6 && j + 1 == E - 1) ? ed.yend + ((size_t)ke * 2 + 1) * np : ed.curt + (size_t)x1 * np;
                 a0v = F[kFacAl * ed.Ecap + j];
                 a1v = F[kFacAl * ed.Ecap + j + 1];
                 b0v = F[kFacBe * ed.Ecap + j];
@@ -1375,10 +1374,16 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 ih = F[kFacIh * ed.Ecap + j];
                 i3h = F[kFacI3h * ed.Ecap + j];
                 const int nr = np >> 2;   // double4 per row
-                const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(yb0 + c0));
-                const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(yb1 + c0));
                 const v4d mv0 = Mr[0], mv1 = Mr[nr];
                 const v4d sv0 = S0[0], sv1 = S1[0], sv2 = S2[0];
+                __builtin_amdgcn_sched_barrier(0);   // keep the independent loads above
+                // knot values: the signal at the two knots (sample-major rows), or the
+                // clamped end values (E >= 6), rows T + 2 ke and T + 2 ke + 1 of the same
+                // array
+                const int r0 = (E >= 6 && j == 0) ? T + 2 * ke : x0;
+                const int r1 = (E >= 6 && j + 1 == E - 1) ? T + 2 * ke + 1 : x1;
+                const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r0 * np + c0));
+                const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r1 * np + c0));
                 v0[0] = yv0.x; v0[1] = yv0.y; v0[2] = yv0.z; v0[3] = yv0.w;
                 v1[0] = yv1.x; v1[1] = yv1.y; v1[2] = yv1.z; v1[3] = yv1.w;
                 m0[0] = mv0.x; m0[1] = mv0.y; m0[2] = mv0.z; m0[3] = mv0.w;
