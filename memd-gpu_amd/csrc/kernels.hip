@@ -1223,6 +1223,14 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     if (st->done) return;
     const int n = ed.n, T = ed.T, K = ed.K, W = ed.W;
     const double* curt = ed.curt;   // [T + 4K][np]: the signal sample-major, then the clamped end values
+    // descriptor fields of the record build, read once: the descriptor is reached
+    // through a generic pointer, so every LDS store would otherwise force a reload
+    const int Ecap = ed.Ecap, np = ed.np, nbmax = ed.nbmax, spline = ed.spline;
+    const double* const ysol = ed.ysol;
+    const double* const sepv = ed.sep;
+    const double* const fac = ed.fac;
+    const int* const Xall = ed.X;
+    const double* const mend = ed.mend;
     const int TW = ed.etw;
     const int lgq = 4 + (TW >> 1);       // log2 quads per channel: 4, 5, 6 for TW = 1, 2, 4
     const int CG = 16 / TW;
@@ -1354,25 +1362,24 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 // 4 channels from c0 (a multiple of 4) in rows of np (a multiple of 4)
                 // words: 32-byte aligned; channels past the group are padding or the
                 // next group's and are not used
-                const int np = ed.np;
-                const size_t rb = (size_t)ke * ed.Ecap + j;
-                const GPtr<v4d> Mr = gptr(reinterpret_cast<const v4d*>(ed.ysol + rb * np + c0));
-                const double* Sb = ed.sep + (size_t)ke * ed.nbmax * np + c0;
+                const size_t rb = (size_t)ke * Ecap + j;
+                const GPtr<v4d> Mr = gptr(reinterpret_cast<const v4d*>(ysol + rb * np + c0));
+                const double* Sb = sepv + (size_t)ke * nbmax * np + c0;
                 const GPtr<v4d> S0 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)bj * np));
                 const GPtr<v4d> S1 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 1, J) * np));
                 const GPtr<v4d> S2 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 2, J) * np));
-                const GPtr<double> F = gptr(ed.fac + (size_t)ke * 6 * ed.Ecap);
-                const GPtr<int> Xr = gptr(ed.X + (size_t)ke * ed.Ecap);
+                const GPtr<double> F = gptr(fac + (size_t)ke * 6 * Ecap);
+                const GPtr<int> Xr = gptr(Xall + (size_t)ke * Ecap);
                 // the knot positions first: the knot-value loads below depend on them,
                 // and the other loads are issued while they are in flight
                 x0 = Xr[j];
                 x1 = Xr[j + 1];
-                a0v = F[kFacAl * ed.Ecap + j];
-                a1v = F[kFacAl * ed.Ecap + j + 1];
-                b0v = F[kFacBe * ed.Ecap + j];
-                b1v = F[kFacBe * ed.Ecap + j + 1];
-                ih = F[kFacIh * ed.Ecap + j];
-                i3h = F[kFacI3h * ed.Ecap + j];
+                a0v = F[kFacAl * Ecap + j];
+                a1v = F[kFacAl * Ecap + j + 1];
+                b0v = F[kFacBe * Ecap + j];
+                b1v = F[kFacBe * Ecap + j + 1];
+                ih = F[kFacIh * Ecap + j];
+                i3h = F[kFacI3h * Ecap + j];
                 const int nr = np >> 2;   // double4 per row
                 const v4d mv0 = Mr[0], mv1 = Mr[nr];
                 const v4d sv0 = S0[0], sv1 = S1[0], sv2 = S2[0];
@@ -1409,8 +1416,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 for (int q = 0; q < 4; ++q) {
                     if (q < nch) {
                         double me0 = 0.0, me1 = 0.0;
-                        if (ed.spline) {
-                            const GPtr<double> Me = gptr(ed.mend + ((size_t)ke * n + cg * CG + 4 * qd + q) * 2);
+                        if (spline) {
+                            const GPtr<double> Me = gptr(mend + ((size_t)ke * n + cg * CG + 4 * qd + q) * 2);
                             me0 = Me[0];
                             me1 = Me[1];
                         }
