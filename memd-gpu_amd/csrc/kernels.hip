@@ -1215,6 +1215,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     __shared__ int sCum[NS][4];          // knots before each word of the tile
     __shared__ unsigned long long sMsk[NS][4];
     __shared__ int sNb;
+    __shared__ unsigned sValid;   // bit s: segment s of the batch is valid (both envelopes of its direction)
     __shared__ int cE[2 * kEvalChunk], cPr[2 * kEvalChunk];   // staged formation inputs
     __shared__ unsigned long long cM[2 * kEvalChunk][4];
     const int2 br = tab[blockIdx.x];
@@ -1328,7 +1329,11 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 }
             }
             if (tid <= NS) sIp[tid] = tid < 2 * nb ? pit - nit : total;
-            if (tid == 0) sNb = nb;
+            const unsigned long long vbits = __ballot(valid);
+            if (tid == 0) {
+                sNb = nb;
+                sValid = (unsigned)vbits;
+            }
         }
         __syncthreads();
         const int nb = uniform(sNb);
@@ -1441,9 +1446,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         // phase B.  Samples 0 and T-1 are knots of every direction, so the knots
         // below a sample t in 1..T-1 number 1..E-1: the interval index needs no
         // upper clamp (t = 0 gives -1, clamped to interval 0).
+        const unsigned vmask = (unsigned)uniform((int)sValid);
         for (int d = 0; d < nb; ++d) {
-            const int E0 = uniform(sSegB[2 * d].z), E1 = uniform(sSegB[2 * d + 1].z);
-            if (!(E0 > 3 && E1 > 3)) continue;
+            if (!((vmask >> (2 * d)) & 1u)) continue;
             nv = nv + 1.0;
             if (lane_on) {
                 {
