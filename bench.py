@@ -411,12 +411,13 @@ def main():
         kernels = {k: round(v * 1e3, 2) for k, v in pr["ms_per_launch"].items()}   # us per launch
         steps = max(pr["steps"], 1)
         W = (T + 63) // 64
-        # fused record+evaluation kernel, per launch: cur read+write (16 B/sample); knot value
-        # and block solution per knot-channel (16 B); per knot the position, the two spike
-        # entries and the two reciprocals (36 B); flag word + prefix per (epoch, direction,
-        # envelope, 64 samples) (12 B).  Flops: two 3-fma Horners + mean per evaluation point
-        # (15), record build per knot-channel (10), divide + subtract per sample (2).
-        bytes_eval = (16 * pr["samples"] + 16 * pr["knot_channels"] + 36 * pr["knots"]) / steps \
+        # fused record+evaluation kernel, per launch: cur read+write (16 B/sample) and one read
+        # of the sample-major copy that supplies the knot values (8 B/sample); the block
+        # solution per knot-channel (8 B); per knot the position, the two spike entries and
+        # the two reciprocals (36 B); flag word + prefix per (epoch, direction, envelope,
+        # 64 samples) (12 B).  Flops: two 3-fma Horners + mean per evaluation point (15),
+        # record build per knot-channel (10), divide + subtract per sample (2).
+        bytes_eval = (24 * pr["samples"] + 8 * pr["knot_channels"] + 36 * pr["knots"]) / steps \
             + 12 * 2 * K * W * B
         flops_eval = (15 * pr["eval_points"] + 10 * pr["knot_channels"] + 2 * pr["samples"]) / steps
         t_eval = pr["ms_per_launch"]["k_eval"] * 1e-3   # slot of the fused record+evaluation kernel
