@@ -336,8 +336,9 @@ def bench_c4(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(epochs shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: epochs are "
+              "independent and shard across GPUs with no exchange (one process per GPU, each "
+              "running this benchmark on its share).", file=sys.stderr)
         sys.exit(2)
     import torch
     import memd_amd

@@ -946,7 +946,7 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
     const size_t rounds = bg.size() > tw.size() ? bg.size() : tw.size();
     memd_opts o;
     memd_opts_default(&o);
-    for (size_t r = 0; r <= rounds; ++r) {   // round 0 warms up (plan region, caches)
+    for (size_t r = 0; r <= rounds; ++r) {   // timing pass 0 warms up (plan region, caches)
         const size_t i = r == 0 ? 0 : r - 1;
         const size_t ib = i < bg.size() ? i : bg.size() - 1, it = i < tw.size() ? i : tw.size() - 1;
         memd_tuning cand = def;
