@@ -296,10 +296,24 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
     const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
     const CPtr<double> V = cptr(ed.V);
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < n * TW; idx += kFlagTile) {
-        const int c = idx / TW, i = idx - c * TW;
-        const int t = t0 - 1 + i;
-        curT[idx] = (t >= 0 && t < T) ? (double)cur[(unsigned)c * (unsigned)T + (unsigned)t] : 0.0;
+    {   // stage the tile: G loads per lane in flight before their LDS stores
+        constexpr int G = 8;
+        const int tot = n * TW;
+        for (int i0 = tid; i0 < tot; i0 += kFlagTile * G) {
+            double v[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {   // unconditional loads from clamped addresses
+                const int idx = min(i0 + g * kFlagTile, tot - 1);
+                const int c = idx / TW, i = idx - c * TW;
+                v[g] = cur[(unsigned)c * (unsigned)T + (unsigned)min(max(t0 - 1 + i, 0), T - 1)];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {   // samples outside 0 .. T-1 are 0
+                const int idx = i0 + g * kFlagTile;
+                const int c = idx / TW, t = t0 - 1 + (idx - c * TW);
+                if (idx < tot) curT[idx] = (t >= 0 && t < T) ? v[g] : 0.0;
+            }
+        }
     }
     __syncthreads();
     {   // sample-major copy of the tile for the knot-value gathers: lanes over
