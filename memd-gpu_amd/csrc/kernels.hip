@@ -677,11 +677,13 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                                                          const int2* __restrict__ tab, int rmax, int tile_off) {
     constexpr int L = kSolverBlock;
     extern __shared__ double lds[];
-    double* s_rd = lds;                   // [rmax]
-    double* s_cp = lds + rmax;            // [rmax]
-    double* s_lo = lds + 2 * rmax;        // [rmax]
-    double* s_ih = lds + 3 * rmax;        // [rmax + 2]
-    int* sX = reinterpret_cast<int*>(lds + 4 * rmax + 2);   // [rmax + 3]
+    // per tile row r (rmax + 3 rows; the entries past a row's use are written, never read)
+    const int rw = rmax + 3;
+    double* s_rd = lds;                   // factor rows p0 + r (r < R)
+    double* s_cp = lds + rw;
+    double* s_lo = lds + 2 * rw;
+    double* s_ih = lds + 3 * rw;          // reciprocal of interval p0 - 1 + r (r <= R + 1)
+    int* sX = reinterpret_cast<int*>(lds + 4 * rw);   // position of knot p0 - 1 + r
     // tile [RW][np]: row r = knot p0 - 1 + r (clamped to E - 1), channels innermost;
     // 16-byte aligned (tile_off)
     double* sY = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tile_off);
@@ -715,21 +717,26 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
         __syncthreads();   // the previous item's tile is no longer read
-        // 1a. knot positions, factors of rows p0..qlast, reciprocals of intervals p0-1..qlast+1
-        for (int r = tid; r < RW; r += kBlockThreads) sX[r] = X[min(p0 - 1 + r, E - 1)];
-        for (int r = tid; r < R; r += kBlockThreads) {
-            const int u = p0 + r;
-            const double rd = gptr(fac_row(ed, ke, kFacRd))[u], cpv = gptr(fac_row(ed, ke, kFacCp))[u];
+        // 1a. knot positions of tile rows 0..RW-1, factors of rows p0..qlast, reciprocals of
+        //     intervals p0-1..qlast+1 (the last one: the next separator's right-hand side;
+        //     clamped where the group ends the system): one round of loads per lane row
+        const GPtr<double> frd = gptr(fac_row(ed, ke, kFacRd)), fcp = gptr(fac_row(ed, ke, kFacCp));
+        const GPtr<double> fih = gptr(fac_row(ed, ke, kFacIh));
+        const int spline = ed.spline;
+        for (int r = tid; r < RW; r += kBlockThreads) {
+            const int u = min(p0 + r, qlast);   // factor row (used when r < R)
+            const int xs = X[min(p0 - 1 + r, E - 1)];
+            const double rd = frd[u], cpv = fcp[u];
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
+            const double ihv = fih[min(p0 - 1 + r, E - 2)];
             double lo, di, up;
-            sys_row3(xm, x0, xp, u, E, ed.spline, lo, di, up);
+            sys_row3(xm, x0, xp, u, E, spline, lo, di, up);
+            sX[r] = xs;
+            s_ih[r] = ihv;
             s_rd[r] = rd;
             s_cp[r] = cpv;
             s_lo[r] = lo;
         }
-        // reciprocals of intervals p0-1 .. qlast+1 (the last one: the next separator's
-        // right-hand side; clamped where the group ends the system)
-        for (int r = tid; r <= R + 1; r += kBlockThreads) s_ih[r] = gptr(fac_row(ed, ke, kFacIh))[min(p0 - 1 + r, E - 2)];
         const int X0 = X[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
         const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
         __syncthreads();
@@ -1769,9 +1776,9 @@ BlockLds block_lds(const int* n, const int* bpw, int count) {
     }
     BlockLds L;
     L.rmax = rmax;
-    // rd, cp, lo: rmax doubles each; ih: rmax + 2 doubles; X: rmax + 3 ints; the tile
-    // [rows][np] starts 16-byte aligned (global-to-LDS loads of 16 bytes per lane)
-    L.tile_off = (sizeof(double) * (4 * (size_t)rmax + 2) + ((size_t)rmax + 3) * sizeof(int) + 15) / 16 * 16;
+    // rd, cp, lo, ih: rmax + 3 doubles each; X: rmax + 3 ints; the tile [rows][np] starts
+    // 16-byte aligned (global-to-LDS loads of 16 bytes per lane)
+    L.tile_off = (sizeof(double) * 4 * ((size_t)rmax + 3) + ((size_t)rmax + 3) * sizeof(int) + 15) / 16 * 16;
     L.bytes = L.tile_off + tile * sizeof(double);
     return L;
 }
