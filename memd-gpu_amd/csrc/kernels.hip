@@ -545,11 +545,25 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
         const GMut<double> ih = gmut(fac_row(ed, ke, kFacIh));
         const GMut<double> i3h = gmut(fac_row(ed, ke, kFacI3h));
         const int J = block_geom(E, 0).J;
-        // reciprocals of every interval 0 .. E-2, all lanes (coalesced)
-        for (int u = lane; u < E - 1; u += 64) {
-            const double h = (double)(X[u + 1] - X[u]);
-            ih[u] = 1.0 / h;
-            i3h[u] = 1.0 / (3.0 * h);
+        // reciprocals of every interval 0 .. E-2, all lanes (coalesced), 4 intervals per
+        // lane per round with their loads issued together
+        for (int u0 = lane; u0 < E - 1; u0 += 64 * 4) {
+            int xa[4], xb[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int u = min(u0 + 64 * g, E - 2);
+                xa[g] = X[u];
+                xb[g] = X[u + 1];
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int u = u0 + 64 * g;
+                if (u < E - 1) {
+                    const double h = (double)(xb[g] - xa[g]);
+                    ih[u] = 1.0 / h;
+                    i3h[u] = 1.0 / (3.0 * h);
+                }
+            }
         }
         // one lane per block: the block's rows live in registers, memory sees stores only
         for (int b = lane; b <= J; b += 64) {
@@ -1496,19 +1510,29 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
     }
     if (!(cl < cgn)) return;
-    double* cur = cur_slot(ed, st->imf);
+    const GMut<double> cur = gmut(cur_slot(ed, st->imf));
     const bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
-    double* nxt = cur_slot(ed, st->imf + 1);
+    const GMut<double> nxt = gmut(cur_slot(ed, st->imf + 1));
+    const GMut<double> run = gmut(ed.running);
+    // the lane's samples (and residuals) are loaded together, from in-range indices
+    // (global address space: flat accesses would make every store wait for all loads)
+    double cv[4], rv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) cv[s] = cur[(size_t)c * T + min(ts + s, tlast)];
+    if (last) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) rv[s] = run[(size_t)c * T + min(ts + s, tlast)];
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        int t = ts + s;
+        const int t = ts + s;
         if (t <= tlast) {
-            size_t idx = (size_t)c * T + t;
-            double v = cur[idx] - acc[s] / nv;
+            const size_t idx = (size_t)c * T + t;
+            const double v = cv[s] - acc[s] / nv;
             cur[idx] = v;
             if (last) {
-                double rr = ed.running[idx] - v;
-                ed.running[idx] = rr;
+                const double rr = rv[s] - v;
+                run[idx] = rr;
                 nxt[idx] = rr;
             }
         }
