@@ -209,8 +209,9 @@ __global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps,
 
 __device__ __forceinline__ double project_at(const double* __restrict__ Vk, const double* __restrict__ cur,
                                              int n, int T, int t) {
+    const GPtr<double> v = gptr(Vk), x = gptr(cur);
     double acc = 0.0;
-    for (int c = 0; c < n; ++c) acc = fma(Vk[c], cur[(size_t)c * T + t], acc);
+    for (int c = 0; c < n; ++c) acc = fma(v[c], x[(size_t)c * T + t], acc);
     return acc;
 }
 
@@ -252,8 +253,9 @@ __device__ void flags_wide(const EpochDesc& ed, const EpochState* st, int by) {
     int lane = tid & 63;
     int w = (t0 >> 6) + (tid >> 6);
     if (lane == 0 && w < ed.W) {
-        ed.mask[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-        ed.mask[((size_t)k * 2 + 1) * ed.W + w] = mmin;
+        const GMut<unsigned long long> mk = gmut(ed.mask);
+        mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
+        mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
     }
 }
 
@@ -401,8 +403,11 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
                 const unsigned long long mmax = __ballot(fmax);
                 const unsigned long long mmin = __ballot(fmin);
                 if (lane == 0 && w < ed.W) {
-                    ed.mask[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-                    ed.mask[((size_t)k * 2 + 1) * ed.W + w] = mmin;
+                    // global-address-space stores: a flat store would also hold up the
+                    // LDS waits of the next pass
+                    const GMut<unsigned long long> mk = gmut(ed.mask);
+                    mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
+                    mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
                 }
             }
         }
@@ -1299,11 +1304,12 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 int E = 0, pr = 0;
                 unsigned long long m[4] = {0, 0, 0, 0};
                 if (kk < K) {
-                    E = ed.E[ke];
-                    pr = ed.pre[(size_t)ke * W + w0];
+                    E = gptr(ed.E)[ke];
+                    pr = gptr(ed.pre)[(size_t)ke * W + w0];
+                    const GPtr<unsigned long long> mk = gptr(ed.mask);
 #pragma unroll
                     for (int w = 0; w < 4; ++w)
-                        if (w < TW && w0 + w < W) m[w] = ed.mask[(size_t)ke * W + w0 + w];
+                        if (w < TW && w0 + w < W) m[w] = mk[(size_t)ke * W + w0 + w];
                 }
                 cE[tid] = E;
                 cPr[tid] = pr;
