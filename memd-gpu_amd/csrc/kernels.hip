@@ -1492,20 +1492,13 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             if (!((vmask >> (2 * d)) & 1u)) continue;
             nv = nv + 1.0;
             if (lane_on) {
-                {
-                    const int4 A = sSegA[2 * d];
-                    const int4 B = sSegB[2 * d];
-                    const int jj = max(sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1, 0);
-                    const double4* rr = rec + B.x + __umul24(cl, B.y);
-                    eval_quad<0>(rr, xk + B.w, jj - A.w, ts, tlast, ep, acc);
-                }
-                {
-                    const int4 A = sSegA[2 * d + 1];
-                    const int4 B = sSegB[2 * d + 1];
-                    const int jj = max(sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1, 0);
-                    const double4* rr = rec + B.x + __umul24(cl, B.y);
-                    eval_quad<1>(rr, xk + B.w, jj - A.w, ts, tlast, ep, acc);
-                }
+                // both envelopes' first interval and record offsets, read together
+                const int i0 = max(sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1, 0) - sSegA[2 * d].w;
+                const int i1 =
+                    max(sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1, 0) - sSegA[2 * d + 1].w;
+                const int4 B0 = sSegB[2 * d], B1 = sSegB[2 * d + 1];
+                eval_quad<0>(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, ts, tlast, ep, acc);
+                eval_quad<1>(rec + B1.x + __umul24(cl, B1.y), xk + B1.w, i1, ts, tlast, ep, acc);
             }
         }
         __syncthreads();
