@@ -323,6 +323,7 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
         const int tn = min(kFlagTile, T - t0);
         const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
         const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np);
+#pragma unroll 4
         for (int idx = tid; idx < n * tn; idx += kFlagTile) {
             const int tl = (int)(((unsigned long long)idx * nmagic) >> 32);
             const int c = idx - tl * n;
@@ -342,6 +343,7 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
             const int col = side ? TW - 1 : 0;
             const unsigned vb = (unsigned)(kh + kk) * (unsigned)n;
             double acc = 0.0;
+#pragma unroll 16
             for (int c = 0; c < n; ++c) acc = fma((double)V[vb + c], curT[c * TW + col], acc);
             Ph[kk][side] = acc;
         }
