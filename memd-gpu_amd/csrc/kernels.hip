@@ -1203,11 +1203,14 @@ __device__ __forceinline__ void row_scan16(int& x) {
 // first): each sample uses the record of X_j < t <= X_{j+1}.  ENV 0 stores the
 // upper envelope, ENV 1 adds the mean of both to the accumulator.  The first
 // sample is in range and lies in interval i (its caller counts the knots below
-// it).
+// it).  The knots are exactly the set flag bits, so sample t enters the next
+// interval, whose left knot is t - 1, when bit t - 1 is set: bit s - 1 of
+// `cross` is the flag of sample ts + s - 1 (the caller clears the flag of
+// sample 0, the left knot of the first interval).
 template <int ENV>
 __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, const int* __restrict__ xkrow, int i,
-                                          int ts, int tlast, double (&ep)[4], double (&acc)[4]) {
-    int xj = xkrow[i], xn = xkrow[i + 1];
+                                          unsigned cross, int ts, int tlast, double (&ep)[4], double (&acc)[4]) {
+    int xj = xkrow[i];
     double4 r = recrow[i];
     {
         const double v = horner4(r, (double)(ts - xj));
@@ -1218,10 +1221,9 @@ __device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, co
     for (int s = 1; s < 4; ++s) {
         const int t = ts + s;
         if (t <= tlast) {
-            if (t > xn) {
+            if ((cross >> (s - 1)) & 1u) {
                 ++i;
-                xj = xn;
-                xn = xkrow[i + 1];
+                xj = t - 1;
                 r = recrow[i];
             }
             const double v = horner4(r, (double)(t - xj));
@@ -1254,8 +1256,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
     __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
     __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
-    __shared__ int sCum[NS][4];          // knots before each word of the tile
-    __shared__ unsigned long long sMsk[NS][4];
+    __shared__ int sCum[NS][4];          // knots before the end of each word of the tile
+    __shared__ unsigned long long sMsk[NS][4];   // flag words; sample 0's flag cleared
     __shared__ int sNb;
     __shared__ unsigned sValid;   // bit s: segment s of the batch is valid (both envelopes of its direction)
     __shared__ int cE[2 * kEvalChunk], cPr[2 * kEvalChunk];   // staged formation inputs
@@ -1291,7 +1293,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int c = cg * CG + cl;
     const int ts = t0 + 4 * q;
     const int wi = q >> 4;               // tile word of the quad
-    const unsigned long long below = (1ull << (4 * (q & 15))) - 1ull;
+    const int qsh = 4 * (q & 15);        // bit of sample ts in its flag word
     const bool lane_on = cl < cgn && ts <= tlast;
     const int nq = (cgn + 3) >> 2;              // channel quads of the group
     const int qmag = (65536 + nq - 1) / nq;     // rem / nq as (rem * qmag) >> 16: exact for rem < 2^15
@@ -1333,7 +1335,6 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             int run = pr;
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
-                cum[w] = run;
                 msk[w] = 0;
                 if (have && w < TW && w0 + w < W) {
                     const unsigned long long m = cM[ci][w];
@@ -1342,7 +1343,11 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                     inside += lim >= 64 ? (int)__popcll(m) : (lim > 0 ? (int)__popcll(m & ((1ull << lim) - 1ull)) : 0);
                     run += (int)__popcll(m);
                 }
+                cum[w] = run;
             }
+            // sample 0 is the left knot of interval 0, not a crossing into it: without
+            // its flag, the knots below t = 0 count 1 like those below t = 1
+            if (w0 == 0) msk[0] &= ~1ull;
             const int lo = min(max(pr - 1, 0), E - 2);
             const int hi = min(max(pr + inside - 1, 0), E - 2);
             const int ni = hi - lo + 1;
@@ -1487,20 +1492,23 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         }
         __syncthreads();
         // phase B.  Samples 0 and T-1 are knots of every direction, so the knots
-        // below a sample t in 1..T-1 number 1..E-1: the interval index needs no
-        // upper clamp (t = 0 gives -1, clamped to interval 0).
+        // below a sample t in 1..T-1 number 1..E-1 (and sample 0's cleared flag counts
+        // 1 for t = 0): the interval index knots below - 1 lies in 0..E-2.  The flag
+        // word shifted to the quad's first sample gives the knots at or after it in
+        // the word and, in its low 3 bits, the quad's interval crossings.
         const unsigned vmask = (unsigned)uniform((int)sValid);
         for (int d = 0; d < nb; ++d) {
             if (!((vmask >> (2 * d)) & 1u)) continue;
             nv = nv + 1.0;
             if (lane_on) {
-                // both envelopes' first interval and record offsets, read together
-                const int i0 = max(sCum[2 * d][wi] + (int)__popcll(sMsk[2 * d][wi] & below) - 1, 0) - sSegA[2 * d].w;
-                const int i1 =
-                    max(sCum[2 * d + 1][wi] + (int)__popcll(sMsk[2 * d + 1][wi] & below) - 1, 0) - sSegA[2 * d + 1].w;
+                // both envelopes' first interval, crossing flags and record offsets, read together
+                const unsigned long long h0 = sMsk[2 * d][wi] >> qsh, h1 = sMsk[2 * d + 1][wi] >> qsh;
+                const int i0 = sCum[2 * d][wi] - (int)__popcll(h0) - 1 - sSegA[2 * d].w;
+                const int i1 = sCum[2 * d + 1][wi] - (int)__popcll(h1) - 1 - sSegA[2 * d + 1].w;
+                const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
                 const int4 B0 = sSegB[2 * d], B1 = sSegB[2 * d + 1];
-                eval_quad<0>(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, ts, tlast, ep, acc);
-                eval_quad<1>(rec + B1.x + __umul24(cl, B1.y), xk + B1.w, i1, ts, tlast, ep, acc);
+                eval_quad<0>(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, x0, ts, tlast, ep, acc);
+                eval_quad<1>(rec + B1.x + __umul24(cl, B1.y), xk + B1.w, i1, x1, ts, tlast, ep, acc);
             }
         }
         __syncthreads();
