@@ -1199,36 +1199,36 @@ __device__ __forceinline__ void row_scan16(int& x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);   // row_shr:8
 }
 
-// Phase B of one envelope for a lane's 4 consecutive samples (interval i of the
-// first): each sample uses the record of X_j < t <= X_{j+1}.  ENV 0 stores the
-// upper envelope, ENV 1 adds the mean of both to the accumulator.  The first
-// sample is in range and lies in interval i (its caller counts the knots below
-// it).  The knots are exactly the set flag bits, so sample t enters the next
-// interval, whose left knot is t - 1, when bit t - 1 is set: bit s - 1 of
-// `cross` is the flag of sample ts + s - 1 (the caller clears the flag of
-// sample 0, the left knot of the first interval).
-template <int ENV>
-__device__ __forceinline__ void eval_quad(const double4* __restrict__ recrow, const int* __restrict__ xkrow, int i,
-                                          unsigned cross, int ts, int tlast, double (&ep)[4], double (&acc)[4]) {
-    int xj = xkrow[i];
-    double4 r = recrow[i];
-    {
-        const double v = horner4(r, (double)(ts - xj));
-        if (ENV == 0) ep[0] = v;
-        else acc[0] = acc[0] + (ep[0] + v) * 0.5;
-    }
+// Phase B of one direction for a lane's 4 consecutive samples: both envelopes
+// (upper u, lower l; interval iu / il of the first sample) at each sample, the
+// mean of the two added to the accumulator.  Each sample uses the record of
+// X_j < t <= X_{j+1}.  The first sample is in range and lies in intervals iu and
+// il (its caller counts the knots below it).  The knots are exactly the set flag
+// bits, so sample t enters the next interval, whose left knot is t - 1, when bit
+// t - 1 is set: bit s - 1 of `cu` / `cl` is the flag of sample ts + s - 1 (the
+// caller clears the flag of sample 0, the left knot of the first interval).
+// Both envelopes' crossing loads of a sample are issued before either is used.
+__device__ __forceinline__ void eval_pair(const double4* __restrict__ ru, const int* __restrict__ xu, int iu,
+                                          unsigned cu, const double4* __restrict__ rl, const int* __restrict__ xl,
+                                          int il, unsigned cl, int ts, int tlast, double (&acc)[4]) {
+    int ju = xu[iu], jl = xl[il];
+    double4 qu = ru[iu], ql = rl[il];
+    acc[0] = acc[0] + (horner4(qu, (double)(ts - ju)) + horner4(ql, (double)(ts - jl))) * 0.5;
 #pragma unroll
     for (int s = 1; s < 4; ++s) {
         const int t = ts + s;
         if (t <= tlast) {
-            if ((cross >> (s - 1)) & 1u) {
-                ++i;
-                xj = t - 1;
-                r = recrow[i];
+            if ((cu >> (s - 1)) & 1u) {
+                ++iu;
+                ju = t - 1;
+                qu = ru[iu];
             }
-            const double v = horner4(r, (double)(t - xj));
-            if (ENV == 0) ep[s] = v;
-            else acc[s] = acc[s] + (ep[s] + v) * 0.5;
+            if ((cl >> (s - 1)) & 1u) {
+                ++il;
+                jl = t - 1;
+                ql = rl[il];
+            }
+            acc[s] = acc[s] + (horner4(qu, (double)(t - ju)) + horner4(ql, (double)(t - jl))) * 0.5;
         }
     }
 }
@@ -1298,7 +1298,6 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int nq = (cgn + 3) >> 2;              // channel quads of the group
     const int qmag = (65536 + nq - 1) / nq;     // rem / nq as (rem * qmag) >> 16: exact for rem < 2^15
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    double ep[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
     int kc1 = 0;   // end of the staged chunk of formation inputs
     for (int kb = 0; kb < K;) {
@@ -1507,8 +1506,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int i1 = sCum[2 * d + 1][wi] - (int)__popcll(h1) - 1 - sSegA[2 * d + 1].w;
                 const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
                 const int4 B0 = sSegB[2 * d], B1 = sSegB[2 * d + 1];
-                eval_quad<0>(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, x0, ts, tlast, ep, acc);
-                eval_quad<1>(rec + B1.x + __umul24(cl, B1.y), xk + B1.w, i1, x1, ts, tlast, ep, acc);
+                eval_pair(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, x0, rec + B1.x + __umul24(cl, B1.y),
+                          xk + B1.w, i1, x1, ts, tlast, acc);
             }
         }
         __syncthreads();
