@@ -1256,7 +1256,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
     __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
     __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
-    __shared__ int sCum[NS][4];          // knots before the end of each word of the tile
+    __shared__ int sCum[NS][4];          // knots before the end of each word of the tile, less jlo + 1
     __shared__ unsigned long long sMsk[NS][4];   // flag words; sample 0's flag cleared
     __shared__ int sNb;
     __shared__ unsigned sValid;   // bit s: segment s of the batch is valid (both envelopes of its direction)
@@ -1371,7 +1371,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 sSegB[tid] = make_int4(pneed - need, stride, E, pxk - nxk);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
-                    sCum[tid][w] = cum[w];
+                    sCum[tid][w] = cum[w] - lo - 1;
                     sMsk[tid][w] = msk[w];
                 }
             }
@@ -1500,12 +1500,16 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             if (!((vmask >> (2 * d)) & 1u)) continue;
             nv = nv + 1.0;
             if (lane_on) {
-                // both envelopes' first interval, crossing flags and record offsets, read together
-                const unsigned long long h0 = sMsk[2 * d][wi] >> qsh, h1 = sMsk[2 * d + 1][wi] >> qsh;
-                const int i0 = sCum[2 * d][wi] - (int)__popcll(h0) - 1 - sSegA[2 * d].w;
-                const int i1 = sCum[2 * d + 1][wi] - (int)__popcll(h1) - 1 - sSegA[2 * d + 1].w;
-                const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
+                // both envelopes' flag words, knot counts and record offsets in one
+                // round of LDS reads; then the first interval (in the tile's window)
+                // and the crossing flags
+                const unsigned long long m0 = sMsk[2 * d][wi], m1 = sMsk[2 * d + 1][wi];
+                const int c0 = sCum[2 * d][wi], c1 = sCum[2 * d + 1][wi];
                 const int4 B0 = sSegB[2 * d], B1 = sSegB[2 * d + 1];
+                __builtin_amdgcn_sched_barrier(0);
+                const unsigned long long h0 = m0 >> qsh, h1 = m1 >> qsh;
+                const int i0 = c0 - (int)__popcll(h0), i1 = c1 - (int)__popcll(h1);
+                const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
                 eval_pair(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, x0, rec + B1.x + __umul24(cl, B1.y),
                           xk + B1.w, i1, x1, ts, tlast, acc);
             }
