@@ -43,21 +43,23 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="c3: skip the plain per-epoch pass (profiling runs)")
+    ap.add_argument("--no-plain", action="store_true", help="uneven512: skip the plain per-epoch pass (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in tile shapes (default: memd_autotune before the untimed warmup)")
     ap.add_argument("--tuning", type=str, default=None,
                     help="BLOCK_GROUPS,EVAL_TILE_WORDS: install these shapes for --n instead of autotuning "
                          "(profiling runs repeat a bench run's choice without the tuner's own kernels)")
-    ap.add_argument("--workload", choices=("m", "c1", "c2", "c3", "c4"), default="m",
-                    help="m: 32ch x 4096 K=64 batched (headline); c1: 32ch x 30504 K=64 single epochs "
-                         "vs the CPU oracle; c3: 512 uneven epochs batched vs plain; c4: served "
+    ap.add_argument("--workload", choices=("batched32x4096", "eeg32x30504", "wide128x16384", "uneven512", "served"),
+                    default="batched32x4096",
+                    help="batched32x4096: 64 epochs of 32ch x 4096 K=64 per batched call (headline); eeg32x30504: "
+                         "32ch x 30504 K=64 single epochs vs the CPU oracle; wide128x16384: 128ch x 16384 K=256 "
+                         "single epochs vs the CPU oracle; uneven512: 512 uneven epochs batched vs plain; served: "
                          "continuous batching under Poisson arrivals")
     ap.add_argument("--rates", type=float, nargs="+", default=[10.0, 25.0, 40.0],
-                    help="c4: arrival rates (epochs/s) to sweep")
-    ap.add_argument("--duration", type=float, default=20.0, help="c4: seconds of arrivals per rate")
-    ap.add_argument("--pool-gb", type=int, default=64, help="c4: server HBM pool (GiB)")
-    ap.add_argument("--max-active", type=int, default=128, help="c4: cap on epochs advanced together")
+                    help="served: arrival rates (epochs/s) to sweep")
+    ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of arrivals per rate")
+    ap.add_argument("--pool-gb", type=int, default=64, help="served: server HBM pool (GiB)")
+    ap.add_argument("--max-active", type=int, default=128, help="served: cap on epochs advanced together")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     return ap.parse_args()
@@ -99,7 +101,7 @@ def cpu_baseline(n, T, K, n_imf, S, budget_s):
                       f"OpenMP over channels with {threads} threads"}
 
 
-def bench_c1(a):
+def bench_eeg(a):
     """32 ch x 30504 samples (EEG-like generator), K = 64, 9 slots x 10 sifts:
     plain per-epoch GPU path vs the FP64 CPU oracle on this host."""
     import torch
@@ -142,7 +144,7 @@ def bench_c1(a):
         "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
 
 
-def bench_c2(a):
+def bench_wide(a):
     """128 ch x 16384 samples (EEG-like generator + 40 Hz on a quarter of the
     channels), K = 256, 9 slots x 10 sifts: plain per-epoch GPU path vs the FP64
     CPU oracle on this host."""
@@ -193,7 +195,7 @@ def bench_c2(a):
         "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
 
 
-def bench_c3(a):
+def bench_uneven(a):
     """512 independent epochs, n in {8,16,32,64}, T in [1000,8000], K = 2n (EEG-like
     generator, per-epoch seeds), 9 slots x 10 sifts: one batched call vs one call
     per epoch (plain path timed on the first 32 epochs)."""
@@ -253,7 +255,7 @@ def bench_c3(a):
                        "plain_epoch_p99": round(pct(plat, 99) * 1e3, 3) if plat else None}}), flush=True)
 
 
-def bench_c4(a):
+def bench_served(a):
     """Served continuous batching: Poisson arrivals at each rate in --rates for
     --duration seconds; 90% of requests are uneven epochs (n in {8,16,32,64},
     T in [1000,8000], K = 2n), 10% are 32 ch x 60000, K = 64; 9 slots x 10
@@ -310,7 +312,7 @@ def bench_c4(a):
                 srv.release(rid)
             if time.perf_counter() - last_print > 10:
                 last_print = time.perf_counter()
-                print(f"c4: rate {rate}/s t={done_t:.1f}s done={len(lat)} active={active} queued={queued}",
+                print(f"served: rate {rate}/s t={done_t:.1f}s done={len(lat)} active={active} queued={queued}",
                       file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         lat_s = [x for x, k in zip(lat, kinds) if k == "short"]
@@ -354,14 +356,14 @@ def main():
         print(json.dumps({"copy_bytes": a.copy_bytes}), flush=True)
         return
 
-    if a.workload == "c1":
-        return bench_c1(a)
-    if a.workload == "c4":
-        return bench_c4(a)
-    if a.workload == "c3":
-        return bench_c3(a)
-    if a.workload == "c2":
-        return bench_c2(a)
+    if a.workload == "eeg32x30504":
+        return bench_eeg(a)
+    if a.workload == "served":
+        return bench_served(a)
+    if a.workload == "uneven512":
+        return bench_uneven(a)
+    if a.workload == "wide128x16384":
+        return bench_wide(a)
 
     n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
     dev = "cuda:0"

@@ -3,7 +3,7 @@
 Bar: bit-exact.  The engine and the oracle share one arithmetic contract
 (FP64, contraction off, fma where both write it), so knot counts per direction
 per sift, sift counts, IMF count and every IMF sample must be identical.  The
-numeric tolerance the north star allows (1e-9 relative L2) is asserted too, as
+floating-point tolerance for IMF samples (1e-9 relative L2) is asserted too, as
 a weaker second line.
 """
 import os
