@@ -356,7 +356,8 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     std::vector<int> ns(count), bpws(count);
     for (int e = 0; e < count; ++e) {
         nmax = eds[e].n > nmax ? eds[e].n : nmax;
-        if (eds[e].n <= kFlagLdsMaxCh && eds[e].n > nlds) nlds = eds[e].n;
+        const int tile_ch = eds[e].n <= kFlagLdsMaxCh ? eds[e].n : kFlagChunk;   // chunked above the limit
+        if (tile_ch > nlds) nlds = tile_ch;
         ns[e] = eds[e].n;
         bpws[e] = eds[e].bpw;
     }

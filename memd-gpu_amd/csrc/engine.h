@@ -36,8 +36,12 @@ constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
 constexpr int kEvalBatch = 8;      // directions per batch
 constexpr int kEvalChunk = 32;     // directions whose formation inputs are staged together
 constexpr int kEvalDefault = kEvalFused;
-// Projection/extrema: the LDS-staged kernel holds n x 258 doubles.
-constexpr int kFlagLdsMaxCh = 64;
+// Projection/extrema: the LDS-staged kernel holds n x 258 doubles up to
+// kFlagLdsMaxCh channels; wider epochs pass kFlagChunk channels at a time through
+// the same tile, kFlagGroup directions per workgroup.
+constexpr int kFlagLdsMaxCh = 16;
+constexpr int kFlagChunk = 16;
+constexpr int kFlagGroup = 16;
 constexpr int kFlagDirs = 8;       // directions per k_flags pass
 constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
@@ -181,7 +185,7 @@ struct StepPlan {
     int grid[kNumLists];    // persistent grid size per list
     int mode;               // kModeFix or kModeStop (all epochs of one call)
     int nmax;               // largest channel count in the batch
-    int nlds;               // largest channel count <= kFlagLdsMaxCh (k_flags' LDS tile), 0 if none
+    int nlds;               // channels of k_flags' LDS tile: the largest n <= kFlagLdsMaxCh, at least kFlagChunk if an epoch is wider, 0 if none
     int eval_variant;       // index into kEvalVariants
     int coef_records;       // launch k_coef (the evaluation reads the record array)
     int block_rmax;         // k_block: rows of the longest work item

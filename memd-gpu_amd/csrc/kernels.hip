@@ -207,59 +207,6 @@ __global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps,
 
 // ------------------------------------------------------ projection + flags
 
-__device__ __forceinline__ double project_at(const double* __restrict__ Vk, const double* __restrict__ cur,
-                                             int n, int T, int t) {
-    const GPtr<double> v = gptr(Vk), x = gptr(cur);
-    double acc = 0.0;
-    for (int c = 0; c < n; ++c) acc = fma(v[c], x[(size_t)c * T + t], acc);
-    return acc;
-}
-
-// Wide-channel path of k_flags (n > kFlagLdsMaxCh): one workgroup per (direction,
-// tile), signal read through the caches.
-__device__ void flags_wide(const EpochDesc& ed, const EpochState* st, int by) {
-    const int n = ed.n, T = ed.T;
-    int tiles = (T + kFlagTile - 1) / kFlagTile;
-    int k = by / tiles;
-    int tile = by - k * tiles;
-    const double* cur = cur_slot(ed, st->imf);
-    const double* Vk = ed.V + (size_t)k * n;
-    __shared__ double Ps[kFlagTile + 2];
-    int tid = threadIdx.x;
-    int t0 = tile * kFlagTile;
-    int t = t0 + tid;
-    if (t < T) Ps[tid + 1] = project_at(Vk, cur, n, T, t);
-    if (tid == 0 && t0 > 0) Ps[0] = project_at(Vk, cur, n, T, t0 - 1);
-    if (tid == kFlagTile - 1 && t0 + kFlagTile < T) Ps[kFlagTile + 1] = project_at(Vk, cur, n, T, t0 + kFlagTile);
-    if (k == 0) {   // sample-major copy of the tile for the knot-value gathers
-        const int tn = min(kFlagTile, T - t0);
-        for (int c = 0; c < n; ++c)
-            if (tid < tn) ed.curt[(size_t)(t0 + tid) * ed.np + c] = cur[(size_t)c * T + t0 + tid];
-    }
-    __syncthreads();
-    int lim = window_limit(T, ed.window);
-    bool fmax = false, fmin = false;
-    if (t < T) {
-        if (t == 0 || t == T - 1) {
-            fmax = fmin = true;
-        } else if (t <= lim) {
-            double p = Ps[tid + 1], a = Ps[tid], b = Ps[tid + 2];
-            fmax = (p > a) && (p > b);
-            fmin = (p < a) && (p < b);
-        }
-    }
-    unsigned long long mmax = __ballot(fmax);
-    unsigned long long mmin = __ballot(fmin);
-    int lane = tid & 63;
-    int w = (t0 >> 6) + (tid >> 6);
-    if (lane == 0 && w < ed.W) {
-        const GMut<unsigned long long> mk = gmut(ed.mask);
-        mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-        mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
-    }
-}
-
-
 // Neighbouring lanes' values by DPP wavefront shifts (no LDS round trip); lane 0
 // (from_prev) and lane 63 (from_next) get no value and are filled by the caller.
 __device__ __forceinline__ double wave_from_prev(double v) {   // lane i <- lane i-1
@@ -276,6 +223,133 @@ __device__ __forceinline__ double wave_from_next(double v) {   // lane i <- lane
 // One workgroup per (epoch, 256-sample tile) and all directions: the tile of
 // the current signal (plus one sample each side) is staged in LDS once, then
 // every direction's projection, extremum tests and ballots run from LDS.
+// Channel-chunked path of k_flags (n > kFlagLdsMaxCh): one workgroup per (group
+// of kFlagGroup directions, tile).  The tile's channels pass through the LDS tile
+// kFlagChunk at a time, and every direction's chain (and each halo chain) stays in
+// registers across the chunks in ascending channel order: the same fma chain as
+// with the whole tile staged.  The signal is read once per direction group.
+__device__ void flags_chunked(const EpochDesc& ed, const EpochState* st, int by, double* curT) {
+    constexpr int TW = kFlagTile + 2;
+    constexpr int DG = kFlagGroup;
+    constexpr int CH = kFlagChunk;
+    constexpr int NW = kFlagTile / 64;
+    __shared__ double Qe[DG][NW][2];   // each wave's first and last projection
+    __shared__ double Qh[DG][2];       // projections at t0-1 and t0+kFlagTile
+    const int n = ed.n, T = ed.T, K = ed.K;
+    const int tiles = (T + kFlagTile - 1) / kFlagTile;
+    const int grp = by / tiles;
+    const int tile = by - grp * tiles;
+    const int kg0 = grp * DG;
+    const int kd = min(DG, K - kg0);
+    const int t0 = tile * kFlagTile;
+    const int tn = min(kFlagTile, T - t0);
+    const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
+    const CPtr<double> V = cptr(ed.V);
+    const int tid = threadIdx.x;
+    double acc[DG];
+    unsigned vrow[DG];   // chains past the last direction repeat it and are ignored
+#pragma unroll
+    for (int d = 0; d < DG; ++d) {
+        acc[d] = 0.0;
+        vrow[d] = (unsigned)min(kg0 + d, K - 1) * (unsigned)n;
+    }
+    // halo chains: thread 2d + side, for direction d and column 0 or TW - 1
+    const unsigned hrow = (unsigned)min(kg0 + (tid >> 1), K - 1) * (unsigned)n;
+    const int hcol = (tid & 1) ? TW - 1 : 0;
+    double hacc = 0.0;
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        const int cn = min(CH, n - c0);
+        __syncthreads();   // the previous chunk has been read
+        {   // stage the chunk: G loads per lane in flight before their LDS stores
+            constexpr int G = 8;
+            const int tot = cn * TW;
+            for (int i0 = tid; i0 < tot; i0 += kFlagTile * G) {
+                double v[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {   // unconditional loads from clamped addresses
+                    const int idx = min(i0 + g * kFlagTile, tot - 1);
+                    const int c = idx / TW, i = idx - c * TW;
+                    v[g] = cur[(unsigned)(c0 + c) * (unsigned)T + (unsigned)min(max(t0 - 1 + i, 0), T - 1)];
+                }
+#pragma unroll
+                for (int g = 0; g < G; ++g) {   // samples outside 0 .. T-1 are 0
+                    const int idx = i0 + g * kFlagTile;
+                    const int c = idx / TW, t = t0 - 1 + (idx - c * TW);
+                    if (idx < tot) curT[idx] = (t >= 0 && t < T) ? v[g] : 0.0;
+                }
+            }
+        }
+        __syncthreads();
+        if (grp == 0) {   // sample-major copy of the chunk for the knot-value gathers
+            const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np + c0);
+            for (int idx = tid; idx < CH * tn; idx += kFlagTile) {
+                const int tl = idx / CH, c = idx - tl * CH;
+                if (c < cn) ct[(unsigned)tl * (unsigned)ed.np + (unsigned)c] = curT[c * TW + tl + 1];
+            }
+        }
+        if (tid < 2 * kd) {
+            for (int c = 0; c < cn; ++c) hacc = fma((double)V[hrow + c0 + c], curT[c * TW + hcol], hacc);
+        }
+        // every curT read feeds DG chains
+        int c = 0;
+        for (; c + 4 <= cn; c += 4) {
+            const double x0 = curT[(c + 0) * TW + tid + 1], x1 = curT[(c + 1) * TW + tid + 1];
+            const double x2 = curT[(c + 2) * TW + tid + 1], x3 = curT[(c + 3) * TW + tid + 1];
+#pragma unroll
+            for (int d = 0; d < DG; ++d) {
+                const CPtr<double> v = V + vrow[d] + c0 + c;
+                acc[d] = fma((double)v[0], x0, acc[d]);
+                acc[d] = fma((double)v[1], x1, acc[d]);
+                acc[d] = fma((double)v[2], x2, acc[d]);
+                acc[d] = fma((double)v[3], x3, acc[d]);
+            }
+        }
+        for (; c < cn; ++c) {
+            const double x = curT[c * TW + tid + 1];
+#pragma unroll
+            for (int d = 0; d < DG; ++d) acc[d] = fma((double)V[vrow[d] + c0 + c], x, acc[d]);
+        }
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    if (tid < 2 * kd) Qh[tid >> 1][tid & 1] = hacc;
+    if (lane == 0 || lane == 63) {
+#pragma unroll
+        for (int d = 0; d < DG; ++d) Qe[d][wv][lane == 63 ? 1 : 0] = acc[d];
+    }
+    __syncthreads();
+    const int lim = window_limit(T, ed.window);
+    const int t = t0 + tid;
+    const int w = (t0 >> 6) + wv;
+#pragma unroll
+    for (int d = 0; d < DG; ++d) {
+        if (d < kd) {   // uniform
+            const int k = kg0 + d;
+            // neighbours: adjacent lanes; across waves and tiles through LDS
+            const double p = acc[d];
+            double a = wave_from_prev(p);
+            double b = wave_from_next(p);
+            if (lane == 0) a = wv > 0 ? Qe[d][wv - 1][1] : Qh[d][0];
+            if (lane == 63) b = wv < NW - 1 ? Qe[d][wv + 1][0] : Qh[d][1];
+            bool fmax = false, fmin = false;
+            if (t < T) {
+                if (t == 0 || t == T - 1) {
+                    fmax = fmin = true;
+                } else if (t <= lim) {
+                    fmax = (p > a) && (p > b);
+                    fmin = (p < a) && (p < b);
+                }
+            }
+            const unsigned long long mmax = __ballot(fmax);
+            const unsigned long long mmin = __ballot(fmin);
+            if (lane == 0 && w < ed.W) {
+                const GMut<unsigned long long> mk = gmut(ed.mask);
+                mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
+                mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict__ eps,
                                                      const int2* __restrict__ tab) {
     constexpr int TW = kFlagTile + 2;
@@ -288,8 +362,8 @@ __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
-    if (ed.n > kFlagLdsMaxCh) {   // its blocks are (direction, tile) pairs (blocks_for)
-        flags_wide(ed, st, br.y);
+    if (ed.n > kFlagLdsMaxCh) {   // its blocks are (direction group, tile) pairs (blocks_for)
+        flags_chunked(ed, st, br.y, curT);
         return;
     }
     const int n = ed.n, T = ed.T, K = ed.K;
@@ -1780,7 +1854,7 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
     case kMapInit: return (int)(((size_t)e.n * e.T + 255) / 256);
     case kMapFlags:
         if (e.n <= kFlagLdsMaxCh) return (e.T + kFlagTile - 1) / kFlagTile;
-        return e.K * ((e.T + kFlagTile - 1) / kFlagTile);
+        return ((e.K + kFlagGroup - 1) / kFlagGroup) * ((e.T + kFlagTile - 1) / kFlagTile);
     case kMapCompact: return e.K;
     case kMapBlock: return e.bblocks;
     case kMapEval:
@@ -1865,7 +1939,7 @@ static void launch_eval(const StepPlan& p, hipStream_t s) {
 
 static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     switch (which) {
-    case 0:   // per epoch: LDS-staged tiles (n <= kFlagLdsMaxCh) or (direction, tile) blocks
+    case 0:   // per epoch: LDS-staged tiles (n <= kFlagLdsMaxCh) or (direction group, tile) blocks
         MEMD_TABLE_LAUNCH(k_flags, kMapFlags, kFlagTile, (size_t)p.nlds * (kFlagTile + 2) * sizeof(double));
         break;
     case 1:
@@ -1917,7 +1991,7 @@ hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
 }
 
 hipError_t configure_kernels() {
-    // the LDS-staged projection kernel holds up to 64 x 258 doubles (132 KB)
+    // the projection kernel's LDS tile holds up to kFlagLdsMaxCh x 258 doubles (33 KB)
     hipError_t e0 = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
     if (e0 != hipSuccess) return e0;

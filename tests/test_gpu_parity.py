@@ -84,10 +84,10 @@ def test_headline_shape_full_schedule(ctx, oracle):
 @pytest.mark.parametrize("n,T,K,n_imf,S,seed", [
     (128, 2048, 256, 3, 2, 60),   # the 128 x 16384, K=256 configuration's channels and directions
     (200, 700, 16, 3, 2, 61),     # above 128 channels, ragged evaluation groups
-    (65, 1500, 130, 3, 2, 62),    # just past the LDS-staged projection kernel's 64 channels
+    (65, 1500, 130, 3, 2, 62),    # 4 full 16-channel chunks and 1 channel; 130 directions: a partial last group
 ])
 def test_high_density_matches_oracle(ctx, oracle, n, T, K, n_imf, S, seed):
-    # more than 64 channels: the projection/extrema pass runs per direction from HBM
+    # wide epochs: the projection/extrema pass takes the channels 16 at a time
     import synth
     _check_epoch(ctx, oracle, synth.eeg_epoch(n, T, seed, gamma40=True), K, n_imf, S)
 
@@ -241,8 +241,8 @@ def test_batch_many_epochs_xcd_groups(ctx, oracle):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_batch_mixing_wide_and_narrow_epochs(ctx, oracle, mode):
-    # epochs above and below the 64-channel LDS projection limit in one batch: each
-    # takes its own projection path (the batch's widest epoch must not decide it)
+    # epochs above and below the 16-channel limit of the whole-tile projection path in
+    # one batch: each takes its own path (the batch's widest epoch must not decide it)
     import synth
     eps = []
     for i, (n, T, K) in enumerate([(96, 700, 12), (16, 1300, 32), (64, 900, 16), (8, 1100, 16), (72, 600, 8)]):
