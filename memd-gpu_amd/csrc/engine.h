@@ -41,7 +41,7 @@ constexpr int kEvalDefault = kEvalFused;
 // the same tile, kFlagGroup directions per workgroup.
 constexpr int kFlagLdsMaxCh = 16;
 constexpr int kFlagChunk = 16;
-constexpr int kFlagGroup = 16;
+constexpr int kFlagGroup = 32;
 constexpr int kFlagDirs = 8;       // directions per k_flags pass
 constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
