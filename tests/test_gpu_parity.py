@@ -85,6 +85,7 @@ def test_headline_shape_full_schedule(ctx, oracle):
     (128, 2048, 256, 3, 2, 60),   # the 128 x 16384, K=256 configuration's channels and directions
     (200, 700, 16, 3, 2, 61),     # above 128 channels, ragged evaluation groups
     (65, 1500, 130, 3, 2, 62),    # 4 full 16-channel chunks and 1 channel; 130 directions: a partial last group
+    (23, 900, 40, 3, 2, 63),      # a 7-channel chunk (4 + 3 channels); 40 directions: groups of 32 and 8
 ])
 def test_high_density_matches_oracle(ctx, oracle, n, T, K, n_imf, S, seed):
     # wide epochs: the projection/extrema pass takes the channels 16 at a time
