@@ -1991,13 +1991,15 @@ hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
 }
 
 hipError_t configure_kernels() {
-    // the projection kernel's LDS tile holds up to kFlagLdsMaxCh x 258 doubles (33 KB)
+    // the projection kernel's LDS tile holds up to max(kFlagLdsMaxCh, kFlagChunk) x 258
+    // doubles (33 KB)
     hipError_t e0 = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
     if (e0 != hipSuccess) return e0;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flags),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       kFlagLdsMaxCh * (kFlagTile + 2) * (int)sizeof(double));
+                                       (kFlagLdsMaxCh > kFlagChunk ? kFlagLdsMaxCh : kFlagChunk) * (kFlagTile + 2) *
+                                           (int)sizeof(double));
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_stop),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
