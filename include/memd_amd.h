@@ -202,8 +202,14 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
 memd_status memd_ctx_set_tuning(memd_ctx* ctx, const memd_tuning* t);
 memd_status memd_ctx_get_tuning(memd_ctx* ctx, int32_t n, memd_tuning* out);
 
-/* Device bytes the engine needs for one epoch of this shape (workspace only). */
+/* Device bytes the engine needs for one fix-mode epoch of this shape with at
+ * most 64 output slots (workspace only; no reference counterpart: the reference
+ * allocates inside every call, memReducedEpochs_dev.cu:1077-1165). */
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K);
+/* The same for a given number of output slots and options (opts == NULL: the
+ * defaults).  Stop mode adds its per-interval record array, so size a stop-mode
+ * server pool with this call. */
+size_t memd_workspace_bytes_opts(size_t n, size_t T, size_t K, size_t N_imf, const memd_opts* opts);
 
 /*
  * Raw sample files: float32, channel-major [n][T_file], no header -- the
@@ -237,8 +243,8 @@ memd_status memd_write_bin_f32(const char* path, const float* h, size_t count);
 typedef struct memd_server memd_server;
 enum { MEMD_REQ_QUEUED = 0, MEMD_REQ_RUNNING = 1, MEMD_REQ_DONE = 2 };
 
-/* hbm_pool_bytes: workspace pool (sum of memd_workspace_bytes of the epochs
- * active at once); max_active: cap on epochs advanced together (0 = only the
+/* hbm_pool_bytes: workspace pool (sum of memd_workspace_bytes_opts of the
+ * epochs active at once, with the server's options); max_active: cap on epochs advanced together (0 = only the
  * pool limits it).  The options' mode, window and thresholds apply to every
  * request; the diagnostics fields must be unset.  Uses the context's stream. */
 memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max_active, const memd_opts* opts,

@@ -96,7 +96,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     const size_t np = (size_t)channel_stride(n);
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
-    L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // only the record-reading evaluations
+    L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // stop mode's record array only
     L.sep = take((size_t)K * 2 * np * nb * 8);
     L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
     L.mend = take((size_t)K * 2 * n * 2 * 8);
@@ -148,16 +148,6 @@ memd_status validate_shape(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N
 }
 
 hipStream_t work_stream(memd_ctx* ctx) { return ctx->user; }
-
-// Evaluation-kernel tile variant: MEMD_EVAL_VARIANT (tuning runs) or the default.
-int eval_variant_choice() {
-    const char* v = std::getenv("MEMD_EVAL_VARIANT");
-    if (v && *v) {
-        int i = std::atoi(v);
-        if (i >= 0 && i < kNumEvalVariants) return i;
-    }
-    return kEvalDefault;
-}
 
 // Built-in tile shapes for n channels: k_block groups of blocks_per_item(n)
 // solver blocks; evaluation tiles of 16 / TW channels close to n.
@@ -238,13 +228,13 @@ struct PlanLayout {
 
 // grp[e] in [0, groups) is the XCD group of epoch e (groups == 1: no affinity).
 memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const std::vector<int>& grp, int groups,
-                        int variant, size_t start, PlanLayout* P) {
+                        size_t start, PlanLayout* P) {
     const int count = (int)eds.size();
     *P = PlanLayout{};
     P->groups = groups;
     for (int e = 0; e < count; ++e) {
         for (int f = 0; f < kNumMaps; ++f) {
-            long v = (long)P->nblk[f] + blocks_for(f, eds[e], variant);
+            long v = (long)P->nblk[f] + blocks_for(f, eds[e]);
             if (v > 0x7fffffffL) return fail(ctx, MEMD_ERR_UNSUPPORTED, "batch too large for one launch");
             P->nblk[f] = (int)v;
         }
@@ -291,7 +281,7 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
 // (falling back to any queue once its own is drained).  tabs is host staging
 // that must outlive the asynchronous copies (the caller synchronises later).
 memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::vector<int>& grp,
-                       const PlanLayout& P, char* region, int mode, int variant, bool records,
+                       const PlanLayout& P, char* region, int mode, bool records,
                        std::vector<int2>& tabs, hipStream_t s, StepPlan* out) {
     const int count = (int)eds.size();
     const int groups = P.groups;
@@ -308,7 +298,7 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     for (int f = 0; f < kNumMaps; ++f) {
         std::vector<std::vector<int2>> q(groups);
         for (int e = 0; e < count; ++e) {
-            int nb = blocks_for(f, eds[e], variant);
+            int nb = blocks_for(f, eds[e]);
             for (int b = 0; b < nb; ++b) q[grp[e]].push_back(make_int2(e, b));
         }
         std::vector<size_t> head(groups, 0);
@@ -368,7 +358,6 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     plan.mode = mode;
     plan.nmax = nmax;
     plan.nlds = nlds;
-    plan.eval_variant = variant;
     plan.coef_records = records ? 1 : 0;
     *out = plan;
     return MEMD_OK;
@@ -397,6 +386,36 @@ void xcd_pack(const std::vector<EpochIn>& in, std::vector<int>& grp) {
     }
 }
 
+// Every field of a step plan, in order: the launch arguments of one captured step.
+std::string plan_key(const StepPlan& p) {
+    std::string k;
+    auto add = [&](unsigned long long v) { k += std::to_string(v); k += ','; };
+    add((uintptr_t)p.eps);
+    add((unsigned long long)p.count);
+    for (int f = 0; f < kNumMaps; ++f) {
+        add((uintptr_t)p.tab[f]);
+        add((unsigned long long)p.blocks[f]);
+    }
+    for (int l = 0; l < kNumLists; ++l) {
+        add((uintptr_t)p.wl.item[l]);
+        for (int x = 0; x < kXcds; ++x) {
+            add((unsigned long long)p.wl.off[l][x]);
+            add((unsigned long long)p.wl.cap[l][x]);
+        }
+        add((unsigned long long)p.grid[l]);
+    }
+    add((uintptr_t)p.wl.count);
+    add((unsigned long long)p.wl.groups);
+    add((unsigned long long)p.mode);
+    add((unsigned long long)p.nmax);
+    add((unsigned long long)p.nlds);
+    add((unsigned long long)p.coef_records);
+    add((unsigned long long)p.block_rmax);
+    add((unsigned long long)p.block_tile_off);
+    add((unsigned long long)p.block_lds);
+    return k;
+}
+
 // Core: run a set of epochs (all already FP64 and on the device) to completion.
 memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
                        memd_kernel_profile* prof = nullptr) {
@@ -418,8 +437,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %d)", kStopMaxChannels, nmax);
 
     // ---- workspace plan: every epoch's workspace, then the plan region
-    const int variant = eval_variant_choice();
-    const bool records = (o.mode == MEMD_MODE_STOP || variant != kEvalFused);
+    const bool records = (o.mode == MEMD_MODE_STOP);
     std::vector<EpochLayout> lay(count);
     std::vector<size_t> base(count);
     std::vector<EpochDesc> eds(count);
@@ -435,7 +453,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     std::vector<int> grp(count, 0);
     if (groups > 1) xcd_pack(in, grp);
     PlanLayout P;
-    memd_status rs = plan_layout(ctx, eds, grp, groups, variant, total, &P);
+    memd_status rs = plan_layout(ctx, eds, grp, groups, total, &P);
     if (rs != MEMD_OK) return rs;
     rs = ensure_arena(ctx, P.total);
     if (rs != MEMD_OK) return rs;
@@ -453,7 +471,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, hipMemsetAsync(ndone_dev, 0, sizeof(int), s));
     StepPlan plan{};
     std::vector<int2> tabs;
-    rs = write_plan(ctx, eds, grp, P, ctx->arena, o.mode, variant, records, tabs, s, &plan);
+    rs = write_plan(ctx, eds, grp, P, ctx->arena, o.mode, records, tabs, s, &plan);
     if (rs != MEMD_OK) return rs;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
@@ -510,19 +528,11 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, launch_init(plan, s));
     HIP_TRY(ctx, hipEventRecord(ctx->t0, s));
     if (o.use_graph && steps > 1) {
-        // capture one step on the owned stream; the plan lives in the arena, so the
-        // key is the arena address plus the launch geometry
-        std::string key = std::to_string((uintptr_t)ctx->arena) + "/" + std::to_string(count) + "/" +
-                          std::to_string(plan.mode) + "/" + std::to_string(plan.nmax) + "/" +
-                          std::to_string(plan.nlds) + "/" +
-                          std::to_string(plan.eval_variant) + "/b" + std::to_string(plan.block_lds) + ":" +
-                          std::to_string(plan.block_rmax);
-        for (int f = 0; f < kNumMaps; ++f) key += "/" + std::to_string(plan.blocks[f]);
-        key += "/g" + std::to_string(plan.wl.groups);
-        for (int l = 0; l < kNumLists; ++l) {
-            key += "/" + std::to_string(plan.grid[l]);
-            for (int x = 0; x < kXcds; ++x) key += ":" + std::to_string(plan.wl.cap[l][x]);
-        }
+        // capture one step on the owned stream.  The graph bakes in every launch
+        // argument, all of which come from the plan (descriptor array, block tables,
+        // work lists and their counters, grids, LDS sizes): it is replayed only for
+        // a plan equal to the captured one in every field.
+        const std::string key = plan_key(plan);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) {
                 HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
@@ -669,8 +679,13 @@ void memd_opts_default(memd_opts* o) {
 
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K) {
     if (n < 1 || T < 4 || K < 1) return 0;
-    const bool records = eval_variant_choice() != kEvalFused;
-    return layout_for((int)n, (int)T, (int)K, 1, records).total;
+    return layout_for((int)n, (int)T, (int)K, 1, false).total;
+}
+
+size_t memd_workspace_bytes_opts(size_t n, size_t T, size_t K, size_t N_imf, const memd_opts* opts) {
+    if (n < 1 || T < 4 || K < 1 || N_imf < 1) return 0;
+    const bool records = opts && opts->mode == MEMD_MODE_STOP;
+    return layout_for((int)n, (int)T, (int)K, (int)N_imf, records).total;
 }
 
 memd_status memd_ctx_create(int device, memd_ctx** out) {
@@ -1017,7 +1032,6 @@ struct memd_server {
     };
     memd_ctx* ctx = nullptr;
     memd_opts opts{};
-    int variant = 0;
     bool records = false;
     char* pool = nullptr;
     size_t pool_bytes = 0, pool_used = 0;
@@ -1118,7 +1132,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
         S->grp[e] = groups > 1 ? r.grp : 0;
         index[S->active[e]] = e;
     }
-    memd_status rs = plan_layout(ctx, S->eds, S->grp, groups, S->variant, 0, &S->P);
+    memd_status rs = plan_layout(ctx, S->eds, S->grp, groups, 0, &S->P);
     if (rs != MEMD_OK) return rs;
     if (S->P.total > S->region_bytes) {
         HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1137,7 +1151,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     }
     int* ndone = (int*)(S->region + S->P.ndone_at);
     for (EpochDesc& d : S->eds) d.ndone = ndone;
-    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->opts.mode, S->variant, S->records, S->tabs, s, &S->plan);
+    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->opts.mode, S->records, S->tabs, s, &S->plan);
     if (rs != MEMD_OK) return rs;
     // counters are zero between steps; a new region starts from zero too
     HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, kCountBytes, s));
@@ -1146,7 +1160,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     S->fresh.clear();
     for (int64_t id : joined) {
         int e = index[id];
-        int nb = blocks_for(kMapInit, S->eds[e], S->variant);
+        int nb = blocks_for(kMapInit, S->eds[e]);
         for (int b = 0; b < nb; ++b) S->fresh.push_back(make_int2(e, b));
     }
     if (!S->fresh.empty()) {
@@ -1210,8 +1224,7 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     memd_server* S = new memd_server();
     S->ctx = ctx;
     S->opts = o;
-    S->variant = eval_variant_choice();
-    S->records = (o.mode == MEMD_MODE_STOP || S->variant != kEvalFused);
+    S->records = (o.mode == MEMD_MODE_STOP);
     S->max_active = max_active;
     S->pool_bytes = hbm_pool_bytes / kAlign * kAlign;
     hipError_t e = hipMalloc((void**)&S->pool, S->pool_bytes);

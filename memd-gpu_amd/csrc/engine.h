@@ -13,16 +13,9 @@ namespace memd {
 constexpr int kSolverBlock = 32;
 // Samples per projection/extrema tile (4 wavefronts of 64).
 constexpr int kFlagTile = 256;
-// Fused evaluate/mean/update kernel: one wavefront per 64 samples x CPL
-// channels with U directions' records in flight; (CPL, U) is a tuning choice
-// that never changes the arithmetic.
-struct EvalVariant { int cpl, unroll; };
-constexpr EvalVariant kEvalVariants[] = {{4, 1}, {4, 2}, {2, 4}, {8, 1}, {2, 2}, {1, 4}, {0, 0}};
-constexpr int kNumEvalVariants = 7;
-// Variant kEvalFused builds the spline records of each tile in LDS (no record
-// array in HBM); tiles of 64 * TW samples x 16 / TW channels, TW = EpochDesc::etw
-// in {1, 2, 4}.
-constexpr int kEvalFused = 6;
+// The fused evaluate/mean/update kernel builds the spline records of each tile
+// in LDS (no record array in HBM); tiles of 64 * TW samples x 16 / TW channels,
+// TW = EpochDesc::etw in {1, 2, 4}.
 // Intervals meeting a tile of 64 * TW samples: at most 32 * TW knots of one kind
 // lie in its first 64 * TW - 1 samples (strict extrema are never adjacent; the
 // forced knot at t = 0 next to an extremum at t = 1 still leaves at most 32 in
@@ -35,7 +28,6 @@ constexpr int kEvalCap = 1056;
 constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
 constexpr int kEvalBatch = 8;      // directions per batch
 constexpr int kEvalChunk = 32;     // directions whose formation inputs are staged together
-constexpr int kEvalDefault = kEvalFused;
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles up to
 // kFlagLdsMaxCh channels; wider epochs pass kFlagChunk channels at a time through
 // the same tile, kFlagGroup directions per workgroup.
@@ -133,7 +125,7 @@ struct EpochDesc {
     EpochState* st;
     unsigned long long* stats; // optional [4]: knots, knot*channels, eval points, samples
     int* ndone;                // shared by the batch: epochs finished (stop mode)
-    int coef_records;          // 1: k_coef writes the record array (stop mode / record eval variants)
+    int coef_records;          // 1: k_coef writes the record array (stop mode)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
     int bpw;                   // solver blocks per k_block work item
     // k_block work items (ke, group, E, 0) of this epoch, appended by k_compact and
@@ -186,15 +178,14 @@ struct StepPlan {
     int mode;               // kModeFix or kModeStop (all epochs of one call)
     int nmax;               // largest channel count in the batch
     int nlds;               // channels of k_flags' LDS tile: the largest n <= kFlagLdsMaxCh, at least kFlagChunk if an epoch is wider, 0 if none
-    int eval_variant;       // index into kEvalVariants
-    int coef_records;       // launch k_coef (the evaluation reads the record array)
+    int coef_records;       // launch k_coef (stop mode: the evaluation reads the record array)
     int block_rmax;         // k_block: rows of the longest work item
     size_t block_tile_off;  // k_block: LDS byte offset of the tile
     size_t block_lds;       // k_block: dynamic LDS bytes
 };
 
 // Host-side block counts for one epoch (must match the kernels' decoding).
-int blocks_for(int map, const EpochDesc& e, int eval_variant);
+int blocks_for(int map, const EpochDesc& e);
 // Worst-case work items of one epoch in a list.
 long list_capacity(int list, const EpochDesc& e);
 // Kernel slots of one sift step (profiling order): flags, compact, factor,

@@ -1047,8 +1047,8 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
     }
 }
 
-// Phase 3 + coefficients for the record readers (stop mode and the record
-// evaluation variants): the spline record (a, b, c, d) of every interval of every
+// Phase 3 + coefficients for the record reader (stop mode's evaluation): the
+// spline record (a, b, c, d) of every interval of every
 // channel (memReducedEpochs_dev.cu:775-793) with the shared reciprocals, from M of
 // its two knots (knot_m_select).  A work item is 256 * kCoefItems (channel,
 // interval) pairs of one (direction, envelope); lanes run over consecutive
@@ -1181,85 +1181,6 @@ __device__ __forceinline__ int knot_interval(const EpochDesc& ed, int ke, int t,
 
 __device__ __forceinline__ double horner4(const double4& r, double tau) {
     return fma(fma(fma(r.w, tau, r.z), tau, r.y), tau, r.x);
-}
-
-// One wavefront = 64 consecutive samples x CPL channels.  Directions are
-// processed U at a time: all their spline records are loaded before any is
-// used, then the means are added in ascending direction order.
-template <int CPL, int U>
-__global__ __launch_bounds__(64) void k_eval(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab) {
-    const int2 br = tab[blockIdx.x];
-    const EpochDesc& ed = eps[br.x];
-    const EpochState* st = ed.st;
-    if (st->done) return;
-    const int n = ed.n, T = ed.T, K = ed.K;
-    int groups = (n + CPL - 1) / CPL;
-    int tile = br.y / groups;
-    int cg = br.y - tile * groups;
-    const int lane = threadIdx.x;
-    const int t = tile * 64 + lane;
-    const int tq = min(t, T - 1);   // lookups stay in range on a ragged last tile
-    const int c0 = cg * CPL;
-    const double4* coef = reinterpret_cast<const double4*>(ed.coef);
-    double acc[CPL];
-#pragma unroll
-    for (int i = 0; i < CPL; ++i) acc[i] = 0.0;
-    double nv = 0.0;
-    for (int k0 = 0; k0 < K; k0 += U) {
-        double4 r[U][2][CPL];
-        double tau[U][2];
-        bool ok[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            int k = min(k0 + u, K - 1);
-            ok[u] = (k0 + u < K) && dir_valid(ed.E, k);
-#pragma unroll
-            for (int env = 0; env < 2; ++env) {
-                int ke = 2 * k + env;
-                int E = max(ed.E[ke], 2);
-                int j = knot_interval(ed, ke, tq, E);
-                tau[u][env] = (double)(tq - ed.X[(size_t)ke * ed.Ecap + j]);
-#pragma unroll
-                for (int i = 0; i < CPL; ++i) {
-                    int c = min(c0 + i, n - 1);
-                    r[u][env][i] = coef[((size_t)ke * n + c) * ed.Ecap + j];
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (!ok[u]) continue;
-            nv = nv + 1.0;
-#pragma unroll
-            for (int i = 0; i < CPL; ++i) {
-                double ep = horner4(r[u][0][i], tau[u][0]);
-                double em = horner4(r[u][1][i], tau[u][1]);
-                acc[i] = acc[i] + (ep + em) * 0.5;
-            }
-        }
-    }
-    if (ed.stats && br.y == 0 && lane == 0) {
-        atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
-        atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
-    }
-    if (t >= T) return;
-    double* cur = cur_slot(ed, st->imf);
-    bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
-    double* nxt = cur_slot(ed, st->imf + 1);
-#pragma unroll
-    for (int i = 0; i < CPL; ++i) {
-        int c = c0 + i;
-        if (c < n) {
-            size_t idx = (size_t)c * T + t;
-            double v = cur[idx] - acc[i] / nv;
-            cur[idx] = v;
-            if (last) {
-                double rr = ed.running[idx] - v;
-                ed.running[idx] = rr;
-                nxt[idx] = rr;
-            }
-        }
-    }
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -1849,7 +1770,7 @@ int solver_block_capacity(int Ecap) {
     return (N >= 1) ? (N - 1) / kSolverBlock + 1 : 1;
 }
 
-int blocks_for(int map, const EpochDesc& e, int eval_variant) {
+int blocks_for(int map, const EpochDesc& e) {
     switch (map) {
     case kMapInit: return (int)(((size_t)e.n * e.T + 255) / 256);
     case kMapFlags:
@@ -1859,9 +1780,7 @@ int blocks_for(int map, const EpochDesc& e, int eval_variant) {
     case kMapBlock: return e.bblocks;
     case kMapEval:
         if (e.mode == kModeStop) return (e.T + 63) / 64;
-        if (eval_variant == kEvalFused)
-            return ((e.W + e.etw - 1) / e.etw) * ((e.n + 16 / e.etw - 1) / (16 / e.etw));
-        return ((e.T + 63) / 64) * ((e.n + kEvalVariants[eval_variant].cpl - 1) / kEvalVariants[eval_variant].cpl);
+        return ((e.W + e.etw - 1) / e.etw) * ((e.n + 16 / e.etw - 1) / (16 / e.etw));
     }
     return 0;
 }
@@ -1925,18 +1844,6 @@ hipError_t launch_gather_states(const StepPlan& p, EpochState* out, hipStream_t 
     return hipGetLastError();
 }
 
-static void launch_eval(const StepPlan& p, hipStream_t s) {
-    switch (p.eval_variant) {
-    case 1: MEMD_TABLE_LAUNCH((k_eval<4, 2>), kMapEval, 64, 0); break;
-    case 2: MEMD_TABLE_LAUNCH((k_eval<2, 4>), kMapEval, 64, 0); break;
-    case 3: MEMD_TABLE_LAUNCH((k_eval<8, 1>), kMapEval, 64, 0); break;
-    case 4: MEMD_TABLE_LAUNCH((k_eval<2, 2>), kMapEval, 64, 0); break;
-    case 5: MEMD_TABLE_LAUNCH((k_eval<1, 4>), kMapEval, 64, 0); break;
-    case kEvalFused: MEMD_TABLE_LAUNCH(k_eval_fused, kMapEval, 256, 0); break;
-    default: MEMD_TABLE_LAUNCH((k_eval<4, 1>), kMapEval, 64, 0); break;
-    }
-}
-
 static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     switch (which) {
     case 0:   // per epoch: LDS-staged tiles (n <= kFlagLdsMaxCh) or (direction group, tile) blocks
@@ -1951,14 +1858,14 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
                            p.tab[kMapBlock], p.block_rmax, (int)p.block_tile_off);
         break;
     case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 64); break;
-    case 5:   // the fused evaluation forms M itself; records only for the record readers
+    case 5:   // the fused evaluation forms M itself; records only for stop mode's evaluation
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
         break;
     case 6:
         if (p.mode == kModeStop)
             MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
         else
-            launch_eval(p, s);
+            MEMD_TABLE_LAUNCH(k_eval_fused, kMapEval, 256, 0);
         break;
     case 7:
         if (p.mode == kModeStop)

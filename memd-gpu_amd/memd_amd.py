@@ -79,13 +79,13 @@ EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_l
            "memd_opts_default", "memd_version", "memd_directions_f64", "memd_directions_f32", "memd_run_f64",
            "memd_run_f32",
            "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
-           "memd_workspace_bytes", "memd_server_create", "memd_server_destroy", "memd_server_submit",
+           "memd_workspace_bytes", "memd_workspace_bytes_opts", "memd_server_create", "memd_server_destroy", "memd_server_submit",
            "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
            "memd_server_pool_used", "memd_bin_length", "memd_read_bin_f32", "memd_write_bin_f32",
            "memd_autotune", "memd_ctx_set_tuning", "memd_ctx_get_tuning")
 #: exports that do not return a memd_status
 _NOT_STATUS = ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes",
-               "memd_server_pool_used")
+               "memd_workspace_bytes_opts", "memd_server_pool_used")
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
 _lib = None
@@ -118,6 +118,8 @@ def load_library(path=LIB_PATH):
     L.memd_opts_default.restype = None
     L.memd_workspace_bytes.argtypes = [_sz, _sz, _sz]
     L.memd_workspace_bytes.restype = _sz
+    L.memd_workspace_bytes_opts.argtypes = [_sz, _sz, _sz, _sz, ctypes.POINTER(MemdOpts)]
+    L.memd_workspace_bytes_opts.restype = _sz
     L.memd_directions_f64.argtypes = [_vp, _sz, _sz, ctypes.POINTER(_i32), _vp]
     L.memd_directions_f32.argtypes = L.memd_directions_f64.argtypes
     L.memd_run_f64.argtypes = [_vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, _vp, _vp,
@@ -162,6 +164,30 @@ def default_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _shape(t):
+    return tuple(int(v) for v in t.shape)
+
+
+def check_epoch_shapes(y, V, n_imf, S, out=None, x=None):
+    """Shapes of one epoch's buffers, checked before the C ABI sees them (it only
+    receives pointers and sizes, so a mis-shaped buffer would be read or written
+    past its end on the device): y [n][T], V [K][n], out [n_imf][n][T], x [T].
+    Returns (n, T, K)."""
+    if len(y.shape) != 2:
+        raise ValueError(f"d_y must be [n][T], got shape {_shape(y)}")
+    n, T = _shape(y)
+    if len(V.shape) != 2 or int(V.shape[1]) != n:
+        raise ValueError(f"d_DirectionVectors must be [K][{n}], got shape {_shape(V)}")
+    K = int(V.shape[0])
+    if int(n_imf) < 1 or int(S) < 1:
+        raise ValueError(f"n_imf and S must be >= 1 (got {n_imf}, {S})")
+    if out is not None and _shape(out) != (int(n_imf), n, T):
+        raise ValueError(f"d_IMFs must be [{n_imf}][{n}][{T}], got shape {_shape(out)}")
+    if x is not None and (len(x.shape) != 1 or int(x.shape[0]) != T):
+        raise ValueError(f"d_x must be [{T}], got shape {_shape(x)}")
+    return n, T, K
 
 
 def _require_device(t, dtype, name):
@@ -245,8 +271,7 @@ class MemdContext:
         Returns (imfs [n_imf][n][T], info) with info = elapsed_s, sift_counts,
         n_imf_out and, when trace_passes > 0, knot_counts [passes][K][2]."""
         import torch
-        n, T = y.shape
-        K = V.shape[0]
+        n, T, K = check_epoch_shapes(y, V, n_imf, S, out, x)
         f32 = y.dtype == torch.float32
         dt = torch.float32 if f32 else torch.float64
         _require_device(y, dt, "d_y")
@@ -278,6 +303,17 @@ class MemdContext:
     # ---- mexFunction argument layout (memdMatlabCuda.cu:996-1039)
     def mex_compat(self, signalDim, numDirVector, numIMFs, maxiter, data, samplesIndex, primesSet, outputData):
         import torch
+        n, N_imf = int(signalDim), int(numIMFs)
+        if n < 1 or data.numel() % n != 0:
+            raise ValueError(f"data holds {data.numel()} values, not a multiple of signalDim = {n}")
+        T = data.numel() // n   # memdMatlabCuda.cu:1038-1039
+        if outputData.numel() != N_imf * n * T:
+            raise ValueError(f"outputData must hold numIMFs * signalDim * T = {N_imf * n * T} values, "
+                             f"got {outputData.numel()}")
+        if primesSet.numel() != n:
+            raise ValueError(f"primesSet must hold signalDim = {n} values, got {primesSet.numel()}")
+        if samplesIndex is not None and samplesIndex.numel() != T:
+            raise ValueError(f"samplesIndex must hold T = {T} values, got {samplesIndex.numel()}")
         f32 = data.dtype == torch.float32
         dt = torch.float32 if f32 else torch.float64
         _require_device(data, dt, "data")
@@ -301,15 +337,15 @@ class MemdContext:
         outs, scs, nouts = [], [], []
         for i, ep in enumerate(epochs):
             y, V = ep["y"], ep["V"]
+            n, T, K = check_epoch_shapes(y, V, ep["n_imf"], ep["S"], ep.get("out"))
             _require_device(y, torch.float64, "d_y")
             _require_device(V, torch.float64, "d_V")
-            n, T = y.shape
             out = ep.get("out")
             if out is None:
                 out = torch.empty((ep["n_imf"], n, T), dtype=torch.float64, device=y.device)
             sc = (ctypes.c_int32 * ep["n_imf"])()
             no = ctypes.c_int32(0)
-            arr[i] = MemdEpoch(n, T, V.shape[0], ep["n_imf"], ep["S"], y.data_ptr(), V.data_ptr(),
+            arr[i] = MemdEpoch(n, T, K, ep["n_imf"], ep["S"], y.data_ptr(), V.data_ptr(),
                                out.data_ptr(), ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32)),
                                ctypes.pointer(no))
             outs.append(out)
@@ -365,15 +401,15 @@ class MemdServer:
 
     def submit(self, y, V, n_imf, S, out=None):
         import torch
+        n, T, K = check_epoch_shapes(y, V, n_imf, S, out)
         _require_device(y, torch.float64, "d_y")
         _require_device(V, torch.float64, "d_V")
-        n, T = y.shape
         if out is None:
             out = torch.empty((n_imf, n, T), dtype=torch.float64, device=y.device)
         _require_device(out, torch.float64, "d_IMFs")
         sc = (ctypes.c_int32 * n_imf)()
         no = ctypes.c_int32(0)
-        ep = MemdEpoch(n, T, V.shape[0], n_imf, S, y.data_ptr(), V.data_ptr(), out.data_ptr(),
+        ep = MemdEpoch(n, T, K, n_imf, S, y.data_ptr(), V.data_ptr(), out.data_ptr(),
                        ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32)), ctypes.pointer(no))
         rid = ctypes.c_int64(0)
         self.ctx._check(self.lib.memd_server_submit(self.h, ctypes.byref(ep), ctypes.byref(rid)))

@@ -136,3 +136,90 @@ def test_cli_error_paths(tmp_path):
     assert r.returncode == 8 and "cannot open" in r.stderr
     r = subprocess.run([cli, "--no-such-flag", "1"], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+class _NoLibrary:
+    """Stands in for the loaded library: any call means a buffer reached the C ABI."""
+
+    def __getattr__(self, name):
+        raise AssertionError(f"{name} was called with mis-shaped buffers")
+
+
+def _offline_ctx():
+    import memd_amd
+    ctx = object.__new__(memd_amd.MemdContext)
+    ctx.lib, ctx.h, ctx.device = _NoLibrary(), None, 0
+    return ctx
+
+
+@pytest.mark.parametrize("case", ["V_channels", "V_rank", "out_shape", "out_slots", "x_len", "y_rank", "n_imf"])
+def test_run_refuses_mis_shaped_buffers(case):
+    # the C ABI receives pointers and sizes only, so the mirror checks the shapes
+    # before any library call (a mis-shaped buffer would be read or written past its end)
+    import torch
+    import memd_amd
+    n, T, K = 4, 100, 8
+    y, V = torch.zeros(n, T, dtype=torch.float64), torch.zeros(K, n, dtype=torch.float64)
+    out, x, n_imf = torch.zeros(3, n, T, dtype=torch.float64), torch.arange(T, dtype=torch.int32), 3
+    if case == "V_channels":
+        V = torch.zeros(K, n + 1, dtype=torch.float64)
+    elif case == "V_rank":
+        V = torch.zeros(K * n, dtype=torch.float64)
+    elif case == "out_shape":
+        out = torch.zeros(3, n, T - 1, dtype=torch.float64)
+    elif case == "out_slots":
+        n_imf = 4
+    elif case == "x_len":
+        x = torch.arange(T - 1, dtype=torch.int32)
+    elif case == "y_rank":
+        y = torch.zeros(n * T, dtype=torch.float64)
+    elif case == "n_imf":
+        n_imf, out = 0, None
+    with pytest.raises(ValueError):
+        _offline_ctx().run(y, V, n_imf, 2, x=x, out=out)
+
+
+def test_batch_and_server_refuse_mis_shaped_buffers():
+    import torch
+    import memd_amd
+    y, V = torch.zeros(4, 100, dtype=torch.float64), torch.zeros(8, 5, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        _offline_ctx().run_batch([{"y": y, "V": V, "n_imf": 3, "S": 2}])
+    srv = object.__new__(memd_amd.MemdServer)
+    srv.ctx, srv.lib, srv.h, srv._req = _offline_ctx(), _NoLibrary(), None, {}
+    with pytest.raises(ValueError):
+        srv.submit(y, V, 3, 2)
+    with pytest.raises(ValueError):
+        srv.submit(y, torch.zeros(8, 4, dtype=torch.float64), 3, 2, out=torch.zeros(2, 4, 100, dtype=torch.float64))
+
+
+@pytest.mark.parametrize("case", ["numel", "out", "primes", "index"])
+def test_mex_layout_refuses_mis_sized_buffers(case):
+    import torch
+    n, K, N_imf, S, T = 3, 6, 4, 2, 50
+    data, out = torch.zeros(n * T, dtype=torch.float32), torch.zeros(N_imf * n * T, dtype=torch.float32)
+    primes, index = torch.tensor([2, 3, 5], dtype=torch.int32), torch.arange(T, dtype=torch.int32)
+    if case == "numel":
+        data = torch.zeros(n * T + 1, dtype=torch.float32)
+    elif case == "out":
+        out = torch.zeros(N_imf * n * T - 1, dtype=torch.float32)
+    elif case == "primes":
+        primes = torch.tensor([2, 3], dtype=torch.int32)
+    elif case == "index":
+        index = torch.arange(T + 1, dtype=torch.int32)
+    with pytest.raises(ValueError):
+        _offline_ctx().mex_compat(n, K, N_imf, S, data, index, primes, out)
+
+
+def test_workspace_bytes_follow_mode_and_slots():
+    import memd_amd
+    lib = memd_amd.load_library()
+    fix = memd_amd.default_opts()
+    stop = memd_amd.default_opts(mode=memd_amd.MODE_STOP)
+    base = lib.memd_workspace_bytes(32, 4096, 64)
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, None) == base
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 64, ctypes.byref(fix)) == base
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 65, ctypes.byref(fix)) > base
+    # stop mode's record array: 2K * n * (T/2 + 2) intervals of 32 bytes
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, ctypes.byref(stop)) >= base + 128 * 32 * 2050 * 32
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 0, None) == 0

@@ -135,6 +135,23 @@ def test_graph_and_direct_launch_agree(ctx, oracle):
     assert np.array_equal(a, b)
 
 
+def test_graph_replays_only_for_an_equal_plan(ctx, oracle):
+    # consecutive single-epoch calls with the same launch geometry (block counts,
+    # grids, list capacities) but different workspace layouts: T = 1000 and 1010 move
+    # every region after the signal, 10 and 70 output slots move the plan region.
+    # Each call must capture its own step instead of replaying one whose kernel
+    # arguments point into the previous layout.
+    import synth
+    V = oracle.directions(8, 16)
+    for T, n_imf in [(1000, 10), (1010, 10), (1010, 70), (1000, 10)]:
+        y = synth.sines_epoch(8, T, 70 + T)
+        print(f"graph case T={T} n_imf={n_imf}", flush=True)
+        imfs_o, info_o = oracle.memd(y, V, n_imf, 2)
+        imfs_g, info_g = ctx.run(_dev(y), _dev(V), n_imf, 2, use_graph=True)
+        assert np.array_equal(info_g["sift_counts"], info_o["sift_counts"])
+        assert np.array_equal(imfs_g.cpu().numpy(), imfs_o, equal_nan=True)
+
+
 def test_eeg_sample_reduced_schedule(ctx, oracle):
     # the reference's own input file, 32 x 30504, K = 64 (2 IMFs x 2 sifts to keep the oracle fast)
     import synth
