@@ -118,7 +118,7 @@ struct Blocked {
     std::vector<double> rd, cp;                  // per row, block-local forward factors
     std::vector<double> al, be;                  // spike vectors, per row of their block
     std::vector<double> Rlo, Rdi, Rup, Rrd, Rcp; // reduced system rows 1..J
-    std::vector<double> z, g, y;                 // scratch
+    std::vector<double> z, g;                    // factor() scratch
 
     int bp_(int b) const { return b * L + 1; }
     int bq_(int b) const { return (b < J) ? (b + 1) * L - 1 : N; }
@@ -128,7 +128,7 @@ struct Blocked {
         J = (N >= 1) ? (N - 1) / L : 0;
         rd.assign(N + 2, 0.0); cp.assign(N + 2, 0.0);
         al.assign(N + 2, 0.0); be.assign(N + 2, 0.0);
-        z.assign(N + 2, 0.0); g.assign(N + 2, 0.0); y.assign(N + 2, 0.0);
+        z.assign(N + 2, 0.0); g.assign(N + 2, 0.0);
         if (N <= 0) return;
         for (int b = 0; b <= J; ++b) tridiag_factor(bp_(b), bq_(b), lo, di, up, rd.data(), cp.data());
         if (J == 0) return;
@@ -158,14 +158,18 @@ struct Blocked {
         tridiag_factor(1, J, Rlo.data(), Rdi.data(), Rup.data(), Rrd.data(), Rcp.data());
     }
 
-    /* r: rows 1..N (index 0 and N+1 ignored); out: x[0..N+1] with x[0]=x[N+1]=0 */
-    void solve(const double* r, double* out) {
+    /* r: rows 1..N (index 0 and N+1 ignored); out: x[0..N+1] with x[0]=x[N+1]=0.
+     * Reads only the factorisation: concurrent solves of one system (one per
+     * channel) each bring their own scratch. */
+    void solve(const double* r, double* out) const {
         out[0] = 0.0; out[N + 1] = 0.0;
         if (N <= 0) return;
+        std::vector<double> z(N + 2, 0.0);
         if (J == 0) {
             tridiag_apply(1, N, lo, rd.data(), cp.data(), r, z.data(), out);
             return;
         }
+        std::vector<double> y(N + 2, 0.0);
         std::vector<double> D(J + 2, 0.0), Zr(J + 2), Xs(J + 2, 0.0);
         for (int b = 0; b <= J; ++b)     // phase 1
             tridiag_apply(bp_(b), bq_(b), lo, rd.data(), cp.data(), r, z.data(), y.data());
@@ -230,9 +234,10 @@ struct KnotSystem {
         }
         solver.factor(N, lo.data(), di.data(), up.data());
     }
-    /* one channel: Y (clamped, E values) -> coefficients [E-1][4] */
+    /* one channel: Y (clamped, E values) -> coefficients [E-1][4]; sl, r, M are the
+     * caller's scratch (one set per thread) */
     void channel(const double* Y, double* coef, std::vector<double>& sl, std::vector<double>& r,
-                 std::vector<double>& M) {
+                 std::vector<double>& M) const {
         int N = E - 2;
         sl.resize(std::max(E - 1, 1));
         r.assign(E + 1, 0.0);

@@ -196,3 +196,18 @@ def test_not_a_knot_memd_reconstructs(oracle):
     np.testing.assert_allclose(imfs.sum(0), y, rtol=0, atol=1e-12 * np.abs(y).max())
     nat, _ = oracle.memd(y, V, 4, 3)
     assert not np.array_equal(nat, imfs)   # the option changes the envelopes
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_threaded_oracle_equals_serial(oracle, mode):
+    # the OpenMP path (channels in parallel, one solver factorisation shared by them)
+    # must give the serial path's results bit for bit: it is the checker of the
+    # full-size GPU tests and the CPU baseline; 8000 samples give many solver blocks
+    import synth
+    y = synth.eeg_epoch(16, 8000, 3)
+    V = oracle.directions(16, 8)
+    a, ia = oracle.memd(y, V, 3, 2, mode=mode, threads=1)
+    for th in (2, 8):
+        b, ib = oracle.memd(y, V, 3, 2, mode=mode, threads=th)
+        assert np.array_equal(ia["knot_counts"], ib["knot_counts"])
+        assert np.array_equal(a, b)
