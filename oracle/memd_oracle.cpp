@@ -45,10 +45,13 @@
  *     coefficients are exact in FP64.
  *
  * Stop mode (not in the reference, which only has fix mode) follows the public
- * memd.m criterion: sx(t) = |mean(t)|_2 / amp(t), stop when
+ * memd.m criterion (oracle_stop_decision): sx(t) = |mean(t)|_2 / amp(t), stop when
  * !((frac(sx > sd) > tol || any(sx > sd2)) && any(nem > 2)); an IMF is not started
- * when every direction has fewer than 3 interior extrema.  PARITY UNPINNED against
- * the reference (memd.m is not available here); only GPU-vs-oracle agreement is tested.
+ * when every direction has fewer than 3 interior extrema.  The reference cannot
+ * pin it (it has no stop mode); the tests pin it against an independent numpy/scipy
+ * restatement of memd.m's loop (tests/golden/make_golden.py), known-answer cases
+ * at every threshold and the limit cases sd = sd2 = tol = 0 (= fix mode) and
+ * sd = sd2 = inf (no sift).
  */
 #include "memd_oracle.h"
 
@@ -520,6 +523,33 @@ void oracle_envelope_mean(int n, int T, int K, const double* cur, const double* 
     if (amp) for (int t = 0; t < T; ++t) amp[t] = ampsum[t] / nv;
 }
 
+int oracle_stop_decision(int n, int T, int K, const double* mean, const double* amp, const int32_t* counts,
+                         double sd, double sd2, double tol) {
+    // memd.m stop(): sx(t) = |mean(t)|_2, divided by amp(t) only when every amp(t)
+    // is nonzero; keep sifting while (frac(sx > sd) > tol || any(sx > sd2)) holds
+    // and some direction has more than 2 interior extrema
+    bool amp_nonzero = true;
+    for (int t = 0; t < T; ++t)
+        if (!(amp[t] != 0.0)) amp_nonzero = false;
+    long cnt1 = 0;
+    bool any2 = false;
+    for (int t = 0; t < T; ++t) {
+        double s = 0.0;
+        for (int c = 0; c < n; ++c) {
+            double m = mean[(size_t)c * T + t];
+            s = s + m * m;
+        }
+        double sx = std::sqrt(s);
+        if (amp_nonzero) sx = sx / amp[t];
+        if (sx > sd) ++cnt1;
+        if (sx > sd2) any2 = true;
+    }
+    bool anynem = false;
+    for (int k = 0; k < K; ++k)
+        if ((counts[2 * k] - 2) + (counts[2 * k + 1] - 2) > 2) anynem = true;
+    return !((((double)cnt1 / (double)T) > tol || any2) && anynem) ? 1 : 0;
+}
+
 int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const double* V,
                 double* imfs, const oracle_opts* o, oracle_trace* tr) {
     if (n < 1 || T < 4 || K < 1 || N_imf < 1 || S < 1) return -1;
@@ -529,7 +559,8 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
     int nth = (o && o->threads > 1) ? o->threads : 1;
     double sd = o ? o->sd : 0.075, sd2 = o ? o->sd2 : 0.75, tol = o ? o->tol : 0.075;
     size_t nT = (size_t)n * T;
-    std::vector<double> running(y, y + nT), sum(nT), ampsum(T), q(T);
+    std::vector<double> running(y, y + nT), sum(nT), ampsum(T), meanv(mode == ORACLE_MODE_STOP ? nT : 0),
+        ampv(mode == ORACLE_MODE_STOP ? T : 0);
     std::vector<int32_t> counts((size_t)2 * K);
     int pass = 0;
     auto record = [&]() {
@@ -574,29 +605,9 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
                 if (nv == 0.0) {
                     stop = true;   // memd.m: no valid direction -> zero mean, stop
                 } else {
-                    bool amp_nonzero = true;
-                    for (int t = 0; t < T; ++t) {
-                        double a = ampsum[t] / nv;
-                        ampsum[t] = a;
-                        if (!(a != 0.0)) amp_nonzero = false;
-                    }
-                    long cnt1 = 0;
-                    bool any2 = false;
-                    for (int t = 0; t < T; ++t) {
-                        double s = 0.0;
-                        for (int c = 0; c < n; ++c) {
-                            double m = sum[(size_t)c * T + t] / nv;
-                            s = s + m * m;
-                        }
-                        double sx = std::sqrt(s);
-                        if (amp_nonzero) sx = sx / ampsum[t];
-                        if (sx > sd) ++cnt1;
-                        if (sx > sd2) any2 = true;
-                    }
-                    bool anynem = false;
-                    for (int k = 0; k < K; ++k)
-                        if ((counts[2 * k] - 2) + (counts[2 * k + 1] - 2) > 2) anynem = true;
-                    stop = !((((double)cnt1 / (double)T) > tol || any2) && anynem);
+                    for (size_t i = 0; i < nT; ++i) meanv[i] = sum[i] / nv;
+                    for (int t = 0; t < T; ++t) ampv[t] = ampsum[t] / nv;
+                    stop = oracle_stop_decision(n, T, K, meanv.data(), ampv.data(), counts.data(), sd, sd2, tol) != 0;
                 }
                 if (stop) break;
                 for (size_t i = 0; i < nT; ++i) cur[i] = cur[i] - sum[i] / nv;

@@ -84,6 +84,14 @@ void oracle_tridiag_blocked(int N, const double* lo, const double* di, const dou
 void oracle_envelope_mean(int n, int T, int K, const double* cur, const double* V,
                           const oracle_opts* o, double* mean, double* amp, int32_t* counts);
 
+/* ---- stop mode's decision for one pass (memd.m stop(); not in the reference):
+ * mean [n][T] (sum over valid directions / nvalid), amp [T] (memd.m amplitude),
+ * counts [K][2] knot counts (E+, E-, endpoints included).  Returns 1 to stop
+ * sifting, 0 to subtract the mean and sift again.  oracle_memd calls it on every
+ * pass with at least one valid direction. */
+int oracle_stop_decision(int n, int T, int K, const double* mean, const double* amp, const int32_t* counts,
+                         double sd, double sd2, double tol);
+
 /* ---- whole epoch (memReducedEpochs_dev.cu:1074-1368) ----
  * y [n][T], V [K][n], imfs [N_imf][n][T]. Returns 0 on success. */
 int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const double* V,

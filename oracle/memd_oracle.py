@@ -60,6 +60,7 @@ def lib():
         L.oracle_envelope_mean.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, ctypes.POINTER(Opts), _dp, _dp, _ip]
         L.oracle_memd.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp, ctypes.POINTER(Opts), ctypes.POINTER(Trace)]
         L.oracle_window_limit.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.oracle_stop_decision.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip] + [ctypes.c_double] * 3
         _lib = L
     return _lib
 
@@ -181,6 +182,16 @@ def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1, sp
     lib().oracle_envelope_mean(n, T, K, _d(cur), _d(V), ctypes.byref(o), _d(mean),
                                _d(amp) if want_amp else None, _i(counts))
     return mean, amp, counts
+
+
+def stop_decision(mean, amp, counts, sd=0.075, sd2=0.75, tol=0.075):
+    """Stop mode's decision on one pass: True = stop sifting this IMF."""
+    mean = np.ascontiguousarray(mean, np.float64)
+    amp = np.ascontiguousarray(amp, np.float64)
+    counts = np.ascontiguousarray(counts, np.int32)
+    n, T = mean.shape
+    K = counts.shape[0]
+    return bool(lib().oracle_stop_decision(n, T, K, _d(mean), _d(amp), _i(counts), sd, sd2, tol))
 
 
 def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075,

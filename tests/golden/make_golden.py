@@ -18,7 +18,16 @@ index, transposed operand) disagrees with at least one vector:
   m-1,m-2) and scipy.interpolate.CubicSpline(bc_type='natural') evaluated at the
   integer samples, instead of the blocked Thomas solve + Horner.
 * tridiagonal: dense np.linalg.solve of the natural-spline system.
-* one envelope-mean pass of a small epoch: the whole chain above with V @ y.
+* one envelope-mean pass of a small epoch: the whole chain above with V @ y,
+  including memd.m's amplitude sum_valid sqrt(sum_c (e+ - e-)^2) / 2 / nvalid.
+* whole decompositions: the fixed-count loop of memReducedEpochs_dev.cu:1174-1179,
+  1328-1338 and the stop-mode loop of the public memd.m (stop(): sx = |mean|_2,
+  divided by amp only when every amp(t) != 0, sift while (mean(sx > sd) > tol |
+  any(sx > sd2)) & any(nem > 2); stop_emd(): no IMF is started once every
+  direction has fewer than 3 interior extrema), written as plain Python loops over
+  the stage restatements above.  The generator checks that no decision is within
+  rounding of a threshold (sx near sd or sd2, a fraction near tol), so the
+  oracle's different rounding cannot flip one.
 """
 import os
 from fractions import Fraction
@@ -105,11 +114,13 @@ def spline_env(T, X, Y):
     return CubicSpline(X.astype(float), Y, bc_type="natural")(np.arange(T, dtype=float))
 
 
-def envelope_mean_reference(y, V):
+def envelope_stats_reference(y, V):
+    """One pass: mean over valid directions, memd.m amplitude, knot counts, nvalid."""
     n, T = y.shape
     K = V.shape[0]
     P = V @ y
     acc = np.zeros((n, T))
+    amp = np.zeros(T)
     nv = 0
     counts = np.zeros((K, 2), np.int32)
     for k in range(K):
@@ -118,13 +129,79 @@ def envelope_mean_reference(y, V):
         if not (len(Xs[0]) > 3 and len(Xs[1]) > 3):
             continue
         nv += 1
-        for c in range(n):
-            envs = []
-            for e in range(2):
-                Yc = clamp_polyfit(Xs[e], y[c, Xs[e]], e)
-                envs.append(spline_env(T, Xs[e], Yc))
-            acc[c] += (envs[0] + envs[1]) / 2.0
-    return acc / nv, counts
+        emax = np.stack([spline_env(T, Xs[0], clamp_polyfit(Xs[0], y[c, Xs[0]], 0)) for c in range(n)])
+        emin = np.stack([spline_env(T, Xs[1], clamp_polyfit(Xs[1], y[c, Xs[1]], 1)) for c in range(n)])
+        acc += (emax + emin) / 2.0
+        amp += np.sqrt(np.sum((emax - emin) ** 2, axis=0)) / 2.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return acc / nv, amp / nv, counts, nv
+
+
+def envelope_mean_reference(y, V):
+    mean, _, counts, _ = envelope_stats_reference(y, V)
+    return mean, counts
+
+
+def memd_fix_reference(y, V, n_imf, S):
+    """memReducedEpochs_dev.cu:1174-1179, 1328-1338: S sifts per IMF, N_imf - 1 IMFs + residual."""
+    running = y.copy()
+    out = []
+    for _ in range(n_imf - 1):
+        cur = running.copy()
+        for _ in range(S):
+            cur = cur - envelope_stats_reference(cur, V)[0]
+        out.append(cur)
+        running = running - cur
+    out.append(running)
+    return np.stack(out)
+
+
+class Fragile(Exception):
+    pass
+
+
+def stop_reference(mean, amp, counts, nv, sd, sd2, tol, margin=1e-9):
+    """memd.m stop() on one pass; raises Fragile when an sx(t) is within `margin`
+    (relative) of sd or sd2, where the oracle's rounding (about 1e-13 relative
+    from scipy's) could put it on the other side."""
+    if nv == 0:
+        return True   # no valid direction: memd.m's zero mean, stop
+    sx = np.sqrt(np.sum(mean ** 2, axis=0))
+    if np.all(amp != 0):
+        sx = sx / amp
+    for thr in (sd, sd2):
+        if np.any(np.abs(sx - thr) <= margin * thr):
+            raise Fragile(f"sx within {margin} of {thr}")
+    T = mean.shape[1]
+    frac = np.count_nonzero(sx > sd) / T
+    nem = (counts[:, 0] - 2) + (counts[:, 1] - 2)
+    return not ((frac > tol or np.any(sx > sd2)) and np.any(nem > 2))
+
+
+def memd_stop_reference(y, V, n_imf, S, sd, sd2, tol):
+    """memd.m's loop: stop_emd before each IMF, stop() before each subtraction, at
+    most S subtractions per IMF, at most n_imf - 1 IMFs; returns (imfs, sifts, used)."""
+    running = y.copy()
+    out, sifts = [], []
+    for _ in range(n_imf - 1):
+        cur = running.copy()
+        mean, amp, counts, nv = envelope_stats_reference(cur, V)
+        if np.all((counts[:, 0] - 2) + (counts[:, 1] - 2) < 3):
+            break   # stop_emd
+        nsift = 0
+        while not stop_reference(mean, amp, counts, nv, sd, sd2, tol):
+            cur = cur - mean
+            nsift += 1
+            if nsift == S:
+                break
+            mean, amp, counts, nv = envelope_stats_reference(cur, V)
+        out.append(cur)
+        sifts.append(nsift)
+        running = running - cur
+    out.append(running)
+    sc = np.zeros(n_imf, np.int32)
+    sc[:len(sifts)] = sifts
+    return np.stack(out), sc, len(out)
 
 
 def main():
@@ -172,11 +249,40 @@ def main():
     t = np.arange(T)
     y = np.stack([np.sin(2 * np.pi * t / 37.0 + c) + 0.3 * rng.standard_normal(T) for c in range(n)])
     V = directions_closed_form(n, K, [-K] + primes_upto(n - 1))
-    mean, counts = envelope_mean_reference(y, V)
+    mean, amp, counts, _ = envelope_stats_reference(y, V)
     out["pass_y"] = y
     out["pass_V"] = V
     out["pass_mean"] = mean
+    out["pass_amp"] = amp
     out["pass_counts"] = counts
+    # the fixed-count loop of a small epoch
+    y = np.stack([np.sin(2 * np.pi * t / 23.0 + c) + 0.5 * np.sin(2 * np.pi * t / 71.0 + 2 * c)
+                  + 0.2 * rng.standard_normal(T) for c in range(n)])
+    out["fix_y"] = y
+    out["fix_V"] = V
+    out["fix_imfs"] = memd_fix_reference(y, V, 4, 3)
+    # stop-mode decompositions (memd.m's loop)
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "..", "..", "memd-gpu_amd"))
+    import synth
+    cases = [
+        ("c0", lambda: synth.sines_epoch(8, 1000, 2), 16, 12, 50, (0.075, 0.75, 0.075)),   # the 8 x 1000 host case
+        ("uni", lambda: synth.sines_epoch(1, 700, 21), 4, 10, 30, (0.075, 0.75, 0.075)),   # univariate
+        ("end", lambda: synth.sines_epoch(3, 64, 23), 6, 8, 10, (0.075, 0.75, 0.075)),     # ends by stop_emd
+        ("thr", lambda: synth.eeg_epoch(4, 1500, 22), 8, 8, 20, (0.05, 0.5, 0.05)),        # other thresholds
+    ]
+    for name, gen, K, n_imf, S, (sd, sd2, tol) in cases:
+        y = gen()
+        n = y.shape[0]
+        V = directions_closed_form(n, K, [-K] + primes_upto(n - 1))
+        imfs, sc, used = memd_stop_reference(y, V, n_imf, S, sd, sd2, tol)
+        out[f"stop_{name}_y"] = y
+        out[f"stop_{name}_V"] = V
+        out[f"stop_{name}_cfg"] = np.array([n_imf, S, sd, sd2, tol])
+        out[f"stop_{name}_imfs"] = imfs
+        out[f"stop_{name}_sifts"] = sc
+        out[f"stop_{name}_used"] = np.array(used)
+        print(name, "sifts", sc.tolist(), "slots used", used)
     np.savez_compressed(os.path.join(HERE, "golden_small.npz"), **out)
     print("wrote", len(out), "arrays")
 

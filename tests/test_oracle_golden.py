@@ -211,3 +211,147 @@ def test_threaded_oracle_equals_serial(oracle, mode):
         b, ib = oracle.memd(y, V, 3, 2, mode=mode, threads=th)
         assert np.array_equal(ia["knot_counts"], ib["knot_counts"])
         assert np.array_equal(a, b)
+
+
+# ---- memd.m amplitude, the IMF loops and the stop criterion ---------------------
+
+def test_envelope_amplitude_matches_independent_pipeline(oracle, gold):
+    # memd.m's amp(t) = sum over valid directions of |e+(t) - e-(t)|_2 / 2, / nvalid
+    _, amp, _ = oracle.envelope_mean(gold["pass_y"], gold["pass_V"], want_amp=True)
+    ref = gold["pass_amp"]
+    np.testing.assert_allclose(amp, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def test_fix_loop_matches_independent_pipeline(oracle, gold):
+    # memReducedEpochs_dev.cu:1174-1179, 1328-1338: 3 IMFs of 3 sifts + residual
+    imfs, info = oracle.memd(gold["fix_y"], gold["fix_V"], 4, 3)
+    ref = gold["fix_imfs"]
+    assert info["n_imf_out"] == 4 and info["sift_counts"].tolist() == [3, 3, 3, 0]
+    np.testing.assert_allclose(imfs, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+def test_fix_loop_composes_single_sifts(oracle):
+    # the loop against compositions of the (separately pinned) envelope-mean pass,
+    # bit for bit: a sift is cur - mean(cur); the next IMF starts from running - IMF
+    import synth
+    y = synth.sines_epoch(4, 800, 41)
+    V = oracle.directions(4, 8)
+    m = lambda x: oracle.envelope_mean(x, V)[0]
+    one, _ = oracle.memd(y, V, 2, 1)
+    assert np.array_equal(one[0], y - m(y))
+    assert np.array_equal(one[1], y - one[0])        # residual slot
+    two, _ = oracle.memd(y, V, 2, 2)
+    s1 = y - m(y)
+    assert np.array_equal(two[0], s1 - m(s1))         # S = 2: the second sift acts on the first's result
+    three, _ = oracle.memd(y, V, 3, 1)
+    r1 = y - one[0]
+    assert np.array_equal(three[0], one[0])
+    assert np.array_equal(three[1], r1 - m(r1))       # IMF 2 sifts the residual, not the signal
+    assert np.array_equal(three[2], r1 - three[1])
+
+
+@pytest.mark.parametrize("name", ["c0", "uni", "end", "thr"])
+def test_stop_mode_matches_memd_m_restatement(oracle, gold, name):
+    # memd.m's loop (stop_emd, stop(), at most S subtractions per IMF) written as
+    # plain loops over numpy/scipy stages; c0 is the 8 x 1000, K = 16 host case
+    y, V = gold[f"stop_{name}_y"], gold[f"stop_{name}_V"]
+    n_imf, S, sd, sd2, tol = gold[f"stop_{name}_cfg"]
+    imfs, info = oracle.memd(y, V, int(n_imf), int(S), mode=oracle.MODE_STOP, sd=sd, sd2=sd2, tol=tol)
+    used = int(gold[f"stop_{name}_used"])
+    assert info["n_imf_out"] == used
+    assert info["sift_counts"].tolist() == gold[f"stop_{name}_sifts"].tolist()
+    ref = gold[f"stop_{name}_imfs"]
+    for i in range(used):
+        assert np.linalg.norm(imfs[i] - ref[i]) <= 1e-9 * np.linalg.norm(y), f"slot {i}"
+
+
+def _counts(nem_per_dir):
+    # knot counts (endpoints included) giving these interior-extrema totals
+    return np.array([[2 + v // 2, 2 + v - v // 2] for v in nem_per_dir], np.int32)
+
+
+def test_stop_decision_known_answers(oracle):
+    T = 40
+    amp = np.ones(T)
+    mean = np.zeros((2, T))
+    rich = _counts([5, 4])
+    # zero mean: sx = 0 everywhere, stop
+    assert oracle.stop_decision(mean, amp, rich)
+    # one sample above sd2 keeps sifting (any(sx > sd2))
+    m = mean.copy(); m[0, 7] = 0.8
+    assert not oracle.stop_decision(m, amp, rich)
+    # ... unless no direction has more than 2 interior extrema
+    assert oracle.stop_decision(m, amp, _counts([2, 1]))
+    assert not oracle.stop_decision(m, amp, _counts([2, 3]))
+    # fraction above sd: 3 of 40 is exactly tol = 0.075, not above it -> stop;
+    # 4 of 40 is above -> sift
+    m = mean.copy(); m[1, :3] = 0.1
+    assert oracle.stop_decision(m, amp, rich)
+    m[1, 3] = 0.1
+    assert not oracle.stop_decision(m, amp, rich)
+    # sx is compared strictly: a sample exactly at sd2 does not count as above it,
+    # nor do 4 of 40 samples exactly at sd (sqrt(x * x) == x, so sx is exactly 0.075)
+    m = mean.copy(); m[0, 0] = 0.75
+    assert oracle.stop_decision(m, amp, rich)
+    m = mean.copy(); m[0, :4] = 0.075
+    assert oracle.stop_decision(m, amp, rich)
+    m[0, 4] = 0.0750000000000001
+    assert oracle.stop_decision(m, amp, rich)          # 1 of 40 above sd
+    m[0, :4] = 0.0750000000000001
+    assert not oracle.stop_decision(m, amp, rich)      # 5 of 40 above sd
+    # sx is the 2-norm over channels: 0.6 and 0.6 give 0.849 > sd2
+    m = mean.copy(); m[0, 5] = 0.6; m[1, 5] = 0.6
+    assert not oracle.stop_decision(m, amp, rich)
+    # amplitude scaling: |mean| = 0.5 everywhere (above sd at every sample) ...
+    m = np.zeros((1, T)); m[0, :] = 0.5
+    assert not oracle.stop_decision(m, np.ones(T), _counts([4]))
+    # ... divided by amp = 10 it is 0.05 < sd: stop
+    assert oracle.stop_decision(m, np.full(T, 10.0), _counts([4]))
+    # ... but with one amp(t) = 0 memd.m does not divide at all: 0.5 > sd, sift
+    # (dividing anyway would give 0.05 elsewhere and 0/0 at the zero: stop)
+    a = np.full(T, 10.0); a[9] = 0.0
+    m[0, 9] = 0.0
+    assert not oracle.stop_decision(m, a, _counts([4]))
+
+
+def test_stop_mode_with_zero_thresholds_is_fix_mode(oracle):
+    # sd = sd2 = tol = 0: every pass with any nonzero mean sample keeps sifting, so
+    # each IMF takes exactly S subtractions, like fix mode, bit for bit
+    import synth
+    y = synth.sines_epoch(4, 900, 42)
+    V = oracle.directions(4, 8)
+    fix, finfo = oracle.memd(y, V, 4, 3)
+    stop, sinfo = oracle.memd(y, V, 4, 3, mode=oracle.MODE_STOP, sd=0.0, sd2=0.0, tol=0.0)
+    assert sinfo["n_imf_out"] == 4 and sinfo["sift_counts"].tolist() == finfo["sift_counts"].tolist()
+    assert np.array_equal(stop, fix)
+
+
+def test_stop_mode_with_infinite_thresholds_never_sifts(oracle):
+    # sd = sd2 = inf: no sample is ever above a threshold, the first IMF is the
+    # signal itself after 0 sifts, and the zero residual has no extrema left
+    import synth
+    y = synth.sines_epoch(4, 900, 43)
+    V = oracle.directions(4, 8)
+    imfs, info = oracle.memd(y, V, 6, 5, mode=oracle.MODE_STOP, sd=np.inf, sd2=np.inf)
+    assert info["n_imf_out"] == 2
+    assert info["sift_counts"].tolist() == [0] * 6
+    assert np.array_equal(imfs[0], y)
+    assert not imfs[1].any()
+
+
+@pytest.mark.parametrize("periods,slots", [(1.0, 1), (1.6, 2)])
+def test_stop_mode_ends_below_three_interior_extrema(oracle, periods, slots):
+    # memd.m stop_emd: no IMF is started once every direction has fewer than 3
+    # interior extrema.  One channel (every direction is [1]): one sine period has
+    # one maximum and one minimum inside, so the decomposition ends at once and the
+    # only slot is the residual; 1.6 periods have 3 and start an IMF
+    T = 64
+    t = np.arange(T)
+    y = np.sin(2 * np.pi * periods * t / T + 0.3)[None, :]
+    nint = len(oracle.extrema(y[0], 0)) - 2 + len(oracle.extrema(y[0], 1)) - 2
+    assert nint == (2 if periods == 1.0 else 3)
+    V = oracle.directions(1, 4)
+    imfs, info = oracle.memd(y, V, 5, 4, mode=oracle.MODE_STOP)
+    assert info["n_imf_out"] == slots
+    if slots == 1:
+        assert np.array_equal(imfs[0], y)
