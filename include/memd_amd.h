@@ -111,6 +111,25 @@ memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_i
                          const int32_t* d_x, const double* d_y, const double* d_V, double* d_IMFs,
                          const memd_opts* opts, double* elapsed_s);
 
+/*
+ * memd_run_f64 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code running under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a static per-device array of MEMD_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every device it runs on until the process ends; the step is
+ * launched directly (no graph: instantiating one allocates) and no hipMalloc,
+ * hipFree or other allocating call is made.  Shapes whose workspace does not fit
+ * (memd_workspace_bytes_opts plus the plan region, e.g. up to 32 ch x 4096, K =
+ * 64 in fix mode) are refused with MEMD_ERR_UNSUPPORTED.  Results are
+ * bit-identical to memd_run_f64.  Every call on one device shares that device's
+ * array: calls that overlap on one device (any context, any stream) race and
+ * must be serialised by the caller.
+ */
+#define MEMD_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+memd_status memd_run_noalloc_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S,
+                                 const int32_t* d_x, const double* d_y, const double* d_V, double* d_IMFs,
+                                 const memd_opts* opts, double* elapsed_s);
+
 /* Same call with the reference's memd<int,float> argument types
  * (memReducedEpochs_dev.cu:1522); converts to FP64 on the device. */
 memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S,

@@ -20,10 +20,20 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace memd;
+
+// Workspace of the allocation-free entry point (memd_run_noalloc_f64): one
+// array in every device's copy of the code object, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel or a copy (on the
+// host the name is a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress returns for the current device, resolved once per device
+// when the first context on it is created (scratch_address).
+constexpr size_t kScratchBytes = (size_t)MEMD_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) char scratch[kScratchBytes];
 
 struct memd_ctx {
     int device = 0;
@@ -40,9 +50,28 @@ struct memd_ctx {
     int* flag = nullptr;            // device word for the coordinate check
     hipEvent_t pev[kNumStepKernels + 1] = {};   // per-kernel timing events (profiling entry point)
     std::map<int, memd_tuning> tuning;          // tile shapes per channel count (memd_autotune / set_tuning)
+    char* scratch = nullptr;        // this device's copy of `scratch` (allocation-free entry point)
 };
 
 namespace {
+
+// Device address of `scratch` per device, resolved with that device current.
+hipError_t scratch_address(int device, char** out) {
+    static std::mutex m;
+    static std::map<int, char*> table;
+    std::lock_guard<std::mutex> lock(m);
+    auto it = table.find(device);
+    if (it != table.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    table[device] = (char*)p;
+    *out = (char*)p;
+    return hipSuccess;
+}
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -416,9 +445,17 @@ std::string plan_key(const StepPlan& p) {
     return k;
 }
 
+// Device memory a run works in: the context's arena (grown on demand) or a
+// fixed buffer that is never reallocated (the allocation-free entry point; the
+// step is then launched directly, since instantiating a graph allocates).
+struct RunArena {
+    char* fixed = nullptr;   // nullptr: the context's arena
+    size_t fixed_bytes = 0;
+};
+
 // Core: run a set of epochs (all already FP64 and on the device) to completion.
 memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
-                       memd_kernel_profile* prof = nullptr) {
+                       memd_kernel_profile* prof = nullptr, RunArena where = RunArena{}) {
     memd_opts o;
     if (opts) o = *opts; else memd_opts_default(&o);
     if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP)
@@ -455,23 +492,33 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     PlanLayout P;
     memd_status rs = plan_layout(ctx, eds, grp, groups, total, &P);
     if (rs != MEMD_OK) return rs;
-    rs = ensure_arena(ctx, P.total);
-    if (rs != MEMD_OK) return rs;
+    char* arena = nullptr;
+    if (where.fixed) {
+        if (P.total > where.fixed_bytes)
+            return fail(ctx, MEMD_ERR_UNSUPPORTED,
+                        "this call needs %zu bytes of workspace; the allocation-free entry point holds %zu",
+                        P.total, where.fixed_bytes);
+        arena = where.fixed;
+    } else {
+        rs = ensure_arena(ctx, P.total);
+        if (rs != MEMD_OK) return rs;
+        arena = ctx->arena;
+    }
     const size_t stats_at = P.stats_at;
-    int* ndone_dev = (int*)(ctx->arena + P.ndone_at);
+    int* ndone_dev = (int*)(arena + P.ndone_at);
     for (int e = 0; e < count; ++e) {
         EpochDesc& d = eds[e];
-        bind_workspace(d, ctx->arena + base[e], lay[e]);
+        bind_workspace(d, arena + base[e], lay[e]);
         d.d_trace = o.d_knot_counts;
         d.max_passes = o.d_knot_counts ? o.max_passes : 0;
-        d.stats = prof ? (unsigned long long*)(ctx->arena + stats_at) : nullptr;
+        d.stats = prof ? (unsigned long long*)(arena + stats_at) : nullptr;
         d.ndone = ndone_dev;
     }
     hipStream_t s = work_stream(ctx);
-    HIP_TRY(ctx, hipMemsetAsync(ndone_dev, 0, sizeof(int), s));
+    HIP_TRY(ctx, launch_zero_ints(ndone_dev, 1, s));   // a kernel, not a memset node: callers may capture
     StepPlan plan{};
     std::vector<int2> tabs;
-    rs = write_plan(ctx, eds, grp, P, ctx->arena, o.mode, records, tabs, s, &plan);
+    rs = write_plan(ctx, eds, grp, P, arena, o.mode, records, tabs, s, &plan);
     if (rs != MEMD_OK) return rs;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
@@ -485,7 +532,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
 
     if (prof) {
         std::memset(prof, 0, sizeof *prof);
-        HIP_TRY(ctx, hipMemsetAsync(ctx->arena + stats_at, 0, sizeof(unsigned long long) * 4, s));
+        HIP_TRY(ctx, hipMemsetAsync(arena + stats_at, 0, sizeof(unsigned long long) * 4, s));
         HIP_TRY(ctx, hipEventRecord(ctx->pev[0], s));
         HIP_TRY(ctx, launch_init(plan, s));
         HIP_TRY(ctx, hipEventRecord(ctx->pev[1], s));
@@ -519,7 +566,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         }
         prof->steps = steps;
         unsigned long long st4[4];
-        HIP_TRY(ctx, hipMemcpy(st4, ctx->arena + stats_at, sizeof st4, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(st4, arena + stats_at, sizeof st4, hipMemcpyDeviceToHost));
         prof->knots = (int64_t)st4[0];
         prof->knot_channels = (int64_t)st4[1];
         prof->eval_points = (int64_t)st4[2];
@@ -527,7 +574,8 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     } else {
     HIP_TRY(ctx, launch_init(plan, s));
     HIP_TRY(ctx, hipEventRecord(ctx->t0, s));
-    if (o.use_graph && steps > 1) {
+    const bool graph = o.use_graph && steps > 1 && !where.fixed;
+    if (graph) {
         // capture one step on the owned stream.  The graph bakes in every launch
         // argument, all of which come from the plan (descriptor array, block tables,
         // work lists and their counters, grids, LDS sizes): it is replayed only for
@@ -557,7 +605,6 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         }
     }
     {
-        const bool graph = o.use_graph && steps > 1;
         auto launch_steps = [&](long nsteps) -> memd_status {
             if (graph) {
                 HIP_TRY(ctx, hipEventRecord(ctx->ev_in, s));
@@ -653,9 +700,9 @@ memd_status cached_directions(memd_ctx* ctx, int n, int K, const int32_t* P, con
     return MEMD_OK;
 }
 
-struct Scratch {  // FP64 staging for the f32 entry points (separate allocation)
+struct Staging {  // FP64 staging for the f32 entry points and the autotuner (separate allocations)
     double* p = nullptr;
-    ~Scratch() { if (p) (void)hipFree(p); }
+    ~Staging() { if (p) (void)hipFree(p); }
 };
 
 }  // namespace
@@ -717,6 +764,7 @@ memd_status memd_ctx_create(int device, memd_ctx** out) {
     if (he == hipSuccess) he = hipEventCreate(&ctx->t1);
     if (he == hipSuccess) { step = "hipMalloc"; he = hipMalloc((void**)&ctx->flag, sizeof(int)); }
     if (he == hipSuccess) { step = "hipFuncSetAttribute"; he = configure_kernels(); }
+    if (he == hipSuccess) { step = "hipGetSymbolAddress(scratch)"; he = scratch_address(ctx->device, &ctx->scratch); }
     if (he != hipSuccess) {
         memd_ctx_destroy(ctx);
         return fail(nullptr, MEMD_ERR_HIP, "context set-up on device %d: %s failed: %s", device, step,
@@ -806,6 +854,26 @@ memd_status memd_run_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_i
     return run_epochs(ctx, in, opts, elapsed_s);
 }
 
+memd_status memd_run_noalloc_f64(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S,
+                                 const int32_t* d_x, const double* d_y, const double* d_V, double* d_IMFs,
+                                 const memd_opts* opts, double* elapsed_s) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    memd_status rs = validate_shape(ctx, n, T, K, N_imf, S);
+    if (rs != MEMD_OK) return rs;
+    if (!d_y || !d_V || !d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
+    if (!ctx->scratch) return fail(ctx, MEMD_ERR_HIP, "the allocation-free workspace was not resolved on device %d",
+                                   ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    rs = check_coords(ctx, d_x, T);
+    if (rs != MEMD_OK) return rs;
+    std::vector<EpochIn> in{{(int)n, (int)T, (int)K, (int)N_imf, (int)S, d_y, d_V, d_IMFs,
+                             opts ? opts->h_sift_counts : nullptr, opts ? opts->h_n_imf_out : nullptr}};
+    RunArena where;
+    where.fixed = ctx->scratch;
+    where.fixed_bytes = kScratchBytes;
+    return run_epochs(ctx, in, opts, elapsed_s, nullptr, where);
+}
+
 memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_imf, size_t S, const int32_t* d_x,
                          const float* d_y, const float* d_V, float* d_IMFs, const memd_opts* opts, double* elapsed_s) {
     if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
@@ -814,7 +882,7 @@ memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_i
     if (!d_y || !d_V || !d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     size_t ny = n * T, nv = K * n, no = N_imf * n * T;
-    Scratch sc;
+    Staging sc;
     hipError_t e = hipMalloc((void**)&sc.p, (ny + nv + no) * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "FP64 staging allocation failed: %s", hipGetErrorString(e));
     hipStream_t s = work_stream(ctx);
@@ -867,7 +935,7 @@ memd_status memd_mex_compat_f32(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_i
     memd_status rs = mex_common(ctx, n, K, N_imf, S, numel, d_index, d_primes, &T, &dV);
     if (rs != MEMD_OK) return rs;
     size_t ny = (size_t)n * T, no = (size_t)N_imf * n * T;
-    Scratch sc;
+    Staging sc;
     hipError_t e = hipMalloc((void**)&sc.p, (ny + no) * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "FP64 staging allocation failed: %s", hipGetErrorString(e));
     hipStream_t s = work_stream(ctx);
@@ -941,7 +1009,7 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
             hy[c * T + t] = std::sin(0.031 * (double)t + 0.7 * (double)c) + 0.5 * std::sin(0.173 * (double)t + (double)c) +
                             0.25 * std::sin(0.61 * (double)t) + 0.2 * u;
         }
-    Scratch y, V, imfs;
+    Staging y, V, imfs;
     hipError_t e = hipMalloc((void**)&y.p, n * T * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&V.p, K * n * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&imfs.p, count * 2 * n * T * sizeof(double));

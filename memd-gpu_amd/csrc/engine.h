@@ -207,6 +207,8 @@ hipError_t launch_step(const StepPlan& p, hipStream_t s);
 // Same launches with an event recorded after each kernel (ev[0] before the first).
 hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev /*[kNumStepKernels+1]*/);
 hipError_t launch_check_coords(const int32_t* x, int T, int* flag, hipStream_t s);
+// p[0 .. count) = 0 by a kernel (no memset node in a caller's captured graph)
+hipError_t launch_zero_ints(int* p, int count, hipStream_t s);
 hipError_t launch_f32_to_f64(const float* in, double* out, size_t count, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* in, float* out, size_t count, hipStream_t s);
 

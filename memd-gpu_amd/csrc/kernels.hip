@@ -1745,6 +1745,11 @@ __global__ void k_gather_states(const EpochDesc* __restrict__ eps, int count, Ep
 }
 
 
+__global__ void k_zero_ints(int* __restrict__ p, int count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
 __global__ void k_check_coords(const int32_t* __restrict__ x, int T, int* flag) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < T && x[t] != t) atomicOr(flag, 1);
@@ -1915,6 +1920,12 @@ hipError_t configure_kernels() {
 
 hipError_t launch_check_coords(const int32_t* x, int T, int* flag, hipStream_t s) {
     hipLaunchKernelGGL(k_check_coords, dim3((T + 255) / 256), dim3(256), 0, s, x, T, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* p, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_ints, dim3((count + 255) / 256), dim3(256), 0, s, p, count);
     return hipGetLastError();
 }
 

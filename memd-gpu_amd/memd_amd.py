@@ -77,7 +77,7 @@ class MemdError(RuntimeError):
 #: every symbol include/memd_amd.h declares (checked by the CPU test-suite)
 EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_last_error",
            "memd_opts_default", "memd_version", "memd_directions_f64", "memd_directions_f32", "memd_run_f64",
-           "memd_run_f32",
+           "memd_run_f32", "memd_run_noalloc_f64",
            "memd_mex_compat_f32", "memd_mex_compat_f64", "memd_run_batch_f64", "memd_profile_batch_f64",
            "memd_workspace_bytes", "memd_workspace_bytes_opts", "memd_server_create", "memd_server_destroy", "memd_server_submit",
            "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
@@ -87,6 +87,8 @@ EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_l
 _NOT_STATUS = ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes",
                "memd_workspace_bytes_opts", "memd_server_pool_used")
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+#: MEMD_NOALLOC_SCRATCH_BYTES: the per-device workspace of memd_run_noalloc_f64
+NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
 
 _lib = None
 
@@ -125,6 +127,7 @@ def load_library(path=LIB_PATH):
     L.memd_run_f64.argtypes = [_vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, _vp, _vp,
                                ctypes.POINTER(MemdOpts), ctypes.POINTER(ctypes.c_double)]
     L.memd_run_f32.argtypes = L.memd_run_f64.argtypes
+    L.memd_run_noalloc_f64.argtypes = L.memd_run_f64.argtypes
     L.memd_mex_compat_f32.argtypes = [_vp, _i32, _i32, _i32, _i32, _sz, _vp, _vp, _vp, _vp]
     L.memd_mex_compat_f64.argtypes = L.memd_mex_compat_f32.argtypes
     L.memd_run_batch_f64.argtypes = [_vp, ctypes.POINTER(MemdEpoch), _sz, ctypes.POINTER(MemdOpts),
@@ -265,14 +268,19 @@ class MemdContext:
 
     # ---- memd (memReducedEpochs_dev.cu:1074-1368)
     def run(self, y, V, n_imf, S, x=None, out=None, mode=MODE_FIX, window=WINDOW_REFWARP32,
-            sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0, spline=SPLINE_NATURAL):
+            sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0, spline=SPLINE_NATURAL,
+            noalloc=False):
         """Decompose one epoch y [n][T] (float64 or float32 device tensor).
 
         Returns (imfs [n_imf][n][T], info) with info = elapsed_s, sift_counts,
-        n_imf_out and, when trace_passes > 0, knot_counts [passes][K][2]."""
+        n_imf_out and, when trace_passes > 0, knot_counts [passes][K][2].
+        noalloc=True calls memd_run_noalloc_f64 (float64, out given)."""
         import torch
         n, T, K = check_epoch_shapes(y, V, n_imf, S, out, x)
         f32 = y.dtype == torch.float32
+        if noalloc and (f32 or out is None or trace_passes):
+            raise ValueError("the allocation-free entry point takes float64 buffers, a preallocated out "
+                             "and no knot-count trace buffer")
         dt = torch.float32 if f32 else torch.float64
         _require_device(y, dt, "d_y")
         _require_device(V, dt, "d_DirectionVectors")
@@ -293,6 +301,8 @@ class MemdContext:
             o.max_passes = trace_passes
         el = ctypes.c_double(0.0)
         fn = self.lib.memd_run_f32 if f32 else self.lib.memd_run_f64
+        if noalloc:
+            fn = self.lib.memd_run_noalloc_f64
         self._check(fn(self.h, n, T, K, n_imf, S, _ptr(x), _ptr(y), _ptr(V), _ptr(out), ctypes.byref(o),
                        ctypes.byref(el)))
         info = {"elapsed_s": el.value, "sift_counts": np.array(sc[:], np.int32), "n_imf_out": nout.value}
