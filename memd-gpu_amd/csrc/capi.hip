@@ -105,7 +105,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, X, fac, redsys, ysol, coef, sep, sepin, mend, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, X, fac, redsys, ysol, coef, sep, sepin, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -121,14 +121,13 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.pre = take((size_t)K * 2 * W * 4);
     L.E = take((size_t)K * 2 * 4);
     L.X = take((size_t)K * 2 * Ecap * 4);
-    L.fac = take((size_t)K * 2 * 6 * Ecap * 8);
+    L.fac = take((size_t)K * 2 * kFacPlanes * Ecap * 2 * 8);
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     const size_t np = (size_t)channel_stride(n);
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // stop mode's record array only
-    L.sep = take((size_t)K * 2 * np * nb * 8);
+    L.sep = take((size_t)K * 2 * np * (nb + 1) * 8);
     L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
-    L.mend = take((size_t)K * 2 * n * 2 * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
     L.red = take(64);
@@ -233,7 +232,6 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.coef = (double*)(b + L.coef);
     d.sep = (double*)(b + L.sep);
     d.sepin = (double*)(b + L.sepin);
-    d.mend = (double*)(b + L.mend);
     d.mean = (double*)(b + L.mean);
     d.tstat = (double*)(b + L.tstat);
     d.stopred = (int*)(b + L.red);

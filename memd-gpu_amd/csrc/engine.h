@@ -62,7 +62,12 @@ int block_bpw_fit(int n, int want);
 constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;
-enum : int { kFacRd = 0, kFacCp = 1, kFacAl = 2, kFacBe = 3, kFacIh = 4, kFacI3h = 5 };
+// Per knot row u of one (direction, envelope), shared by its channels: three
+// planes of Ecap pairs.  kFacAB: al, be, the phase-3 coefficients of the blocked
+// solve (the block's spike entries; 0 where a block has no left or right spike
+// and at the rows outside blocks: the end rows and the separators); kFacIH:
+// ih = 1/h_u, i3h = 1/(3 h_u) of interval u; kFacRC: rd, cp, the block factors.
+enum : int { kFacAB = 0, kFacIH = 1, kFacRC = 2, kFacPlanes = 3 };
 // Largest channel count: one solver workgroup of 256 lanes holds >= 1 block.
 constexpr int kMaxChannels = 256;
 
@@ -104,17 +109,19 @@ struct EpochDesc {
     int* pre;              // [K][2][W] knots before each word
     int* E;                // [K][2] knot counts
     int* X;                // [K][2][Ecap] knot sample positions
-    double* fac;           // [K][2][6][Ecap] per knot row, shared by channels:
-                           //   rd, cp (block factors), al, be (spikes), ih = 1/h, i3h = 1/(3h)
+    double* fac;           // [K][2][kFacPlanes][Ecap][2] per knot row, shared by channels (kFacAB ..)
     double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
     // Knot-major and separator-major arrays: rows of np = n rounded up to a
     // multiple of 4 words (32-byte aligned 4-channel groups; padding is never read
     // into results)
     double* yend;          // [K][2][2][np] clamped end-knot values Y_0, Y_{E-1} (E >= 6): rows T ..
                            //   T + 4K - 1 of curt; the other knot values are the signal at the knot
-    double* ysol;          // [K][2][Ecap][np] block solutions y at block rows (separator rows unused)
+    double* ysol;          // [K][2][Ecap][np] per knot row the value y^ of M_u = fma(-Z[i1], be_u,
+                           //   fma(-Z[i0], al_u, y^_u)): the block solution at block rows (i0, i1 =
+                           //   b, b + 1 for block b), the separator value at separator rows and 0 (or the
+                           //   not-a-knot end value) at the end rows (i0 = i1 = 0 at both)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
-    double* sep;           // [K][2][nbmax][np] separator values (z, then X_j)
+    double* sep;           // [K][2][nbmax + 1][np] Z: Z[0] = Z[J+1] = 0, Z[j] = separator value j
     double* sepin;         // [K][2][nbmax][3][n] y before, right-hand side, y after each separator
     double* mean;          // [n][T] stop mode: envelope mean
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
@@ -137,7 +144,6 @@ struct EpochDesc {
     int bblocks;
     int etw;                   // k_eval_fused tile: 64 * etw samples x 16 / etw channels (etw in {1, 2, 4})
     int spline;                // 0 natural ends, 1 not-a-knot
-    double* mend;              // [K][2][n][2] not-a-knot: M_0 and M_{E-1} per channel
 };
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its

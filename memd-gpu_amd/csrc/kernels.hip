@@ -150,8 +150,29 @@ __device__ __forceinline__ void sys_row3(int xm, int x0, int xp, int u, int E, i
     }
 }
 
-__device__ __forceinline__ double* fac_row(const EpochDesc& ed, int ke, int field) {
-    return ed.fac + ((size_t)ke * 6 + field) * ed.Ecap;
+// Plane `plane` (kFacAB, kFacIH, kFacRC) of the per-knot pairs of (direction,
+// envelope) ke: the pair of row u at [2 u], [2 u + 1].
+__device__ __forceinline__ double* fac_plane(const EpochDesc& ed, int ke, int plane) {
+    return ed.fac + ((size_t)ke * kFacPlanes + plane) * ed.Ecap * 2;
+}
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// Z of (direction, envelope) ke: Z[j] at [j * np], j = 0 .. J + 1.
+__device__ __forceinline__ double* zsep_base(const EpochDesc& ed, int ke) {
+    return ed.sep + (size_t)ke * (ed.nbmax + 1) * ed.np;
+}
+
+// Knot rows outside the blocks of the blocked solve: the end rows 0 and E-1 and
+// the separators u = 32 j, 1 <= j <= J.  Their M is y^ itself (i0 = i1 = 0).
+__device__ __forceinline__ bool outside_blocks(int u, int E, int J) {
+    return u == 0 || u == E - 1 || (u >= kSolverBlock && (u & (kSolverBlock - 1)) == 0 && (u >> 5) <= J);
+}
+
+// Z indices of knot row u: (b, b + 1) for a row of block b, (0, 0) outside blocks.
+__device__ __forceinline__ int2 zsep_index(int u, int E, int J) {
+    if (outside_blocks(u, E, J)) return make_int2(0, 0);
+    const int b = (u - 1) >> 5;
+    return make_int2(b, b + 1);
 }
 
 __device__ __forceinline__ void zero_counters(const WorkLists& wl) {
@@ -619,16 +640,14 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
         const int E = ed.E[ke];
         const int spline = ed.spline;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
-        const GMut<double> rd = gmut(fac_row(ed, ke, kFacRd));
-        const GMut<double> cp = gmut(fac_row(ed, ke, kFacCp));
-        const GMut<double> al = gmut(fac_row(ed, ke, kFacAl));
-        const GMut<double> be = gmut(fac_row(ed, ke, kFacBe));
-        const GMut<double> ih = gmut(fac_row(ed, ke, kFacIh));
-        const GMut<double> i3h = gmut(fac_row(ed, ke, kFacI3h));
+        const GMut<v2d> AB = gmut(reinterpret_cast<v2d*>(fac_plane(ed, ke, kFacAB)));   // (al, be) of row u
+        const GMut<v2d> IH = gmut(reinterpret_cast<v2d*>(fac_plane(ed, ke, kFacIH)));   // (ih, i3h)
+        const GMut<v2d> RC = gmut(reinterpret_cast<v2d*>(fac_plane(ed, ke, kFacRC)));   // (rd, cp)
         const int J = block_geom(E, 0).J;
-        // reciprocals of every interval 0 .. E-2, all lanes (coalesced), 4 intervals per
-        // lane per round with their loads issued together
-        for (int u0 = lane; u0 < E - 1; u0 += 64 * 4) {
+        // every row: the reciprocals of interval u (u < E-1) and, at the rows outside
+        // the blocks, al = be = 0; all lanes, 4 rows per lane per round with their
+        // loads issued together
+        for (int u0 = lane; u0 < E; u0 += 64 * 4) {
             int xa[4], xb[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
@@ -639,10 +658,15 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int u = u0 + 64 * g;
-                if (u < E - 1) {
-                    const double h = (double)(xb[g] - xa[g]);
-                    ih[u] = 1.0 / h;
-                    i3h[u] = 1.0 / (3.0 * h);
+                if (u < E) {
+                    if (u < E - 1) {
+                        const double h = (double)(xb[g] - xa[g]);
+                        v2d v;
+                        v.x = 1.0 / h;
+                        v.y = 1.0 / (3.0 * h);
+                        IH[u] = v;
+                    }
+                    if (outside_blocks(u, E, J)) AB[u] = v2d{0.0, 0.0};
                 }
             }
         }
@@ -664,8 +688,7 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
                     double den = (i == 0) ? di : fma(-lo, cprev, di);
                     double r = 1.0 / den;
                     cprev = up * r;
-                    rd[p + i] = r;
-                    cp[p + i] = cprev;
+                    RC[p + i] = v2d{r, cprev};
                     cpr[i] = cprev;
                     // left spike A_b^{-1}(lo_p e_first), forward sweep
                     z = (i == 0) ? lo * r : fma(-lo, z, 0.0) * r;
@@ -675,24 +698,15 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
                     if (i == len - 1) zr = up * r;
                 }
             }
-            if (left) {
-                double x = 0.0;
+            // phase-3 coefficients: the spikes, 0 where the block has none (the
+            // first block has no left spike, the last no right spike)
+            double x = 0.0, y = 0.0;
 #pragma unroll
-                for (int i = L - 1; i >= 0; --i) {
-                    if (i < len) {
-                        x = (i == len - 1) ? zl[i] : fma(-cpr[i], x, zl[i]);
-                        al[p + i] = x;
-                    }
-                }
-            }
-            if (right) {
-                double x = 0.0;
-#pragma unroll
-                for (int i = L - 1; i >= 0; --i) {
-                    if (i < len) {
-                        x = (i == len - 1) ? zr : fma(-cpr[i], x, 0.0);
-                        be[p + i] = x;
-                    }
+            for (int i = L - 1; i >= 0; --i) {
+                if (i < len) {
+                    x = (i == len - 1) ? zl[i] : fma(-cpr[i], x, zl[i]);
+                    y = (i == len - 1) ? zr : fma(-cpr[i], y, 0.0);
+                    AB[p + i] = v2d{left ? x : 0.0, right ? y : 0.0};
                 }
             }
         }
@@ -711,9 +725,10 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
                     int xm = X[sp - 1], x0 = X[sp], xp = X[sp + 1];
                     double lo = (double)(x0 - xm), up = (double)(xp - x0);
                     double di = (double)(2 * ((x0 - xm) + (xp - x0)));
-                    s_rlo[lane] = (j >= 2) ? -(lo * al[sp - 1]) : 0.0;
-                    s_rdi[lane] = fma(-up, al[sp + 1], fma(-lo, be[sp - 1], di));
-                    s_rup[lane] = (j <= J - 1) ? -(up * be[sp + 1]) : 0.0;
+                    const v2d abm = AB[sp - 1], abp = AB[sp + 1];   // (al, be) beside the separator
+                    s_rlo[lane] = (j >= 2) ? -(lo * abm.x) : 0.0;
+                    s_rdi[lane] = fma(-up, abp.x, fma(-lo, abm.y, di));
+                    s_rup[lane] = (j <= J - 1) ? -(up * abp.y) : 0.0;
                 }
                 __syncthreads();
                 if (lane == 0) {
@@ -815,15 +830,16 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         // 1a. knot positions of tile rows 0..RW-1, factors of rows p0..qlast, reciprocals of
         //     intervals p0-1..qlast+1 (the last one: the next separator's right-hand side;
         //     clamped where the group ends the system): one round of loads per lane row
-        const GPtr<double> frd = gptr(fac_row(ed, ke, kFacRd)), fcp = gptr(fac_row(ed, ke, kFacCp));
-        const GPtr<double> fih = gptr(fac_row(ed, ke, kFacIh));
+        const GPtr<v2d> RCp = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacRC)));
+        const GPtr<double> IHp = gptr(fac_plane(ed, ke, kFacIH));
         const int spline = ed.spline;
         for (int r = tid; r < RW; r += kBlockThreads) {
             const int u = min(p0 + r, qlast);   // factor row (used when r < R)
             const int xs = X[min(p0 - 1 + r, E - 1)];
-            const double rd = frd[u], cpv = fcp[u];
+            const v2d rc = RCp[u];
+            const double rd = rc.x, cpv = rc.y;
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
-            const double ihv = fih[min(p0 - 1 + r, E - 2)];
+            const double ihv = IHp[2 * (unsigned)min(p0 - 1 + r, E - 2)];
             double lo, di, up;
             sys_row3(xm, x0, xp, u, E, spline, lo, di, up);
             sX[r] = xs;
@@ -858,6 +874,20 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         // rE repeat knot E-1)
         const int rE = E - 1 - (p0 - 1);
         const bool has_left = p0 == 1, has_right = rE <= RW - 1;
+        {   // y^ = 0 at the end rows (natural ends; not-a-knot end values replace them
+            // in k_reduced) and the zero padding Z[0], Z[J+1] of the separator values
+            const GMut<double> Z = gmut(zsep_base(ed, ke));
+            for (int c = tid; c < np; c += kBlockThreads) {
+                if (has_left) {
+                    ys[c] = 0.0;
+                    Z[c] = 0.0;
+                }
+                if (b0 + nb - 1 == J) {
+                    ys[(unsigned)(E - 1) * (unsigned)np + (unsigned)c] = 0.0;
+                    Z[(unsigned)(J + 1) * (unsigned)np + (unsigned)c] = 0.0;
+                }
+            }
+        }
         if (E >= 6 && (has_left || has_right)) {
             for (int c = tid; c < n; c += kBlockThreads) {
                 if (has_left) {
@@ -942,22 +972,21 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
 // values (0 for natural ends, me[0] / me[1] for not-a-knot).
 // row[u * rst]: the channel's block solution at row u; sv[j * sst]: its separator
 // value j.
-__device__ __forceinline__ double knot_M(const double* row, int rst, const double* sv, int sst, const double* al,
-                                         const double* be, int u, int E, int J, const double* me) {
+__device__ __forceinline__ double knot_M(const double* row, int rst, const double* sv, int sst, const double* AB,
+                                         int u, int E, int J) {
     constexpr int L = kSolverBlock;
-    if (u == 0) return me ? me[0] : 0.0;
-    if (u == E - 1) return me ? me[1] : 0.0;
+    if (u == 0 || u == E - 1) return 0.0;
     if (u % L == 0 && u / L <= J) return sv[(u / L) * sst];
     int b = (u - 1) / L;
     double m = row[(size_t)u * rst];
-    if (b >= 1) m = fma(-sv[b * sst], al[u], m);
-    if (b < J) m = fma(-sv[(b + 1) * sst], be[u], m);
+    if (b >= 1) m = fma(-sv[b * sst], AB[2 * (size_t)u], m);
+    if (b < J) m = fma(-sv[(b + 1) * sst], AB[2 * (size_t)u + 1], m);
     return m;
 }
 
 // Phase 2: separator values, one lane per channel, one wave per (direction,
 // envelope) (no division: the reduced factorisation comes from k_factor).
-__global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
+__global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
     const ListCursor cu = list_cursor(wl, kListReduced);
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
@@ -975,10 +1004,11 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
             const int np = ed.np;
             const double* row = ed.ysol + (size_t)ke * ed.Ecap * np + c;   // row[u * np]
             // separator-major arrays: lanes (channels) read and write consecutive words
-            double* sv = ed.sep + (size_t)ke * ed.nbmax * np + c;   // sv[j * np]
+            double* sv = zsep_base(ed, ke) + c;   // sv[j * np] = Z[j]
             const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * n + c;   // sip[(3j + v) * n]
             const GPtr<double> sing = gptr(sip);
             const GMut<double> svg = gmut(sv);
+            const GMut<double> rowg = gmut(const_cast<double*>(row));   // y^ of the separator rows
             const GPtr<double> Rlog = gptr(Rlo), Rrdg = gptr(Rrd), Rcpg = gptr(Rcp);
             const GPtr<int> Xg = gptr(X);
             // forward sweep over the separators, inputs of RC separators loaded together
@@ -1009,8 +1039,10 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                     svg[j * np] = z;
                 }
             }
-            // back substitution, RC separators at a time
+            // back substitution, RC separators at a time; every separator value also
+            // becomes y^ of its knot row
             double x = z;
+            if (J > 0) rowg[(unsigned)(J * L) * (unsigned)np] = x;
             for (int j1 = J - 1; j1 >= 1; j1 -= RC) {
                 double cpj[RC], zj[RC];
 #pragma unroll
@@ -1025,23 +1057,22 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
                     if (j < 1) break;
                     x = fma(-cpj[a], x, zj[a]);
                     svg[j * np] = x;
+                    rowg[(unsigned)(j * L) * (unsigned)np] = x;
                 }
             }
             if (ed.spline) {
-                // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2), read
-                // before k_coef overwrites the rows with M
-                const double* al = fac_row(ed, ke, kFacAl);
-                const double* be = fac_row(ed, ke, kFacBe);
-                const double* ihv = fac_row(ed, ke, kFacIh);
-                double M1 = knot_M(row, np, sv, np, al, be, 1, E, J, nullptr);
-                double M2 = knot_M(row, np, sv, np, al, be, 2, E, J, nullptr);
-                double MN1 = knot_M(row, np, sv, np, al, be, E - 3, E, J, nullptr);
-                double MN = knot_M(row, np, sv, np, al, be, E - 2, E, J, nullptr);
+                // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2): y^ of
+                // the end rows (k_block wrote 0 there)
+                const double* AB = fac_plane(ed, ke, kFacAB);
+                const double* IH = fac_plane(ed, ke, kFacIH);
+                double M1 = knot_M(row, np, sv, np, AB, 1, E, J);
+                double M2 = knot_M(row, np, sv, np, AB, 2, E, J);
+                double MN1 = knot_M(row, np, sv, np, AB, E - 3, E, J);
+                double MN = knot_M(row, np, sv, np, AB, E - 2, E, J);
                 double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
                 double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
-                double* me = ed.mend + ((size_t)ke * n + c) * 2;
-                me[0] = ((h0 + h1) * M1 - h0 * M2) * ihv[1];
-                me[1] = ((a + b) * MN - b * MN1) * ihv[E - 3];
+                rowg[0] = ((h0 + h1) * M1 - h0 * M2) * IH[2 * 1];
+                rowg[(unsigned)(E - 1) * (unsigned)np] = ((a + b) * MN - b * MN1) * IH[2 * (size_t)(E - 3)];
             }
         }
     }
@@ -1050,42 +1081,18 @@ __global__ __launch_bounds__(64) void k_reduced(const EpochDesc* __restrict__ ep
 // Phase 3 + coefficients for the record reader (stop mode's evaluation): the
 // spline record (a, b, c, d) of every interval of every
 // channel (memReducedEpochs_dev.cu:775-793) with the shared reciprocals, from M of
-// its two knots (knot_m_select).  A work item is 256 * kCoefItems (channel,
+// its two knots (knot_m).  A work item is 256 * kCoefItems (channel,
 // interval) pairs of one (direction, envelope); lanes run over consecutive
 // intervals, and the loads of G pairs are issued together.  The fused fix-mode
 // evaluation forms M the same way inside its record build.
 
-// M of knot u of one channel row after phase 2, from values loaded up front:
-// knot_M's operations, selected without branches.
-struct KnotMIn {
-    double y, al, be, s0, s1, ss, me;
-};
-
-__device__ __forceinline__ KnotMIn knot_m_load(const double* row, int rst, const GPtr<double>& al,
-                                                const GPtr<double>& be, const double* sv, int sst, const double* me,
-                                                int u, int E, int J) {
-    const int b = max(u - 1, 0) >> 5;
-    KnotMIn v;
-    v.y = row[(size_t)u * rst];
-    v.al = al[u];
-    v.be = be[u];
-    v.s0 = sv[b * sst];
-    v.s1 = sv[min(b + 1, J) * sst];
-    v.ss = sv[min(u >> 5, J) * sst];
-    v.me = me ? me[u == 0 ? 0 : 1] : 0.0;
-    return v;
-}
-
-__device__ __forceinline__ double knot_m_select(const KnotMIn& v, int u, int E, int J) {
-    constexpr int L = kSolverBlock;
-    const int b = max(u - 1, 0) >> 5;
-    double x = v.y;
-    x = (b >= 1) ? fma(-v.s0, v.al, x) : x;
-    x = (b < J) ? fma(-v.s1, v.be, x) : x;
-    const bool sep = u >= L && (u & (L - 1)) == 0 && (u >> 5) <= J;
-    x = sep ? v.ss : x;
-    x = (u == 0 || u == E - 1) ? v.me : x;
-    return x;
+// M of a knot row after phase 2 from its y^, its phase-3 coefficients al, be
+// and the separator values Z[i0], Z[i1] (zsep_index): the blocked solve's
+// x_u = fma(-X_{b+1}, be_u, fma(-X_b, al_u, y_u)) at block rows, where a missing
+// spike has al or be = 0 and Z[0] = Z[J+1] = +0 (so the term adds -0: x + (-0) = x),
+// and y^ itself at the end and separator rows (both Z = +0, al = be = 0).
+__device__ __forceinline__ double knot_m(double yh, double al, double be, double z0, double z1) {
+    return fma(-z1, be, fma(-z0, al, yh));
 }
 
 __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps, WorkLists wl) {
@@ -1096,12 +1103,12 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
-        const int n = ed.n;
+        const int n = ed.n, np = ed.np;
         const int E = ed.E[ke];
         const int J = (E - 3) / L;
         const int base = it.z * 256 * kCoefItems;
-        const GPtr<double> al = gptr(fac_row(ed, ke, kFacAl));
-        const GPtr<double> be = gptr(fac_row(ed, ke, kFacBe));
+        const GPtr<v2d> AB = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacAB)));
+        const GPtr<v2d> IH = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacIH)));
         const size_t kerow = (size_t)ke * n;
         // idx / D as (idx * ceil(2^40 / D)) >> 40: exact for idx < n * D when n * D^2 < 2^40
         // (the error term idx * (magic * D - 2^40) stays below 2^40); otherwise divide
@@ -1116,11 +1123,12 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const int nI = E - 1;
         const int items = n * nI;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
-        const GPtr<double> ihv = gptr(fac_row(ed, ke, kFacIh));
-        const GPtr<double> i3hv = gptr(fac_row(ed, ke, kFacI3h));
+        const GPtr<double> Yh = gptr(ed.ysol + (size_t)ke * ed.Ecap * np);
+        const GPtr<double> Z = gptr(zsep_base(ed, ke));
         double4* out = reinterpret_cast<double4*>(ed.coef);
         for (int r0 = 0; r0 < kCoefItems; r0 += G) {
-            KnotMIn v0[G], v1[G];
+            // y^, al, be and Z[i0], Z[i1] of both knots; knot values, positions, reciprocals
+            double h0[G], hk[G], a0[G], e0[G], a1[G], e1[G], z00[G], z01[G], z10[G], z11[G];
             double y0[G], y1[G], ih[G], i3h[G];
             int cs[G], js[G], x0[G], x1[G];
 #pragma unroll
@@ -1130,29 +1138,36 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 const int j = idx - c * nI;
                 cs[a] = c;
                 js[a] = j;
-                const size_t rr = kerow + c;
-                const double* row = ed.ysol + (size_t)ke * ed.Ecap * ed.np + c;   // row[u * np]
-                const double* sv = ed.sep + (size_t)ke * ed.nbmax * ed.np + cs[a];   // sv[j * np]
-                const double* me = ed.spline ? ed.mend + rr * 2 : nullptr;
-                v0[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j, E, J);
-                v1[a] = knot_m_load(row, ed.np, al, be, sv, ed.np, me, j + 1, E, J);
+                const int2 zj = zsep_index(j, E, J), zk = zsep_index(j + 1, E, J);
+                h0[a] = Yh[(unsigned)j * (unsigned)np + (unsigned)c];
+                hk[a] = Yh[(unsigned)(j + 1) * (unsigned)np + (unsigned)c];
+                const v2d ab0 = AB[j], ab1 = AB[j + 1];
+                a0[a] = ab0.x;
+                e0[a] = ab0.y;
+                a1[a] = ab1.x;
+                e1[a] = ab1.y;
+                z00[a] = Z[(unsigned)zj.x * (unsigned)np + (unsigned)c];
+                z01[a] = Z[(unsigned)zj.y * (unsigned)np + (unsigned)c];
+                z10[a] = Z[(unsigned)zk.x * (unsigned)np + (unsigned)c];
+                z11[a] = Z[(unsigned)zk.y * (unsigned)np + (unsigned)c];
                 // knot values: the signal at the knot, or the clamped end values (rows
                 // T + 2 ke and T + 2 ke + 1 of the same array)
-                const int r0 = (E >= 6 && j == 0) ? ed.T + 2 * ke : X[j];
-                const int r1 = (E >= 6 && j + 1 == E - 1) ? ed.T + 2 * ke + 1 : X[j + 1];
-                y0[a] = ed.curt[(size_t)r0 * ed.np + c];
-                y1[a] = ed.curt[(size_t)r1 * ed.np + c];
+                const int r0w = (E >= 6 && j == 0) ? ed.T + 2 * ke : X[j];
+                const int r1w = (E >= 6 && j + 1 == E - 1) ? ed.T + 2 * ke + 1 : X[j + 1];
+                y0[a] = ed.curt[(size_t)r0w * np + c];
+                y1[a] = ed.curt[(size_t)r1w * np + c];
                 x0[a] = X[j];
                 x1[a] = X[j + 1];
-                ih[a] = ihv[j];
-                i3h[a] = i3hv[j];
+                const v2d ihp = IH[j];
+                ih[a] = ihp.x;
+                i3h[a] = ihp.y;
             }
 #pragma unroll
             for (int a = 0; a < G; ++a) {
                 const int idx = base + (r0 + a) * 256 + (int)threadIdx.x;
                 if (idx >= items) continue;
-                const double m0 = knot_m_select(v0[a], js[a], E, J);
-                const double m1 = knot_m_select(v1[a], js[a] + 1, E, J);
+                const double m0 = knot_m(h0[a], a0[a], e0[a], z00[a], z01[a]);
+                const double m1 = knot_m(hk[a], a1[a], e1[a], z10[a], z11[a]);
                 const double h = (double)(x1[a] - x0[a]);
                 const double sl = (y1[a] - y0[a]) * ih[a];
                 double4 r;
@@ -1195,35 +1210,48 @@ __device__ __forceinline__ void row_scan16(int& x) {
 }
 
 // Phase B of one direction for a lane's 4 consecutive samples: both envelopes
-// (upper u, lower l; interval iu / il of the first sample) at each sample, the
-// mean of the two added to the accumulator.  Each sample uses the record of
-// X_j < t <= X_{j+1}.  The first sample is in range and lies in intervals iu and
-// il (its caller counts the knots below it).  The knots are exactly the set flag
-// bits, so sample t enters the next interval, whose left knot is t - 1, when bit
-// t - 1 is set: bit s - 1 of `cu` / `cl` is the flag of sample ts + s - 1 (the
+// (upper u, lower l) at each sample, the mean of the two added to the
+// accumulator.  Each sample uses the record of X_j < t <= X_{j+1}.  On entry ru /
+// rl point at the records of the first sample's intervals and xu / xl at their
+// left knots' positions (all four loads are issued together).  The knots are exactly the set flag bits,
+// so sample t enters the next interval, whose left knot is t - 1 (offset 1), when
+// bit t - 1 is set: bit s - 1 of `cu` / `cl` is the flag of sample ts + s - 1 (the
 // caller clears the flag of sample 0, the left knot of the first interval).
-// Both envelopes' crossing loads of a sample are issued before either is used.
-__device__ __forceinline__ void eval_pair(const double4* __restrict__ ru, const int* __restrict__ xu, int iu,
-                                          unsigned cu, const double4* __restrict__ rl, const int* __restrict__ xl,
-                                          int il, unsigned cl, int ts, int tlast, double (&acc)[4]) {
-    int ju = xu[iu], jl = xl[il];
-    double4 qu = ru[iu], ql = rl[il];
-    acc[0] = acc[0] + (horner4(qu, (double)(ts - ju)) + horner4(ql, (double)(ts - jl))) * 0.5;
+// Offsets are integers below 2^53, so the running offset tau + 1 is the same
+// double as (double)(t - X_j).
+__device__ __forceinline__ double4 rec_load(const v2d* ab, const v2d* cd) {
+    const v2d x = ab[0], y = cd[0];
+    double4 r;
+    r.x = x.x; r.y = x.y; r.z = y.x; r.w = y.y;
+    return r;
+}
+
+__device__ __forceinline__ void eval_pair(const v2d* __restrict__ rua, const v2d* __restrict__ ruc,
+                                          const int* __restrict__ xu, unsigned cu, const v2d* __restrict__ rla,
+                                          const v2d* __restrict__ rlc, const int* __restrict__ xl, unsigned cl,
+                                          int ts, int tlast, double (&acc)[4]) {
+    const int ju = xu[0], jl = xl[0];
+    double4 qu = rec_load(rua, ruc), ql = rec_load(rla, rlc);
+    double tu = (double)(ts - ju), tl = (double)(ts - jl);
+    acc[0] = acc[0] + (horner4(qu, tu) + horner4(ql, tl)) * 0.5;
 #pragma unroll
     for (int s = 1; s < 4; ++s) {
-        const int t = ts + s;
-        if (t <= tlast) {
+        if (ts + s <= tlast) {
+            tu = tu + 1.0;
+            tl = tl + 1.0;
             if ((cu >> (s - 1)) & 1u) {
-                ++iu;
-                ju = t - 1;
-                qu = ru[iu];
+                ++rua;
+                ++ruc;
+                qu = rec_load(rua, ruc);
+                tu = 1.0;
             }
             if ((cl >> (s - 1)) & 1u) {
-                ++il;
-                jl = t - 1;
-                ql = rl[il];
+                ++rla;
+                ++rlc;
+                ql = rec_load(rla, rlc);
+                tl = 1.0;
             }
-            acc[s] = acc[s] + (horner4(qu, (double)(t - ju)) + horner4(ql, (double)(t - jl))) * 0.5;
+            acc[s] = acc[s] + (horner4(qu, tu) + horner4(ql, tl)) * 0.5;
         }
     }
 }
@@ -1238,15 +1266,19 @@ __device__ __forceinline__ void eval_pair(const double4* __restrict__ ru, const 
 //     close the batch;
 //   phase A: every (direction, envelope, interval, quad of <= 4 channels) item of the batch
 //     builds the spline record (a,b,c,d) of its interval in LDS from M at its two
-//     knots (formed from the block solutions, spikes and separator values as
-//     knot_m_select does), the knot values and the shared reciprocals -- the same
-//     operations as the record path, so the records are bit-identical;
+//     knots (knot_m: y^ and the phase-3 coefficients of the knot row, the separator
+//     values), the knot values and the shared reciprocals -- the same operations
+//     as the record path, so the records are bit-identical;
 //   phase B: each lane evaluates both envelopes of every valid direction of the
 //     batch at its 4 samples and adds their mean in ascending direction order.
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
-    __shared__ double4 rec[kEvalCap];    // segment s: [channel][stride] from its record base
+    // spline records, segment s: [channel][stride] from its record base, as two
+    // arrays of 16-byte halves (a, b) and (c, d): a wave's record reads and writes
+    // are then 16 bytes apart, not every other 16 bytes of 32-byte records
+    __shared__ v2d recAB[kEvalCap];
+    __shared__ v2d recCD[kEvalCap];
     __shared__ int xk[kEvalXkCap];       // segment s: knot positions jlo .. jlo + nint
     __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
     __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
@@ -1265,12 +1297,11 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const double* curt = ed.curt;   // [T + 4K][np]: the signal sample-major, then the clamped end values
     // descriptor fields of the record build, read once: the descriptor is reached
     // through a generic pointer, so every LDS store would otherwise force a reload
-    const int Ecap = ed.Ecap, np = ed.np, nbmax = ed.nbmax, spline = ed.spline;
+    const int Ecap = ed.Ecap, np = ed.np, nbmax = ed.nbmax;
     const double* const ysol = ed.ysol;
-    const double* const sepv = ed.sep;
+    const double* const zsep = ed.sep;
     const double* const fac = ed.fac;
     const int* const Xall = ed.X;
-    const double* const mend = ed.mend;
     const int TW = ed.etw;
     const int lgq = 4 + (TW >> 1);       // log2 quads per channel: 4, 5, 6 for TW = 1, 2, 4
     const int CG = 16 / TW;
@@ -1282,7 +1313,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int t0 = 64 * w0;
     const int tlast = min(t0 + 64 * TW - 1, T - 1);
     const int tid = threadIdx.x;
-    const int wbase = tid & ~63;
+    const int wbase = uniform(tid & ~63);
     const int cl = tid >> lgq;
     const int q = tid & ((1 << lgq) - 1);
     const int c = cg * CG + cl;
@@ -1384,17 +1415,19 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         for (int s = 0; s <= NS; ++s) ip[s] = uniform(sIp[s]);
         const int total = ip[NS];
         // phase A: items (interval, quad of <= 4 channels), quad fastest.  The
-        // interval's positions, spikes and reciprocals are loaded once for its
-        // channels; the channel rows of the knot-major arrays are adjacent words.
+        // interval's positions, per-row records and reciprocals are loaded once for
+        // its channels; the channel rows of the knot-major arrays are adjacent words.
         for (int a0 = 0; wbase + 256 * a0 < total; ++a0) {   // wave-uniform bound
             const int f = tid + 256 * a0;
-            double v0[4], v1[4], m0[4], m1[4], s0v[4], s1v[4], s2v[4];
-            double a0v = 0.0, a1v = 0.0, b0v = 0.0, b1v = 0.0, ih = 0.0, i3h = 0.0;
+            double v0[4], v1[4], y0[4], y1[4], z00[4], z01[4], z10[4], z11[4];
+            double al0 = 0.0, be0 = 0.0, al1 = 0.0, be1 = 0.0, ih = 0.0, i3h = 0.0;
             int x0 = 0, x1 = 0, seg = -1, j = 0, qd = 0, nch = 0;
             if (f < total) {
+                // the item's segment: the last s with ip[s] <= f (ip in scalar registers:
+                // a fully unrolled count keeps them there)
                 seg = 0;
 #pragma unroll
-                for (int s = 1; s < NS; ++s) seg += (f >= ip[s]) ? 1 : 0;
+                for (int s2 = 1; s2 < NS; ++s2) seg += (f >= ip[s2]) ? 1 : 0;
                 const int4 A = sSegA[seg];
                 const int E = sSegB[seg].z;
                 const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
@@ -1405,31 +1438,28 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int c0 = cg * CG + 4 * qd;
                 nch = min(4, cgn - 4 * qd);
                 const int J = (E - 3) / kSolverBlock;
-                const int bj = max(j - 1, 0) >> 5;
                 // 4 channels from c0 (a multiple of 4) in rows of np (a multiple of 4)
                 // words: 32-byte aligned; channels past the group are padding or the
                 // next group's and are not used
-                const size_t rb = (size_t)ke * Ecap + j;
-                const GPtr<v4d> Mr = gptr(reinterpret_cast<const v4d*>(ysol + rb * np + c0));
-                const double* Sb = sepv + (size_t)ke * nbmax * np + c0;
-                const GPtr<v4d> S0 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)bj * np));
-                const GPtr<v4d> S1 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 1, J) * np));
-                const GPtr<v4d> S2 = gptr(reinterpret_cast<const v4d*>(Sb + (size_t)min(bj + 2, J) * np));
-                const GPtr<double> F = gptr(fac + (size_t)ke * 6 * Ecap);
-                const GPtr<int> Xr = gptr(Xall + (size_t)ke * Ecap);
+                const size_t kE = (size_t)ke * Ecap;
+                const GPtr<v4d> Yr = gptr(reinterpret_cast<const v4d*>(ysol + (kE + j) * np + c0));
+                const double* Zb = zsep + (size_t)ke * (nbmax + 1) * np + c0;
+                const int2 zj = zsep_index(j, E, J), zk = zsep_index(j + 1, E, J);
+                const double* Fb = fac + (size_t)ke * kFacPlanes * Ecap * 2;   // planes of (direction, envelope) ke
+                const GPtr<int> Xr = gptr(Xall + kE);
                 // the knot positions first: the knot-value loads below depend on them,
                 // and the other loads are issued while they are in flight
                 x0 = Xr[j];
                 x1 = Xr[j + 1];
-                a0v = F[kFacAl * Ecap + j];
-                a1v = F[kFacAl * Ecap + j + 1];
-                b0v = F[kFacBe * Ecap + j];
-                b1v = F[kFacBe * Ecap + j + 1];
-                ih = F[kFacIh * Ecap + j];
-                i3h = F[kFacI3h * Ecap + j];
+                const GPtr<v2d> ABr = gptr(reinterpret_cast<const v2d*>(Fb + (size_t)kFacAB * Ecap * 2));
+                const GPtr<v2d> IHr = gptr(reinterpret_cast<const v2d*>(Fb + (size_t)kFacIH * Ecap * 2));
+                const v2d ab0 = ABr[j], ab1 = ABr[j + 1], ihp = IHr[j];
                 const int nr = np >> 2;   // double4 per row
-                const v4d mv0 = Mr[0], mv1 = Mr[nr];
-                const v4d sv0 = S0[0], sv1 = S1[0], sv2 = S2[0];
+                const v4d yh0 = Yr[0], yh1 = Yr[nr];
+                const v4d za = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zj.x * np));
+                const v4d zb = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zj.y * np));
+                const v4d zc = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zk.x * np));
+                const v4d zd = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zk.y * np));
                 __builtin_amdgcn_sched_barrier(0);   // keep the independent loads above
                 // knot values: the signal at the two knots (sample-major rows), or the
                 // clamped end values (E >= 6), rows T + 2 ke and T + 2 ke + 1 of the same
@@ -1438,48 +1468,35 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int r1 = (E >= 6 && j + 1 == E - 1) ? T + 2 * ke + 1 : x1;
                 const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r0 * np + c0));
                 const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r1 * np + c0));
+                al0 = ab0.x; be0 = ab0.y; ih = ihp.x; i3h = ihp.y;
+                al1 = ab1.x; be1 = ab1.y;
                 v0[0] = yv0.x; v0[1] = yv0.y; v0[2] = yv0.z; v0[3] = yv0.w;
                 v1[0] = yv1.x; v1[1] = yv1.y; v1[2] = yv1.z; v1[3] = yv1.w;
-                m0[0] = mv0.x; m0[1] = mv0.y; m0[2] = mv0.z; m0[3] = mv0.w;
-                m1[0] = mv1.x; m1[1] = mv1.y; m1[2] = mv1.z; m1[3] = mv1.w;
-                s0v[0] = sv0.x; s0v[1] = sv0.y; s0v[2] = sv0.z; s0v[3] = sv0.w;
-                s1v[0] = sv1.x; s1v[1] = sv1.y; s1v[2] = sv1.z; s1v[3] = sv1.w;
-                s2v[0] = sv2.x; s2v[1] = sv2.y; s2v[2] = sv2.z; s2v[3] = sv2.w;
+                y0[0] = yh0.x; y0[1] = yh0.y; y0[2] = yh0.z; y0[3] = yh0.w;
+                y1[0] = yh1.x; y1[1] = yh1.y; y1[2] = yh1.z; y1[3] = yh1.w;
+                z00[0] = za.x; z00[1] = za.y; z00[2] = za.z; z00[3] = za.w;
+                z01[0] = zb.x; z01[1] = zb.y; z01[2] = zb.z; z01[3] = zb.w;
+                z10[0] = zc.x; z10[1] = zc.y; z10[2] = zc.z; z10[3] = zc.w;
+                z11[0] = zd.x; z11[1] = zd.y; z11[2] = zd.z; z11[3] = zd.w;
             }
             if (seg >= 0) {
                 const int4 A = sSegA[seg];
                 const int4 B = sSegB[seg];
-                const int E = B.z, J = (E - 3) / kSolverBlock;
                 const int i = j - A.w;
                 if (qd == 0) {   // first quad of the group: knot positions
                     xk[B.w + i] = x0;
                     if (i == A.z - 1) xk[B.w + A.z] = x1;
                 }
-                // M_j and M_{j+1}: knot_m_select's operations
-                const int bj = max(j - 1, 0) >> 5, bn = j >> 5;   // blocks of j and j+1
-                const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
                 const double h = (double)(x1 - x0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (q < nch) {
-                        double me0 = 0.0, me1 = 0.0;
-                        if (spline) {
-                            const GPtr<double> Me = gptr(mend + ((size_t)ke * n + cg * CG + 4 * qd + q) * 2);
-                            me0 = Me[0];
-                            me1 = Me[1];
-                        }
-                        const KnotMIn in0{m0[q], a0v, b0v, s0v[q], s1v[q], s1v[q], me0};
-                        const KnotMIn in1{m1[q], a1v, b1v, bn == bj ? s0v[q] : s1v[q],
-                                          bn == bj ? s1v[q] : s2v[q], bn == bj ? s1v[q] : s2v[q], me1};
-                        const double mj = knot_m_select(in0, j, E, J);
-                        const double mk = knot_m_select(in1, j + 1, E, J);
-                        const double slp = (v1[q] - v0[q]) * ih;
-                        double4 r;
-                        r.x = v0[q];
-                        r.y = slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0);
-                        r.z = mj;
-                        r.w = (mk - mj) * i3h;
-                        rec[B.x + (4 * qd + q) * B.y + i] = r;
+                for (int q2 = 0; q2 < 4; ++q2) {
+                    if (q2 < nch) {
+                        const double mj = knot_m(y0[q2], al0, be0, z00[q2], z01[q2]);
+                        const double mk = knot_m(y1[q2], al1, be1, z10[q2], z11[q2]);
+                        const double slp = (v1[q2] - v0[q2]) * ih;
+                        const int at = B.x + (4 * qd + q2) * B.y + i;
+                        recAB[at] = v2d{v0[q2], slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0)};
+                        recCD[at] = v2d{mj, (mk - mj) * i3h};
                     }
                 }
             }
@@ -1505,8 +1522,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const unsigned long long h0 = m0 >> qsh, h1 = m1 >> qsh;
                 const int i0 = c0 - (int)__popcll(h0), i1 = c1 - (int)__popcll(h1);
                 const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
-                eval_pair(rec + B0.x + __umul24(cl, B0.y), xk + B0.w, i0, x0, rec + B1.x + __umul24(cl, B1.y),
-                          xk + B1.w, i1, x1, ts, tlast, acc);
+                const int r0 = B0.x + __umul24(cl, B0.y) + i0, r1 = B1.x + __umul24(cl, B1.y) + i1;
+                eval_pair(recAB + r0, recCD + r0, xk + B0.w + i0, x0, recAB + r1, recCD + r1, xk + B1.w + i1, x1, ts,
+                          tlast, acc);
             }
         }
         __syncthreads();
