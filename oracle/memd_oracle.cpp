@@ -52,6 +52,18 @@
  * restatement of memd.m's loop (tests/golden/make_golden.py), known-answer cases
  * at every threshold and the limit cases sd = sd2 = tol = 0 (= fix mode) and
  * sd = sd2 = inf (no sift).
+ *
+ * Mirrored ends (option, not in the reference) follow memd.m's
+ * boundary_conditions with nbsym = 2 (mirror_ends below): on the reference's
+ * own interior extrema, with the reference's direction validity (both lists with
+ * at least two interior extrema), no clamp, and the spline through the mirrored
+ * knots (positions may lie left of 0 or right of T-1; a knot carries the signal
+ * at its source sample).  Knot counts reported per pass stay the clamp lists'
+ * (interior extrema + 2), so stop mode's nem is unchanged.  Pinned by a 1-based
+ * MATLAB-style transcription of boundary_conditions knot by knot over every
+ * reachable branch, scipy CubicSplines through the mirrored knots (one pass,
+ * the fixed-count loop, and memd.m's own stop / mirror / not-a-knot setting),
+ * the reflection invariants, and flat envelopes for level extrema.
  */
 #include "memd_oracle.h"
 
@@ -306,6 +318,111 @@ int extrema(int T, const double* P, int env, int window, int32_t* X) {
     return E;
 }
 
+/* memd.m boundary_conditions (Rehman & Mandic's MEMD toolbox, after Rilling,
+ * Flandrin & Goncalves' EMD boundary handling), nbsym = 2, in 0-based samples
+ * (t(i) = i, so the first sample is 0 and the last T-1).  imax[0..a-1] and
+ * imin[0..b-1] are the interior extrema (a, b >= 2), x the projection.  Per end:
+ * up to two extrema of each kind, nearest the end, are reflected about the end
+ * extremum (when the end sample lies beyond the first extremum of the other
+ * kind) or about the end sample, which then joins one list as a knot; when a
+ * reflection about an extremum does not reach the end sample the reflection
+ * point becomes the end sample.  The output lists hold the sources (samples)
+ * of the added knots in position order (position = 2 * sym - source). */
+struct MirrorEnds {
+    int nl[2], nr[2];        // knots added at the left / right end, per envelope (0 max, 1 min)
+    int ls[2][3], rs[2][3];  // their sources, in position order
+    int lsym, rsym;
+};
+
+int take_reversed(const int32_t* v, int lo, int hi, int* out) {   // v[max(lo,0) .. hi-1], last first
+    int m = 0;
+    for (int i = hi - 1; i >= std::max(lo, 0); --i) out[m++] = v[i];
+    return m;
+}
+
+void mirror_ends(int T, const double* x, const int32_t* imax, int a, const int32_t* imin, int b, MirrorEnds& r) {
+    const int last = T - 1;
+    int* lmax = r.ls[0]; int* lmin = r.ls[1];
+    int* rmax = r.rs[0]; int* rmin = r.rs[1];
+    int& nlmax = r.nl[0]; int& nlmin = r.nl[1];
+    int& nrmax = r.nr[0]; int& nrmin = r.nr[1];
+    if (imax[0] < imin[0]) {
+        if (x[0] > x[imin[0]]) {
+            nlmax = take_reversed(imax, 1, std::min(a, 3), lmax);
+            nlmin = take_reversed(imin, 0, std::min(b, 2), lmin);
+            r.lsym = imax[0];
+        } else {
+            nlmax = take_reversed(imax, 0, std::min(a, 2), lmax);
+            nlmin = take_reversed(imin, 0, std::min(b, 1), lmin);
+            lmin[nlmin++] = 0;
+            r.lsym = 0;
+        }
+    } else {
+        if (x[0] < x[imax[0]]) {
+            nlmax = take_reversed(imax, 0, std::min(a, 2), lmax);
+            nlmin = take_reversed(imin, 1, std::min(b, 3), lmin);
+            r.lsym = imin[0];
+        } else {
+            nlmax = take_reversed(imax, 0, std::min(a, 1), lmax);
+            lmax[nlmax++] = 0;
+            nlmin = take_reversed(imin, 0, std::min(b, 2), lmin);
+            r.lsym = 0;
+        }
+    }
+    // the first mirrored knot of either list must not lie right of sample 0
+    if (2 * r.lsym - lmin[0] > 0 || 2 * r.lsym - lmax[0] > 0) {
+        if (r.lsym == imax[0]) nlmax = take_reversed(imax, 0, std::min(a, 2), lmax);
+        else nlmin = take_reversed(imin, 0, std::min(b, 2), lmin);
+        r.lsym = 0;
+    }
+    if (imax[a - 1] < imin[b - 1]) {
+        if (x[last] < x[imax[a - 1]]) {
+            nrmax = take_reversed(imax, a - 2, a, rmax);
+            nrmin = take_reversed(imin, b - 3, b - 1, rmin);
+            r.rsym = imin[b - 1];
+        } else {
+            rmax[0] = last;
+            nrmax = 1 + take_reversed(imax, a - 1, a, rmax + 1);
+            nrmin = take_reversed(imin, b - 2, b, rmin);
+            r.rsym = last;
+        }
+    } else {
+        if (x[last] > x[imin[b - 1]]) {
+            nrmax = take_reversed(imax, a - 3, a - 1, rmax);
+            nrmin = take_reversed(imin, b - 2, b, rmin);
+            r.rsym = imax[a - 1];
+        } else {
+            nrmax = take_reversed(imax, a - 2, a, rmax);
+            rmin[0] = last;
+            nrmin = 1 + take_reversed(imin, b - 1, b, rmin + 1);
+            r.rsym = last;
+        }
+    }
+    // the last mirrored knot of either list must not lie left of sample T-1
+    if (2 * r.rsym - rmin[nrmin - 1] < last || 2 * r.rsym - rmax[nrmax - 1] < last) {
+        if (r.rsym == imax[a - 1]) nrmax = take_reversed(imax, a - 2, a, rmax);
+        else nrmin = take_reversed(imin, b - 2, b, rmin);
+        r.rsym = last;
+    }
+}
+
+/* Mirrored knot lists of one valid direction from the clamp-mode lists X[e]
+ * (E[e] >= 4, ends included): positions P[e] and sources S[e]; returns E'. */
+void mirror_lists(int T, const double* Pk, const int32_t* const X[2], const int E[2], int32_t* const P[2],
+                  int32_t* const S[2], int Em[2], int sym[2]) {
+    MirrorEnds m;
+    mirror_ends(T, Pk, X[0] + 1, E[0] - 2, X[1] + 1, E[1] - 2, m);
+    sym[0] = m.lsym;
+    sym[1] = m.rsym;
+    for (int e = 0; e < 2; ++e) {
+        int k = 0;
+        for (int i = 0; i < m.nl[e]; ++i, ++k) { S[e][k] = m.ls[e][i]; P[e][k] = 2 * m.lsym - m.ls[e][i]; }
+        for (int i = 1; i + 1 < E[e]; ++i, ++k) { S[e][k] = X[e][i]; P[e][k] = X[e][i]; }
+        for (int i = 0; i < m.nr[e]; ++i, ++k) { S[e][k] = m.rs[e][i]; P[e][k] = 2 * m.rsym - m.rs[e][i]; }
+        Em[e] = k;
+    }
+}
+
 struct Workspace {
     std::vector<double> P;
     std::vector<int32_t> X[2];
@@ -314,12 +431,14 @@ struct Workspace {
 
 /* Envelopes of cur for all directions; accumulate mean into sum (ascending k),
  * optional memd.m amplitude.  nthreads>1 parallelises over channels / samples. */
-void envelope_pass(int n, int T, int K, const double* cur, const double* V, int window, int spline,
+void envelope_pass(int n, int T, int K, const double* cur, const double* V, int window, int spline, int boundary,
                    bool want_amp, int nthreads, double* sum, double* ampsum, double* nvalid_out,
                    int32_t* counts) {
     std::vector<double> Pk(T);
-    std::vector<int32_t> X[2];
+    std::vector<int32_t> X[2], MP[2], MS[2];
     X[0].resize(T); X[1].resize(T);
+    const bool mirror = boundary == ORACLE_BOUNDARY_MIRROR;
+    if (mirror) for (int e = 0; e < 2; ++e) { MP[e].resize(T + 4); MS[e].resize(T + 4); }
     KnotSystem sys[2];
     std::vector<double> diff;
     if (want_amp) diff.assign((size_t)n * T, 0.0);
@@ -341,25 +460,38 @@ void envelope_pass(int n, int T, int K, const double* cur, const double* V, int 
         bool valid = (E[0] > 3) && (E[1] > 3);   // memReducedEpochs_dev.cu:840,857
         if (!valid) continue;
         nvalid = nvalid + 1.0;
-        for (int e = 0; e < 2; ++e) sys[e].build(E[e], X[e].data(), spline);
+        // knot positions (Xp) and the samples whose values they carry (Xs)
+        const int32_t* Xp[2] = {X[0].data(), X[1].data()};
+        const int32_t* Xs[2] = {X[0].data(), X[1].data()};
+        int Ek[2] = {E[0], E[1]};
+        if (mirror) {
+            const int32_t* Xr[2] = {X[0].data(), X[1].data()};
+            int32_t* P2[2] = {MP[0].data(), MP[1].data()};
+            int32_t* S2[2] = {MS[0].data(), MS[1].data()};
+            int sym[2];
+            mirror_lists(T, Pk.data(), Xr, E, P2, S2, Ek, sym);
+            for (int e = 0; e < 2; ++e) { Xp[e] = P2[e]; Xs[e] = S2[e]; }
+        }
+        for (int e = 0; e < 2; ++e) sys[e].build(Ek[e], Xp[e], spline);
 #pragma omp parallel num_threads(nthreads) if (nthreads > 1)
         {
             std::vector<double> Y[2], coef[2], sl, r, M, env[2];
             for (int e = 0; e < 2; ++e) {
-                Y[e].resize(E[e]); coef[e].resize(4 * (size_t)std::max(E[e] - 1, 1)); env[e].resize(T);
+                Y[e].resize(Ek[e]); coef[e].resize(4 * (size_t)std::max(Ek[e] - 1, 1)); env[e].resize(T);
             }
 #pragma omp for schedule(static)
             for (int c = 0; c < n; ++c) {
                 const double* cc = cur + (size_t)c * T;
                 for (int e = 0; e < 2; ++e) {
-                    for (int i = 0; i < E[e]; ++i) Y[e][i] = cc[X[e][i]];
-                    clamp_ends(E[e], X[e].data(), Y[e].data(), e);
+                    for (int i = 0; i < Ek[e]; ++i) Y[e][i] = cc[Xs[e][i]];
+                    if (!mirror) clamp_ends(Ek[e], Xp[e], Y[e].data(), e);
                     sys[e].channel(Y[e].data(), coef[e].data(), sl, r, M);
-                    // evaluation: interval j with X_j < t <= X_{j+1}; t = 0 uses j = 0
+                    // evaluation: interval j with X_j < t <= X_{j+1} (j = 0 for t <= X_1):
+                    // the first knot is at or left of sample 0 in both end treatments
                     int j = 0;
-                    const int32_t* Xe = X[e].data();
+                    const int32_t* Xe = Xp[e];
                     for (int t = 0; t < T; ++t) {
-                        while (j + 1 < E[e] - 1 && Xe[j + 1] < t) ++j;
+                        while (j + 1 < Ek[e] - 1 && Xe[j + 1] < t) ++j;
                         double tau = (double)(t - Xe[j]);
                         env[e][t] = horner(&coef[e][4 * (size_t)j], tau);
                     }
@@ -470,6 +602,20 @@ int oracle_extrema(int T, const double* Pk, int env, int window, int32_t* X) {
 
 void oracle_clamp(int E, const int32_t* X, double* Y, int env) { clamp_ends(E, X, Y, env); }
 
+int oracle_mirror_knots(int T, const double* Pk, const int32_t* Xmax, int Emax, const int32_t* Xmin, int Emin,
+                        int32_t* Pmax, int32_t* Smax, int32_t* Pmin, int32_t* Smin, int32_t* E, int32_t* sym) {
+    if (Emax < 4 || Emin < 4) return -1;
+    const int32_t* X[2] = {Xmax, Xmin};
+    const int Er[2] = {Emax, Emin};
+    int32_t* P[2] = {Pmax, Pmin};
+    int32_t* S[2] = {Smax, Smin};
+    int Em[2], sy[2];
+    mirror_lists(T, Pk, X, Er, P, S, Em, sy);
+    E[0] = Em[0]; E[1] = Em[1];
+    sym[0] = sy[0]; sym[1] = sy[1];
+    return 0;
+}
+
 void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, double* M) {
     oracle_spline_solve_bc(E, X, nrhs, Y, M, ORACLE_SPLINE_NATURAL);
 }
@@ -515,10 +661,11 @@ void oracle_envelope_mean(int n, int T, int K, const double* cur, const double* 
                           const oracle_opts* o, double* mean, double* amp, int32_t* counts) {
     int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
     int spline = o ? o->spline : ORACLE_SPLINE_NATURAL;
+    int boundary = o ? o->boundary : ORACLE_BOUNDARY_CLAMP;
     int nth = (o && o->threads > 1) ? o->threads : 1;
     std::vector<double> ampsum(T);
     double nv = 0.0;
-    envelope_pass(n, T, K, cur, V, window, spline, amp != nullptr, nth, mean, ampsum.data(), &nv, counts);
+    envelope_pass(n, T, K, cur, V, window, spline, boundary, amp != nullptr, nth, mean, ampsum.data(), &nv, counts);
     for (size_t i = 0; i < (size_t)n * T; ++i) mean[i] = mean[i] / nv;
     if (amp) for (int t = 0; t < T; ++t) amp[t] = ampsum[t] / nv;
 }
@@ -556,6 +703,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
     int mode = o ? o->mode : ORACLE_MODE_FIX;
     int window = o ? o->window : ORACLE_WINDOW_REFWARP32;
     int spline = o ? o->spline : ORACLE_SPLINE_NATURAL;
+    int boundary = o ? o->boundary : ORACLE_BOUNDARY_CLAMP;
     int nth = (o && o->threads > 1) ? o->threads : 1;
     double sd = o ? o->sd : 0.075, sd2 = o ? o->sd2 : 0.75, tol = o ? o->tol : 0.075;
     size_t nT = (size_t)n * T;
@@ -577,7 +725,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
             std::memcpy(cur, running.data(), sizeof(double) * nT);
             for (int s = 0; s < S; ++s) {
                 double nv = 0.0;
-                envelope_pass(n, T, K, cur, V, window, spline, false, nth, sum.data(), nullptr, &nv, counts.data());
+                envelope_pass(n, T, K, cur, V, window, spline, boundary, false, nth, sum.data(), nullptr, &nv, counts.data());
                 record();
                 for (size_t i = 0; i < nT; ++i) cur[i] = cur[i] - sum[i] / nv;
             }
@@ -592,7 +740,7 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
             int nsift = 0;
             for (;;) {
                 double nv = 0.0;
-                envelope_pass(n, T, K, cur, V, window, spline, true, nth, sum.data(), ampsum.data(), &nv,
+                envelope_pass(n, T, K, cur, V, window, spline, boundary, true, nth, sum.data(), ampsum.data(), &nv,
                               counts.data());
                 record();
                 if (nsift == 0) {   // memd.m stop_emd: every direction < 3 interior extrema

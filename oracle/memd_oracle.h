@@ -32,12 +32,21 @@ enum { ORACLE_WINDOW_REFWARP32 = 0, ORACLE_WINDOW_FULL = 1 };
  * (the reference's dead tridiagonal_setup_nak, :738-773, in a pivot-free form). */
 enum { ORACLE_SPLINE_NATURAL = 0, ORACLE_SPLINE_NOT_A_KNOT = 1 };
 
+/* End treatment of the knot lists. CLAMP = the reference's: samples 0 and T-1 are
+ * knots, raised / lowered to the linear extrapolation of the next two knots when
+ * E >= 6 (memReducedEpochs_dev.cu:99-104, 447-486, 545-584). MIRROR = memd.m's
+ * boundary_conditions with nbsym = 2 (not in the reference): the interior extrema
+ * are extended by up to two mirror images per end, reflected about the end
+ * extremum or the end sample, with no clamp. */
+enum { ORACLE_BOUNDARY_CLAMP = 0, ORACLE_BOUNDARY_MIRROR = 1 };
+
 typedef struct oracle_opts {
     int mode;        /* ORACLE_MODE_* */
     int window;      /* ORACLE_WINDOW_* */
     double sd, sd2, tol; /* stop thresholds (stop mode only) */
     int threads;     /* OpenMP threads; <=1 runs the serial code path */
     int spline;      /* ORACLE_SPLINE_* */
+    int boundary;    /* ORACLE_BOUNDARY_* */
 } oracle_opts;
 
 typedef struct oracle_trace {
@@ -64,6 +73,14 @@ void oracle_project(int n, int T, int K, const double* cur, const double* V, dou
 int oracle_extrema(int T, const double* Pk, int env, int window, int32_t* X);
 /* end-knot clamp of memReducedEpochs_dev.cu:447-486 / :545-584; Y has E values */
 void oracle_clamp(int E, const int32_t* X, double* Y, int env);
+/* mirrored knot lists of one direction (ORACLE_BOUNDARY_MIRROR). Xmax / Xmin are
+ * the clamp-mode lists of oracle_extrema (Emax, Emin >= 4: both ends included),
+ * Pk the projection row. Writes per envelope e the positions P[e] and source
+ * samples S[e] (capacity Emax + 2 / Emin + 2; a knot at position p carries the
+ * signal at sample S) and returns the counts in E[2]; sym[2] gets the left and
+ * right mirror points (samples). Returns 0, or -1 when a list has fewer than 4 knots. */
+int oracle_mirror_knots(int T, const double* Pk, const int32_t* Xmax, int Emax, const int32_t* Xmin, int Emin,
+                        int32_t* Pmax, int32_t* Smax, int32_t* Pmin, int32_t* Smin, int32_t* E, int32_t* sym);
 /* natural-spline second-derivative-type solve for nrhs channels:
  * Y is [nrhs][E] (already clamped), M is [nrhs][E] output. */
 void oracle_spline_solve(int E, const int32_t* X, int nrhs, const double* Y, double* M);

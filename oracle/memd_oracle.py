@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmemd_oracle.so")
 MODE_FIX, MODE_STOP = 0, 1
 WINDOW_REFWARP32, WINDOW_FULL = 0, 1
 SPLINE_NATURAL, SPLINE_NOT_A_KNOT = 0, 1
+BOUNDARY_CLAMP, BOUNDARY_MIRROR = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -23,7 +24,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 class Opts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("window", ctypes.c_int),
                 ("sd", ctypes.c_double), ("sd2", ctypes.c_double), ("tol", ctypes.c_double),
-                ("threads", ctypes.c_int), ("spline", ctypes.c_int)]
+                ("threads", ctypes.c_int), ("spline", ctypes.c_int), ("boundary", ctypes.c_int)]
 
 
 class Trace(ctypes.Structure):
@@ -60,6 +61,7 @@ def lib():
         L.oracle_envelope_mean.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, ctypes.POINTER(Opts), _dp, _dp, _ip]
         L.oracle_memd.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp, ctypes.POINTER(Opts), ctypes.POINTER(Trace)]
         L.oracle_window_limit.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.oracle_mirror_knots.argtypes = [ctypes.c_int, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int] + [_ip] * 6
         L.oracle_stop_decision.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip] + [ctypes.c_double] * 3
         _lib = L
     return _lib
@@ -74,8 +76,8 @@ def _i(a):
 
 
 def make_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, threads=1,
-              spline=SPLINE_NATURAL):
-    return Opts(mode, window, sd, sd2, tol, threads, spline)
+              spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP):
+    return Opts(mode, window, sd, sd2, tol, threads, spline, boundary)
 
 
 def solver_block():
@@ -125,6 +127,25 @@ def extrema(Pk, env, window=WINDOW_REFWARP32):
     return X[:E].copy()
 
 
+def mirror_knots(Pk, Xmax, Xmin):
+    """Mirrored knot lists of one direction from its clamp-mode lists (ends included).
+
+    Returns ((P_max, S_max), (P_min, S_min), (lsym, rsym)): knot positions, the
+    samples whose values they carry, and the two mirror points."""
+    Pk = np.ascontiguousarray(Pk, np.float64)
+    Xmax = np.ascontiguousarray(Xmax, np.int32)
+    Xmin = np.ascontiguousarray(Xmin, np.int32)
+    out = [np.zeros(len(X) + 2, np.int32) for X in (Xmax, Xmax, Xmin, Xmin)]
+    E = np.zeros(2, np.int32)
+    sym = np.zeros(2, np.int32)
+    rc = lib().oracle_mirror_knots(len(Pk), _d(Pk), _i(Xmax), len(Xmax), _i(Xmin), len(Xmin),
+                                   *[_i(a) for a in out], _i(E), _i(sym))
+    if rc != 0:
+        raise ValueError("mirroring needs at least 4 knots per list")
+    return ((out[0][:E[0]].copy(), out[1][:E[0]].copy()), (out[2][:E[1]].copy(), out[3][:E[1]].copy()),
+            (int(sym[0]), int(sym[1])))
+
+
 def clamp(X, Y, env):
     X = np.ascontiguousarray(X, np.int32)
     Y = np.array(Y, np.float64)
@@ -170,7 +191,8 @@ def tridiag_blocked(lo, di, up, r):
     return x
 
 
-def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1, spline=SPLINE_NATURAL):
+def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1, spline=SPLINE_NATURAL,
+                  boundary=BOUNDARY_CLAMP):
     cur = np.ascontiguousarray(cur, np.float64)
     V = np.ascontiguousarray(V, np.float64)
     n, T = cur.shape
@@ -178,7 +200,7 @@ def envelope_mean(cur, V, window=WINDOW_REFWARP32, want_amp=False, threads=1, sp
     mean = np.zeros((n, T), np.float64)
     amp = np.zeros(T, np.float64) if want_amp else None
     counts = np.zeros((K, 2), np.int32)
-    o = make_opts(window=window, threads=threads, spline=spline)
+    o = make_opts(window=window, threads=threads, spline=spline, boundary=boundary)
     lib().oracle_envelope_mean(n, T, K, _d(cur), _d(V), ctypes.byref(o), _d(mean),
                                _d(amp) if want_amp else None, _i(counts))
     return mean, amp, counts
@@ -195,7 +217,7 @@ def stop_decision(mean, amp, counts, sd=0.075, sd2=0.75, tol=0.075):
 
 
 def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075,
-         threads=1, trace=True, spline=SPLINE_NATURAL):
+         threads=1, trace=True, spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP):
     """Whole epoch. Returns (imfs [N_imf][n][T], info dict)."""
     y = np.ascontiguousarray(y, np.float64)
     V = np.ascontiguousarray(V, np.float64)
@@ -208,7 +230,7 @@ def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0
     nout = np.zeros(1, np.int32)
     npass = np.zeros(1, np.int32)
     tr = Trace(_i(kc) if trace else None, max_passes, _i(sc), _i(nout), _i(npass))
-    o = make_opts(mode, window, sd, sd2, tol, threads, spline)
+    o = make_opts(mode, window, sd, sd2, tol, threads, spline, boundary)
     rc = lib().oracle_memd(n, T, K, n_imf, S, _d(y), _d(V), _d(imfs), ctypes.byref(o), ctypes.byref(tr))
     if rc != 0:
         raise ValueError("oracle_memd rejected its arguments")

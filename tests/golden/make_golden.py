@@ -28,6 +28,11 @@ index, transposed operand) disagrees with at least one vector:
   the stage restatements above.  The generator checks that no decision is within
   rounding of a threshold (sx near sd or sd2, a fraction near tol), so the
   oracle's different rounding cannot flip one.
+* mirrored ends (memd.m's boundary_conditions, nbsym = 2; not in the reference):
+  a transcription with MATLAB's 1-based indices, inclusive slices and
+  min(end, .) / max(end - ., 1) bounds, instead of the oracle's 0-based
+  reversed-range helper; the envelopes are scipy CubicSplines (natural or
+  not-a-knot) through the mirrored knots, evaluated at the integer samples.
 """
 import os
 from fractions import Fraction
@@ -114,6 +119,109 @@ def spline_env(T, X, Y):
     return CubicSpline(X.astype(float), Y, bc_type="natural")(np.arange(T, dtype=float))
 
 
+def boundary_conditions(indmin, indmax, x, nbsym=2):
+    """memd.m's boundary_conditions for one direction, with 1-based sample indices
+    as in MATLAB (t(i) = i).  indmin / indmax: 0-based interior extrema; x: the
+    projection.  Returns 0-based (positions, sources) for the min and max lists
+    and the mirror points (lsym, rsym), and the branch taken at each end."""
+    lx = len(x)
+    X = lambda i: x[i - 1]
+    imin = [int(i) + 1 for i in indmin]
+    imax = [int(i) + 1 for i in indmax]
+    fl = lambda v: v[::-1]
+
+    def sl(v, a, b):   # MATLAB v(a:b), inclusive, empty when b < a
+        return list(v[a - 1:b]) if b >= a else []
+
+    nmax, nmin = len(imax), len(imin)
+    if imax[0] < imin[0]:
+        if X(1) > X(imin[0]):
+            lmax = fl(sl(imax, 2, min(nmax, nbsym + 1)))
+            lmin = fl(sl(imin, 1, min(nmin, nbsym)))
+            lsym, lb = imax[0], 0
+        else:
+            lmax = fl(sl(imax, 1, min(nmax, nbsym)))
+            lmin = fl(sl(imin, 1, min(nmin, nbsym - 1))) + [1]
+            lsym, lb = 1, 1
+    else:
+        if X(1) < X(imax[0]):
+            lmax = fl(sl(imax, 1, min(nmax, nbsym)))
+            lmin = fl(sl(imin, 2, min(nmin, nbsym + 1)))
+            lsym, lb = imin[0], 2
+        else:
+            lmax = fl(sl(imax, 1, min(nmax, nbsym - 1))) + [1]
+            lmin = fl(sl(imin, 1, min(nmin, nbsym)))
+            lsym, lb = 1, 3
+    if imax[-1] < imin[-1]:
+        if X(lx) < X(imax[-1]):
+            rmax = fl(sl(imax, max(nmax - nbsym + 1, 1), nmax))
+            rmin = fl(sl(imin, max(nmin - nbsym, 1), nmin - 1))
+            rsym, rb = imin[-1], 0
+        else:
+            rmax = [lx] + fl(sl(imax, max(nmax - nbsym + 2, 1), nmax))
+            rmin = fl(sl(imin, max(nmin - nbsym + 1, 1), nmin))
+            rsym, rb = lx, 1
+    else:
+        if X(lx) > X(imin[-1]):
+            rmax = fl(sl(imax, max(nmax - nbsym, 1), nmax - 1))
+            rmin = fl(sl(imin, max(nmin - nbsym + 1, 1), nmin))
+            rsym, rb = imax[-1], 2
+        else:
+            rmax = fl(sl(imax, max(nmax - nbsym + 1, 1), nmax))
+            rmin = [lx] + fl(sl(imin, max(nmin - nbsym + 2, 1), nmin))
+            rsym, rb = lx, 3
+    tl = lambda L: [2 * lsym - i for i in L]
+    tr = lambda L: [2 * rsym - i for i in L]
+    if tl(lmin)[0] > 1 or tl(lmax)[0] > 1:   # symmetrized parts do not reach t(1)
+        if lsym == imax[0]:
+            lmax = fl(sl(imax, 1, min(nmax, nbsym)))
+        else:
+            lmin = fl(sl(imin, 1, min(nmin, nbsym)))
+        assert lsym != 1
+        lsym, lb = 1, lb + 4
+    if tr(rmin)[-1] < lx or tr(rmax)[-1] < lx:
+        if rsym == imax[-1]:
+            rmax = fl(sl(imax, max(nmax - nbsym + 1, 1), nmax))
+        else:
+            rmin = fl(sl(imin, max(nmin - nbsym + 1, 1), nmin))
+        assert rsym != lx
+        rsym, rb = lx, rb + 4
+    tmin = np.array(tl(lmin) + imin + tr(rmin)) - 1
+    tmax = np.array(tl(lmax) + imax + tr(rmax)) - 1
+    smin = np.array(lmin + imin + rmin) - 1
+    smax = np.array(lmax + imax + rmax) - 1
+    return (tmin, smin), (tmax, smax), (lsym - 1, rsym - 1), (lb, rb)
+
+
+def spline_env_bc(T, X, Y, bc):
+    return CubicSpline(X.astype(float), Y, bc_type=bc)(np.arange(T, dtype=float))
+
+
+def envelope_stats_mirror(y, V, bc="natural"):
+    """One pass with mirrored ends: the reference's extrema (ends dropped), memd.m's
+    boundary_conditions, CubicSpline envelopes through the mirrored knots, no clamp."""
+    n, T = y.shape
+    K = V.shape[0]
+    P = V @ y
+    acc = np.zeros((n, T))
+    amp = np.zeros(T)
+    nv = 0
+    counts = np.zeros((K, 2), np.int32)
+    for k in range(K):
+        Xs = [simulate_extrema_kernel(P[k], 0), simulate_extrema_kernel(P[k], 1)]
+        counts[k] = [len(Xs[0]), len(Xs[1])]
+        if not (len(Xs[0]) > 3 and len(Xs[1]) > 3):
+            continue
+        nv += 1
+        (tmin, smin), (tmax, smax), _, _ = boundary_conditions(Xs[1][1:-1], Xs[0][1:-1], P[k])
+        emax = spline_env_bc(T, tmax, y[:, smax].T, bc).T
+        emin = spline_env_bc(T, tmin, y[:, smin].T, bc).T
+        acc += (emax + emin) / 2.0
+        amp += np.sqrt(np.sum((emax - emin) ** 2, axis=0)) / 2.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return acc / nv, amp / nv, counts, nv
+
+
 def envelope_stats_reference(y, V):
     """One pass: mean over valid directions, memd.m amplitude, knot counts, nvalid."""
     n, T = y.shape
@@ -142,14 +250,15 @@ def envelope_mean_reference(y, V):
     return mean, counts
 
 
-def memd_fix_reference(y, V, n_imf, S):
+def memd_fix_reference(y, V, n_imf, S, stats=None):
     """memReducedEpochs_dev.cu:1174-1179, 1328-1338: S sifts per IMF, N_imf - 1 IMFs + residual."""
+    stats = stats or envelope_stats_reference
     running = y.copy()
     out = []
     for _ in range(n_imf - 1):
         cur = running.copy()
         for _ in range(S):
-            cur = cur - envelope_stats_reference(cur, V)[0]
+            cur = cur - stats(cur, V)[0]
         out.append(cur)
         running = running - cur
     out.append(running)
@@ -178,14 +287,15 @@ def stop_reference(mean, amp, counts, nv, sd, sd2, tol, margin=1e-9):
     return not ((frac > tol or np.any(sx > sd2)) and np.any(nem > 2))
 
 
-def memd_stop_reference(y, V, n_imf, S, sd, sd2, tol):
+def memd_stop_reference(y, V, n_imf, S, sd, sd2, tol, stats=None):
     """memd.m's loop: stop_emd before each IMF, stop() before each subtraction, at
     most S subtractions per IMF, at most n_imf - 1 IMFs; returns (imfs, sifts, used)."""
+    stats = stats or envelope_stats_reference
     running = y.copy()
     out, sifts = [], []
     for _ in range(n_imf - 1):
         cur = running.copy()
-        mean, amp, counts, nv = envelope_stats_reference(cur, V)
+        mean, amp, counts, nv = stats(cur, V)
         if np.all((counts[:, 0] - 2) + (counts[:, 1] - 2) < 3):
             break   # stop_emd
         nsift = 0
@@ -194,7 +304,7 @@ def memd_stop_reference(y, V, n_imf, S, sd, sd2, tol):
             nsift += 1
             if nsift == S:
                 break
-            mean, amp, counts, nv = envelope_stats_reference(cur, V)
+            mean, amp, counts, nv = stats(cur, V)
         out.append(cur)
         sifts.append(nsift)
         running = running - cur
@@ -283,6 +393,72 @@ def main():
         out[f"stop_{name}_sifts"] = sc
         out[f"stop_{name}_used"] = np.array(used)
         print(name, "sifts", sc.tolist(), "slots used", used)
+    # mirrored ends: knot lists of short projections, every branch of both ends
+    rng2 = np.random.default_rng(20261204)
+    T = 64
+    rows, seen = [], set()
+    for trial in range(8000):
+        tt = np.arange(T)
+        Pk = np.sin(2 * np.pi * tt / rng2.uniform(6, 30) + rng2.uniform(0, 6.3)) + rng2.uniform(0, 0.8) * rng2.standard_normal(T)
+        if trial % 2:   # monotone ramps at the ends: reflections that fall short of the end sample
+            a, b = rng2.integers(0, 24, 2)
+            Pk[:a] = Pk[a] + rng2.uniform(-1.5, 1.5) * (a - tt[:a]) / max(a, 1)
+            Pk[T - b:] = Pk[T - b - 1] + rng2.uniform(-1.5, 1.5) * (tt[T - b:] - (T - b - 1)) / max(b, 1)
+        Xa, Xi = simulate_extrema_kernel(Pk, 0), simulate_extrema_kernel(Pk, 1)
+        if len(Xa) < 4 or len(Xi) < 4:
+            continue
+        res = boundary_conditions(Xi[1:-1], Xa[1:-1], Pk)
+        key = res[3]
+        if key in seen and trial % 50:
+            continue
+        seen.add(key)
+        rows.append((Pk, res))
+    lbs = {r[1][3][0] for r in rows}
+    rbs = {r[1][3][1] for r in rows}
+    assert lbs >= {0, 1, 2, 3, 4, 6} and rbs >= {0, 1, 2, 3, 4, 6}, (lbs, rbs)
+    C, W = len(rows), T // 2 + 4
+    mk = {k: np.zeros((C, W), np.int32) for k in ("pmax", "smax", "pmin", "smin")}
+    mk_e = np.zeros((C, 2), np.int32)
+    mk_sym = np.zeros((C, 2), np.int32)
+    mk_br = np.zeros((C, 2), np.int32)
+    for i, (Pk, ((tmin, smin), (tmax, smax), sym, br)) in enumerate(rows):
+        mk["pmax"][i, :len(tmax)], mk["smax"][i, :len(smax)] = tmax, smax
+        mk["pmin"][i, :len(tmin)], mk["smin"][i, :len(smin)] = tmin, smin
+        mk_e[i] = [len(tmax), len(tmin)]
+        mk_sym[i] = sym
+        mk_br[i] = br
+    out["mir_P"] = np.stack([r[0] for r in rows])
+    for k, v in mk.items():
+        out[f"mir_{k}"] = v
+    out["mir_E"] = mk_e
+    out["mir_sym"] = mk_sym
+    out["mir_branch"] = mk_br
+    print("mirror knot cases", C, "left branches", sorted(lbs), "right branches", sorted(rbs))
+    # mirrored ends: one pass (both end conditions), a fixed-count loop and memd.m's own
+    # configuration (stop criterion, mirrored ends, not-a-knot splines)
+    n, T, K = 3, 300, 8
+    t = np.arange(T)
+    y = np.stack([np.sin(2 * np.pi * t / 29.0 + 1.3 * c) + 0.4 * np.sin(2 * np.pi * t / 83.0 + c)
+                  + 0.3 * rng2.standard_normal(T) for c in range(n)])
+    V = directions_closed_form(n, K, [-K] + primes_upto(n - 1))
+    out["mpass_y"] = y
+    out["mpass_V"] = V
+    for bc, tag in (("natural", "nat"), ("not-a-knot", "nak")):
+        mean, amp, counts, _ = envelope_stats_mirror(y, V, bc)
+        out[f"mpass_mean_{tag}"] = mean
+        out[f"mpass_amp_{tag}"] = amp
+        out[f"mpass_counts_{tag}"] = counts
+    out["mfix_imfs"] = memd_fix_reference(y, V, 4, 3, stats=lambda c, v: envelope_stats_mirror(c, v, "natural"))
+    y = synth.sines_epoch(4, 900, 31)
+    V = directions_closed_form(4, 12, [-12] + primes_upto(3))
+    imfs, sc, used = memd_stop_reference(y, V, 6, 20, 0.075, 0.75, 0.075,
+                                         stats=lambda c, v: envelope_stats_mirror(c, v, "not-a-knot"))
+    out["mstop_y"] = y
+    out["mstop_V"] = V
+    out["mstop_imfs"] = imfs
+    out["mstop_sifts"] = sc
+    out["mstop_used"] = np.array(used)
+    print("memd.m configuration sifts", sc.tolist(), "slots used", used)
     np.savez_compressed(os.path.join(HERE, "golden_small.npz"), **out)
     print("wrote", len(out), "arrays")
 
