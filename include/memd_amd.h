@@ -47,6 +47,16 @@ enum {
                                   diagonally dominant (the reference's form has a zero pivot
                                   whenever the first two knot gaps are equal) */
 };
+/* End treatment of the knot lists. */
+enum {
+    MEMD_BOUNDARY_CLAMP = 0,   /* the reference's: samples 0 and T-1 are knots, raised / lowered to
+                                  the linear extrapolation of the next two knots when a list has
+                                  >= 6 knots (memReducedEpochs_dev.cu:99-104, 447-486, 545-584) */
+    MEMD_BOUNDARY_MIRROR = 1   /* memd.m's boundary_conditions (nbsym = 2), not in the reference:
+                                  up to two extrema per end and list mirrored about the end
+                                  extremum or the end sample, no clamp.  Runs the record-path
+                                  kernels in both sifting modes: n <= 128 channels */
+};
 /* Extrema window. */
 enum {
     MEMD_WINDOW_REFWARP32 = 0, /* the reference's coverage: interior t <= 240*ceil(T/242)
@@ -65,6 +75,7 @@ typedef struct memd_opts {
     int32_t* h_sift_counts;  /* host [N_imf]: sifts per IMF */
     int32_t* h_n_imf_out;    /* host: output slots used (IMFs + residual) */
     int32_t spline;          /* MEMD_SPLINE_* (default NATURAL) */
+    int32_t boundary;        /* MEMD_BOUNDARY_* (default CLAMP) */
 } memd_opts;
 
 typedef struct memd_ctx memd_ctx;

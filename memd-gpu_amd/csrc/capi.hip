@@ -105,7 +105,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, X, fac, redsys, ysol, coef, sep, sepin, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, mir, X, fac, redsys, ysol, coef, sep, sepin, mean, tstat, red, sift, st, total;
 };
 
 EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
@@ -119,13 +119,14 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.curt = take(((size_t)T + 4 * (size_t)K) * channel_stride(n) * 8);   // then yend
     L.mask = take((size_t)K * 2 * W * 8);
     L.pre = take((size_t)K * 2 * W * 4);
-    L.E = take((size_t)K * 2 * 4);
+    L.E = take((size_t)K * 2 * 4 * 2);   // E, then Eraw
+    L.mir = take((size_t)K * 2 * 16);
     L.X = take((size_t)K * 2 * Ecap * 4);
     L.fac = take((size_t)K * 2 * kFacPlanes * Ecap * 2 * 8);
     L.redsys = take((size_t)K * 2 * 3 * nb * 8);
     const size_t np = (size_t)channel_stride(n);
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
-    L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // stop mode's record array only
+    L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // the record path's array only
     L.sep = take((size_t)K * 2 * np * (nb + 1) * 8);
     L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
     L.mean = take((size_t)n * T * 8);
@@ -206,6 +207,7 @@ EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& 
     d.y = I.y; d.V = I.V; d.imfs = I.imfs;
     d.coef_records = records ? 1 : 0;
     d.spline = o.spline;
+    d.boundary = o.boundary;
     const memd_tuning tu = tuning_for(ctx, I.n);
     d.bpw = tu.block_groups;
     d.etw = tu.eval_tile_words;
@@ -223,6 +225,8 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.mask = (unsigned long long*)(b + L.mask);
     d.pre = (int*)(b + L.pre);
     d.E = (int*)(b + L.E);
+    d.Eraw = d.E + 2 * (size_t)d.K;   // desc_geometry set K
+    d.mir = (int4*)(b + L.mir);
     d.X = (int*)(b + L.X);
     d.fac = (double*)(b + L.fac);
     d.red = (double*)(b + L.redsys);
@@ -451,28 +455,48 @@ struct RunArena {
     size_t fixed_bytes = 0;
 };
 
-// Core: run a set of epochs (all already FP64 and on the device) to completion.
-memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
-                       memd_kernel_profile* prof = nullptr, RunArena where = RunArena{}) {
-    memd_opts o;
-    if (opts) o = *opts; else memd_opts_default(&o);
+// Option values both the run and the server entry points accept.
+memd_status check_opts(memd_ctx* ctx, const memd_opts& o) {
     if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP)
         return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
     if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
         return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
     if (o.spline != MEMD_SPLINE_NATURAL && o.spline != MEMD_SPLINE_NOT_A_KNOT)
         return fail(ctx, MEMD_ERR_ARG, "unknown spline end condition %d", o.spline);
+    if (o.boundary != MEMD_BOUNDARY_CLAMP && o.boundary != MEMD_BOUNDARY_MIRROR)
+        return fail(ctx, MEMD_ERR_ARG, "unknown end treatment %d", o.boundary);
+    return MEMD_OK;
+}
+
+// The record-path kernels (spline records in HBM, one lane per sample walking the
+// channels; at most kStopMaxChannels channels) serve stop mode, and both modes
+// with mirrored ends; fix mode with the reference's ends uses the fused evaluation.
+bool record_path(const memd_opts& o) { return o.mode == MEMD_MODE_STOP || o.boundary == MEMD_BOUNDARY_MIRROR; }
+
+memd_status check_record_channels(memd_ctx* ctx, const memd_opts& o, int n) {
+    if (record_path(o) && n > kStopMaxChannels)
+        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode and mirrored ends support at most %d channels (got %d)",
+                    kStopMaxChannels, n);
+    return MEMD_OK;
+}
+
+// Core: run a set of epochs (all already FP64 and on the device) to completion.
+memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd_opts* opts, double* elapsed,
+                       memd_kernel_profile* prof = nullptr, RunArena where = RunArena{}) {
+    memd_opts o;
+    if (opts) o = *opts; else memd_opts_default(&o);
+    memd_status rs = check_opts(ctx, o);
+    if (rs != MEMD_OK) return rs;
     const int count = (int)in.size();
     if (count == 0) return MEMD_OK;
     if (o.d_knot_counts && count != 1)
         return fail(ctx, MEMD_ERR_ARG, "knot-count tracing is only available for single-epoch calls");
     int nmax = 0;
     for (const EpochIn& I : in) nmax = I.n > nmax ? I.n : nmax;
-    if (o.mode == MEMD_MODE_STOP && nmax > kStopMaxChannels)
-        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %d)", kStopMaxChannels, nmax);
+    if ((rs = check_record_channels(ctx, o, nmax)) != MEMD_OK) return rs;
 
     // ---- workspace plan: every epoch's workspace, then the plan region
-    const bool records = (o.mode == MEMD_MODE_STOP);
+    const bool records = record_path(o);
     std::vector<EpochLayout> lay(count);
     std::vector<size_t> base(count);
     std::vector<EpochDesc> eds(count);
@@ -488,7 +512,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     std::vector<int> grp(count, 0);
     if (groups > 1) xcd_pack(in, grp);
     PlanLayout P;
-    memd_status rs = plan_layout(ctx, eds, grp, groups, total, &P);
+    rs = plan_layout(ctx, eds, grp, groups, total, &P);
     if (rs != MEMD_OK) return rs;
     char* arena = nullptr;
     if (where.fixed) {
@@ -516,7 +540,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, launch_zero_ints(ndone_dev, 1, s));   // a kernel, not a memset node: callers may capture
     StepPlan plan{};
     std::vector<int2> tabs;
-    rs = write_plan(ctx, eds, grp, P, arena, o.mode, records, tabs, s, &plan);
+    rs = write_plan(ctx, eds, grp, P, arena, records ? kModeStop : kModeFix, records, tabs, s, &plan);
     if (rs != MEMD_OK) return rs;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
@@ -729,7 +753,7 @@ size_t memd_workspace_bytes(size_t n, size_t T, size_t K) {
 
 size_t memd_workspace_bytes_opts(size_t n, size_t T, size_t K, size_t N_imf, const memd_opts* opts) {
     if (n < 1 || T < 4 || K < 1 || N_imf < 1) return 0;
-    const bool records = opts && opts->mode == MEMD_MODE_STOP;
+    const bool records = opts && record_path(*opts);
     return layout_for((int)n, (int)T, (int)K, (int)N_imf, records).total;
 }
 
@@ -1217,7 +1241,8 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     }
     int* ndone = (int*)(S->region + S->P.ndone_at);
     for (EpochDesc& d : S->eds) d.ndone = ndone;
-    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->opts.mode, S->records, S->tabs, s, &S->plan);
+    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->records ? kModeStop : kModeFix, S->records, S->tabs, s,
+                    &S->plan);
     if (rs != MEMD_OK) return rs;
     // counters are zero between steps; a new region starts from zero too
     HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, kCountBytes, s));
@@ -1278,11 +1303,8 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     *out = nullptr;
     memd_opts o;
     if (opts) o = *opts; else memd_opts_default(&o);
-    if (o.mode != MEMD_MODE_FIX && o.mode != MEMD_MODE_STOP) return fail(ctx, MEMD_ERR_ARG, "unknown mode %d", o.mode);
-    if (o.window != MEMD_WINDOW_REFWARP32 && o.window != MEMD_WINDOW_FULL)
-        return fail(ctx, MEMD_ERR_ARG, "unknown extrema window %d", o.window);
-    if (o.spline != MEMD_SPLINE_NATURAL && o.spline != MEMD_SPLINE_NOT_A_KNOT)
-        return fail(ctx, MEMD_ERR_ARG, "unknown spline end condition %d", o.spline);
+    memd_status rs = check_opts(ctx, o);
+    if (rs != MEMD_OK) return rs;
     if (o.d_knot_counts || o.h_sift_counts || o.h_n_imf_out)
         return fail(ctx, MEMD_ERR_ARG, "per-call diagnostics are not available on a server (use the request's outputs)");
     if (hbm_pool_bytes < kAlign || max_active < 0) return fail(ctx, MEMD_ERR_ARG, "invalid pool size or active cap");
@@ -1290,7 +1312,7 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     memd_server* S = new memd_server();
     S->ctx = ctx;
     S->opts = o;
-    S->records = (o.mode == MEMD_MODE_STOP);
+    S->records = record_path(o);
     S->max_active = max_active;
     S->pool_bytes = hbm_pool_bytes / kAlign * kAlign;
     hipError_t e = hipMalloc((void**)&S->pool, S->pool_bytes);
@@ -1322,8 +1344,7 @@ memd_status memd_server_submit(memd_server* S, const memd_epoch* ep, int64_t* id
     memd_status rs = validate_shape(ctx, ep->n, ep->T, ep->K, ep->N_imf, ep->S);
     if (rs != MEMD_OK) return rs;
     if (!ep->d_y || !ep->d_V || !ep->d_IMFs) return fail(ctx, MEMD_ERR_ARG, "null device pointer");
-    if (S->opts.mode == MEMD_MODE_STOP && ep->n > (size_t)kStopMaxChannels)
-        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode supports at most %d channels (got %zu)", kStopMaxChannels, ep->n);
+    if ((rs = check_record_channels(ctx, S->opts, (int)ep->n)) != MEMD_OK) return rs;
     memd_server::Req r;
     r.in = {(int)ep->n, (int)ep->T, (int)ep->K, (int)ep->N_imf, (int)ep->S, ep->d_y, ep->d_V, ep->d_IMFs,
             ep->h_sift_counts, ep->h_n_imf_out};

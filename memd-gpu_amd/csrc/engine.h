@@ -95,7 +95,8 @@ struct EpochState {
 struct EpochDesc {
     int n, T, K, W;        // W = ceil(T/64) bitmask words per projection row
     int np;                // row stride of curt, yend, ysol, sep: n rounded up to a multiple of 4
-    int Ecap;              // knot capacity per (direction, envelope) = T/2 + 2
+    int Ecap;              // knot capacity per (direction, envelope) = T/2 + 4 (mirrored ends add
+                           //   up to 2 knots; their lists are compacted one slot in)
     int nbmax;             // solver blocks per (direction, envelope) at Ecap knots
     int N_imf, S, mode, window;
     double sd, sd2, tol;
@@ -107,8 +108,14 @@ struct EpochDesc {
                            //   followed by yend
     unsigned long long* mask; // [K][2][W] extrema flags (bit t%64 of word t/64)
     int* pre;              // [K][2][W] knots before each word
-    int* E;                // [K][2] knot counts
-    int* X;                // [K][2][Ecap] knot sample positions
+    int* E;                // [K][2] knot counts of the lists the solver uses
+    int* Eraw;             // [K][2] knot counts of the clamp lists (interior extrema + 2): the
+                           //   trace and stop mode's nem; equal to E unless ends are mirrored
+    int* X;                // [K][2][Ecap] knot positions (samples; mirrored ends may lie left of
+                           //   0 or right of T-1)
+    int4* mir;             // [K][2] mirrored ends: (knots added left, knots added right, left mirror
+                           //   point, right mirror point); an added knot at position p carries
+                           //   the signal at sample 2 * point - p
     double* fac;           // [K][2][kFacPlanes][Ecap][2] per knot row, shared by channels (kFacAB ..)
     double* red;           // [K][2][3][nbmax] reduced system per separator: lo, rd, cp
     // Knot-major and separator-major arrays: rows of np = n rounded up to a
@@ -144,6 +151,7 @@ struct EpochDesc {
     int bblocks;
     int etw;                   // k_eval_fused tile: 64 * etw samples x 16 / etw channels (etw in {1, 2, 4})
     int spline;                // 0 natural ends, 1 not-a-knot
+    int boundary;              // 0 the reference's clamped ends, 1 mirrored ends (record path only)
 };
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its

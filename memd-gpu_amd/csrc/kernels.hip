@@ -528,9 +528,111 @@ __device__ __forceinline__ void list_sizes(const EpochDesc& ed, int E, int* m) {
     m[kListCoef] = ed.coef_records ? (int)(((long)ed.n * E + 256 * kCoefItems - 1) / (256 * kCoefItems)) : 0;
 }
 
+// ------------------------------------------------------------ mirrored ends
+//
+// memd.m's boundary_conditions with nbsym = 2 (Rehman & Mandic's MEMD toolbox,
+// after Rilling, Flandrin & Goncalves), in 0-based samples, on the interior
+// extrema of one direction: per end, up to two extrema of each list nearest the
+// end are reflected about the end extremum (when the end sample lies beyond the
+// first extremum of the other kind) or about the end sample, which then joins
+// one list; a reflection about an extremum that falls short of the end sample is
+// redone about the end sample.  Same decisions as the oracle's mirror_ends.
+
+// The interior extrema a list's end rules can reach: the first three and the
+// last three (back[2] is the last), a >= 2 of them in all.
+struct EndExtrema {
+    int front[3], back[3], a;
+    __device__ int at(int i) const { return i < 3 ? front[i] : back[i - (a - 3)]; }
+};
+
+// out[] = interior elements hi-1 down to max(lo, 0); returns the count.
+__device__ __forceinline__ int take_reversed(const EndExtrema& v, int lo, int hi, int* out) {
+    int m = 0;
+    for (int i = hi - 1; i >= max(lo, 0); --i) out[m++] = v.at(i);
+    return m;
+}
+
+struct MirrorEnds {
+    int nl[2], nr[2];        // knots added left / right (0 max list, 1 min list)
+    int ls[2][3], rs[2][3];  // their source samples, in position order
+    int lsym, rsym;
+};
+
+// x0 / xl: projection at samples 0 and T-1; xa0, xi0: at the first maximum and
+// minimum; xal, xil: at the last ones.
+__device__ void mirror_ends(int last, double x0, double xl, double xa0, double xi0, double xal, double xil,
+                            const EndExtrema& A, const EndExtrema& I, MirrorEnds& r) {
+    const int a = A.a, b = I.a;
+    const int imax0 = A.front[0], imin0 = I.front[0], imaxl = A.back[2], iminl = I.back[2];
+    if (imax0 < imin0) {
+        if (x0 > xi0) {
+            r.nl[0] = take_reversed(A, 1, min(a, 3), r.ls[0]);
+            r.nl[1] = take_reversed(I, 0, min(b, 2), r.ls[1]);
+            r.lsym = imax0;
+        } else {
+            r.nl[0] = take_reversed(A, 0, min(a, 2), r.ls[0]);
+            r.nl[1] = take_reversed(I, 0, min(b, 1), r.ls[1]);
+            r.ls[1][r.nl[1]++] = 0;
+            r.lsym = 0;
+        }
+    } else {
+        if (x0 < xa0) {
+            r.nl[0] = take_reversed(A, 0, min(a, 2), r.ls[0]);
+            r.nl[1] = take_reversed(I, 1, min(b, 3), r.ls[1]);
+            r.lsym = imin0;
+        } else {
+            r.nl[0] = take_reversed(A, 0, min(a, 1), r.ls[0]);
+            r.ls[0][r.nl[0]++] = 0;
+            r.nl[1] = take_reversed(I, 0, min(b, 2), r.ls[1]);
+            r.lsym = 0;
+        }
+    }
+    if (2 * r.lsym - r.ls[1][0] > 0 || 2 * r.lsym - r.ls[0][0] > 0) {
+        if (r.lsym == imax0) r.nl[0] = take_reversed(A, 0, min(a, 2), r.ls[0]);
+        else r.nl[1] = take_reversed(I, 0, min(b, 2), r.ls[1]);
+        r.lsym = 0;
+    }
+    if (imaxl < iminl) {
+        if (xl < xal) {
+            r.nr[0] = take_reversed(A, a - 2, a, r.rs[0]);
+            r.nr[1] = take_reversed(I, b - 3, b - 1, r.rs[1]);
+            r.rsym = iminl;
+        } else {
+            r.rs[0][0] = last;
+            r.nr[0] = 1 + take_reversed(A, a - 1, a, r.rs[0] + 1);
+            r.nr[1] = take_reversed(I, b - 2, b, r.rs[1]);
+            r.rsym = last;
+        }
+    } else {
+        if (xl > xil) {
+            r.nr[0] = take_reversed(A, a - 3, a - 1, r.rs[0]);
+            r.nr[1] = take_reversed(I, b - 2, b, r.rs[1]);
+            r.rsym = imaxl;
+        } else {
+            r.nr[0] = take_reversed(A, a - 2, a, r.rs[0]);
+            r.rs[1][0] = last;
+            r.nr[1] = 1 + take_reversed(I, b - 1, b, r.rs[1] + 1);
+            r.rsym = last;
+        }
+    }
+    if (2 * r.rsym - r.rs[1][r.nr[1] - 1] < last || 2 * r.rsym - r.rs[0][r.nr[0] - 1] < last) {
+        if (r.rsym == imaxl) r.nr[0] = take_reversed(A, a - 2, a, r.rs[0]);
+        else r.nr[1] = take_reversed(I, b - 2, b, r.rs[1]);
+        r.rsym = last;
+    }
+}
+
+// Sample whose value knot j (at position p) of a mirrored list carries.
+__device__ __forceinline__ int knot_source(const int4& m, int j, int E, int p) {
+    return j < m.x ? 2 * m.z - p : (j >= E - m.y ? 2 * m.w - p : p);
+}
+
 // One workgroup per (epoch, direction), one wave per envelope: the knot list
 // (exclusive prefix of the flag words' popcounts, a wave scan), then the work
-// items of a valid direction appended to the solver lists.
+// items of a valid direction appended to the solver lists.  With mirrored ends
+// the lists are compacted one slot in, and a valid direction's lists are then
+// rebuilt around their interior extrema (mirror_ends) before the work is sized:
+// the interior moves left by one when one knot is added at the left end.
 __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab,
                                                  WorkLists wl) {
     const int2 br = tab[blockIdx.x];
@@ -541,12 +643,14 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
     const int W = ed.W;
     const int tid = threadIdx.x;
     const int env = tid >> 6, lane = tid & 63;   // wave e compacts envelope e
+    const bool mirror = ed.boundary != 0;
     __shared__ int sE[2];
+    __shared__ int sEnds[2][6];   // mirrored ends: raw slots 1..3 and E-4..E-2 of each list
     {
         const int ke = 2 * k + env;
         const unsigned long long* m = ed.mask + (size_t)ke * W;
         int* pre = ed.pre + (size_t)ke * W;
-        int* X = ed.X + (size_t)ke * ed.Ecap;
+        int* X = ed.X + (size_t)ke * ed.Ecap + (mirror ? 1 : 0);
         const int per = (W + 63) / 64;   // consecutive words per lane
         const int w0 = lane * per;
         const int w1 = min(w0 + per, W);
@@ -558,20 +662,26 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
             const int v = __shfl_up(incl, d);
             if (lane >= d) incl += v;
         }
+        const int Etot = __shfl(incl, 63);
         int base = incl - mine;
         for (int w = w0; w < w1; ++w) {
             unsigned long long bits = m[w];
             pre[w] = base;
             while (bits) {
                 const int b = __ffsll((long long)bits) - 1;
-                X[base++] = (w << 6) + b;
+                const int pos = (w << 6) + b;
+                if (mirror) {
+                    if (base >= 1 && base <= 3) sEnds[env][base - 1] = pos;
+                    if (base >= Etot - 4 && base <= Etot - 2) sEnds[env][3 + base - (Etot - 4)] = pos;
+                }
+                X[base++] = pos;
                 bits &= bits - 1;
             }
         }
         if (lane == 63) {
-            const int Etot = incl;
             sE[env] = Etot;
             ed.E[ke] = Etot;
+            ed.Eraw[ke] = Etot;
             if (ed.d_trace && st->pass < ed.max_passes)
                 ed.d_trace[(size_t)st->pass * 2 * ed.K + ke] = Etot;
             if (ed.stats) {
@@ -582,6 +692,60 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
     }
     __syncthreads();
     if (!(sE[0] > 3 && sE[1] > 3)) return;
+    if (mirror) {
+        __shared__ double sx[6];
+        // the end rules' lists live in LDS: thread 0 indexes them at run time, which
+        // private arrays would turn into scratch
+        __shared__ EndExtrema sA, sI;
+        __shared__ MirrorEnds sR;
+        if (tid < 6) {   // the projection (as k_flags: fma chain over ascending channels) at the
+                         // samples the end rules compare
+            const int T = ed.T;
+            const int t = tid == 0 ? 0 : tid == 1 ? T - 1 : tid == 2 ? sEnds[0][0] : tid == 3 ? sEnds[1][0]
+                        : tid == 4 ? sEnds[0][5] : sEnds[1][5];
+            const double* Vk = ed.V + (size_t)k * ed.n;
+            const double* row = ed.curt + (size_t)t * ed.np;
+            double acc = 0.0;
+            for (int c = 0; c < ed.n; ++c) acc = fma(Vk[c], row[c], acc);
+            sx[tid] = acc;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 0; i < 3; ++i) {
+                sA.front[i] = sEnds[0][i]; sA.back[i] = sEnds[0][3 + i];
+                sI.front[i] = sEnds[1][i]; sI.back[i] = sEnds[1][3 + i];
+            }
+            sA.a = sE[0] - 2;
+            sI.a = sE[1] - 2;
+            mirror_ends(ed.T - 1, sx[0], sx[1], sx[2], sx[3], sx[4], sx[5], sA, sI, sR);
+        }
+        __syncthreads();
+        {
+            const int ke = 2 * k + env;
+            int* X = ed.X + (size_t)ke * ed.Ecap;   // raw list at X[1 .. Eraw]
+            const int Eraw = sE[env], nin = Eraw - 2;
+            const int nl = sR.nl[env], nr = sR.nr[env], lsym = sR.lsym, rsym = sR.rsym;
+            if (nl == 1) {   // interior X[2 .. Eraw-1] -> X[1 .. Eraw-2]; each pass reads only
+                             // slots above the ones it writes, which later passes never read
+                for (int i0 = 0; i0 < nin; i0 += 64) {
+                    const int i = i0 + lane;
+                    const int v = i < nin ? X[2 + i] : 0;
+                    if (i < nin) X[1 + i] = v;
+                }
+            }
+            const int Em = nl + nin + nr;
+            if (lane == 0) {   // after the interior has moved (the right knots may land on its
+                               // old last slot)
+                for (int i = 0; i < nl; ++i) X[i] = 2 * lsym - sR.ls[env][i];
+                for (int i = 0; i < nr; ++i) X[nl + nin + i] = 2 * rsym - sR.rs[env][i];
+                ed.E[ke] = Em;
+                ed.mir[ke] = make_int4(nl, nr, lsym, rsym);
+            }
+        }
+        __syncthreads();
+        if (tid == 0 || tid == 64) sE[tid >> 6] = sR.nl[tid >> 6] + (sE[tid >> 6] - 2) + sR.nr[tid >> 6];
+        __syncthreads();
+    }
     __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
     const int g = wl.groups > 1 ? ed.xcd : 0;
     if (tid == 0) {   // one counter update per list for both envelopes
@@ -833,9 +997,12 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const GPtr<v2d> RCp = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacRC)));
         const GPtr<double> IHp = gptr(fac_plane(ed, ke, kFacIH));
         const int spline = ed.spline;
+        const bool mirror = ed.boundary != 0;
+        const int4 mir = mirror ? ed.mir[ke] : make_int4(0, 0, 0, 0);
         for (int r = tid; r < RW; r += kBlockThreads) {
             const int u = min(p0 + r, qlast);   // factor row (used when r < R)
-            const int xs = X[min(p0 - 1 + r, E - 1)];
+            const int jr = min(p0 - 1 + r, E - 1);
+            const int xs = mirror ? knot_source(mir, jr, E, X[jr]) : X[jr];   // the sample the knot carries
             const v2d rc = RCp[u];
             const double rd = rc.x, cpv = rc.y;
             const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
@@ -888,7 +1055,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                 }
             }
         }
-        if (E >= 6 && (has_left || has_right)) {
+        if (!mirror && E >= 6 && (has_left || has_right)) {
             for (int c = tid; c < n; c += kBlockThreads) {
                 if (has_left) {
                     const double y = clamp_end(sY[c], sY[np + c], sY[2 * np + c], X0, X1, X2, env, true);
@@ -1125,6 +1292,8 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const GPtr<double> Yh = gptr(ed.ysol + (size_t)ke * ed.Ecap * np);
         const GPtr<double> Z = gptr(zsep_base(ed, ke));
+        const bool mirror = ed.boundary != 0;
+        const int4 mir = mirror ? ed.mir[ke] : make_int4(0, 0, 0, 0);
         double4* out = reinterpret_cast<double4*>(ed.coef);
         for (int r0 = 0; r0 < kCoefItems; r0 += G) {
             // y^, al, be and Z[i0], Z[i1] of both knots; knot values, positions, reciprocals
@@ -1150,10 +1319,17 @@ __global__ __launch_bounds__(256) void k_coef(const EpochDesc* __restrict__ eps,
                 z01[a] = Z[(unsigned)zj.y * (unsigned)np + (unsigned)c];
                 z10[a] = Z[(unsigned)zk.x * (unsigned)np + (unsigned)c];
                 z11[a] = Z[(unsigned)zk.y * (unsigned)np + (unsigned)c];
-                // knot values: the signal at the knot, or the clamped end values (rows
-                // T + 2 ke and T + 2 ke + 1 of the same array)
-                const int r0w = (E >= 6 && j == 0) ? ed.T + 2 * ke : X[j];
-                const int r1w = (E >= 6 && j + 1 == E - 1) ? ed.T + 2 * ke + 1 : X[j + 1];
+                // knot values: the signal at the knot, the clamped end values (rows
+                // T + 2 ke and T + 2 ke + 1 of the same array), or with mirrored ends the
+                // signal at the knot's source sample
+                int r0w, r1w;
+                if (mirror) {
+                    r0w = knot_source(mir, j, E, X[j]);
+                    r1w = knot_source(mir, j + 1, E, X[j + 1]);
+                } else {
+                    r0w = (E >= 6 && j == 0) ? ed.T + 2 * ke : X[j];
+                    r1w = (E >= 6 && j + 1 == E - 1) ? ed.T + 2 * ke + 1 : X[j + 1];
+                }
                 y0[a] = ed.curt[(size_t)r0w * np + c];
                 y1[a] = ed.curt[(size_t)r1w * np + c];
                 x0[a] = X[j];
@@ -1188,6 +1364,23 @@ __device__ __forceinline__ int knot_interval(const EpochDesc& ed, int ke, int t,
     int w = t >> 6, bit = t & 63;
     unsigned long long m = ed.mask[(size_t)ke * ed.W + w];
     int before = ed.pre[(size_t)ke * ed.W + w] + __popcll(m & ((1ull << bit) - 1ull));
+    int j = before - 1;
+    j = j < 0 ? 0 : j;
+    j = j > E - 2 ? E - 2 : j;
+    return j;
+}
+
+// The same interval with mirrored ends: the knots left of t are the added ones
+// (positions X[0 .. nl-1] and X[E-nr .. E-1]) and the interior extrema, which are
+// the flag bits before t less the flag of sample 0.
+__device__ __forceinline__ int knot_interval_mirror(const EpochDesc& ed, int ke, int t, int E) {
+    const int* X = ed.X + (size_t)ke * ed.Ecap;
+    const int4 m = ed.mir[ke];
+    int w = t >> 6, bit = t & 63;
+    unsigned long long mk = ed.mask[(size_t)ke * ed.W + w];
+    int before = ed.pre[(size_t)ke * ed.W + w] + __popcll(mk & ((1ull << bit) - 1ull)) - (t > 0 ? 1 : 0);
+    for (int i = 0; i < m.x; ++i) before += X[i] < t ? 1 : 0;
+    for (int i = E - m.y; i < E; ++i) before += X[i] < t ? 1 : 0;
     int j = before - 1;
     j = j < 0 ? 0 : j;
     j = j > E - 2 ? E - 2 : j;
@@ -1614,8 +1807,14 @@ __global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ 
         if (!dir_valid(ed.E, k)) continue;
         nv = nv + 1.0;
         int ke0 = 2 * k, ke1 = 2 * k + 1;
-        int j0 = knot_interval(ed, ke0, tq, ed.E[ke0]);
-        int j1 = knot_interval(ed, ke1, tq, ed.E[ke1]);
+        int j0, j1;
+        if (ed.boundary != 0) {
+            j0 = knot_interval_mirror(ed, ke0, tq, ed.E[ke0]);
+            j1 = knot_interval_mirror(ed, ke1, tq, ed.E[ke1]);
+        } else {
+            j0 = knot_interval(ed, ke0, tq, ed.E[ke0]);
+            j1 = knot_interval(ed, ke1, tq, ed.E[ke1]);
+        }
         double tau0 = (double)(tq - ed.X[(size_t)ke0 * ed.Ecap + j0]);
         double tau1 = (double)(tq - ed.X[(size_t)ke1 * ed.Ecap + j1]);
         double s2 = 0.0;
@@ -1631,7 +1830,7 @@ __global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ 
     if (t >= T) return;
     double q = 0.0;
     for (int c = 0; c < n; ++c) {
-        double m = (nv > 0.0) ? sums[c * 64 + lane] / nv : 0.0;
+        double m = sums[c * 64 + lane] / nv;   // 0/0 = NaN with no valid direction, as fix mode
         ed.mean[(size_t)c * T + t] = m;
         q = q + m * m;
     }
@@ -1640,7 +1839,9 @@ __global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ 
     if (t == 0) ed.stopred[0] = (int)nv;
 }
 
-// One workgroup per epoch: the stop decision and the state transition.
+// One workgroup per epoch: the stop decision and the state transition.  Epochs in
+// fix mode on these kernels (mirrored ends) sift exactly S times per IMF, as
+// k_advance does.
 __global__ __launch_bounds__(256) void k_stop_decide(const EpochDesc* __restrict__ eps, int count) {
     int e = blockIdx.x;
     if (e >= count) return;
@@ -1656,6 +1857,25 @@ __global__ __launch_bounds__(256) void k_stop_decide(const EpochDesc* __restrict
     }
     __syncthreads();
     if (s_done) return;
+    if (ed.mode == kModeFix) {
+        if (threadIdx.x != 0) return;
+        const int imf = st->imf;
+        st->act_imf = imf;
+        st->pass += 1;
+        st->action = kActContinue;
+        st->nsift += 1;
+        if (st->nsift == ed.S) {
+            ed.sift_counts[imf] = ed.S;
+            st->nsift = 0;
+            st->imf = imf + 1;
+            if (imf + 1 == ed.N_imf - 1) {
+                st->done = 1;
+                st->n_imf_out = ed.N_imf;
+                atomicAdd(ed.ndone, 1);
+            }
+        }
+        return;
+    }
     const int T = ed.T, K = ed.K;
     const bool nvzero = ed.stopred[0] == 0;
     // every amp(t) nonzero? (memd.m divides by amp only then)
@@ -1678,7 +1898,7 @@ __global__ __launch_bounds__(256) void k_stop_decide(const EpochDesc* __restrict
     if (threadIdx.x != 0) return;
     bool all_small = true, anynem = false;
     for (int k = 0; k < K; ++k) {
-        int nem = (ed.E[2 * k] - 2) + (ed.E[2 * k + 1] - 2);
+        int nem = (ed.Eraw[2 * k] - 2) + (ed.Eraw[2 * k + 1] - 2);
         if (nem >= 3) all_small = false;
         if (nem > 2) anynem = true;
     }
@@ -1787,7 +2007,7 @@ __global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ 
 
 // ------------------------------------------------------------ host helpers
 
-int knot_capacity(int T) { return T / 2 + 2; }
+int knot_capacity(int T) { return T / 2 + 4; }
 int solver_block_capacity(int Ecap) {
     int N = Ecap - 2;
     return (N >= 1) ? (N - 1) / kSolverBlock + 1 : 1;
@@ -1802,7 +2022,7 @@ int blocks_for(int map, const EpochDesc& e) {
     case kMapCompact: return e.K;
     case kMapBlock: return e.bblocks;
     case kMapEval:
-        if (e.mode == kModeStop) return (e.T + 63) / 64;
+        if (e.coef_records) return (e.T + 63) / 64;
         return ((e.W + e.etw - 1) / e.etw) * ((e.n + 16 / e.etw - 1) / (16 / e.etw));
     }
     return 0;

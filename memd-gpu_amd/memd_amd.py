@@ -29,6 +29,7 @@ STATUS_NAMES = {0: "MEMD_OK", 1: "MEMD_ERR_ARG", 2: "MEMD_ERR_HIP", 3: "MEMD_ERR
 MODE_FIX, MODE_STOP = 0, 1
 WINDOW_REFWARP32, WINDOW_FULL = 0, 1
 SPLINE_NATURAL, SPLINE_NOT_A_KNOT = 0, 1
+BOUNDARY_CLAMP, BOUNDARY_MIRROR = 0, 1
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -39,7 +40,7 @@ class MemdOpts(ctypes.Structure):
     _fields_ = [("mode", _i32), ("window", _i32), ("sd", ctypes.c_double), ("sd2", ctypes.c_double),
                 ("tol", ctypes.c_double), ("use_graph", _i32), ("d_knot_counts", _vp),
                 ("max_passes", _i32), ("h_sift_counts", ctypes.POINTER(_i32)),
-                ("h_n_imf_out", ctypes.POINTER(_i32)), ("spline", _i32)]
+                ("h_n_imf_out", ctypes.POINTER(_i32)), ("spline", _i32), ("boundary", _i32)]
 
 
 class MemdEpoch(ctypes.Structure):
@@ -157,11 +158,12 @@ def load_library(path=LIB_PATH):
 
 
 def default_opts(mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75, tol=0.075, use_graph=True,
-                 spline=SPLINE_NATURAL):
+                 spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP):
     o = MemdOpts()
     load_library().memd_opts_default(ctypes.byref(o))
     o.mode, o.window, o.sd, o.sd2, o.tol, o.use_graph = mode, window, sd, sd2, tol, int(bool(use_graph))
     o.spline = spline
+    o.boundary = boundary
     return o
 
 
@@ -269,7 +271,7 @@ class MemdContext:
     # ---- memd (memReducedEpochs_dev.cu:1074-1368)
     def run(self, y, V, n_imf, S, x=None, out=None, mode=MODE_FIX, window=WINDOW_REFWARP32,
             sd=0.075, sd2=0.75, tol=0.075, use_graph=True, trace_passes=0, spline=SPLINE_NATURAL,
-            noalloc=False):
+            noalloc=False, boundary=BOUNDARY_CLAMP):
         """Decompose one epoch y [n][T] (float64 or float32 device tensor).
 
         Returns (imfs [n_imf][n][T], info) with info = elapsed_s, sift_counts,
@@ -289,7 +291,7 @@ class MemdContext:
         _require_device(out, dt, "d_IMFs")
         if x is not None:
             _require_device(x, torch.int32, "d_x")
-        o = default_opts(mode, window, sd, sd2, tol, use_graph, spline)
+        o = default_opts(mode, window, sd, sd2, tol, use_graph, spline, boundary)
         sc = (ctypes.c_int32 * n_imf)()
         nout = ctypes.c_int32(0)
         o.h_sift_counts = ctypes.cast(sc, ctypes.POINTER(ctypes.c_int32))
@@ -338,7 +340,7 @@ class MemdContext:
 
     # ---- batched epochs (no reference counterpart)
     def run_batch(self, epochs, mode=MODE_FIX, window=WINDOW_REFWARP32, use_graph=True, profile=False,
-                  spline=SPLINE_NATURAL):
+                  spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP, sd=0.075, sd2=0.75, tol=0.075):
         """epochs: list of dicts with y, V, n_imf, S (and optional out).
         Returns (list of imfs tensors, info); with profile=True info["profile"]
         holds per-kernel average launch times (ms) and the work counters."""
@@ -361,7 +363,7 @@ class MemdContext:
             outs.append(out)
             scs.append(sc)
             nouts.append(no)
-        o = default_opts(mode, window, use_graph=use_graph, spline=spline)
+        o = default_opts(mode, window, sd, sd2, tol, use_graph=use_graph, spline=spline, boundary=boundary)
         el = ctypes.c_double(0.0)
         if profile:
             pr = MemdKernelProfile()
@@ -388,10 +390,10 @@ class MemdServer:
     sift counts of a finished request."""
 
     def __init__(self, ctx, pool_bytes, max_active=0, mode=MODE_FIX, window=WINDOW_REFWARP32,
-                 sd=0.075, sd2=0.75, tol=0.075, spline=SPLINE_NATURAL):
+                 sd=0.075, sd2=0.75, tol=0.075, spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP):
         self.ctx = ctx
         self.lib = ctx.lib
-        o = default_opts(mode, window, sd, sd2, tol, spline=spline)
+        o = default_opts(mode, window, sd, sd2, tol, spline=spline, boundary=boundary)
         h = ctypes.c_void_p()
         ctx._check(self.lib.memd_server_create(ctx.h, int(pool_bytes), int(max_active), ctypes.byref(o),
                                                ctypes.byref(h)))

@@ -9,6 +9,10 @@
 //             2049 + 2048 i (i = 0..117) that the file holds, decompose the
 //             first L samples of every channel (8 output slots unless --imfs)
 //             and print "Channel: n, Length: L, DirVec: K, Time: seconds".
+//             With --files A,B,C and --channel-list 32,64,128 it sweeps the
+//             files in turn as runBenchmark's file table does (:1377-1383:
+//             32, 64 and 128 channels with K = 64, 128, 256, i.e. K = 2n unless
+//             --directions is given).
 //
 // The engine computes in FP64: the float32 samples are widened on the host,
 // directions are the FP64 set of GenerateDirectionVector (:1029-1072), and the
@@ -30,16 +34,32 @@ struct Args {
     std::string input = "./EEGsample_32CH_30504.bin";
     std::string output = "./mode_matrix.bin";
     size_t n = 32, K = 64, imfs = 0, sifts = 10, length = 0;
-    int mode = MEMD_MODE_FIX, window = MEMD_WINDOW_REFWARP32;
-    bool bench = false;
-    std::vector<size_t> lengths;
+    int mode = MEMD_MODE_FIX, window = MEMD_WINDOW_REFWARP32, spline = MEMD_SPLINE_NATURAL,
+        boundary = MEMD_BOUNDARY_CLAMP;
+    bool bench = false, K_given = false;
+    std::vector<size_t> lengths, channel_list;
+    std::vector<std::string> files;
 };
+
+std::vector<std::string> split(const std::string& s) {
+    std::vector<std::string> out;
+    size_t pos = 0;
+    while (pos < s.size()) {
+        size_t c = s.find(',', pos);
+        if (c == std::string::npos) c = s.size();
+        out.push_back(s.substr(pos, c - pos));
+        pos = c + 1;
+    }
+    return out;
+}
 
 void usage() {
     std::fprintf(stderr,
                  "usage: memd_cli [--input FILE] [--output FILE] [--channels n] [--directions K]\n"
                  "                [--imfs N] [--sifts S] [--length T] [--mode fix|stop]\n"
-                 "                [--window refwarp32|full] [--bench [--lengths L1,L2,...]]\n");
+                 "                [--window refwarp32|full] [--spline natural|not-a-knot]\n"
+                 "                [--boundary clamp|mirror]\n"
+                 "                [--bench [--lengths L1,L2,...] [--files F1,F2,... --channel-list n1,n2,...]]\n");
 }
 
 bool parse(int argc, char** argv, Args& a) {
@@ -56,7 +76,7 @@ bool parse(int argc, char** argv, Args& a) {
         if (k == "--input") a.input = v;
         else if (k == "--output") a.output = v;
         else if (k == "--channels") a.n = std::strtoull(v, nullptr, 10);
-        else if (k == "--directions") a.K = std::strtoull(v, nullptr, 10);
+        else if (k == "--directions") { a.K = std::strtoull(v, nullptr, 10); a.K_given = true; }
         else if (k == "--imfs") a.imfs = std::strtoull(v, nullptr, 10);
         else if (k == "--sifts") a.sifts = std::strtoull(v, nullptr, 10);
         else if (k == "--length") a.length = std::strtoull(v, nullptr, 10);
@@ -68,15 +88,20 @@ bool parse(int argc, char** argv, Args& a) {
             if (!std::strcmp(v, "refwarp32")) a.window = MEMD_WINDOW_REFWARP32;
             else if (!std::strcmp(v, "full")) a.window = MEMD_WINDOW_FULL;
             else return false;
+        } else if (k == "--spline") {
+            if (!std::strcmp(v, "natural")) a.spline = MEMD_SPLINE_NATURAL;
+            else if (!std::strcmp(v, "not-a-knot")) a.spline = MEMD_SPLINE_NOT_A_KNOT;
+            else return false;
+        } else if (k == "--boundary") {
+            if (!std::strcmp(v, "clamp")) a.boundary = MEMD_BOUNDARY_CLAMP;
+            else if (!std::strcmp(v, "mirror")) a.boundary = MEMD_BOUNDARY_MIRROR;
+            else return false;
         } else if (k == "--lengths") {
-            std::string s = v;
-            size_t pos = 0;
-            while (pos < s.size()) {
-                size_t c = s.find(',', pos);
-                if (c == std::string::npos) c = s.size();
-                a.lengths.push_back(std::strtoull(s.substr(pos, c - pos).c_str(), nullptr, 10));
-                pos = c + 1;
-            }
+            for (const std::string& t : split(v)) a.lengths.push_back(std::strtoull(t.c_str(), nullptr, 10));
+        } else if (k == "--files") {
+            a.files = split(v);
+        } else if (k == "--channel-list") {
+            for (const std::string& t : split(v)) a.channel_list.push_back(std::strtoull(t.c_str(), nullptr, 10));
         } else {
             return false;
         }
@@ -97,8 +122,8 @@ struct DevBuf {
 };
 
 // One decomposition of h_y [n][T]; writes the float32 IMF stack to `output`.
-int run_one(memd_ctx* ctx, const Args& a, size_t T, size_t N_imf, const std::vector<float>& h_y, double* secs) {
-    const size_t n = a.n, K = a.K;
+int run_one(memd_ctx* ctx, const Args& a, size_t n, size_t K, size_t T, size_t N_imf, const std::vector<float>& h_y,
+            double* secs) {
     std::vector<double> y64(h_y.begin(), h_y.end());
     std::vector<int32_t> h_x(T);
     for (size_t i = 0; i < T; ++i) h_x[i] = (int32_t)i;   // sample index 0..T-1 (:1504-1506)
@@ -119,6 +144,8 @@ int run_one(memd_ctx* ctx, const Args& a, size_t T, size_t N_imf, const std::vec
     memd_opts_default(&o);
     o.mode = a.mode;
     o.window = a.window;
+    o.spline = a.spline;
+    o.boundary = a.boundary;
     rc = memd_run_f64(ctx, n, T, K, N_imf, a.sifts, d_x.p, d_y.p, d_V.p, d_IMFs.p, &o, secs);
     if (rc != MEMD_OK) return die(ctx, rc, "memd_run_f64");
     std::vector<double> imfs(N_imf * n * T);
@@ -132,21 +159,12 @@ int run_one(memd_ctx* ctx, const Args& a, size_t T, size_t N_imf, const std::vec
     return MEMD_OK;
 }
 
-}  // namespace
-
-int main(int argc, char** argv) {
-    Args a;
-    if (!parse(argc, argv, a) || a.n == 0 || a.K == 0 || a.sifts == 0) {
-        usage();
-        return MEMD_ERR_ARG;
-    }
+// Decompose the file's first L samples per channel for every length of the list
+// (the bench skips lengths past the file, like runBenchmark's list).
+int run_file(memd_ctx* ctx, const Args& a, const std::string& path, size_t n, size_t K) {
     size_t T_file = 0;
-    int rc = memd_bin_length(a.input.c_str(), a.n, &T_file);
+    int rc = memd_bin_length(path.c_str(), n, &T_file);
     if (rc != MEMD_OK) return die(nullptr, rc, "memd_bin_length");
-    memd_ctx* ctx = nullptr;
-    rc = memd_ctx_create(0, &ctx);
-    if (rc != MEMD_OK) return die(nullptr, rc, "memd_ctx_create");
-
     std::vector<size_t> lengths;
     size_t N_imf = a.imfs;
     if (a.bench) {
@@ -162,24 +180,45 @@ int main(int argc, char** argv) {
         if (L > T_file) {
             if (a.bench) continue;   // the list runs past the file: those lengths are skipped
             std::fprintf(stderr, "memd_cli: --length %zu exceeds the file's %zu samples per channel\n", L, T_file);
-            memd_ctx_destroy(ctx);
             return MEMD_ERR_ARG;
         }
-        std::vector<float> h_y(a.n * L);
-        rc = memd_read_bin_f32(a.input.c_str(), a.n, T_file, L, h_y.data());
-        if (rc != MEMD_OK) {
-            memd_ctx_destroy(ctx);
-            return die(nullptr, rc, "memd_read_bin_f32");
-        }
+        std::vector<float> h_y(n * L);
+        rc = memd_read_bin_f32(path.c_str(), n, T_file, L, h_y.data());
+        if (rc != MEMD_OK) return die(nullptr, rc, "memd_read_bin_f32");
         double secs = 0.0;
-        rc = run_one(ctx, a, L, N_imf, h_y, &secs);
-        if (rc != MEMD_OK) {
-            memd_ctx_destroy(ctx);
-            return rc;
-        }
-        std::printf("Channel: %zu, Length: %zu, DirVec: %zu, Time: %f \n", a.n, L, a.K, secs);
+        rc = run_one(ctx, a, n, K, L, N_imf, h_y, &secs);
+        if (rc != MEMD_OK) return rc;
+        std::printf("Channel: %zu, Length: %zu, DirVec: %zu, Time: %f \n", n, L, K, secs);
         std::fflush(stdout);
     }
+    return MEMD_OK;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args a;
+    if (!parse(argc, argv, a) || a.n == 0 || a.K == 0 || a.sifts == 0 ||
+        (!a.files.empty() && (!a.bench || a.files.size() != a.channel_list.size()))) {
+        usage();
+        return MEMD_ERR_ARG;
+    }
+    for (size_t c : a.channel_list)
+        if (c == 0) {
+            usage();
+            return MEMD_ERR_ARG;
+        }
+    memd_ctx* ctx = nullptr;
+    int rc = memd_ctx_create(0, &ctx);
+    if (rc != MEMD_OK) return die(nullptr, rc, "memd_ctx_create");
+    if (a.files.empty()) {
+        rc = run_file(ctx, a, a.input, a.n, a.K);
+    } else {   // runBenchmark's file table: K = 2n per file unless --directions
+        for (size_t f = 0; f < a.files.size() && rc == MEMD_OK; ++f) {
+            const size_t n = a.channel_list[f];
+            rc = run_file(ctx, a, a.files[f], n, a.K_given ? a.K : 2 * n);
+        }
+    }
     memd_ctx_destroy(ctx);
-    return 0;
+    return rc;
 }
