@@ -86,6 +86,17 @@ memd_status memd_ctx_create(int device, memd_ctx** out);
 memd_status memd_ctx_destroy(memd_ctx* ctx);
 /* Run on the given hipStream_t (passed as void*); NULL = legacy default stream. */
 memd_status memd_ctx_set_stream(memd_ctx* ctx, void* hip_stream);
+/* Direction cache (no reference counterpart: the reference regenerates the set on
+ * every call, memReducedEpochs_dev.cu:1029-1072).  Sets are kept per (n, K,
+ * primes) and the least recently used are freed once they would exceed
+ * max_bytes (default 256 MiB; a single larger set is still kept).  Setting a
+ * smaller cap evicts at once. */
+memd_status memd_ctx_set_direction_cache(memd_ctx* ctx, size_t max_bytes);
+size_t memd_ctx_direction_cache_bytes(memd_ctx* ctx);
+/* Testing hook (no reference counterpart): the next `count` device allocations
+ * the library makes for this context fail as out of memory, so callers can
+ * exercise their MEMD_ERR_NOMEM handling.  0 clears it. */
+memd_status memd_debug_fail_allocations(memd_ctx* ctx, int32_t count);
 /* Message of the last failure on this context (never NULL).  With ctx == NULL:
  * the last failure of a context-free call (file I/O, memd_ctx_create) on this
  * host thread. */
@@ -271,7 +282,23 @@ memd_status memd_write_bin_f32(const char* path, const float* h, size_t count);
  * proceeds; read it once the request is MEMD_REQ_DONE.
  */
 typedef struct memd_server memd_server;
-enum { MEMD_REQ_QUEUED = 0, MEMD_REQ_RUNNING = 1, MEMD_REQ_DONE = 2 };
+enum {
+    MEMD_REQ_QUEUED = 0,
+    MEMD_REQ_RUNNING = 1,
+    MEMD_REQ_DONE = 2,
+    MEMD_REQ_CANCELLED = 3,   /* withdrawn by memd_server_cancel */
+    MEMD_REQ_TIMED_OUT = 4    /* withdrawn at its deadline (memd_server_set_deadline) */
+};
+/* Server counters (memd_server_get_info). */
+typedef struct memd_server_info {
+    int64_t steps;      /* sift steps queued (graph replays) */
+    int64_t probes;     /* state probes taken (one per memd_server_step that queued steps) */
+    int64_t rebuilds;   /* plans built, each with its captured step graph */
+    int64_t retired;    /* requests finished */
+    int64_t withdrawn;  /* requests cancelled or timed out */
+    int64_t deadlines;  /* deadlines set */
+    double rebuild_s;   /* host seconds spent building plans and capturing their graphs */
+} memd_server_info;
 
 /* hbm_pool_bytes: workspace pool (sum of memd_workspace_bytes_opts of the
  * epochs active at once, with the server's options); max_active: cap on epochs advanced together (0 = only the
@@ -284,14 +311,32 @@ memd_status memd_server_destroy(memd_server* srv);
  * (h_sift_counts, h_n_imf_out, written at completion) must stay valid until the
  * request is done.  MEMD_ERR_ADMISSION: its workspace can never fit the pool. */
 memd_status memd_server_submit(memd_server* srv, const memd_epoch* ep, int64_t* id);
-/* Admit what fits, run `steps` sift steps over the active set, retire finished
- * epochs.  Blocking.  Optional outputs: active and queued counts afterwards. */
+/* Withdraw expired requests, admit what fits (rebuilding the plan and its step
+ * graph when epochs join), queue `steps` replays of the step graph over the
+ * active set with a state probe behind them, then wait for the PREVIOUS call's
+ * probe and retire the epochs it shows finished.  The call returns while its own
+ * steps run, so a loop of calls keeps the device busy; an epoch is reported done
+ * by the call after the one whose steps finished it (or by a call with
+ * steps = 0, which waits for every probe).  Optional outputs: active and queued
+ * counts afterwards. */
 memd_status memd_server_step(memd_server* srv, int32_t steps, int32_t* n_active, int32_t* n_queued);
-/* Pop up to cap ids of requests finished since the last call, in completion order. */
+/* Withdraw a request.  Queued: removed at once.  Running: the device is told to
+ * skip it behind the queued work and the call waits for that, so its buffers
+ * are no longer touched when it returns; its workspace returns to the pool and
+ * its IMF buffer holds a partial result.  It then reads MEMD_REQ_CANCELLED and
+ * is listed by memd_server_completed.  MEMD_ERR_ARG if it has already ended. */
+memd_status memd_server_cancel(memd_server* srv, int64_t id);
+/* Deadline `seconds` from now (host steady clock): a request that has not ended
+ * by then is withdrawn as by memd_server_cancel at the next memd_server_step and
+ * reads MEMD_REQ_TIMED_OUT.  seconds <= 0 clears the deadline. */
+memd_status memd_server_set_deadline(memd_server* srv, int64_t id, double seconds);
+memd_status memd_server_get_info(memd_server* srv, memd_server_info* out);
+/* Pop up to cap ids of requests that ended (done, cancelled or timed out) since
+ * the last call, in order. */
 memd_status memd_server_completed(memd_server* srv, int64_t* ids, int32_t cap, int32_t* n);
 /* MEMD_REQ_* state of a request; MEMD_ERR_NOT_FOUND for unknown or released ids. */
 memd_status memd_server_poll(memd_server* srv, int64_t id, int32_t* state);
-/* Forget a finished request. */
+/* Forget a request that has ended. */
 memd_status memd_server_release(memd_server* srv, int64_t id);
 /* Pool bytes currently held by active epochs. */
 size_t memd_server_pool_used(memd_server* srv);

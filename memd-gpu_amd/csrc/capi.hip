@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -43,7 +44,16 @@ struct memd_ctx {
     std::string err = "no error";
     char* arena = nullptr;
     size_t arena_bytes = 0;
-    std::map<std::string, double*> dircache;
+    // direction sets by (n, K, primes), least recently used evicted beyond dircache_cap bytes
+    struct DirEntry {
+        double* p = nullptr;
+        size_t bytes = 0;
+        unsigned long long used = 0;
+    };
+    std::map<std::string, DirEntry> dircache;
+    size_t dircache_bytes = 0, dircache_cap = (size_t)256 << 20;
+    unsigned long long dircache_clock = 0;
+    int fail_allocs = 0;            // testing hook: the next fail_allocs device allocations fail
     // last captured step graph and the plan it was captured for
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
@@ -75,6 +85,17 @@ hipError_t scratch_address(int device, char** out) {
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// Every device allocation the library makes for a context goes through here, so
+// that memd_debug_fail_allocations can make the next ones fail as out of memory.
+hipError_t dev_alloc(memd_ctx* ctx, void** p, size_t bytes) {
+    if (ctx && ctx->fail_allocs > 0) {
+        --ctx->fail_allocs;
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
 
 memd_status fail(memd_ctx* ctx, memd_status code, const char* fmt, ...) {
     char buf[512];
@@ -152,7 +173,7 @@ memd_status ensure_arena(memd_ctx* ctx, size_t bytes) {
         ctx->arena_bytes = 0;
     }
     size_t want = bytes + bytes / 8;   // headroom so similar shapes reuse the arena
-    hipError_t e = hipMalloc((void**)&ctx->arena, want);
+    hipError_t e = dev_alloc(ctx, (void**)&ctx->arena, want);
     if (e != hipSuccess) {
         ctx->arena = nullptr;
         (void)hipGetLastError();
@@ -250,6 +271,8 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
 struct PlanLayout {
     size_t desc_at = 0, tab_at[kNumMaps] = {}, list_at[kNumLists] = {};
     size_t cnt_at = 0, stats_at = 0, ndone_at = 0, fresh_at = 0, states_at = 0, total = 0;
+    size_t sift_at = 0, siftoff_at = 0;   // server probes: every epoch's sift counts and their offsets
+    int sift_total = 0;
     size_t bitem_at = 0, bcnt_at = 0;   // per-epoch k_block items (int4) and counters (one line each)
     int nblk[kNumMaps] = {};
     long lcap[kNumLists] = {};
@@ -303,6 +326,11 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
     total += align_up(sizeof(int2) * (size_t)P->nblk[kMapInit]);
     P->states_at = total;
     total += align_up(sizeof(EpochState) * (size_t)count);
+    for (int e = 0; e < count; ++e) P->sift_total += eds[e].N_imf;
+    P->sift_at = total;
+    total += align_up(sizeof(int) * (size_t)P->sift_total);
+    P->siftoff_at = total;
+    total += align_up(sizeof(int) * (size_t)count);
     P->total = total;
     return MEMD_OK;
 }
@@ -707,17 +735,55 @@ std::string dir_key(int n, int K, const int32_t* P) {
     return k;
 }
 
+// Evict least recently used direction sets until `incoming` more bytes fit the
+// cap (an entry larger than the cap alone is still cached).  Work queued on the
+// context's stream may read an entry, so the stream drains first.
+memd_status dircache_trim(memd_ctx* ctx, size_t incoming) {
+    bool synced = false;
+    while (!ctx->dircache.empty() && ctx->dircache_bytes + incoming > ctx->dircache_cap) {
+        auto lru = ctx->dircache.begin();
+        for (auto it = ctx->dircache.begin(); it != ctx->dircache.end(); ++it)
+            if (it->second.used < lru->second.used) lru = it;
+        if (!synced) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->user));
+            synced = true;
+        }
+        HIP_TRY(ctx, hipFree(lru->second.p));
+        ctx->dircache_bytes -= lru->second.bytes;
+        ctx->dircache.erase(lru);
+    }
+    return MEMD_OK;
+}
+
 memd_status cached_directions(memd_ctx* ctx, int n, int K, const int32_t* P, const double** out) {
     std::string key = dir_key(n, K, P);
     auto it = ctx->dircache.find(key);
-    if (it != ctx->dircache.end()) { *out = it->second; return MEMD_OK; }
+    if (it != ctx->dircache.end()) {
+        it->second.used = ++ctx->dircache_clock;
+        *out = it->second.p;
+        return MEMD_OK;
+    }
     std::vector<double> V((size_t)K * n);
     directions_host(n, K, P, V.data());
+    const size_t bytes = V.size() * sizeof(double);
+    memd_status rs = dircache_trim(ctx, bytes);
+    if (rs != MEMD_OK) return rs;
     double* d = nullptr;
-    hipError_t e = hipMalloc((void**)&d, V.size() * sizeof(double));
-    if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "direction cache allocation failed: %s", hipGetErrorString(e));
-    HIP_TRY(ctx, hipMemcpy(d, V.data(), V.size() * sizeof(double), hipMemcpyHostToDevice));
-    ctx->dircache[key] = d;
+    hipError_t e = dev_alloc(ctx, (void**)&d, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, MEMD_ERR_NOMEM, "direction cache allocation failed: %s", hipGetErrorString(e));
+    }
+    e = hipMemcpy(d, V.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(ctx, MEMD_ERR_HIP, "direction upload failed: %s", hipGetErrorString(e));
+    }
+    memd_ctx::DirEntry& en = ctx->dircache[key];
+    en.p = d;
+    en.bytes = bytes;
+    en.used = ++ctx->dircache_clock;
+    ctx->dircache_bytes += bytes;
     *out = d;
     return MEMD_OK;
 }
@@ -806,7 +872,7 @@ memd_status memd_ctx_destroy(memd_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    for (auto& kv : ctx->dircache) (void)hipFree(kv.second);
+    for (auto& kv : ctx->dircache) (void)hipFree(kv.second.p);
     if (ctx->arena) (void)hipFree(ctx->arena);
     if (ctx->flag) (void)hipFree(ctx->flag);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
@@ -827,6 +893,22 @@ memd_status memd_ctx_set_stream(memd_ctx* ctx, void* hip_stream) {
 }
 
 const char* memd_last_error(memd_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
+
+memd_status memd_ctx_set_direction_cache(memd_ctx* ctx, size_t max_bytes) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->dircache_cap = max_bytes;
+    return dircache_trim(ctx, 0);
+}
+
+size_t memd_ctx_direction_cache_bytes(memd_ctx* ctx) { return ctx ? ctx->dircache_bytes : 0; }
+
+memd_status memd_debug_fail_allocations(memd_ctx* ctx, int32_t count) {
+    if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
+    if (count < 0) return fail(ctx, MEMD_ERR_ARG, "negative count");
+    ctx->fail_allocs = count;
+    return MEMD_OK;
+}
 
 memd_status memd_directions_f64(memd_ctx* ctx, size_t n, size_t K, const int32_t* h_primes, double* d_V) {
     if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
@@ -905,7 +987,7 @@ memd_status memd_run_f32(memd_ctx* ctx, size_t n, size_t T, size_t K, size_t N_i
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     size_t ny = n * T, nv = K * n, no = N_imf * n * T;
     Staging sc;
-    hipError_t e = hipMalloc((void**)&sc.p, (ny + nv + no) * sizeof(double));
+    hipError_t e = dev_alloc(ctx, (void**)&sc.p, (ny + nv + no) * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "FP64 staging allocation failed: %s", hipGetErrorString(e));
     hipStream_t s = work_stream(ctx);
     HIP_TRY(ctx, launch_f32_to_f64(d_y, sc.p, ny, s));
@@ -958,7 +1040,7 @@ memd_status memd_mex_compat_f32(memd_ctx* ctx, int32_t n, int32_t K, int32_t N_i
     if (rs != MEMD_OK) return rs;
     size_t ny = (size_t)n * T, no = (size_t)N_imf * n * T;
     Staging sc;
-    hipError_t e = hipMalloc((void**)&sc.p, (ny + no) * sizeof(double));
+    hipError_t e = dev_alloc(ctx, (void**)&sc.p, (ny + no) * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "FP64 staging allocation failed: %s", hipGetErrorString(e));
     hipStream_t s = work_stream(ctx);
     HIP_TRY(ctx, launch_f32_to_f64(d_data, sc.p, ny, s));
@@ -1032,9 +1114,9 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
                             0.25 * std::sin(0.61 * (double)t) + 0.2 * u;
         }
     Staging y, V, imfs;
-    hipError_t e = hipMalloc((void**)&y.p, n * T * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&V.p, K * n * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&imfs.p, count * 2 * n * T * sizeof(double));
+    hipError_t e = dev_alloc(ctx, (void**)&y.p, n * T * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(ctx, (void**)&V.p, K * n * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(ctx, (void**)&imfs.p, count * 2 * n * T * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "autotune buffers: %s", hipGetErrorString(e));
     HIP_TRY(ctx, hipMemcpy(y.p, hy.data(), n * T * sizeof(double), hipMemcpyHostToDevice));
     rs = memd_directions_f64(ctx, n, K, nullptr, V.p);
@@ -1105,10 +1187,17 @@ memd_status memd_profile_batch_f64(memd_ctx* ctx, const memd_epoch* epochs, size
 // ======================================================================= server
 //
 // Served continuous batching.  Workspaces live in one pool allocated at
-// creation; the plan region (descriptors, tables, lists) is rebuilt whenever
-// the active set changes.  Continuing epochs keep their workspace (and thus
-// their state) across rebuilds; joining epochs are initialised through a
-// table that covers only them.
+// creation.  The plan region (descriptors, tables, lists) is rebuilt when epochs
+// join (or are withdrawn), and one sift step of the plan is captured as a graph
+// that every later call replays.  A call does not wait for its own steps: it
+// queues them, takes a probe (every epoch's state and sift counts, copied to
+// pinned host memory, and an event) and then waits only for the previous
+// call's probe, so the device always has the next steps queued while the host
+// retires what the older probe shows finished.  Finished epochs leave the
+// active set at once (their pool block can be reused by the next join) and stay
+// in the plan, where every kernel skips them, until the next rebuild.
+// Continuing epochs keep their workspace (and thus their state) across
+// rebuilds; joining epochs are initialised through a table that covers only them.
 
 struct memd_server {
     struct Req {
@@ -1119,6 +1208,17 @@ struct memd_server {
         int grp = 0;
         long weight = 0;      // n * T * K, the XCD load estimate
         long skipped = 0;     // admission rounds this request was passed over
+        double deadline = 0;  // host clock (seconds), 0 = none
+    };
+    struct Probe {
+        hipEvent_t ev = nullptr;
+        EpochState* h_states = nullptr;   // pinned
+        int* h_sift = nullptr;            // pinned
+        size_t cap_states = 0, cap_sift = 0;
+        std::vector<int64_t> ids;         // the plan's epochs, in plan order, when the probe was taken
+        std::vector<int> sift_off;
+        long seq = 0;
+        bool pending = false;
     };
     memd_ctx* ctx = nullptr;
     memd_opts opts{};
@@ -1131,18 +1231,28 @@ struct memd_server {
     int max_active = 0;
     std::map<int64_t, Req> reqs;
     std::vector<int64_t> queue;             // FIFO
-    std::vector<int64_t> active;
+    std::vector<int64_t> active;            // running epochs (a subset of plan_ids)
     std::vector<int64_t> finished;          // completion order, popped by memd_server_completed
     size_t finished_head = 0;
     int64_t next_id = 1;
     long load[kXcds] = {};
-    // current plan and its host staging (kept alive across asynchronous copies)
+    // current plan, its graph and its host staging (kept alive across asynchronous copies)
     std::vector<EpochDesc> eds;
     std::vector<int> grp;
     std::vector<int2> tabs, fresh;
-    std::vector<EpochState> states;
+    std::vector<int64_t> plan_ids;
+    std::vector<int> sift_off;
     PlanLayout P;
     StepPlan plan{};
+    bool dirty = false;                     // rebuild before the next launch
+    hipGraphExec_t graph = nullptr;
+    std::vector<std::pair<long, hipGraphExec_t>> old_graphs;   // destroyed once probe seq completes
+    hipEvent_t staged = nullptr;            // the last rebuild's host-to-device copies
+    hipStream_t cap = nullptr;              // graph capture (replays run on the context's own stream)
+    Probe probe[2];
+    int probe_slot = 0;
+    long probe_seq = 0, probe_done = 0;
+    memd_server_info info{};
 };
 
 namespace {
@@ -1150,6 +1260,10 @@ namespace {
 // A queued request passed over this many admission rounds blocks backfill
 // behind it until it fits.
 constexpr long kMaxSkip = 8;
+
+double host_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 bool pool_alloc(memd_server* S, size_t bytes, size_t* off) {
     for (auto it = S->free_blocks.begin(); it != S->free_blocks.end(); ++it) {
@@ -1207,12 +1321,41 @@ void server_admit(memd_server* S, std::vector<int64_t>& joined) {
     S->queue.swap(keep);
 }
 
-// Rebuild the plan over the active set and initialise the joining epochs.
+// Undo an admission whose plan could not be built: the requests go back to the
+// front of the queue, their pool blocks are returned.
+void server_unadmit(memd_server* S, const std::vector<int64_t>& joined) {
+    std::vector<int64_t> q(joined.begin(), joined.end());
+    for (int64_t id : joined) {
+        memd_server::Req& r = S->reqs[id];
+        pool_free(S, r.off, r.lay.total);
+        S->load[r.grp] -= r.weight;
+        r.state = MEMD_REQ_QUEUED;
+        S->active.erase(std::find(S->active.begin(), S->active.end(), id));
+    }
+    q.insert(q.end(), S->queue.begin(), S->queue.end());
+    S->queue.swap(q);
+}
+
+// Destroy replaced graphs whose last launch precedes a completed probe.
+void server_drop_graphs(memd_server* S, bool all) {
+    std::vector<std::pair<long, hipGraphExec_t>> keep;
+    for (auto& g : S->old_graphs) {
+        if (all || g.first <= S->probe_done) (void)hipGraphExecDestroy(g.second);
+        else keep.push_back(g);
+    }
+    S->old_graphs.swap(keep);
+}
+
+// Rebuild the plan over the active set, initialise the joining epochs and
+// capture the step graph.
 memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, hipStream_t s) {
     memd_ctx* ctx = S->ctx;
+    const double t0 = host_seconds();
     const int count = (int)S->active.size();
+    if (S->staged) HIP_TRY(ctx, hipEventSynchronize(S->staged));   // the previous plan's staging is free
     S->eds.resize(count);
     S->grp.resize(count);
+    S->plan_ids = S->active;
     const int groups = count >= kXcds ? kXcds : 1;
     std::map<int64_t, int> index;
     for (int e = 0; e < count; ++e) {
@@ -1225,12 +1368,19 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     memd_status rs = plan_layout(ctx, S->eds, S->grp, groups, 0, &S->P);
     if (rs != MEMD_OK) return rs;
     if (S->P.total > S->region_bytes) {
+        // the region is still read by queued steps and probes: drain, then replace it
         HIP_TRY(ctx, hipStreamSynchronize(s));
+        S->probe_done = S->probe_seq;
+        server_drop_graphs(S, true);
+        if (S->graph) {
+            (void)hipGraphExecDestroy(S->graph);
+            S->graph = nullptr;
+        }
         if (S->region) HIP_TRY(ctx, hipFree(S->region));
         S->region = nullptr;
         S->region_bytes = 0;
         size_t want = S->P.total + S->P.total / 4;
-        hipError_t e = hipMalloc((void**)&S->region, want);
+        hipError_t e = dev_alloc(ctx, (void**)&S->region, want);
         if (e != hipSuccess) {
             S->region = nullptr;
             (void)hipGetLastError();
@@ -1244,10 +1394,19 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->records ? kModeStop : kModeFix, S->records, S->tabs, s,
                     &S->plan);
     if (rs != MEMD_OK) return rs;
-    // counters are zero between steps; a new region starts from zero too
-    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.cnt_at, 0, kCountBytes, s));
-    HIP_TRY(ctx, hipMemsetAsync(S->region + S->P.bcnt_at, 0, kBcountStride * S->eds.size(), s));
-    HIP_TRY(ctx, hipMemsetAsync(ndone, 0, sizeof(int), s));
+    S->sift_off.resize(count);
+    for (int e = 0, at = 0; e < count; ++e) {
+        S->sift_off[e] = at;
+        at += S->eds[e].N_imf;
+    }
+    if (count)
+        HIP_TRY(ctx, hipMemcpyAsync(S->region + S->P.siftoff_at, S->sift_off.data(), sizeof(int) * count,
+                                    hipMemcpyHostToDevice, s));
+    // counters are zero between steps; a new region starts from zero too (kernels, not
+    // memset nodes: the step graph is replayed in the same stream order)
+    HIP_TRY(ctx, launch_zero_ints((int*)(S->region + S->P.cnt_at), (int)(kCountBytes / sizeof(int)), s));
+    HIP_TRY(ctx, launch_zero_ints((int*)(S->region + S->P.bcnt_at), (int)(kBcountStride * count / sizeof(int)), s));
+    HIP_TRY(ctx, launch_zero_ints(ndone, 1, s));
     S->fresh.clear();
     for (int64_t id : joined) {
         int e = index[id];
@@ -1259,36 +1418,146 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
         HIP_TRY(ctx, hipMemcpyAsync(ft, S->fresh.data(), sizeof(int2) * S->fresh.size(), hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, launch_init_table(S->plan, ft, (int)S->fresh.size(), s));
     }
+    HIP_TRY(ctx, hipEventRecord(S->staged, s));
+    // one sift step of this plan as a graph (kernels only); the replaced graph lives
+    // until the probe after its last launch has completed
+    if (S->graph) S->old_graphs.push_back({S->probe_seq, S->graph});
+    S->graph = nullptr;
+    if (count) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(S->cap, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_step(S->plan, S->cap);
+        hipError_t ce = hipStreamEndCapture(S->cap, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(ctx, MEMD_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(le),
+                        hipGetErrorString(ce));
+        }
+        hipError_t ie = hipGraphInstantiate(&S->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            S->graph = nullptr;
+            return fail(ctx, MEMD_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+        }
+    }
+    S->dirty = false;
+    S->info.rebuilds += 1;
+    S->info.rebuild_s += host_seconds() - t0;
     return MEMD_OK;
 }
 
-// Snapshot the states and retire every finished epoch.
-memd_status server_retire(memd_server* S, hipStream_t s) {
+void server_consume(memd_server* S, memd_server::Probe& pr);
+
+// Queue `steps` replays of the step graph and a probe behind them.
+memd_status server_launch(memd_server* S, int steps, hipStream_t s) {
     memd_ctx* ctx = S->ctx;
-    const int count = (int)S->active.size();
-    S->states.resize(count);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_in, s));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->own, ctx->ev_in, 0));
+    for (int i = 0; i < steps; ++i) HIP_TRY(ctx, hipGraphLaunch(S->graph, ctx->own));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_out, ctx->own));
+    HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_out, 0));
+    S->info.steps += steps;
+    memd_server::Probe& pr = S->probe[S->probe_slot];
+    S->probe_slot ^= 1;
+    if (pr.pending) {   // normally consumed by the previous call already
+        HIP_TRY(ctx, hipEventSynchronize(pr.ev));
+        server_consume(S, pr);
+    }
+    const int count = S->plan.count;
+    const size_t nsift = (size_t)S->P.sift_total;
+    if (pr.cap_states < (size_t)count || pr.cap_sift < nsift) {
+        if (pr.h_states) HIP_TRY(ctx, hipHostFree(pr.h_states));
+        if (pr.h_sift) HIP_TRY(ctx, hipHostFree(pr.h_sift));
+        pr.h_states = nullptr;
+        pr.h_sift = nullptr;
+        pr.cap_states = pr.cap_sift = 0;
+        const size_t cs = (size_t)count * 2 + 16, cf = nsift * 2 + 64;
+        HIP_TRY(ctx, hipHostMalloc((void**)&pr.h_states, sizeof(EpochState) * cs, hipHostMallocDefault));
+        HIP_TRY(ctx, hipHostMalloc((void**)&pr.h_sift, sizeof(int) * cf, hipHostMallocDefault));
+        pr.cap_states = cs;
+        pr.cap_sift = cf;
+    }
     EpochState* dst = (EpochState*)(S->region + S->P.states_at);
-    HIP_TRY(ctx, launch_gather_states(S->plan, dst, s));
-    HIP_TRY(ctx, hipMemcpyAsync(S->states.data(), dst, sizeof(EpochState) * count, hipMemcpyDeviceToHost, s));
-    HIP_TRY(ctx, hipStreamSynchronize(s));
-    std::vector<int64_t> still;
-    for (int e = 0; e < count; ++e) {
-        int64_t id = S->active[e];
-        memd_server::Req& r = S->reqs[id];
-        if (!S->states[e].done) {
-            still.push_back(id);
-            continue;
-        }
+    int* dsift = (int*)(S->region + S->P.sift_at);
+    HIP_TRY(ctx, launch_gather_probe(S->plan, dst, dsift, (const int*)(S->region + S->P.siftoff_at), s));
+    HIP_TRY(ctx, hipMemcpyAsync(pr.h_states, dst, sizeof(EpochState) * count, hipMemcpyDeviceToHost, s));
+    if (nsift) HIP_TRY(ctx, hipMemcpyAsync(pr.h_sift, dsift, sizeof(int) * nsift, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipEventRecord(pr.ev, s));
+    pr.ids = S->plan_ids;
+    pr.sift_off = S->sift_off;
+    pr.seq = ++S->probe_seq;
+    pr.pending = true;
+    S->info.probes += 1;
+    return MEMD_OK;
+}
+
+// Retire every running epoch a completed probe shows finished.
+void server_consume(memd_server* S, memd_server::Probe& pr) {
+    pr.pending = false;
+    if (pr.seq > S->probe_done) S->probe_done = pr.seq;
+    bool retired = false;
+    for (size_t e = 0; e < pr.ids.size(); ++e) {
+        const EpochState& st = pr.h_states[e];
+        if (!st.done) continue;
+        auto it = S->reqs.find(pr.ids[e]);
+        if (it == S->reqs.end() || it->second.state != MEMD_REQ_RUNNING) continue;
+        memd_server::Req& r = it->second;
         if (r.in.h_sift_counts)
-            HIP_TRY(ctx, hipMemcpy(r.in.h_sift_counts, S->eds[e].sift_counts, sizeof(int32_t) * r.in.N_imf,
-                                   hipMemcpyDeviceToHost));
-        if (r.in.h_n_imf_out) *r.in.h_n_imf_out = S->states[e].n_imf_out;
+            std::memcpy(r.in.h_sift_counts, pr.h_sift + pr.sift_off[e], sizeof(int32_t) * r.in.N_imf);
+        if (r.in.h_n_imf_out) *r.in.h_n_imf_out = st.n_imf_out;
         r.state = MEMD_REQ_DONE;
+        // the block stays intact until a rebuild drops this epoch from the plan: a
+        // new epoch placed there is initialised in stream order after that rebuild
         pool_free(S, r.off, r.lay.total);
         S->load[r.grp] -= r.weight;
-        S->finished.push_back(id);
+        S->active.erase(std::find(S->active.begin(), S->active.end(), pr.ids[e]));
+        S->finished.push_back(pr.ids[e]);
+        S->info.retired += 1;
+        retired = true;
     }
-    S->active.swap(still);
+    (void)retired;
+}
+
+// Wait for (blocking) or check (non-blocking) the probes older than `keep_seq`.
+memd_status server_poll_probes(memd_server* S, bool block, long keep_seq) {
+    memd_ctx* ctx = S->ctx;
+    for (int pass = 0; pass < 2; ++pass) {
+        // the older pending probe first
+        memd_server::Probe* p = nullptr;
+        for (auto& pr : S->probe)
+            if (pr.pending && pr.seq < keep_seq && (!p || pr.seq < p->seq)) p = &pr;
+        if (!p) break;
+        if (block) {
+            HIP_TRY(ctx, hipEventSynchronize(p->ev));
+        } else {
+            hipError_t q = hipEventQuery(p->ev);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) HIP_TRY(ctx, q);
+        }
+        server_consume(S, *p);
+    }
+    return MEMD_OK;
+}
+
+// Withdraw a queued or running request (cancel or deadline).
+memd_status server_withdraw(memd_server* S, int64_t id, int state) {
+    memd_ctx* ctx = S->ctx;
+    memd_server::Req& r = S->reqs[id];
+    if (r.state == MEMD_REQ_QUEUED) {
+        S->queue.erase(std::find(S->queue.begin(), S->queue.end(), id));
+    } else {
+        // mark it done on the device behind everything queued, and wait: afterwards no
+        // kernel reads or writes its buffers
+        hipStream_t s = work_stream(ctx);
+        HIP_TRY(ctx, launch_mark_done((EpochState*)(S->pool + r.off + r.lay.st), s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        pool_free(S, r.off, r.lay.total);
+        S->load[r.grp] -= r.weight;
+        S->active.erase(std::find(S->active.begin(), S->active.end(), id));
+    }
+    r.state = state;
+    S->finished.push_back(id);
+    S->info.withdrawn += 1;
     return MEMD_OK;
 }
 
@@ -1315,12 +1584,15 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     S->records = record_path(o);
     S->max_active = max_active;
     S->pool_bytes = hbm_pool_bytes / kAlign * kAlign;
-    hipError_t e = hipMalloc((void**)&S->pool, S->pool_bytes);
+    hipError_t e = dev_alloc(ctx, (void**)&S->pool, S->pool_bytes);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->staged, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->cap, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&S->probe[i].ev, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        delete S;
-        return fail(ctx, MEMD_ERR_NOMEM, "server pool allocation of %zu bytes failed: %s", hbm_pool_bytes,
-                    hipGetErrorString(e));
+        memd_server_destroy(S);
+        return fail(ctx, e == hipErrorOutOfMemory ? MEMD_ERR_NOMEM : MEMD_ERR_HIP,
+                    "server set-up (pool of %zu bytes) failed: %s", hbm_pool_bytes, hipGetErrorString(e));
     }
     S->free_blocks[0] = S->pool_bytes;
     *out = S;
@@ -1331,6 +1603,15 @@ memd_status memd_server_destroy(memd_server* S) {
     if (!S) return MEMD_OK;
     (void)hipSetDevice(S->ctx->device);
     (void)hipStreamSynchronize(work_stream(S->ctx));
+    server_drop_graphs(S, true);
+    if (S->graph) (void)hipGraphExecDestroy(S->graph);
+    for (auto& pr : S->probe) {
+        if (pr.h_states) (void)hipHostFree(pr.h_states);
+        if (pr.h_sift) (void)hipHostFree(pr.h_sift);
+        if (pr.ev) (void)hipEventDestroy(pr.ev);
+    }
+    if (S->staged) (void)hipEventDestroy(S->staged);
+    if (S->cap) (void)hipStreamDestroy(S->cap);
     if (S->pool) (void)hipFree(S->pool);
     if (S->region) (void)hipFree(S->region);
     delete S;
@@ -1365,22 +1646,72 @@ memd_status memd_server_step(memd_server* S, int32_t steps, int32_t* n_active, i
     if (steps < 0) return fail(ctx, MEMD_ERR_ARG, "negative step count");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t s = work_stream(ctx);
+    memd_status rs;
+    // 1. deadlines
+    if (S->info.deadlines) {
+        const double now = host_seconds();
+        std::vector<int64_t> late;
+        for (auto& kv : S->reqs)
+            if (kv.second.deadline > 0 && now >= kv.second.deadline &&
+                (kv.second.state == MEMD_REQ_QUEUED || kv.second.state == MEMD_REQ_RUNNING))
+                late.push_back(kv.first);
+        for (int64_t id : late)
+            if ((rs = server_withdraw(S, id, MEMD_REQ_TIMED_OUT)) != MEMD_OK) return rs;
+    }
+    // 2. what the device has already reported
+    if ((rs = server_poll_probes(S, false, S->probe_seq + 1)) != MEMD_OK) return rs;
+    // 3. admissions; the plan is rebuilt when epochs join, after a failed rebuild, or
+    //    when finished epochs outnumber the running ones in it
     std::vector<int64_t> joined;
-    const size_t before = S->active.size();
     server_admit(S, joined);
-    // the plan is rebuilt when the set changed: epochs joined now, or retired in
-    // the previous call (the active list then shrank below the plan's count)
-    if (!joined.empty() || before != (size_t)S->plan.count) {
-        memd_status rs = server_rebuild(S, joined, s);
-        if (rs != MEMD_OK) return rs;
+    const bool stale = S->plan.count > 2 * (int)S->active.size() + 8;
+    if (!joined.empty() || S->dirty || stale || (S->plan.count == 0 && !S->active.empty())) {
+        rs = server_rebuild(S, joined, s);
+        if (rs != MEMD_OK) {
+            server_unadmit(S, joined);
+            S->dirty = true;
+            return rs;
+        }
     }
-    if (!S->active.empty()) {
-        for (int i = 0; i < steps; ++i) HIP_TRY(ctx, launch_step(S->plan, s));
-        memd_status rs = server_retire(S, s);
-        if (rs != MEMD_OK) return rs;
+    // 4. queue the steps and a probe behind them
+    long keep = S->probe_seq + 1;
+    if (steps > 0 && !S->active.empty() && S->graph) {
+        if ((rs = server_launch(S, steps, s)) != MEMD_OK) return rs;
+        keep = S->probe_seq;   // this call's probe stays in flight
     }
+    // 5. wait for the older probes and retire what they show
+    if ((rs = server_poll_probes(S, true, keep)) != MEMD_OK) return rs;
+    server_drop_graphs(S, false);
     if (n_active) *n_active = (int32_t)S->active.size();
     if (n_queued) *n_queued = (int32_t)S->queue.size();
+    return MEMD_OK;
+}
+
+memd_status memd_server_cancel(memd_server* S, int64_t id) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(S->ctx, MEMD_ERR_NOT_FOUND, "unknown request id %lld", (long long)id);
+    if (it->second.state != MEMD_REQ_QUEUED && it->second.state != MEMD_REQ_RUNNING)
+        return fail(S->ctx, MEMD_ERR_ARG, "request %lld has already ended (state %d)", (long long)id, it->second.state);
+    HIP_TRY(S->ctx, hipSetDevice(S->ctx->device));
+    return server_withdraw(S, id, MEMD_REQ_CANCELLED);
+}
+
+memd_status memd_server_set_deadline(memd_server* S, int64_t id, double seconds) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(S->ctx, MEMD_ERR_NOT_FOUND, "unknown request id %lld", (long long)id);
+    if (it->second.state != MEMD_REQ_QUEUED && it->second.state != MEMD_REQ_RUNNING)
+        return fail(S->ctx, MEMD_ERR_ARG, "request %lld has already ended (state %d)", (long long)id, it->second.state);
+    it->second.deadline = seconds > 0 ? host_seconds() + seconds : 0;
+    if (seconds > 0) S->info.deadlines += 1;
+    return MEMD_OK;
+}
+
+memd_status memd_server_get_info(memd_server* S, memd_server_info* out) {
+    if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
+    if (!out) return fail(S->ctx, MEMD_ERR_ARG, "null output pointer");
+    *out = S->info;
     return MEMD_OK;
 }
 
@@ -1410,8 +1741,8 @@ memd_status memd_server_release(memd_server* S, int64_t id) {
     if (!S) return fail(nullptr, MEMD_ERR_ARG, "null server");
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(S->ctx, MEMD_ERR_NOT_FOUND, "unknown request id %lld", (long long)id);
-    if (it->second.state != MEMD_REQ_DONE)
-        return fail(S->ctx, MEMD_ERR_ARG, "request %lld is not finished", (long long)id);
+    if (it->second.state == MEMD_REQ_QUEUED || it->second.state == MEMD_REQ_RUNNING)
+        return fail(S->ctx, MEMD_ERR_ARG, "request %lld has not ended", (long long)id);
     S->reqs.erase(it);
     return MEMD_OK;
 }

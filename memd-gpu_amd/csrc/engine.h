@@ -215,14 +215,18 @@ hipError_t configure_kernels();
 hipError_t launch_init(const StepPlan& p, hipStream_t s);
 // k_init over an explicit block table (epochs joining a running set).
 hipError_t launch_init_table(const StepPlan& p, const int2* tab, int blocks, hipStream_t s);
-// Copy every epoch's EpochState into out[count].
-hipError_t launch_gather_states(const StepPlan& p, EpochState* out, hipStream_t s);
 hipError_t launch_step(const StepPlan& p, hipStream_t s);
 // Same launches with an event recorded after each kernel (ev[0] before the first).
 hipError_t launch_step_timed(const StepPlan& p, hipStream_t s, hipEvent_t* ev /*[kNumStepKernels+1]*/);
 hipError_t launch_check_coords(const int32_t* x, int T, int* flag, hipStream_t s);
 // p[0 .. count) = 0 by a kernel (no memset node in a caller's captured graph)
 hipError_t launch_zero_ints(int* p, int count, hipStream_t s);
+// Server probe: every epoch's state into states[e] and its sift counts into
+// sift[sift_off[e] ..] (one copy to the host carries both).
+hipError_t launch_gather_probe(const StepPlan& p, EpochState* states, int* sift, const int* sift_off, hipStream_t s);
+// Withdraw one epoch: its state becomes done with no pending action, so no later
+// step touches its buffers.
+hipError_t launch_mark_done(EpochState* st, hipStream_t s);
 hipError_t launch_f32_to_f64(const float* in, double* out, size_t count, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* in, float* out, size_t count, hipStream_t s);
 

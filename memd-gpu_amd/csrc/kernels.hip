@@ -1976,12 +1976,23 @@ __global__ void k_stop_reset(const EpochDesc* __restrict__ eps, int count, WorkL
 
 // ------------------------------------------------------------ misc kernels
 
-// Snapshot of every epoch's state into one array (one copy to the host).
-__global__ void k_gather_states(const EpochDesc* __restrict__ eps, int count, EpochState* __restrict__ out) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < count) out[e] = *eps[e].st;
+
+__global__ void k_gather_probe(const EpochDesc* __restrict__ eps, int count, EpochState* __restrict__ states,
+                               int* __restrict__ sift, const int* __restrict__ sift_off) {
+    const int e = blockIdx.x;
+    if (e >= count) return;
+    const EpochDesc& ed = eps[e];
+    if (threadIdx.x == 0) states[e] = *ed.st;
+    const int at = sift_off[e];
+    for (int i = threadIdx.x; i < ed.N_imf; i += blockDim.x) sift[at + i] = ed.sift_counts[i];
 }
 
+__global__ void k_mark_done(EpochState* st) {
+    if (threadIdx.x == 0) {
+        st->done = 1;
+        st->action = kActNone;
+    }
+}
 
 __global__ void k_zero_ints(int* __restrict__ p, int count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2081,9 +2092,14 @@ hipError_t launch_init_table(const StepPlan& p, const int2* tab, int blocks, hip
     return hipGetLastError();
 }
 
-hipError_t launch_gather_states(const StepPlan& p, EpochState* out, hipStream_t s) {
+hipError_t launch_gather_probe(const StepPlan& p, EpochState* states, int* sift, const int* sift_off, hipStream_t s) {
     if (p.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_states, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count, out);
+    hipLaunchKernelGGL(k_gather_probe, dim3(p.count), dim3(64), 0, s, p.eps, p.count, states, sift, sift_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_done(EpochState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_mark_done, dim3(1), dim3(64), 0, s, st);
     return hipGetLastError();
 }
 

@@ -43,6 +43,12 @@ class MemdOpts(ctypes.Structure):
                 ("h_n_imf_out", ctypes.POINTER(_i32)), ("spline", _i32), ("boundary", _i32)]
 
 
+class MemdServerInfo(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("probes", ctypes.c_int64), ("rebuilds", ctypes.c_int64),
+                ("retired", ctypes.c_int64), ("withdrawn", ctypes.c_int64), ("deadlines", ctypes.c_int64),
+                ("rebuild_s", ctypes.c_double)]
+
+
 class MemdEpoch(ctypes.Structure):
     _fields_ = [("n", _sz), ("T", _sz), ("K", _sz), ("N_imf", _sz), ("S", _sz),
                 ("d_y", _vp), ("d_V", _vp), ("d_IMFs", _vp),
@@ -83,11 +89,13 @@ EXPORTS = ("memd_ctx_create", "memd_ctx_destroy", "memd_ctx_set_stream", "memd_l
            "memd_workspace_bytes", "memd_workspace_bytes_opts", "memd_server_create", "memd_server_destroy", "memd_server_submit",
            "memd_server_step", "memd_server_completed", "memd_server_poll", "memd_server_release",
            "memd_server_pool_used", "memd_bin_length", "memd_read_bin_f32", "memd_write_bin_f32",
-           "memd_autotune", "memd_ctx_set_tuning", "memd_ctx_get_tuning")
+           "memd_autotune", "memd_ctx_set_tuning", "memd_ctx_get_tuning", "memd_server_cancel",
+           "memd_server_set_deadline", "memd_server_get_info", "memd_ctx_set_direction_cache",
+           "memd_ctx_direction_cache_bytes", "memd_debug_fail_allocations")
 #: exports that do not return a memd_status
 _NOT_STATUS = ("memd_last_error", "memd_version", "memd_opts_default", "memd_workspace_bytes",
-               "memd_workspace_bytes_opts", "memd_server_pool_used")
-REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+               "memd_workspace_bytes_opts", "memd_server_pool_used", "memd_ctx_direction_cache_bytes")
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_TIMED_OUT = 0, 1, 2, 3, 4
 #: MEMD_NOALLOC_SCRATCH_BYTES: the per-device workspace of memd_run_noalloc_f64
 NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
 
@@ -144,6 +152,13 @@ def load_library(path=LIB_PATH):
     L.memd_server_release.argtypes = [_vp, ctypes.c_int64]
     L.memd_server_pool_used.argtypes = [_vp]
     L.memd_server_pool_used.restype = _sz
+    L.memd_server_cancel.argtypes = [_vp, ctypes.c_int64]
+    L.memd_server_set_deadline.argtypes = [_vp, ctypes.c_int64, ctypes.c_double]
+    L.memd_server_get_info.argtypes = [_vp, ctypes.POINTER(MemdServerInfo)]
+    L.memd_ctx_set_direction_cache.argtypes = [_vp, _sz]
+    L.memd_ctx_direction_cache_bytes.argtypes = [_vp]
+    L.memd_ctx_direction_cache_bytes.restype = _sz
+    L.memd_debug_fail_allocations.argtypes = [_vp, _i32]
     L.memd_bin_length.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]
     L.memd_read_bin_f32.argtypes = [ctypes.c_char_p, _sz, _sz, _sz, _vp]
     L.memd_write_bin_f32.argtypes = [ctypes.c_char_p, _vp, _sz]
@@ -235,6 +250,17 @@ class MemdContext:
     def set_stream(self, stream):
         """Bind a torch.cuda.Stream (or None for the legacy default stream)."""
         self._check(self.lib.memd_ctx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
+
+    # ---- direction cache and the allocation-failure hook (no reference counterpart)
+    def set_direction_cache(self, max_bytes):
+        self._check(self.lib.memd_ctx_set_direction_cache(self.h, int(max_bytes)))
+
+    def direction_cache_bytes(self):
+        return int(self.lib.memd_ctx_direction_cache_bytes(self.h))
+
+    def fail_allocations(self, count):
+        """Make the next `count` device allocations of this context fail (MEMD_ERR_NOMEM)."""
+        self._check(self.lib.memd_debug_fail_allocations(self.h, int(count)))
 
     # ---- tile shapes (no reference counterpart; never change results)
     def autotune(self, n, T, K):
@@ -457,6 +483,17 @@ class MemdServer:
 
     def pool_used(self):
         return int(self.lib.memd_server_pool_used(self.h))
+
+    def cancel(self, rid):
+        self.ctx._check(self.lib.memd_server_cancel(self.h, int(rid)))
+
+    def set_deadline(self, rid, seconds):
+        self.ctx._check(self.lib.memd_server_set_deadline(self.h, int(rid), float(seconds)))
+
+    def info(self):
+        i = MemdServerInfo()
+        self.ctx._check(self.lib.memd_server_get_info(self.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in MemdServerInfo._fields_}
 
 
 # ---- raw sample files (memReducedEpochs_dev.cu:896-932, used at :1484-1501) ----

@@ -208,6 +208,13 @@ int main(int argc, char** argv) {
             usage();
             return MEMD_ERR_ARG;
         }
+    // every input file is checked before the device is touched
+    const std::vector<std::string> inputs = a.files.empty() ? std::vector<std::string>{a.input} : a.files;
+    for (size_t f = 0; f < inputs.size(); ++f) {
+        size_t T_file = 0;
+        int rc = memd_bin_length(inputs[f].c_str(), a.files.empty() ? a.n : a.channel_list[f], &T_file);
+        if (rc != MEMD_OK) return die(nullptr, rc, "memd_bin_length");
+    }
     memd_ctx* ctx = nullptr;
     int rc = memd_ctx_create(0, &ctx);
     if (rc != MEMD_OK) return die(nullptr, rc, "memd_ctx_create");
