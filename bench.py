@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--K", type=int, default=64)
     ap.add_argument("--n-imf", type=int, default=9)
     ap.add_argument("--sifts", type=int, default=10)
-    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="eeg32x30504 / wide128x16384: whole CPU epochs timed (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="uneven512: skip the plain per-epoch pass (profiling runs)")
@@ -87,23 +88,75 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, float), q)) if len(xs) else None
 
 
-def cpu_baseline(n, T, K, n_imf, S, budget_s):
-    """The FP64 oracle (OpenMP over channels, bit-identical to its serial path)
-    on the same workload, whole epochs, until the budget is spent."""
+def cpu_info():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff}
+
+
+def cpu_threads():
+    """Threads for the CPU legs: the CPUs this process may use, at most 16 (one GPU's share)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, aff))
+
+
+def cpu_whole_epochs(gen, n, K, n_imf, S, reps, threads):
+    """Median seconds of `reps` whole epochs (distinct inputs from gen(i)), FP64
+    oracle with OpenMP over channels, after a one-sift warm-up."""
+    from oracle import memd_oracle
+    V = memd_oracle.directions(n, K)
+    memd_oracle.memd(gen(0), V, 2, 1, threads=threads, trace=False)
+    times = []
+    for r in range(reps):
+        y = gen(1 + r)
+        t0 = time.perf_counter()
+        memd_oracle.memd(y, V, n_imf, S, threads=threads, trace=False)
+        times.append(time.perf_counter() - t0)
+        print(f"cpu: whole epoch {r + 1}/{reps} of {n} x {y.shape[1]} in {times[-1]:.1f} s", file=sys.stderr, flush=True)
+    return float(np.median(times)), times
+
+
+def cpu_epoch_parallel(ys_Vs, n_imf, S, threads):
+    """Epochs per second of the oracle's epoch-parallel batch (one epoch per thread)."""
+    from oracle import memd_oracle
+    t0 = time.perf_counter()
+    memd_oracle.memd_batch(ys_Vs, n_imf, S, threads=threads)
+    return len(ys_Vs) / (time.perf_counter() - t0)
+
+
+def cpu_baseline(n, T, K, n_imf, S):
+    """The FP64 oracle on the same workload, whole epochs, three ways: OpenMP over
+    channels (median of 5 epochs after a warm-up), epochs in parallel (one per
+    thread, as a batched CPU run would), and one thread.  value = the faster of the
+    two parallel throughputs."""
     from oracle import memd_oracle
     import synth
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
+    gen = lambda i: synth.sines_epoch(n, T, 5000 + i)
+    med, times = cpu_whole_epochs(gen, n, K, n_imf, S, 5, threads)
+    chan = 1.0 / med
     V = memd_oracle.directions(n, K)
-    done, t0 = 0, time.perf_counter()
-    while True:
-        memd_oracle.memd(synth.sines_epoch(n, T, 5000 + done), V, n_imf, S, threads=threads, trace=False)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= 8:
-            break
-    return {"value": done / el, "unit": "epochs/s", "cores": threads, "kind": "port",
-            "sample": f"{done} whole epochs of the same workload ({el:.1f} s), FP64 oracle, "
-                      f"OpenMP over channels with {threads} threads"}
+    par = cpu_epoch_parallel([(gen(100 + i), V) for i in range(threads)], n_imf, S, threads)
+    t0 = time.perf_counter()
+    memd_oracle.memd(gen(200), V, n_imf, S, threads=1, trace=False)
+    single = 1.0 / (time.perf_counter() - t0)
+    best = max(chan, par)
+    return {"value": best, "unit": "epochs/s", "cores": threads, "kind": "port",
+            "sample": f"whole epochs of the same workload, FP64 oracle: median of 5 with OpenMP over channels "
+                      f"({chan:.3f}/s), {threads} epochs in parallel one per thread ({par:.3f}/s), "
+                      f"one epoch on one thread ({single:.4f}/s); value = the faster parallel form",
+            "channel_parallel_epochs_per_s": round(chan, 4), "channel_parallel_times_s": [round(t, 3) for t in times],
+            "epoch_parallel_epochs_per_s": round(par, 4), "single_core_epochs_per_s": round(single, 5),
+            "cpu": cpu_info()}
 
 
 def bench_eeg(a):
@@ -128,15 +181,19 @@ def bench_eeg(a):
         lat.append(time.perf_counter() - s0)
     gpu_eps = len(lat) / sum(lat)
     ctx.close()
-    # CPU: the oracle for 2 sifts of the first IMF, scaled to the 80-sift epoch (the
-    # per-sift cost is dominated by the K*n*T envelope evaluations, the same in every sift)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    y0 = synth.eeg_epoch(n, T, 3000)
+    if a.no_cpu:
+        print(json.dumps({"metric": "plain GPU only (--no-cpu)", "value": round(gpu_eps, 3), "unit": "epochs/s",
+                          "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3)}), flush=True)
+        return
+    # CPU: whole epochs of the same shape (median after a one-sift warm-up), and one
+    # sift of IMF 1 on one thread for the single-core rate
+    threads = cpu_threads()
+    med, times = cpu_whole_epochs(lambda i: synth.eeg_epoch(n, T, 3000 + i), n, K, N_imf, S, a.cpu_reps, threads)
+    cpu_eps = 1.0 / med
     Vh = memd_oracle.directions(n, K)
     t0 = time.perf_counter()
-    memd_oracle.memd(y0, Vh, 2, 2, threads=threads, trace=False)
-    per_sift = (time.perf_counter() - t0) / 2
-    cpu_eps = 1.0 / (per_sift * (N_imf - 1) * S)
+    memd_oracle.memd(synth.eeg_epoch(n, T, 3000), Vh, 2, 1, threads=1, trace=False)
+    single_sift = time.perf_counter() - t0
     print(json.dumps({
         "metric": "MEMD epochs/s, 32ch x 30504, K=64 (plain GPU vs FP64 CPU)", "value": round(gpu_eps, 3),
         "unit": "epochs/s", "n_gpus": 1, "steps": len(lat), "warmup": a.warmup,
@@ -145,7 +202,10 @@ def bench_eeg(a):
         "config": {"workload": "memd_32ch_x30504_K64_fix9x10_plain", "generator": "eeg pink+alpha, fs=512"},
         "latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
         "cpu_baseline": {"value": round(cpu_eps, 5), "unit": "epochs/s", "cores": threads, "kind": "port",
-                         "sample": f"2 sifts of IMF 1 of one epoch ({per_sift * 2:.1f} s) scaled to 80 sifts"},
+                         "sample": f"median of {len(times)} whole epochs of the same shape after a one-sift "
+                                   f"warm-up, FP64 oracle, OpenMP over channels with {threads} threads",
+                         "times_s": [round(t, 2) for t in times],
+                         "single_core_one_sift_s": round(single_sift, 2), "cpu": cpu_info()},
         "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
 
 
@@ -178,14 +238,20 @@ def bench_wide(a):
         lat.append(time.perf_counter() - s0)
     gpu_eps = len(lat) / sum(lat)
     ctx.close()
-    # CPU: the oracle for 1 sift of the first IMF, scaled to the 80-sift epoch
-    threads = max(1, min(16, os.cpu_count() or 1))
-    y0 = synth.eeg_epoch(n, T, 4000, gamma40=True)
+    if a.no_cpu:
+        print(json.dumps({"metric": "plain GPU only (--no-cpu)", "value": round(gpu_eps, 3), "unit": "epochs/s",
+                          "ms_per_step": round(sum(lat) / len(lat) * 1e3, 3)}), flush=True)
+        return
+    # CPU: whole epochs of the same shape (median after a one-sift warm-up), and one
+    # sift of IMF 1 on one thread for the single-core rate
+    threads = cpu_threads()
+    med, times = cpu_whole_epochs(lambda i: synth.eeg_epoch(n, T, 4000 + i, gamma40=True), n, K, N_imf, S,
+                                  a.cpu_reps, threads)
+    cpu_eps = 1.0 / med
     Vh = memd_oracle.directions(n, K)
     t0 = time.perf_counter()
-    memd_oracle.memd(y0, Vh, 2, 1, threads=threads, trace=False)
-    per_sift = time.perf_counter() - t0
-    cpu_eps = 1.0 / (per_sift * (N_imf - 1) * S)
+    memd_oracle.memd(synth.eeg_epoch(n, T, 4000, gamma40=True), Vh, 2, 1, threads=1, trace=False)
+    single_sift = time.perf_counter() - t0
     print(json.dumps({
         "metric": "MEMD epochs/s, 128ch x 16384, K=256 (plain GPU vs FP64 CPU)", "value": round(gpu_eps, 3),
         "unit": "epochs/s", "n_gpus": 1, "steps": len(lat), "warmup": a.warmup,
@@ -196,7 +262,10 @@ def bench_wide(a):
         "latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
         "tuning": tuning, "autotune_s": round(tune_s, 3),
         "cpu_baseline": {"value": round(cpu_eps, 6), "unit": "epochs/s", "cores": threads, "kind": "port",
-                         "sample": f"1 sift of IMF 1 of one epoch ({per_sift:.1f} s) scaled to 80 sifts"},
+                         "sample": f"median of {len(times)} whole epochs of the same shape after a one-sift "
+                                   f"warm-up, FP64 oracle, OpenMP over channels with {threads} threads",
+                         "times_s": [round(t, 2) for t in times],
+                         "single_core_one_sift_s": round(single_sift, 2), "cpu": cpu_info()},
         "gpu_over_cpu": round(gpu_eps / cpu_eps, 1)}), flush=True)
 
 
@@ -234,8 +303,12 @@ def bench_uneven(a):
         ctx.run_batch(eps)
         lat.append(time.perf_counter() - s0)
     batched = count * len(lat) / sum(lat)
+    # every 16th epoch's batched result, kept to compare with its plain call
+    sample = list(range(0, count, 16))
+    kept = {i: eps[i]["out"].cpu().numpy().copy() for i in sample}
     plain_n = 0 if a.no_plain else count   # every epoch once, one call each
     plat, plain = [], None
+    same = None
     if plain_n:
         ctx.run(eps[0]["y"], eps[0]["V"], 9, 10, out=eps[0]["out"])
         torch.cuda.synchronize()
@@ -245,6 +318,16 @@ def bench_uneven(a):
             ctx.run(ep["y"], ep["V"], 9, 10, out=ep["out"])
             plat.append(time.perf_counter() - s1)
         plain = plain_n / (time.perf_counter() - s0)
+        same = all(np.array_equal(kept[i], eps[i]["out"].cpu().numpy()) for i in sample)
+    # CPU: the oracle's epoch-parallel batch (one epoch per thread) on the first 32 epochs
+    cpu = None
+    if not a.no_cpu:
+        threads = cpu_threads()
+        sub = [(eps[i]["y"].cpu().numpy(), eps[i]["V"].cpu().numpy()) for i in range(32)]
+        cpu_eps = cpu_epoch_parallel(sub, 9, 10, threads)
+        cpu = {"value": round(cpu_eps, 4), "unit": "epochs/s", "cores": threads, "kind": "port",
+               "sample": f"the first 32 epochs of the batch, FP64 oracle, {threads} epochs in parallel one per thread",
+               "cpu": cpu_info()}
     ctx.close()
     print(json.dumps({
         "metric": "MEMD epochs/s, 512 uneven epochs (batched vs plain GPU)", "value": round(batched, 2),
@@ -254,6 +337,8 @@ def bench_uneven(a):
         "config": {"workload": "memd_512_uneven_fix9x10", "n": [8, 16, 32, 64], "T": [1000, 8000], "K": "2n"},
         "plain_gpu_epochs_per_s": round(plain, 2) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,
+        "batched_equals_plain": {"epochs_checked": len(sample), "bit_identical": same} if plain_n else None,
+        "cpu_baseline": cpu, "batched_over_cpu": round(batched / cpu["value"], 1) if cpu else None,
         "tuning": {str(k): v for k, v in tunings.items()}, "autotune_s": round(tune_s, 3),
         "latency_ms": {"batch_p50": round(pct(lat, 50) * 1e3, 1),
                        "plain_epoch_p50": round(pct(plat, 50) * 1e3, 3) if plat else None,
@@ -490,7 +575,7 @@ def main():
                          "alg_bytes_per_launch": int(bytes_eval), "alg_flops_per_launch": int(flops_eval),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_frac": round(tfs / FP64_PEAK_TFLOPS, 4)})
 
-    cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S, a.cpu_budget_s)
+    cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S)
     pct_or_none = lambda xs, q: round(pct(xs, q) * 1e3, 3) if xs else None
     ctx.close()
     line = {

@@ -773,4 +773,28 @@ int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const do
     return 0;
 }
 
+int oracle_memd_batch(int count, const int* n, const int* T, const int* K, const int* N_imf, const int* S,
+                      const double* const* y, const double* const* V, double* const* imfs, const oracle_opts* o,
+                      int threads) {
+    // epochs in parallel, each on one thread through the serial path: every epoch's
+    // result is the one oracle_memd gives it alone
+    oracle_opts serial;
+    if (o) {
+        serial = *o;
+    } else {
+        serial.mode = ORACLE_MODE_FIX;
+        serial.window = ORACLE_WINDOW_REFWARP32;
+        serial.sd = 0.075; serial.sd2 = 0.75; serial.tol = 0.075;
+        serial.spline = ORACLE_SPLINE_NATURAL;
+        serial.boundary = ORACLE_BOUNDARY_CLAMP;
+    }
+    serial.threads = 1;
+    int bad = 0;
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1) reduction(| : bad)
+    for (int e = 0; e < count; ++e)
+        bad |= oracle_memd(n[e], T[e], K[e], N_imf[e], S[e], y[e], V[e], imfs[e], &serial, nullptr) != 0 ? 1 : 0;
+    return bad ? -1 : 0;
+}
+
 }  // extern "C"

@@ -114,6 +114,13 @@ int oracle_stop_decision(int n, int T, int K, const double* mean, const double* 
 int oracle_memd(int n, int T, int K, int N_imf, int S, const double* y, const double* V,
                 double* imfs, const oracle_opts* o, oracle_trace* tr);
 
+/* ---- many epochs at once: OpenMP over epochs (threads), each epoch on one thread
+ * through the serial path, so every result equals oracle_memd's for that epoch
+ * alone (the CPU counterpart of a batched call).  o->threads is ignored. */
+int oracle_memd_batch(int count, const int* n, const int* T, const int* K, const int* N_imf, const int* S,
+                      const double* const* y, const double* const* V, double* const* imfs, const oracle_opts* o,
+                      int threads);
+
 /* blocked-solver block length (rows per block incl. the separator) */
 int oracle_solver_block(void);
 /* last interior sample examined by the extrema test */

@@ -61,6 +61,8 @@ def lib():
         L.oracle_envelope_mean.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, ctypes.POINTER(Opts), _dp, _dp, _ip]
         L.oracle_memd.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp, ctypes.POINTER(Opts), ctypes.POINTER(Trace)]
         L.oracle_window_limit.argtypes = [ctypes.c_int, ctypes.c_int]
+        _pp = ctypes.POINTER(ctypes.c_void_p)
+        L.oracle_memd_batch.argtypes = [ctypes.c_int] + [_ip] * 5 + [_pp] * 3 + [ctypes.POINTER(Opts), ctypes.c_int]
         L.oracle_mirror_knots.argtypes = [ctypes.c_int, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int] + [_ip] * 6
         L.oracle_stop_decision.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip] + [ctypes.c_double] * 3
         _lib = L
@@ -238,3 +240,23 @@ def memd(y, V, n_imf, S, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0
     if trace:
         info["knot_counts"] = kc[: int(npass[0])].copy()
     return imfs, info
+
+
+def memd_batch(epochs, n_imf, S, threads=1, mode=MODE_FIX, window=WINDOW_REFWARP32, sd=0.075, sd2=0.75,
+               tol=0.075, spline=SPLINE_NATURAL, boundary=BOUNDARY_CLAMP):
+    """epochs: list of (y [n][T], V [K][n]).  Epochs run in parallel on `threads`
+    threads, each serially; returns the list of imfs [n_imf][n][T]."""
+    ys = [np.ascontiguousarray(y, np.float64) for y, _ in epochs]
+    Vs = [np.ascontiguousarray(V, np.float64) for _, V in epochs]
+    outs = [np.zeros((n_imf, y.shape[0], y.shape[1]), np.float64) for y in ys]
+    c = len(ys)
+    ia = lambda v: np.ascontiguousarray(v, np.int32)
+    n, T, K = ia([y.shape[0] for y in ys]), ia([y.shape[1] for y in ys]), ia([V.shape[0] for V in Vs])
+    nimf, ss = ia([n_imf] * c), ia([S] * c)
+    pa = lambda arrs: (ctypes.c_void_p * c)(*[a.ctypes.data for a in arrs])
+    o = make_opts(mode, window, sd, sd2, tol, 1, spline, boundary)
+    rc = lib().oracle_memd_batch(c, _i(n), _i(T), _i(K), _i(nimf), _i(ss), pa(ys), pa(Vs), pa(outs),
+                                 ctypes.byref(o), threads)
+    if rc != 0:
+        raise ValueError("oracle_memd_batch rejected its arguments")
+    return outs

@@ -355,3 +355,15 @@ def test_stop_mode_ends_below_three_interior_extrema(oracle, periods, slots):
     assert info["n_imf_out"] == slots
     if slots == 1:
         assert np.array_equal(imfs[0], y)
+
+
+def test_epoch_parallel_batch_equals_single_epochs(oracle):
+    # the CPU baseline's batched form: every epoch as oracle_memd gives it alone
+    import synth
+    eps = [(synth.eeg_epoch(n, T, 40 + i), oracle.directions(n, K))
+           for i, (n, T, K) in enumerate([(4, 900, 8), (8, 1500, 16), (2, 700, 4), (16, 1200, 32), (3, 64, 6)])]
+    for mode in (oracle.MODE_FIX, oracle.MODE_STOP):
+        outs = oracle.memd_batch(eps, 4, 3, threads=4, mode=mode)
+        for (y, V), got in zip(eps, outs):
+            want, _ = oracle.memd(y, V, 4, 3, mode=mode)
+            assert np.array_equal(got, want)
