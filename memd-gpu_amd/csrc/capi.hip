@@ -216,8 +216,15 @@ memd_tuning tuning_for(const memd_ctx* ctx, int n) {
     return it != ctx->tuning.end() ? it->second : default_tuning(n);
 }
 
-// Descriptor fields that depend only on the epoch's shape and the options.
-EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, bool records) {
+// k_block workgroups an epoch's items are drained by: enough that the whole
+// launch holds about kBlockWgTarget workgroups (4 per CU) when the batch is small,
+// at most 256 per epoch in larger batches (an XCD then works through one epoch's
+// items at a time, so its signal stays in that XCD's L2).
+constexpr int kBlockWgTarget = 1024;
+
+// Descriptor fields that depend only on the epoch's shape, the options and the
+// number of epochs in the launch.
+EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, bool records, int batch) {
     EpochDesc d{};
     d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
     d.np = channel_stride(I.n);
@@ -233,10 +240,11 @@ EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& 
     d.bpw = tu.block_groups;
     d.etw = tu.eval_tile_words;
     // k_block items: at most 2K * groups; workgroups per epoch: the item count at a
-    // typical knot density (one knot per 16 samples), 1..256
+    // typical knot density (one knot per 16 samples), at most the cap above
     d.bcap = 2 * I.K * ((d.nbmax + d.bpw - 1) / d.bpw);
     const long typical = 2L * I.K * (((long)I.T / 512 + 1 + d.bpw - 1) / d.bpw);
-    d.bblocks = (int)std::min(256L, std::max(1L, typical));
+    const long cap = std::max(256L, (long)kBlockWgTarget / std::max(1, batch));
+    d.bblocks = (int)std::min(cap, std::max(1L, typical));
     return d;
 }
 
@@ -418,6 +426,7 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
     plan.nmax = nmax;
     plan.nlds = nlds;
     plan.coef_records = records ? 1 : 0;
+    plan.factor_wide = P.lcap[kListFactor] <= kFactorWideItems ? 1 : 0;
     *out = plan;
     return MEMD_OK;
 }
@@ -469,6 +478,7 @@ std::string plan_key(const StepPlan& p) {
     add((unsigned long long)p.nmax);
     add((unsigned long long)p.nlds);
     add((unsigned long long)p.coef_records);
+    add((unsigned long long)p.factor_wide);
     add((unsigned long long)p.block_rmax);
     add((unsigned long long)p.block_tile_off);
     add((unsigned long long)p.block_lds);
@@ -533,7 +543,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, records);
         base[e] = total;
         total += lay[e].total;
-        eds[e] = desc_geometry(ctx, in[e], o, records);
+        eds[e] = desc_geometry(ctx, in[e], o, records, count);
     }
     // XCD affinity when the batch has enough epochs to keep every XCD busy
     const int groups = count >= kXcds ? kXcds : 1;
@@ -1360,7 +1370,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     std::map<int64_t, int> index;
     for (int e = 0; e < count; ++e) {
         memd_server::Req& r = S->reqs[S->active[e]];
-        S->eds[e] = desc_geometry(ctx, r.in, S->opts, S->records);
+        S->eds[e] = desc_geometry(ctx, r.in, S->opts, S->records, count);
         bind_workspace(S->eds[e], S->pool + r.off, r.lay);
         S->grp[e] = groups > 1 ? r.grp : 0;
         index[S->active[e]] = e;

@@ -62,6 +62,9 @@ int block_bpw_fit(int n, int want);
 constexpr int kCoefItems = 16;     // (channel, knot) pairs per k_coef lane and work item
 // Persistent grid size of the list-driven solver kernels (workgroups).
 constexpr long kListGrid = 2048;
+// Factor lists of at most this many items (8 epochs at K = 64) use k_factor's
+// 256-thread form.
+constexpr long kFactorWideItems = 1024;
 // Per knot row u of one (direction, envelope), shared by its channels: three
 // planes of Ecap pairs.  kFacAB: al, be, the phase-3 coefficients of the blocked
 // solve (the block's spike entries; 0 where a block has no left or right spike
@@ -193,6 +196,7 @@ struct StepPlan {
     int nmax;               // largest channel count in the batch
     int nlds;               // channels of k_flags' LDS tile: the largest n <= kFlagLdsMaxCh, at least kFlagChunk if an epoch is wider, 0 if none
     int coef_records;       // launch k_coef (stop mode: the evaluation reads the record array)
+    int factor_wide;        // k_factor with 256 threads per item (few items: small batches)
     int block_rmax;         // k_block: rows of the longest work item
     size_t block_tile_off;  // k_block: LDS byte offset of the tile
     size_t block_lds;       // k_block: dynamic LDS bytes

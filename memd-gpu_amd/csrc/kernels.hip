@@ -791,10 +791,14 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
 
 // Shared factors of one (direction, envelope): per block the forward factors,
 // the reciprocals of the intervals it owns and its two spike vectors; then the
-// reduced system's factorisation (one lane, J steps).
-__global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
+// reduced system's factorisation (one lane, J steps).  NT threads per item: 64
+// when many items share the device, 256 when few do (a single epoch has 2K
+// items), which cuts the rounds of the row and block phases by 4.  The
+// arithmetic of every row is the same either way.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
-    __shared__ double s_rlo[64], s_rdi[64], s_rup[64];
+    __shared__ double s_rlo[NT], s_rdi[NT], s_rup[NT];
     const ListCursor cu = list_cursor(wl, kListFactor);
     const int lane = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
@@ -811,17 +815,17 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
         // every row: the reciprocals of interval u (u < E-1) and, at the rows outside
         // the blocks, al = be = 0; all lanes, 4 rows per lane per round with their
         // loads issued together
-        for (int u0 = lane; u0 < E; u0 += 64 * 4) {
+        for (int u0 = lane; u0 < E; u0 += NT * 4) {
             int xa[4], xb[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int u = min(u0 + 64 * g, E - 2);
+                const int u = min(u0 + NT * g, E - 2);
                 xa[g] = X[u];
                 xb[g] = X[u + 1];
             }
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int u = u0 + 64 * g;
+                const int u = u0 + NT * g;
                 if (u < E) {
                     if (u < E - 1) {
                         const double h = (double)(xb[g] - xa[g]);
@@ -835,7 +839,7 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
             }
         }
         // one lane per block: the block's rows live in registers, memory sees stores only
-        for (int b = lane; b <= J; b += 64) {
+        for (int b = lane; b <= J; b += NT) {
             const BlockGeom g = block_geom(E, b);
             const int p = g.p, len = g.q - g.p + 1;
             int xs[L + 2];   // X[p-1 .. q+1]
@@ -882,7 +886,7 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
             double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
             double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
             double cprev = 0.0;
-            for (int j0 = 1; j0 <= J; j0 += 64) {
+            for (int j0 = 1; j0 <= J; j0 += NT) {
                 const int j = j0 + lane;
                 if (j <= J) {
                     const int sp = j * L;
@@ -896,7 +900,7 @@ __global__ __launch_bounds__(64) void k_factor(const EpochDesc* __restrict__ eps
                 }
                 __syncthreads();
                 if (lane == 0) {
-                    const int jn = min(64, J - j0 + 1);
+                    const int jn = min(NT, J - j0 + 1);
                     for (int jj = 0; jj < jn; ++jj) {
                         const int jg = j0 + jj;
                         double rlo = s_rlo[jj];
@@ -2111,7 +2115,10 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
     case 1:
         hipLaunchKernelGGL(k_compact, dim3(p.blocks[kMapCompact]), dim3(128), 0, s, p.eps, p.tab[kMapCompact], p.wl);
         break;
-    case 2: MEMD_LIST_LAUNCH(k_factor, kListFactor, 64); break;
+    case 2:
+        if (p.factor_wide) MEMD_LIST_LAUNCH(k_factor<256>, kListFactor, 256);
+        else MEMD_LIST_LAUNCH(k_factor<64>, kListFactor, 64);
+        break;
     case 3:
         hipLaunchKernelGGL(k_block, dim3(p.blocks[kMapBlock]), dim3(kBlockThreads), p.block_lds, s, p.eps,
                            p.tab[kMapBlock], p.block_rmax, (int)p.block_tile_off);
