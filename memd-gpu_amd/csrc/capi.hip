@@ -149,7 +149,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // the record path's array only
     L.sep = take((size_t)K * 2 * np * (nb + 1) * 8);
-    L.sepin = take((size_t)K * 2 * nb * 3 * n * 8);
+    L.sepin = take((size_t)K * 2 * nb * 3 * np * 8);
     L.mean = take((size_t)n * T * 8);
     L.tstat = take((size_t)T * 16);
     L.red = take(64);

@@ -132,7 +132,7 @@ struct EpochDesc {
                            //   not-a-knot end value) at the end rows (i0 = i1 = 0 at both)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][nbmax + 1][np] Z: Z[0] = Z[J+1] = 0, Z[j] = separator value j
-    double* sepin;         // [K][2][nbmax][3][n] y before, right-hand side, y after each separator
+    double* sepin;         // [K][2][nbmax][3][np] y before, right-hand side, y after each separator
     double* mean;          // [n][T] stop mode: envelope mean
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
     int* stopred;          // [4] stop mode: reduction scratch

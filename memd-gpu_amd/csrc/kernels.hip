@@ -993,7 +993,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const double* curt = ed.curt;   // [T][np] current signal, sample-major
         const GMut<double> ye = gmut(ed.yend + (size_t)ke * 2 * np);   // [2][np]: Y_0, Y_{E-1} clamped
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
-        const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * n);   // [separator][3][channel]
+        const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * np);   // [separator][3][np]
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions of tile rows 0..RW-1, factors of rows p0..qlast, reciprocals of
         //     intervals p0-1..qlast+1 (the last one: the next separator's right-hand side;
@@ -1084,7 +1084,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
             const double* Yc = sY + off * np + c;   // Yc[i * np] = Y(p - 1 + i)
             const double sl = (Yc[(len + 2) * np] - Yc[(len + 1) * np]) * s_ih[off + len + 1];
             const double slm = (Yc[(len + 1) * np] - Yc[len * np]) * s_ih[off + len];
-            sip[((size_t)(b0 + slot + 1) * 3 + 1) * n + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
+            sip[((size_t)(b0 + slot + 1) * 3 + 1) * np + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
         }
         __syncthreads();
         for (int lane = tid; lane < n * nb; lane += kBlockThreads) {
@@ -1131,8 +1131,8 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
             const double yv = sY[(r + 1) * np + c2];
             ys[(unsigned)u * (unsigned)np + (unsigned)c2] = yv;
             // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
-            if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * n + c2] = yv;
-            if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * n + c2] = yv;
+            if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * np + c2] = yv;
+            if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * np + c2] = yv;
         }
         it = nit;
     }
@@ -1156,96 +1156,139 @@ __device__ __forceinline__ double knot_M(const double* row, int rst, const doubl
 }
 
 // Phase 2: separator values, one lane per channel, one wave per (direction,
-// envelope) (no division: the reduced factorisation comes from k_factor).
+// envelope) (no division: the reduced factorisation comes from k_factor).  The
+// chain over the J separators is serial per channel, so its inputs are staged in
+// LDS a chunk of separators at a time by 16-byte global-to-LDS loads (they hold
+// no registers: a whole chunk is one round trip), and the chain reads LDS only;
+// the back substitution stages the forward values the same way.  Arithmetic per
+// channel and separator as the oracle's phase 2.
+constexpr int kRedStage = 3072;   // doubles per chunk: separators x 3 x np (forward), x np (back)
 __global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
+    constexpr int kCh = (kMaxChannels + 63) / 64;   // channels per lane
+    __shared__ __align__(16) double sIn[kRedStage];
+    __shared__ double sA[64], sB[64];
+    __shared__ int sLo[64], sUp[64];
     const ListCursor cu = list_cursor(wl, kListReduced);
+    const int lane = threadIdx.x;
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
         const int ke = it.y;
-        const int n = ed.n;
+        const int n = ed.n, np = ed.np;
         const int E = ed.E[ke];
         const int J = (E - 3) / L;
         const int* X = ed.X + (size_t)ke * ed.Ecap;
         const double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
         const double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
         const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            const int np = ed.np;
-            const double* row = ed.ysol + (size_t)ke * ed.Ecap * np + c;   // row[u * np]
-            // separator-major arrays: lanes (channels) read and write consecutive words
-            double* sv = zsep_base(ed, ke) + c;   // sv[j * np] = Z[j]
-            const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * n + c;   // sip[(3j + v) * n]
-            const GPtr<double> sing = gptr(sip);
-            const GMut<double> svg = gmut(sv);
-            const GMut<double> rowg = gmut(const_cast<double*>(row));   // y^ of the separator rows
-            const GPtr<double> Rlog = gptr(Rlo), Rrdg = gptr(Rrd), Rcpg = gptr(Rcp);
-            const GPtr<int> Xg = gptr(X);
-            // forward sweep over the separators, inputs of RC separators loaded together
-            constexpr int RC = 8;
-            double z = 0.0;
-            for (int j0 = 1; j0 <= J; j0 += RC) {
-                double ym[RC], yc[RC], yp[RC], rl[RC], rr[RC];
-                int xm[RC], x0[RC], xp[RC];
+        const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * np;   // [j][3][np]: y before, r, y after
+        double* sv = zsep_base(ed, ke);                                    // [j][np]: Z[j]
+        double* row = ed.ysol + (size_t)ke * ed.Ecap * np;                 // [u][np]: y^ (separator rows)
+        const GPtr<int> Xg = gptr(X);
+        double z[kCh];
 #pragma unroll
-                for (int a = 0; a < RC; ++a) {
-                    const int j = min(j0 + a, J), s0 = j * L;
-                    ym[a] = sing[(3 * j + 0) * n];   // y of row s0-1, r of row s0, y of row s0+1
-                    yc[a] = sing[(3 * j + 1) * n];
-                    yp[a] = sing[(3 * j + 2) * n];
-                    rl[a] = Rlog[j];
-                    rr[a] = Rrdg[j];
-                    xm[a] = Xg[s0 - 1];
-                    x0[a] = Xg[s0];
-                    xp[a] = Xg[s0 + 1];
-                }
-#pragma unroll
-                for (int a = 0; a < RC; ++a) {
-                    const int j = j0 + a;
-                    if (j > J) break;
-                    const double lo = (double)(x0[a] - xm[a]), up = (double)(xp[a] - x0[a]);
-                    const double D = fma(-up, yp[a], fma(-lo, ym[a], yc[a]));
-                    z = (j == 1) ? D * rr[a] : fma(-rl[a], z, D) * rr[a];
-                    svg[j * np] = z;
+        for (int cc = 0; cc < kCh; ++cc) z[cc] = 0.0;
+        // forward sweep
+        const int CH = min(64, kRedStage / (3 * np));
+        for (int j0 = 1; j0 <= J; j0 += CH) {
+            const int cnt = min(CH, J - j0 + 1);
+            __syncthreads();   // the previous chunk is no longer read
+            {
+                const double* src = sip + (size_t)3 * j0 * np;
+                const int pieces = cnt * 3 * np / 2;   // np is a multiple of 4
+                for (int q0 = 0; q0 < pieces; q0 += 64) {
+                    const int q = q0 + lane;
+                    if (q < pieces)
+                        __builtin_amdgcn_global_load_lds(src + 2 * q, (__attribute__((address_space(3))) void*)(sIn + 2 * q0),
+                                                         16, 0, 0);
                 }
             }
-            // back substitution, RC separators at a time; every separator value also
-            // becomes y^ of its knot row
-            double x = z;
-            if (J > 0) rowg[(unsigned)(J * L) * (unsigned)np] = x;
-            for (int j1 = J - 1; j1 >= 1; j1 -= RC) {
-                double cpj[RC], zj[RC];
-#pragma unroll
-                for (int a = 0; a < RC; ++a) {
-                    const int j = max(j1 - a, 1);
-                    cpj[a] = Rcpg[j];
-                    zj[a] = svg[j * np];
-                }
-#pragma unroll
-                for (int a = 0; a < RC; ++a) {
-                    const int j = j1 - a;
-                    if (j < 1) break;
-                    x = fma(-cpj[a], x, zj[a]);
-                    svg[j * np] = x;
-                    rowg[(unsigned)(j * L) * (unsigned)np] = x;
-                }
+            if (lane < cnt) {
+                const int j = j0 + lane, s0 = j * L;
+                const int xm = Xg[s0 - 1], x0 = Xg[s0], xp = Xg[s0 + 1];
+                sLo[lane] = x0 - xm;
+                sUp[lane] = xp - x0;
+                sA[lane] = Rlo[j];
+                sB[lane] = Rrd[j];
             }
-            if (ed.spline) {
-                // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2): y^ of
-                // the end rows (k_block wrote 0 there)
-                const double* AB = fac_plane(ed, ke, kFacAB);
-                const double* IH = fac_plane(ed, ke, kFacIH);
-                double M1 = knot_M(row, np, sv, np, AB, 1, E, J);
-                double M2 = knot_M(row, np, sv, np, AB, 2, E, J);
-                double MN1 = knot_M(row, np, sv, np, AB, E - 3, E, J);
-                double MN = knot_M(row, np, sv, np, AB, E - 2, E, J);
-                double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
-                double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
-                rowg[0] = ((h0 + h1) * M1 - h0 * M2) * IH[2 * 1];
-                rowg[(unsigned)(E - 1) * (unsigned)np] = ((a + b) * MN - b * MN1) * IH[2 * (size_t)(E - 3)];
+            __builtin_amdgcn_s_waitcnt(0);   // the staged chunk has landed (vmcnt) before the barrier
+            __syncthreads();
+#pragma unroll
+            for (int cc = 0; cc < kCh; ++cc) {
+                const int c = lane + 64 * cc;
+                if (c < n) {
+                    double zc = z[cc];
+                    for (int jj = 0; jj < cnt; ++jj) {
+                        const int j = j0 + jj;
+                        const double lo = (double)sLo[jj], up = (double)sUp[jj];
+                        const double* y3 = sIn + (size_t)3 * jj * np + c;
+                        const double D = fma(-up, y3[2 * np], fma(-lo, y3[0], y3[np]));
+                        zc = (j == 1) ? D * sB[jj] : fma(-sA[jj], zc, D) * sB[jj];
+                        sv[(size_t)j * np + c] = zc;
+                    }
+                    z[cc] = zc;
+                }
             }
         }
+        // back substitution: x_J = z_J, x_j = fma(-cp_j, x_{j+1}, z_j); every separator
+        // value also becomes y^ of its knot row
+#pragma unroll
+        for (int cc = 0; cc < kCh; ++cc) {
+            const int c = lane + 64 * cc;
+            if (c < n && J > 0) row[(size_t)(J * L) * np + c] = z[cc];
+        }
+        const int CB = min(64, kRedStage / np);
+        for (int j1 = J - 1; j1 >= 1; j1 -= CB) {
+            const int cnt = min(CB, j1), jlo = j1 - cnt + 1;
+            __syncthreads();   // the forward values are stored (vmcnt) and the previous chunk read
+            {
+                const double* src = sv + (size_t)jlo * np;
+                const int pieces = cnt * np / 2;
+                for (int q0 = 0; q0 < pieces; q0 += 64) {
+                    const int q = q0 + lane;
+                    if (q < pieces)
+                        __builtin_amdgcn_global_load_lds(src + 2 * q, (__attribute__((address_space(3))) void*)(sIn + 2 * q0),
+                                                         16, 0, 0);
+                }
+            }
+            if (lane < cnt) sA[lane] = Rcp[jlo + lane];
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+#pragma unroll
+            for (int cc = 0; cc < kCh; ++cc) {
+                const int c = lane + 64 * cc;
+                if (c < n) {
+                    double xc = z[cc];
+                    for (int j = j1; j >= jlo; --j) {
+                        xc = fma(-sA[j - jlo], xc, sIn[(size_t)(j - jlo) * np + c]);
+                        sv[(size_t)j * np + c] = xc;
+                        row[(size_t)(j * L) * np + c] = xc;
+                    }
+                    z[cc] = xc;
+                }
+            }
+        }
+        if (ed.spline) {
+            // not-a-knot end values from M_1, M_2 and M_{N-1}, M_N (N = E-2): y^ of the
+            // end rows (k_block wrote 0 there); each lane reads back its own channel's
+            // stores
+            const double* AB = fac_plane(ed, ke, kFacAB);
+            const double* IH = fac_plane(ed, ke, kFacIH);
+            for (int c = lane; c < n; c += 64) {
+                const double* rc = row + c;
+                const double* svc = sv + c;
+                double M1 = knot_M(rc, np, svc, np, AB, 1, E, J);
+                double M2 = knot_M(rc, np, svc, np, AB, 2, E, J);
+                double MN1 = knot_M(rc, np, svc, np, AB, E - 3, E, J);
+                double MN = knot_M(rc, np, svc, np, AB, E - 2, E, J);
+                double h0 = (double)(X[1] - X[0]), h1 = (double)(X[2] - X[1]);
+                double a = (double)(X[E - 2] - X[E - 3]), b = (double)(X[E - 1] - X[E - 2]);
+                row[c] = ((h0 + h1) * M1 - h0 * M2) * IH[2 * 1];
+                row[(size_t)(E - 1) * np + c] = ((a + b) * MN - b * MN1) * IH[2 * (size_t)(E - 3)];
+            }
+        }
+        __syncthreads();   // the next item reuses the staging arrays
     }
 }
 
