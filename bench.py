@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--saturate", type=int, default=0,
                     help="served: also submit this many epochs of the mix at once, drain them through the server "
                          "and time the same epochs as one batched call")
+    ap.add_argument("--mode", choices=("fix", "stop"), default="fix",
+                    help="batched32x4096: sifting policy (stop = memd.m criterion 0.075/0.75/0.075, at most --sifts)")
+    ap.add_argument("--boundary", choices=("clamp", "mirror"), default="clamp",
+                    help="batched32x4096: end treatment (mirror = memd.m boundary_conditions)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     return ap.parse_args()
@@ -109,31 +113,31 @@ def cpu_threads():
     return max(1, min(16, aff))
 
 
-def cpu_whole_epochs(gen, n, K, n_imf, S, reps, threads):
+def cpu_whole_epochs(gen, n, K, n_imf, S, reps, threads, **opts):
     """Median seconds of `reps` whole epochs (distinct inputs from gen(i)), FP64
     oracle with OpenMP over channels, after a one-sift warm-up."""
     from oracle import memd_oracle
     V = memd_oracle.directions(n, K)
-    memd_oracle.memd(gen(0), V, 2, 1, threads=threads, trace=False)
+    memd_oracle.memd(gen(0), V, 2, 1, threads=threads, trace=False, **opts)
     times = []
     for r in range(reps):
         y = gen(1 + r)
         t0 = time.perf_counter()
-        memd_oracle.memd(y, V, n_imf, S, threads=threads, trace=False)
+        memd_oracle.memd(y, V, n_imf, S, threads=threads, trace=False, **opts)
         times.append(time.perf_counter() - t0)
         print(f"cpu: whole epoch {r + 1}/{reps} of {n} x {y.shape[1]} in {times[-1]:.1f} s", file=sys.stderr, flush=True)
     return float(np.median(times)), times
 
 
-def cpu_epoch_parallel(ys_Vs, n_imf, S, threads):
+def cpu_epoch_parallel(ys_Vs, n_imf, S, threads, **opts):
     """Epochs per second of the oracle's epoch-parallel batch (one epoch per thread)."""
     from oracle import memd_oracle
     t0 = time.perf_counter()
-    memd_oracle.memd_batch(ys_Vs, n_imf, S, threads=threads)
+    memd_oracle.memd_batch(ys_Vs, n_imf, S, threads=threads, **opts)
     return len(ys_Vs) / (time.perf_counter() - t0)
 
 
-def cpu_baseline(n, T, K, n_imf, S):
+def cpu_baseline(n, T, K, n_imf, S, **opts):
     """The FP64 oracle on the same workload, whole epochs, three ways: OpenMP over
     channels (median of 5 epochs after a warm-up), epochs in parallel (one per
     thread, as a batched CPU run would), and one thread.  value = the faster of the
@@ -142,12 +146,12 @@ def cpu_baseline(n, T, K, n_imf, S):
     import synth
     threads = cpu_threads()
     gen = lambda i: synth.sines_epoch(n, T, 5000 + i)
-    med, times = cpu_whole_epochs(gen, n, K, n_imf, S, 5, threads)
+    med, times = cpu_whole_epochs(gen, n, K, n_imf, S, 5, threads, **opts)
     chan = 1.0 / med
     V = memd_oracle.directions(n, K)
-    par = cpu_epoch_parallel([(gen(100 + i), V) for i in range(threads)], n_imf, S, threads)
+    par = cpu_epoch_parallel([(gen(100 + i), V) for i in range(threads)], n_imf, S, threads, **opts)
     t0 = time.perf_counter()
-    memd_oracle.memd(gen(200), V, n_imf, S, threads=1, trace=False)
+    memd_oracle.memd(gen(200), V, n_imf, S, threads=1, trace=False, **opts)
     single = 1.0 / (time.perf_counter() - t0)
     best = max(chan, par)
     return {"value": best, "unit": "epochs/s", "cores": threads, "kind": "port",
@@ -503,6 +507,8 @@ def main():
         return bench_wide(a)
 
     n, T, K, N_imf, S, B = a.n, a.T, a.K, a.n_imf, a.sifts, a.batch
+    mode = memd_amd.MODE_STOP if a.mode == "stop" else memd_amd.MODE_FIX
+    bnd = memd_amd.BOUNDARY_MIRROR if a.boundary == "mirror" else memd_amd.BOUNDARY_CLAMP
     dev = "cuda:0"
     ctx = memd_amd.MemdContext(0)
     tuning, tune_s = None, 0.0
@@ -520,13 +526,13 @@ def main():
     eps = [{"y": ys[i], "V": V, "n_imf": N_imf, "S": S, "out": outs[i]} for i in range(B)]
 
     for _ in range(a.warmup):
-        ctx.run_batch(eps)
+        ctx.run_batch(eps, mode=mode, boundary=bnd)
     torch.cuda.synchronize()
     lat = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         s0 = time.perf_counter()
-        ctx.run_batch(eps)
+        ctx.run_batch(eps, mode=mode, boundary=bnd)
         lat.append(time.perf_counter() - s0)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
@@ -535,18 +541,19 @@ def main():
     # plain per-epoch path (one call per epoch), latency per epoch
     plain_lat = []
     if a.plain_epochs > 0:
-        ctx.run(ys[0], V, N_imf, S, out=outs[0])
+        ctx.run(ys[0], V, N_imf, S, out=outs[0], mode=mode, boundary=bnd)
         torch.cuda.synchronize()
         for i in range(a.plain_epochs):
             s0 = time.perf_counter()
-            ctx.run(ys[i % B], V, N_imf, S, out=outs[i % B])
+            ctx.run(ys[i % B], V, N_imf, S, out=outs[i % B], mode=mode, boundary=bnd)
             plain_lat.append(time.perf_counter() - s0)
     plain_eps = len(plain_lat) / sum(plain_lat) if plain_lat else None
 
-    workload = f"memd_{n}ch_x{T}_K{K}_fix{N_imf}x{S}_b{B}"
+    workload = f"memd_{n}ch_x{T}_K{K}_{a.mode}{N_imf}x{S}_b{B}" + ("_mirror" if a.boundary == "mirror" else "")
+    fused = a.boundary == "clamp"   # the fused evaluation kernel runs (mirrored ends: the record path)
     roofline, kernels = None, None
     if not a.no_profile:
-        _, info = ctx.run_batch(eps, profile=True)
+        _, info = ctx.run_batch(eps, profile=True, mode=mode, boundary=bnd)
         pr = info["profile"]
         kernels = {k: round(v * 1e3, 2) for k, v in pr["ms_per_launch"].items()}   # us per launch
         steps = max(pr["steps"], 1)
@@ -557,9 +564,24 @@ def main():
         # the two reciprocals (36 B); flag word + prefix per (epoch, direction, envelope,
         # 64 samples) (12 B).  Flops: two 3-fma Horners + mean per evaluation point (15),
         # record build per knot-channel (10), divide + subtract per sample (2).
-        bytes_eval = (24 * pr["samples"] + 8 * pr["knot_channels"] + 36 * pr["knots"]) / steps \
-            + 12 * 2 * K * W * B
-        flops_eval = (15 * pr["eval_points"] + 10 * pr["knot_channels"] + 2 * pr["samples"]) / steps
+        if fused and a.mode == "fix":
+            bytes_eval = (24 * pr["samples"] + 8 * pr["knot_channels"] + 36 * pr["knots"]) / steps \
+                + 12 * 2 * K * W * B
+            flops_eval = (15 * pr["eval_points"] + 10 * pr["knot_channels"] + 2 * pr["samples"]) / steps
+        elif fused:
+            # stop mode: the mean written instead of the signal updated (16 B/sample with the
+            # knot-value reads), and one block sum per (valid direction, 16 channels, sample);
+            # the difference, its square and the group adds per evaluation point (+3)
+            G16 = (n + 15) // 16
+            bytes_eval = (16 * pr["samples"] + 8 * pr["knot_channels"] + 36 * pr["knots"]
+                          + 8 * pr["eval_points"] * G16 / n) / steps + 12 * 2 * K * W * B
+            flops_eval = (18 * pr["eval_points"] + 10 * pr["knot_channels"] + 1 * pr["samples"]) / steps
+        else:
+            # record-path evaluation: one 32-byte record per knot-channel read, the mean
+            # written and |mean|^2, amplitude per sample (8 + 16 B); two Horners, the mean
+            # and the squared difference per evaluation point (18)
+            bytes_eval = (32 * pr["knot_channels"] + 8 * pr["samples"]) / steps + 16 * T * B
+            flops_eval = 18 * pr["eval_points"] / steps
         t_eval = pr["ms_per_launch"]["k_eval"] * 1e-3   # slot of the fused record+evaluation kernel
         gbs = bytes_eval / t_eval / 1e9
         tfs = flops_eval / t_eval / 1e12
@@ -571,11 +593,12 @@ def main():
             roofline = {"bound": "fp64", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tfs / FP64_PEAK_TFLOPS, 4), "traffic": None}
         roofline["traffic"] = measured_traffic(workload, "k_eval_fused")
-        roofline.update({"kernel": "k_eval_fused", "launch_us": round(t_eval * 1e6, 2),
+        roofline.update({"kernel": ("k_eval_fused" if a.mode == "fix" else "k_eval_fused<stop>") if fused
+                         else "k_eval_stop", "launch_us": round(t_eval * 1e6, 2),
                          "alg_bytes_per_launch": int(bytes_eval), "alg_flops_per_launch": int(flops_eval),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_frac": round(tfs / FP64_PEAK_TFLOPS, 4)})
 
-    cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S)
+    cpu = None if a.no_cpu else cpu_baseline(n, T, K, N_imf, S, mode=mode, boundary=bnd)
     pct_or_none = lambda xs, q: round(pct(xs, q) * 1e3, 3) if xs else None
     ctx.close()
     line = {
@@ -584,7 +607,8 @@ def main():
         "ms_per_step": round(total / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "n": n, "T": T, "K": K,
-                   "N_imf": N_imf, "S": S, "mode": "fix", "window": "refwarp32", "batch_epochs": B,
+                   "N_imf": N_imf, "S": S, "mode": a.mode, "boundary": a.boundary, "window": "refwarp32",
+                   "batch_epochs": B,
                    "generator": "sines 5/20/60 Hz, N(0,1)/sqrt(n) mixing, N(0,0.1^2) noise"},
         "latency_ms": {"batched_p50": round(pct(lat, 50) * 1e3, 3), "batched_p99": round(pct(lat, 99) * 1e3, 3),
                        "plain_p50": pct_or_none(plain_lat, 50), "plain_p99": pct_or_none(plain_lat, 99)},

@@ -126,10 +126,29 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, mir, X, fac, redsys, ysol, coef, sep, sepin, mean, tstat, red, sift, st, total;
+    size_t running, curt, mask, pre, E, mir, X, fac, redsys, ysol, coef, sep, sepin, mean, ampb, tstat, red, sift, st,
+        total;
 };
 
-EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
+// Kernel path of a call: stop mode and mirrored ends run the stop-mode kernels
+// (decision and apply); mirrored ends evaluate from the spline record array (the
+// record path), the reference's ends with the fused evaluation, which in stop
+// mode leaves per-block amplitude sums for k_stop_amp.
+struct PathKind {
+    bool stop_kernels = false;
+    bool records = false;
+    bool ampb = false;
+};
+PathKind path_for(const memd_opts& o) {
+    PathKind k;
+    k.records = o.boundary == MEMD_BOUNDARY_MIRROR;
+    k.stop_kernels = o.mode == MEMD_MODE_STOP || k.records;
+    k.ampb = o.mode == MEMD_MODE_STOP && !k.records;
+    return k;
+}
+
+EpochLayout layout_for(int n, int T, int K, int N_imf, const PathKind& pk) {
+    const bool records = pk.records;
     EpochLayout L{};
     size_t W = (size_t)(T + 63) / 64;
     size_t Ecap = (size_t)knot_capacity(T);
@@ -151,6 +170,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, bool records) {
     L.sep = take((size_t)K * 2 * np * (nb + 1) * 8);
     L.sepin = take((size_t)K * 2 * nb * 3 * np * 8);
     L.mean = take((size_t)n * T * 8);
+    L.ampb = take(pk.ampb ? (size_t)K * ((n + 15) / 16) * T * 8 : 0);
     L.tstat = take((size_t)T * 16);
     L.red = take(64);
     L.sift = take((size_t)(N_imf > 0 ? N_imf : 1) * 4);
@@ -224,7 +244,8 @@ constexpr int kBlockWgTarget = 1024;
 
 // Descriptor fields that depend only on the epoch's shape, the options and the
 // number of epochs in the launch.
-EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, bool records, int batch) {
+EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& o, const PathKind& pk, int batch) {
+    const bool records = pk.records;
     EpochDesc d{};
     d.n = I.n; d.T = I.T; d.K = I.K; d.W = (I.T + 63) / 64;
     d.np = channel_stride(I.n);
@@ -238,7 +259,7 @@ EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& 
     d.boundary = o.boundary;
     const memd_tuning tu = tuning_for(ctx, I.n);
     d.bpw = tu.block_groups;
-    d.etw = tu.eval_tile_words;
+    d.etw = pk.ampb ? 1 : tu.eval_tile_words;   // stop mode's block sums take 16-channel tiles
     // k_block items: at most 2K * groups; workgroups per epoch: the item count at a
     // typical knot density (one knot per 16 samples), at most the cap above
     d.bcap = 2 * I.K * ((d.nbmax + d.bpw - 1) / d.bpw);
@@ -266,6 +287,7 @@ void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.sep = (double*)(b + L.sep);
     d.sepin = (double*)(b + L.sepin);
     d.mean = (double*)(b + L.mean);
+    d.ampb = (double*)(b + L.ampb);
     d.tstat = (double*)(b + L.tstat);
     d.stopred = (int*)(b + L.red);
     d.sift_counts = (int*)(b + L.sift);
@@ -507,14 +529,10 @@ memd_status check_opts(memd_ctx* ctx, const memd_opts& o) {
 }
 
 // The record-path kernels (spline records in HBM, one lane per sample walking the
-// channels; at most kStopMaxChannels channels) serve stop mode, and both modes
-// with mirrored ends; fix mode with the reference's ends uses the fused evaluation.
-bool record_path(const memd_opts& o) { return o.mode == MEMD_MODE_STOP || o.boundary == MEMD_BOUNDARY_MIRROR; }
-
+// channels; at most kStopMaxChannels channels) serve mirrored ends in both modes.
 memd_status check_record_channels(memd_ctx* ctx, const memd_opts& o, int n) {
-    if (record_path(o) && n > kStopMaxChannels)
-        return fail(ctx, MEMD_ERR_UNSUPPORTED, "stop mode and mirrored ends support at most %d channels (got %d)",
-                    kStopMaxChannels, n);
+    if (path_for(o).records && n > kStopMaxChannels)
+        return fail(ctx, MEMD_ERR_UNSUPPORTED, "mirrored ends support at most %d channels (got %d)", kStopMaxChannels, n);
     return MEMD_OK;
 }
 
@@ -534,16 +552,17 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     if ((rs = check_record_channels(ctx, o, nmax)) != MEMD_OK) return rs;
 
     // ---- workspace plan: every epoch's workspace, then the plan region
-    const bool records = record_path(o);
+    const PathKind pk = path_for(o);
+    const bool records = pk.records;
     std::vector<EpochLayout> lay(count);
     std::vector<size_t> base(count);
     std::vector<EpochDesc> eds(count);
     size_t total = 0;
     for (int e = 0; e < count; ++e) {
-        lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, records);
+        lay[e] = layout_for(in[e].n, in[e].T, in[e].K, in[e].N_imf, pk);
         base[e] = total;
         total += lay[e].total;
-        eds[e] = desc_geometry(ctx, in[e], o, records, count);
+        eds[e] = desc_geometry(ctx, in[e], o, pk, count);
     }
     // XCD affinity when the batch has enough epochs to keep every XCD busy
     const int groups = count >= kXcds ? kXcds : 1;
@@ -578,7 +597,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, launch_zero_ints(ndone_dev, 1, s));   // a kernel, not a memset node: callers may capture
     StepPlan plan{};
     std::vector<int2> tabs;
-    rs = write_plan(ctx, eds, grp, P, arena, records ? kModeStop : kModeFix, records, tabs, s, &plan);
+    rs = write_plan(ctx, eds, grp, P, arena, pk.stop_kernels ? kModeStop : kModeFix, records, tabs, s, &plan);
     if (rs != MEMD_OK) return rs;
 
     // Fix mode: exactly (N_imf-1)*S steps.  Stop mode: at most (N_imf-1)*(S+1)
@@ -824,13 +843,12 @@ void memd_opts_default(memd_opts* o) {
 
 size_t memd_workspace_bytes(size_t n, size_t T, size_t K) {
     if (n < 1 || T < 4 || K < 1) return 0;
-    return layout_for((int)n, (int)T, (int)K, 1, false).total;
+    return layout_for((int)n, (int)T, (int)K, 1, PathKind{}).total;
 }
 
 size_t memd_workspace_bytes_opts(size_t n, size_t T, size_t K, size_t N_imf, const memd_opts* opts) {
     if (n < 1 || T < 4 || K < 1 || N_imf < 1) return 0;
-    const bool records = opts && record_path(*opts);
-    return layout_for((int)n, (int)T, (int)K, (int)N_imf, records).total;
+    return layout_for((int)n, (int)T, (int)K, (int)N_imf, opts ? path_for(*opts) : PathKind{}).total;
 }
 
 memd_status memd_ctx_create(int device, memd_ctx** out) {
@@ -1232,7 +1250,7 @@ struct memd_server {
     };
     memd_ctx* ctx = nullptr;
     memd_opts opts{};
-    bool records = false;
+    PathKind path;
     char* pool = nullptr;
     size_t pool_bytes = 0, pool_used = 0;
     std::map<size_t, size_t> free_blocks;   // offset -> bytes, coalesced
@@ -1370,7 +1388,7 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     std::map<int64_t, int> index;
     for (int e = 0; e < count; ++e) {
         memd_server::Req& r = S->reqs[S->active[e]];
-        S->eds[e] = desc_geometry(ctx, r.in, S->opts, S->records, count);
+        S->eds[e] = desc_geometry(ctx, r.in, S->opts, S->path, count);
         bind_workspace(S->eds[e], S->pool + r.off, r.lay);
         S->grp[e] = groups > 1 ? r.grp : 0;
         index[S->active[e]] = e;
@@ -1401,8 +1419,8 @@ memd_status server_rebuild(memd_server* S, const std::vector<int64_t>& joined, h
     }
     int* ndone = (int*)(S->region + S->P.ndone_at);
     for (EpochDesc& d : S->eds) d.ndone = ndone;
-    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->records ? kModeStop : kModeFix, S->records, S->tabs, s,
-                    &S->plan);
+    rs = write_plan(ctx, S->eds, S->grp, S->P, S->region, S->path.stop_kernels ? kModeStop : kModeFix, S->path.records,
+                    S->tabs, s, &S->plan);
     if (rs != MEMD_OK) return rs;
     S->sift_off.resize(count);
     for (int e = 0, at = 0; e < count; ++e) {
@@ -1591,7 +1609,7 @@ memd_status memd_server_create(memd_ctx* ctx, size_t hbm_pool_bytes, int32_t max
     memd_server* S = new memd_server();
     S->ctx = ctx;
     S->opts = o;
-    S->records = record_path(o);
+    S->path = path_for(o);
     S->max_active = max_active;
     S->pool_bytes = hbm_pool_bytes / kAlign * kAlign;
     hipError_t e = dev_alloc(ctx, (void**)&S->pool, S->pool_bytes);
@@ -1639,7 +1657,7 @@ memd_status memd_server_submit(memd_server* S, const memd_epoch* ep, int64_t* id
     memd_server::Req r;
     r.in = {(int)ep->n, (int)ep->T, (int)ep->K, (int)ep->N_imf, (int)ep->S, ep->d_y, ep->d_V, ep->d_IMFs,
             ep->h_sift_counts, ep->h_n_imf_out};
-    r.lay = layout_for(r.in.n, r.in.T, r.in.K, r.in.N_imf, S->records);
+    r.lay = layout_for(r.in.n, r.in.T, r.in.K, r.in.N_imf, S->path);
     if (r.lay.total > S->pool_bytes)
         return fail(ctx, MEMD_ERR_ADMISSION, "epoch needs %zu workspace bytes, the pool holds %zu", r.lay.total,
                     S->pool_bytes);

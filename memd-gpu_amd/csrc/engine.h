@@ -79,7 +79,7 @@ enum : int { kModeFix = 0, kModeStop = 1 };
 // finish the IMF without subtracting, or end the decomposition.
 enum : int { kActContinue = 0, kActFinishImf = 1, kActEnd = 2, kActNone = 3 };
 // Stop mode keeps per-sample channel sums in LDS: 64 samples x n channels.
-constexpr int kStopMaxChannels = 128;
+constexpr int kStopMaxChannels = 128;   // the record path (mirrored ends)
 enum : int { kWindowRefWarp32 = 0, kWindowFull = 1 };
 
 // Per-epoch device state, advanced on the device after every sift step.
@@ -134,6 +134,8 @@ struct EpochDesc {
     double* sep;           // [K][2][nbmax + 1][np] Z: Z[0] = Z[J+1] = 0, Z[j] = separator value j
     double* sepin;         // [K][2][nbmax][3][np] y before, right-hand side, y after each separator
     double* mean;          // [n][T] stop mode: envelope mean
+    double* ampb;          // [K][ceil(n/16)][T] stop mode, fused evaluation: per valid direction and
+                           //   block of 16 channels the sum of (e+ - e-)^2 (groups of 4 in order)
     double* tstat;         // [T][2] stop mode: |mean(t)|^2, amplitude(t)
     int* stopred;          // [4] stop mode: reduction scratch
     int* d_trace;          // optional [max_passes][K][2]
@@ -159,7 +161,7 @@ struct EpochDesc {
 
 // Fixed-size launches: block b of a launch belongs to epoch tab[b].x as its
 // local block tab[b].y.
-enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kMapBlock, kNumMaps };
+enum : int { kMapInit = 0, kMapFlags, kMapCompact, kMapEval, kMapBlock, kMapAmp, kNumMaps };
 
 // List-driven launches: k_compact appends the exact work items (epoch, ke, sub)
 // of every valid direction; persistent grids drain the lists.

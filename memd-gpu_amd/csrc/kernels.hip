@@ -1466,14 +1466,20 @@ __device__ __forceinline__ double4 rec_load(const v2d* ab, const v2d* cd) {
     return r;
 }
 
+// DIFF: also dd[s] = e+ - e- at each sample (stop mode's amplitude).
+template <bool DIFF>
 __device__ __forceinline__ void eval_pair(const v2d* __restrict__ rua, const v2d* __restrict__ ruc,
                                           const int* __restrict__ xu, unsigned cu, const v2d* __restrict__ rla,
                                           const v2d* __restrict__ rlc, const int* __restrict__ xl, unsigned cl,
-                                          int ts, int tlast, double (&acc)[4]) {
+                                          int ts, int tlast, double (&acc)[4], double (&dd)[4]) {
     const int ju = xu[0], jl = xl[0];
     double4 qu = rec_load(rua, ruc), ql = rec_load(rla, rlc);
     double tu = (double)(ts - ju), tl = (double)(ts - jl);
-    acc[0] = acc[0] + (horner4(qu, tu) + horner4(ql, tl)) * 0.5;
+    {
+        const double eu = horner4(qu, tu), el = horner4(ql, tl);
+        acc[0] = acc[0] + (eu + el) * 0.5;
+        if (DIFF) dd[0] = eu - el;
+    }
 #pragma unroll
     for (int s = 1; s < 4; ++s) {
         if (ts + s <= tlast) {
@@ -1491,7 +1497,9 @@ __device__ __forceinline__ void eval_pair(const v2d* __restrict__ rua, const v2d
                 ql = rec_load(rla, rlc);
                 tl = 1.0;
             }
-            acc[s] = acc[s] + (horner4(qu, tu) + horner4(ql, tl)) * 0.5;
+            const double eu = horner4(qu, tu), el = horner4(ql, tl);
+            acc[s] = acc[s] + (eu + el) * 0.5;
+            if (DIFF) dd[s] = eu - el;
         }
     }
 }
@@ -1511,9 +1519,16 @@ __device__ __forceinline__ void eval_pair(const v2d* __restrict__ rua, const v2d
 //     as the record path, so the records are bit-identical;
 //   phase B: each lane evaluates both envelopes of every valid direction of the
 //     batch at its 4 samples and adds their mean in ascending direction order.
+// STOP (stop mode, TW = 1, the host guarantees it): the mean goes to ed.mean
+// instead of updating the signal, and per valid direction each wave sums the
+// squared envelope differences of its 4 channels in channel order (shuffles), wave
+// 0 adds the 4 waves' sums in order and stores the block's sum for k_stop_amp:
+// the amplitude's channel order of groups of 4 and blocks of 16.
+template <bool STOP>
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
+    __shared__ double sG[STOP ? 2 * 4 * 64 : 1];   // [buffer][wave][sample]: group sums
     // spline records, segment s: [channel][stride] from its record base, as two
     // arrays of 16-byte halves (a, b) and (c, d): a wave's record reads and writes
     // are then 16 bytes apart, not every other 16 bytes of 32-byte records
@@ -1751,6 +1766,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         for (int d = 0; d < nb; ++d) {
             if (!((vmask >> (2 * d)) & 1u)) continue;
             nv = nv + 1.0;
+            double dd[4] = {0.0, 0.0, 0.0, 0.0};
             if (lane_on) {
                 // both envelopes' flag words, knot counts and record offsets in one
                 // round of LDS reads; then the first interval (in the tile's window)
@@ -1763,8 +1779,32 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 const int i0 = c0 - (int)__popcll(h0), i1 = c1 - (int)__popcll(h1);
                 const unsigned x0 = (unsigned)h0 & 7u, x1 = (unsigned)h1 & 7u;
                 const int r0 = B0.x + __umul24(cl, B0.y) + i0, r1 = B1.x + __umul24(cl, B1.y) + i1;
-                eval_pair(recAB + r0, recCD + r0, xk + B0.w + i0, x0, recAB + r1, recCD + r1, xk + B1.w + i1, x1, ts,
-                          tlast, acc);
+                eval_pair<STOP>(recAB + r0, recCD + r0, xk + B0.w + i0, x0, recAB + r1, recCD + r1, xk + B1.w + i1,
+                                x1, ts, tlast, acc, dd);
+            }
+            if (STOP) {
+                // lanes (channel 4w + j, quad q) of wave w: lane 16 j + q; samples past
+                // the tile and absent channels carry 0, which adds exactly
+                const int buf = d & 1, lw = tid & 63;
+                double g[4];
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    const double x = dd[s2] * dd[s2];
+                    const double x1 = __shfl(x, (lw & 15) + 16), x2 = __shfl(x, (lw & 15) + 32),
+                                 x3 = __shfl(x, (lw & 15) + 48);
+                    g[s2] = ((x + x1) + x2) + x3;
+                }
+                if (lw < 16) {
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) sG[(buf * 4 + (tid >> 6)) * 64 + 4 * lw + s2] = g[s2];
+                }
+                __syncthreads();
+                if (tid < 64 && t0 + tid <= tlast) {
+                    const double* gb = sG + buf * 4 * 64 + tid;
+                    const double b = ((gb[0] + gb[64]) + gb[128]) + gb[192];
+                    const int G16 = (n + 15) >> 4;
+                    ed.ampb[((size_t)(kb + d) * G16 + cg) * T + t0 + tid] = b;
+                }
             }
         }
         __syncthreads();
@@ -1775,6 +1815,12 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
     }
     if (!(cl < cgn)) return;
+    if (STOP) {   // mean = sum / nvalid (0/0 = NaN with no valid direction: the decision then stops)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2)
+            if (ts + s2 <= tlast) ed.mean[(size_t)c * T + ts + s2] = acc[s2] / nv;
+        return;
+    }
     const GMut<double> cur = gmut(cur_slot(ed, st->imf));
     const bool last = (ed.mode == kModeFix) && (st->nsift == ed.S - 1);
     const GMut<double> nxt = gmut(cur_slot(ed, st->imf + 1));
@@ -1864,21 +1910,62 @@ __global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ 
         }
         double tau0 = (double)(tq - ed.X[(size_t)ke0 * ed.Ecap + j0]);
         double tau1 = (double)(tq - ed.X[(size_t)ke1 * ed.Ecap + j1]);
-        double s2 = 0.0;
+        // sum_c (e+ - e-)^2: sequential within groups of 4 channels, 4 groups to a
+        // block of 16, blocks in order (the fused evaluation's order)
+        double s2 = 0.0, bsum = 0.0, gsum = 0.0;
         for (int c = 0; c < n; ++c) {
             double ep = horner4(coef[((size_t)ke0 * n + c) * ed.Ecap + j0], tau0);
             double em = horner4(coef[((size_t)ke1 * n + c) * ed.Ecap + j1], tau1);
             double d = ep - em;
-            s2 = s2 + d * d;
+            const double x = d * d;
+            gsum = (c & 3) == 0 ? x : gsum + x;
+            if ((c & 3) == 3 || c == n - 1) {
+                const int g = (c >> 2) & 3;
+                bsum = g == 0 ? gsum : bsum + gsum;
+                if (g == 3 || c == n - 1) s2 = (c >> 4) == 0 ? bsum : s2 + bsum;
+            }
             sums[c * 64 + lane] = sums[c * 64 + lane] + (ep + em) * 0.5;
         }
         amps = amps + sqrt(s2) * 0.5;
+    }
+    if (ed.stats && br.y == 0 && lane == 0) {
+        atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
+        atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
     }
     if (t >= T) return;
     double q = 0.0;
     for (int c = 0; c < n; ++c) {
         double m = sums[c * 64 + lane] / nv;   // 0/0 = NaN with no valid direction, as fix mode
         ed.mean[(size_t)c * T + t] = m;
+        q = q + m * m;
+    }
+    ed.tstat[2 * (size_t)t + 0] = q;
+    ed.tstat[2 * (size_t)t + 1] = (nv > 0.0) ? amps / nv : 0.0;
+    if (t == 0) ed.stopred[0] = (int)nv;
+}
+
+// Stop mode after the fused evaluation: per sample, the amplitude from the block
+// sums (blocks of 16 channels in order, per valid direction in order) and
+// |mean(t)|^2 (channels in order); the same outputs as k_eval_stop's epilogue.
+__global__ __launch_bounds__(64) void k_stop_amp(const EpochDesc* __restrict__ eps, const int2* __restrict__ tab) {
+    const int2 br = tab[blockIdx.x];
+    const EpochDesc& ed = eps[br.x];
+    if (ed.st->done) return;
+    const int n = ed.n, T = ed.T, K = ed.K, G16 = (n + 15) >> 4;
+    const int t = br.y * 64 + (int)threadIdx.x;
+    if (t >= T) return;
+    double amps = 0.0, nv = 0.0;
+    for (int k = 0; k < K; ++k) {
+        if (!dir_valid(ed.E, k)) continue;
+        nv = nv + 1.0;
+        const double* pb = ed.ampb + (size_t)k * G16 * T + t;
+        double s2 = pb[0];
+        for (int g = 1; g < G16; ++g) s2 = s2 + pb[(size_t)g * T];
+        amps = amps + sqrt(s2) * 0.5;
+    }
+    double q = 0.0;
+    for (int c = 0; c < n; ++c) {
+        const double m = ed.mean[(size_t)c * T + t];
         q = q + m * m;
     }
     ed.tstat[2 * (size_t)t + 0] = q;
@@ -2079,6 +2166,7 @@ int blocks_for(int map, const EpochDesc& e) {
         return ((e.K + kFlagGroup - 1) / kFlagGroup) * ((e.T + kFlagTile - 1) / kFlagTile);
     case kMapCompact: return e.K;
     case kMapBlock: return e.bblocks;
+    case kMapAmp: return (e.mode == kModeStop && !e.coef_records) ? (e.T + 63) / 64 : 0;
     case kMapEval:
         if (e.coef_records) return (e.T + 63) / 64;
         return ((e.W + e.etw - 1) / e.etw) * ((e.n + 16 / e.etw - 1) / (16 / e.etw));
@@ -2171,15 +2259,18 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
         break;
     case 6:
-        if (p.mode == kModeStop)
+        if (p.mode == kModeStop && p.coef_records)   // mirrored ends: the record path
             MEMD_TABLE_LAUNCH(k_eval_stop, kMapEval, 64, (size_t)p.nmax * 64 * sizeof(double));
+        else if (p.mode == kModeStop)
+            MEMD_TABLE_LAUNCH(k_eval_fused<true>, kMapEval, 256, 0);
         else
-            MEMD_TABLE_LAUNCH(k_eval_fused, kMapEval, 256, 0);
+            MEMD_TABLE_LAUNCH(k_eval_fused<false>, kMapEval, 256, 0);
         break;
     case 7:
-        if (p.mode == kModeStop)
+        if (p.mode == kModeStop) {
+            if (!p.coef_records) MEMD_TABLE_LAUNCH(k_stop_amp, kMapAmp, 64, 0);
             hipLaunchKernelGGL(k_stop_decide, dim3(p.count), dim3(256), 0, s, p.eps, p.count);
-        else
+        } else
             hipLaunchKernelGGL(k_advance, dim3((p.count + 63) / 64), dim3(64), 0, s, p.eps, p.count, p.wl);
         break;
     default:

@@ -51,7 +51,10 @@
  * pin it (it has no stop mode); the tests pin it against an independent numpy/scipy
  * restatement of memd.m's loop (tests/golden/make_golden.py), known-answer cases
  * at every threshold and the limit cases sd = sd2 = tol = 0 (= fix mode) and
- * sd = sd2 = inf (no sift).
+ * sd = sd2 = inf (no sift).  The amplitude's sum over channels of (e+ - e-)^2
+ * runs in a fixed order of groups of 4 and blocks of 16 channels (amp_block_sum;
+ * memd.m's sum() order is MATLAB's own), pinned by the block-sum known answers
+ * and the numpy amplitude to 1e-11.
  *
  * Mirrored ends (option, not in the reference) follow memd.m's
  * boundary_conditions with nbsym = 2 (mirror_ends below): on the reference's
@@ -423,6 +426,29 @@ void mirror_lists(int T, const double* Pk, const int32_t* const X[2], const int 
     }
 }
 
+/* The channel reduction of memd.m's amplitude, sum_c d_c^2 (d = e+ - e-), in the
+ * order the fused GPU evaluation forms it: G_g = sequential sum over channels
+ * 4g .. 4g+3, B_b = ((G_4b + G_4b+1) + G_4b+2) + G_4b+3 over the groups of block
+ * b (channels 16b .. 16b+15), then the blocks in order.  For n <= 4 this is the
+ * plain sequential sum.  d[c * stride] is channel c. */
+double amp_block_sum(const double* d, size_t stride, int n) {
+    double s = 0.0;
+    for (int b = 0; 16 * b < n; ++b) {
+        double B = 0.0;
+        for (int g = 0; g < 4 && 16 * b + 4 * g < n; ++g) {
+            double G = 0.0;
+            const int c0 = 16 * b + 4 * g, c1 = std::min(n, c0 + 4);
+            for (int c = c0; c < c1; ++c) {
+                const double x = d[(size_t)c * stride];
+                G = G + x * x;
+            }
+            B = (g == 0) ? G : B + G;
+        }
+        s = (b == 0) ? B : s + B;
+    }
+    return s;
+}
+
 struct Workspace {
     std::vector<double> P;
     std::vector<int32_t> X[2];
@@ -507,12 +533,10 @@ void envelope_pass(int n, int T, int K, const double* cur, const double* V, int 
         if (want_amp) {
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
             for (int t = 0; t < T; ++t) {
-                double s = 0.0;
-                for (int c = 0; c < n; ++c) {
-                    double d = diff[(size_t)c * T + t];
-                    s = s + d * d;
-                }
-                ampsum[t] = ampsum[t] + std::sqrt(s) * 0.5;
+                // sum_c (e+ - e-)^2 in a fixed order: sequentially within groups of 4
+                // channels, the groups added 4 at a time into blocks of 16 channels, the
+                // blocks added in channel order (amp_block_sum)
+                ampsum[t] = ampsum[t] + std::sqrt(amp_block_sum(diff.data() + t, (size_t)T, n)) * 0.5;
             }
         }
     }
@@ -526,6 +550,7 @@ void envelope_pass(int n, int T, int K, const double* cur, const double* V, int 
 extern "C" {
 
 int oracle_solver_block(void) { return SOLVER_BLOCK; }
+double oracle_amp_block_sum(const double* d, int n) { return amp_block_sum(d, 1, n); }
 int oracle_window_limit(int T, int window) { return window_limit(T, window); }
 
 double oracle_radical_inverse(int base, unsigned i) {

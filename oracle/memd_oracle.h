@@ -121,6 +121,10 @@ int oracle_memd_batch(int count, const int* n, const int* T, const int* K, const
                       const double* const* y, const double* const* V, double* const* imfs, const oracle_opts* o,
                       int threads);
 
+/* the amplitude's channel reduction sum_c d[c]^2 in the engine's order: groups of
+ * 4 channels, 4 groups to a block of 16, blocks in order */
+double oracle_amp_block_sum(const double* d, int n);
+
 /* blocked-solver block length (rows per block incl. the separator) */
 int oracle_solver_block(void);
 /* last interior sample examined by the extrema test */

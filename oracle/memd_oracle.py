@@ -64,6 +64,8 @@ def lib():
         _pp = ctypes.POINTER(ctypes.c_void_p)
         L.oracle_memd_batch.argtypes = [ctypes.c_int] + [_ip] * 5 + [_pp] * 3 + [ctypes.POINTER(Opts), ctypes.c_int]
         L.oracle_mirror_knots.argtypes = [ctypes.c_int, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int] + [_ip] * 6
+        L.oracle_amp_block_sum.argtypes = [_dp, ctypes.c_int]
+        L.oracle_amp_block_sum.restype = ctypes.c_double
         L.oracle_stop_decision.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip] + [ctypes.c_double] * 3
         _lib = L
     return _lib
@@ -260,3 +262,9 @@ def memd_batch(epochs, n_imf, S, threads=1, mode=MODE_FIX, window=WINDOW_REFWARP
     if rc != 0:
         raise ValueError("oracle_memd_batch rejected its arguments")
     return outs
+
+
+def amp_block_sum(d):
+    """sum_c d[c]^2 in the engine's order (groups of 4, blocks of 16)."""
+    d = np.ascontiguousarray(d, np.float64)
+    return lib().oracle_amp_block_sum(_d(d), len(d))

@@ -315,3 +315,27 @@ def test_not_a_knot_batch_matches_alone(ctx, oracle):
     for ep, out in zip(eps, outs):
         alone, _ = ctx.run(ep["y"], ep["V"], 4, 3, spline=1)
         assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
+
+
+def test_stop_mode_beyond_the_record_path_cap(ctx, oracle):
+    # stop mode with the reference's ends evaluates in the fused kernel: no 128-channel
+    # cap (160 channels: 10 blocks of 16, the last a partial group of 4)
+    import synth
+    y = synth.eeg_epoch(158, 700, 26)
+    _check_epoch(ctx, oracle, y, 24, 3, 4, mode=1)
+
+
+def test_stop_mode_ignores_wider_tile_tunings(ctx, oracle):
+    # the amplitude's 16-channel blocks take 64-sample tiles whatever the tuning says
+    import synth
+    y = synth.eeg_epoch(8, 1500, 27)
+    V = oracle.directions(8, 16)
+    want, winfo = oracle.memd(y, V, 6, 8, mode=1)
+    ctx.set_tuning(8, 4, 2)
+    try:
+        got, info = ctx.run(_dev(y), _dev(V), 6, 8, mode=1)
+    finally:
+        ctx.set_tuning(8, 4, 1)
+    used = winfo["n_imf_out"]
+    assert info["n_imf_out"] == used and np.array_equal(info["sift_counts"], winfo["sift_counts"])
+    assert np.array_equal(got.cpu().numpy()[:used], want[:used])

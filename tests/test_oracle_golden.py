@@ -367,3 +367,20 @@ def test_epoch_parallel_batch_equals_single_epochs(oracle):
         for (y, V), got in zip(eps, outs):
             want, _ = oracle.memd(y, V, 4, 3, mode=mode)
             assert np.array_equal(got, want)
+
+
+def test_amplitude_channel_sum_order(oracle):
+    # sum_c d^2 in groups of 4 channels (sequential), 4 groups to a block of 16, blocks in
+    # order: known answers where that order shows in the rounding (ulp(2^54) = 4, so a
+    # single 1.0 added to 2^54 is lost while a group's 4.0 is kept)
+    big = 2.0 ** 27                                                        # big^2 = 2^54
+    assert oracle.amp_block_sum(np.array([3.0, 4.0])) == 25.0
+    assert oracle.amp_block_sum(np.ones(37)) == 37.0
+    # plain sequential order would give 2^54 (every 1.0 lost); groups keep the second group's 4
+    assert oracle.amp_block_sum(np.array([big] + [1.0] * 7)) == 2.0 ** 54 + 4.0
+    # block 0: 2^54 + 3 groups of 4; block 1: 4 groups of 4 x 0.25 = 1 each, summed to 4
+    # before it meets block 0 (groups added one by one would lose all four)
+    assert oracle.amp_block_sum(np.array([big] + [1.0] * 15 + [0.5] * 16)) == 2.0 ** 54 + 16.0
+    # n <= 4: the plain sequential sum
+    d = np.array([big, 1.0, 1.0, 1.0])
+    assert oracle.amp_block_sum(d) == 2.0 ** 54
