@@ -310,7 +310,8 @@ struct PlanLayout {
     int groups = 1;
 };
 
-// grp[e] in [0, groups) is the XCD group of epoch e (groups == 1: no affinity).
+// grp[e] in [0, groups) is the XCD group of epoch e (groups == 1: no affinity), -1
+// an epoch spread over all XCDs.
 memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const std::vector<int>& grp, int groups,
                         size_t start, PlanLayout* P) {
     const int count = (int)eds.size();
@@ -324,8 +325,13 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
         }
         for (int l = 0; l < kNumLists; ++l) {
             long c = list_capacity(l, eds[e]);
-            P->lcap[l] += c;
-            P->gcap[l][grp[e]] += c;
+            if (grp[e] >= 0) {
+                P->gcap[l][grp[e]] += c;
+                P->lcap[l] += c;
+            } else {   // spread: any XCD's sub-list may receive its items, and the list holds them all
+                for (int x = 0; x < groups; ++x) P->gcap[l][x] += c;
+                P->lcap[l] += c * groups;
+            }
         }
     }
     size_t total = start;
@@ -388,7 +394,7 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
         std::vector<std::vector<int2>> q(groups);
         for (int e = 0; e < count; ++e) {
             int nb = blocks_for(f, eds[e]);
-            for (int b = 0; b < nb; ++b) q[grp[e]].push_back(make_int2(e, b));
+            for (int b = 0; b < nb; ++b) q[grp[e] >= 0 ? grp[e] : b % groups].push_back(make_int2(e, b));
         }
         std::vector<size_t> head(groups, 0);
         for (int b = 0; b < P.nblk[f]; ++b) {
@@ -425,6 +431,8 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
             plan.wl.cap[l][x] = (int)P.gcap[l][x];
             at += P.gcap[l][x];
         }
+        if (at != P.lcap[l])   // the sub-lists must tile the list's storage exactly
+            return fail(ctx, MEMD_ERR_HIP, "work list %d: sub-lists hold %ld items, storage %ld", l, at, P.lcap[l]);
         // one-wave workgroups for the factor and reduced lists: a larger persistent grid
         const long gmax = (l == kListFactor || l == kListReduced) ? 4L * kListGrid : (long)kListGrid;
         long g = P.lcap[l] < gmax ? P.lcap[l] : gmax;
@@ -454,20 +462,30 @@ memd_status write_plan(memd_ctx* ctx, std::vector<EpochDesc>& eds, const std::ve
 }
 
 // Packing of a batch onto the XCDs: every sift step runs all epochs, so a step
-// lasts as long as the most loaded XCD.  Longest-processing-time first: epochs
-// by decreasing n * T * K (the work of a step; ties keep batch order) go to the
-// least loaded XCD.  Packing never changes an epoch's arithmetic.
+// lasts as long as the most loaded XCD.  An epoch with more than an XCD's fair
+// share of the batch's work (n * T * K) is spread over all XCDs (grp -1: its
+// blocks and work items go to every XCD), since pinned it alone would set the
+// step time.  The rest, longest-processing-time first (by decreasing n * T * K,
+// ties in batch order), go to the least loaded XCD.  Placement never changes an
+// epoch's arithmetic.
 void xcd_pack(const std::vector<EpochIn>& in, std::vector<int>& grp) {
     const int count = (int)in.size();
     std::vector<int> order(count);
     std::vector<long> cost(count);
+    long total = 0;
     for (int e = 0; e < count; ++e) {
         order[e] = e;
         cost[e] = (long)in[e].n * (long)in[e].T * (long)in[e].K;
+        total += cost[e];
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     long load[kXcds] = {};
     for (int e : order) {
+        if (cost[e] * kXcds > total) {
+            grp[e] = -1;
+            for (int x = 0; x < kXcds; ++x) load[x] += cost[e] / kXcds;
+            continue;
+        }
         int g = 0;
         for (int x = 1; x < kXcds; ++x)
             if (load[x] < load[g]) g = x;
@@ -1323,6 +1341,31 @@ void pool_free(memd_server* S, size_t off, size_t bytes) {
     }
 }
 
+// Epochs of at least this much work per step (n * T * K) are spread over all
+// XCDs by the server: pinned to one, a long epoch would set every step's length
+// for the short ones beside it.
+constexpr long kSpreadWeight = 1L << 26;
+
+// XCD placement of an admitted request (least loaded XCD, or all of them), and
+// its removal (sign -1).
+void server_place(memd_server* S, memd_server::Req& r, int sign) {
+    if (sign > 0) {
+        if (r.weight >= kSpreadWeight) {
+            r.grp = -1;
+        } else {
+            int g = 0;
+            for (int x = 1; x < kXcds; ++x)
+                if (S->load[x] < S->load[g]) g = x;
+            r.grp = g;
+        }
+    }
+    if (r.grp < 0) {
+        for (int x = 0; x < kXcds; ++x) S->load[x] += sign * (r.weight / kXcds);
+    } else {
+        S->load[r.grp] += sign * r.weight;
+    }
+}
+
 // Admit queued requests that fit, in FIFO order; returns the admitted ids.
 void server_admit(memd_server* S, std::vector<int64_t>& joined) {
     std::vector<int64_t> keep;
@@ -1334,11 +1377,7 @@ void server_admit(memd_server* S, std::vector<int64_t>& joined) {
         if (!blocked && room && pool_alloc(S, r.lay.total, &off)) {
             r.off = off;
             r.state = MEMD_REQ_RUNNING;
-            int g = 0;
-            for (int x = 1; x < kXcds; ++x)
-                if (S->load[x] < S->load[g]) g = x;
-            r.grp = g;
-            S->load[g] += r.weight;
+            server_place(S, r, +1);
             S->active.push_back(id);
             joined.push_back(id);
             continue;
@@ -1356,7 +1395,7 @@ void server_unadmit(memd_server* S, const std::vector<int64_t>& joined) {
     for (int64_t id : joined) {
         memd_server::Req& r = S->reqs[id];
         pool_free(S, r.off, r.lay.total);
-        S->load[r.grp] -= r.weight;
+        server_place(S, r, -1);
         r.state = MEMD_REQ_QUEUED;
         S->active.erase(std::find(S->active.begin(), S->active.end(), id));
     }
@@ -1537,7 +1576,7 @@ void server_consume(memd_server* S, memd_server::Probe& pr) {
         // the block stays intact until a rebuild drops this epoch from the plan: a
         // new epoch placed there is initialised in stream order after that rebuild
         pool_free(S, r.off, r.lay.total);
-        S->load[r.grp] -= r.weight;
+        server_place(S, r, -1);
         S->active.erase(std::find(S->active.begin(), S->active.end(), pr.ids[e]));
         S->finished.push_back(pr.ids[e]);
         S->info.retired += 1;
@@ -1580,7 +1619,7 @@ memd_status server_withdraw(memd_server* S, int64_t id, int state) {
         HIP_TRY(ctx, launch_mark_done((EpochState*)(S->pool + r.off + r.lay.st), s));
         HIP_TRY(ctx, hipStreamSynchronize(s));
         pool_free(S, r.off, r.lay.total);
-        S->load[r.grp] -= r.weight;
+        server_place(S, r, -1);
         S->active.erase(std::find(S->active.begin(), S->active.end(), id));
     }
     r.state = state;

@@ -145,7 +145,8 @@ struct EpochDesc {
     unsigned long long* stats; // optional [4]: knots, knot*channels, eval points, samples
     int* ndone;                // shared by the batch: epochs finished (stop mode)
     int coef_records;          // 1: k_coef writes the record array (stop mode)
-    int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups)
+    int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups);
+                               //   -1: spread, direction k's items go to sub-list k mod 8
     int bpw;                   // solver blocks per k_block work item
     // k_block work items (ke, group, E, 0) of this epoch, appended by k_compact and
     // drained by the epoch's bblocks workgroups (an XCD works through one epoch's

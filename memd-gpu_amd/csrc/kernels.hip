@@ -747,7 +747,8 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
         __syncthreads();
     }
     __shared__ int s_base[2][kNumLists], s_m[2][kNumLists];
-    const int g = wl.groups > 1 ? ed.xcd : 0;
+    // the epoch's XCD sub-list; a spread epoch (xcd < 0) deals its directions over all
+    const int g = wl.groups > 1 ? (ed.xcd >= 0 ? ed.xcd : (k & (kXcds - 1))) : 0;
     if (tid == 0) {   // one counter update per list for both envelopes
         int m0[kNumLists], m1[kNumLists];
         list_sizes(ed, sE[0], m0);

@@ -339,3 +339,16 @@ def test_stop_mode_ignores_wider_tile_tunings(ctx, oracle):
     used = winfo["n_imf_out"]
     assert info["n_imf_out"] == used and np.array_equal(info["sift_counts"], winfo["sift_counts"])
     assert np.array_equal(got.cpu().numpy()[:used], want[:used])
+
+
+def test_batch_with_a_spread_epoch_equals_alone(ctx, oracle):
+    # one epoch with more than an XCD's share of the batch's work is spread over all
+    # XCDs (its directions dealt over the sub-lists); placement changes no result
+    import synth
+    shapes = [(32, 20000, 64)] + [(4, 900 + 50 * i, 8) for i in range(9)]
+    eps = [{"y": _dev(synth.eeg_epoch(n, T, 120 + i)), "V": _dev(oracle.directions(n, K)), "n_imf": 4, "S": 3}
+           for i, (n, T, K) in enumerate(shapes)]
+    outs, _ = ctx.run_batch(eps)
+    for ep, out in zip(eps, outs):
+        alone, _ = ctx.run(ep["y"], ep["V"], 4, 3)
+        assert np.array_equal(out.cpu().numpy(), alone.cpu().numpy())
