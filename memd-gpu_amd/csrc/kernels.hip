@@ -936,6 +936,49 @@ __device__ __forceinline__ double clamp_end(double ye, double y1, double y2, int
     return take ? tv : ye;
 }
 
+// Global inputs of tile row r of a k_block work item (phase 1a): the sample the
+// knot carries, the factor row's positions and factors, the interval reciprocal.
+struct BlockRowIn {
+    int xs, xm, x0, xp;
+    v2d rc;
+    double ih;
+};
+
+// Geometry of a k_block work item (ke, group, E): its first row p0 and last row
+// qlast of the solve, and the tile rows p0-1 .. qlast+2.
+struct BlockItemGeom {
+    int J, b0, nb, p0, qlast, R, RW;
+};
+__device__ __forceinline__ BlockItemGeom block_item_geom(const int4& it, int bpw) {
+    BlockItemGeom g;
+    g.J = block_geom(it.z, 0).J;
+    g.b0 = it.y * bpw;
+    g.nb = min(bpw, g.J - g.b0 + 1);
+    g.p0 = g.b0 * kSolverBlock + 1;
+    g.qlast = block_geom(it.z, g.b0 + g.nb - 1).q;
+    g.R = g.qlast - g.p0 + 1;
+    g.RW = g.R + 3;
+    return g;
+}
+
+__device__ __forceinline__ BlockRowIn block_row_in(const EpochDesc& ed, const int4& it, const BlockItemGeom& g, int r) {
+    const int ke = it.x, E = it.z;
+    const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
+    const GPtr<v2d> RCp = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacRC)));
+    const GPtr<double> IHp = gptr(fac_plane(ed, ke, kFacIH));
+    const int u = min(g.p0 + r, g.qlast);   // factor row (used when r < R)
+    const int jr = min(g.p0 - 1 + r, E - 1);
+    BlockRowIn v;
+    v.xs = X[jr];
+    if (ed.boundary != 0) v.xs = knot_source(ed.mir[ke], jr, E, v.xs);   // the sample the knot carries
+    v.rc = RCp[u];
+    v.xm = X[u - 1];
+    v.x0 = X[u];
+    v.xp = X[u + 1];
+    v.ih = IHp[2 * (unsigned)min(g.p0 - 1 + r, E - 2)];
+    return v;
+}
+
 // Knot values, right-hand sides and phase 1, per (direction, envelope) and group
 // of EpochDesc::bpw solver blocks (rows p0..qlast, at most 32 * bpw).
 //   1. The group's knot positions, shared factors and reciprocals are staged in
@@ -971,20 +1014,22 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
     const int tid = threadIdx.x;
     const int cnt = min(*ed.bcount, ed.bcap);
     int4 it = br.y < cnt ? ed.bitem[br.y] : make_int4(0, 0, 0, 0);
+    // phase-1a inputs of row tid of the current item: loaded one item ahead, together
+    // with the previous item's tile, so an item costs one round of global loads
+    BlockRowIn pre{};
+    if (br.y < cnt) {   // a workgroup without items loads nothing
+        const BlockItemGeom g0 = block_item_geom(it, ed.bpw);
+        pre = block_row_in(ed, it, g0, min(tid, g0.RW - 1));
+    }
     for (int w = br.y; w < cnt; w += ed.bblocks) {
         // the next item's record is loaded while this one is solved
         const int4 nit = w + ed.bblocks < cnt ? ed.bitem[w + ed.bblocks] : it;
         const int ke = it.x, env = ke & 1;
         const int n = ed.n, np = ed.np;
         const int E = it.z;
-        const int J = block_geom(E, 0).J;
         const int bpw = ed.bpw;
-        const int b0 = it.y * bpw;
-        const int nb = min(bpw, J - b0 + 1);
-        const int p0 = b0 * L + 1;
-        const int qlast = block_geom(E, b0 + nb - 1).q;
-        const int R = qlast - p0 + 1;             // rows of the group, <= rmax
-        const int RW = R + 3;                     // tile rows p0-1 .. qlast+2
+        const BlockItemGeom geo = block_item_geom(it, bpw);
+        const int J = geo.J, b0 = geo.b0, nb = geo.nb, p0 = geo.p0, qlast = geo.qlast, R = geo.R, RW = geo.RW;
         // idx / d as (idx * ceil(2^32 / d)) >> 32 in 64 bits: exact for idx * (d - 1) < 2^32
         // (the tile bounds idx < np * RW by 160 KB / 8 and d <= 256 here); the constant
         // is 2^32 itself when d = 1
@@ -998,31 +1043,30 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions of tile rows 0..RW-1, factors of rows p0..qlast, reciprocals of
         //     intervals p0-1..qlast+1 (the last one: the next separator's right-hand side;
-        //     clamped where the group ends the system): one round of loads per lane row
-        const GPtr<v2d> RCp = gptr(reinterpret_cast<const v2d*>(fac_plane(ed, ke, kFacRC)));
-        const GPtr<double> IHp = gptr(fac_plane(ed, ke, kFacIH));
+        //     clamped where the group ends the system); row tid's inputs were prefetched,
+        //     rows past the workgroup (groups of more than 8 blocks) are loaded here
         const int spline = ed.spline;
         const bool mirror = ed.boundary != 0;
-        const int4 mir = mirror ? ed.mir[ke] : make_int4(0, 0, 0, 0);
-        for (int r = tid; r < RW; r += kBlockThreads) {
-            const int u = min(p0 + r, qlast);   // factor row (used when r < R)
-            const int jr = min(p0 - 1 + r, E - 1);
-            const int xs = mirror ? knot_source(mir, jr, E, X[jr]) : X[jr];   // the sample the knot carries
-            const v2d rc = RCp[u];
-            const double rd = rc.x, cpv = rc.y;
-            const int xm = X[u - 1], x0 = X[u], xp = X[u + 1];
-            const double ihv = IHp[2 * (unsigned)min(p0 - 1 + r, E - 2)];
+        auto put_row = [&](int r, const BlockRowIn& v) {
+            const int u = min(p0 + r, qlast);
             double lo, di, up;
-            sys_row3(xm, x0, xp, u, E, spline, lo, di, up);
-            sX[r] = xs;
-            s_ih[r] = ihv;
-            s_rd[r] = rd;
-            s_cp[r] = cpv;
+            sys_row3(v.xm, v.x0, v.xp, u, E, spline, lo, di, up);
+            sX[r] = v.xs;
+            s_ih[r] = v.ih;
+            s_rd[r] = v.rc.x;
+            s_cp[r] = v.rc.y;
             s_lo[r] = lo;
-        }
+        };
+        if (tid < RW) put_row(tid, pre);
+        for (int r = tid + kBlockThreads; r < RW; r += kBlockThreads) put_row(r, block_row_in(ed, it, geo, r));
         const int X0 = X[0], X1 = X[min(1, E - 1)], X2 = X[min(2, E - 1)];
         const int XE1 = X[E - 1], XE2 = X[max(E - 2, 0)], XE3 = X[max(E - 3, 0)];
         __syncthreads();
+        // the next item's row inputs, in flight with this item's tile
+        {
+            const BlockItemGeom gn = block_item_geom(nit, bpw);   // nit is a valid item (the next, or this one)
+            pre = block_row_in(ed, nit, gn, min(tid, gn.RW - 1));
+        }
         // 1b. the tile, 16 bytes (two channels of one row) per lane: lane l of a wave
         //     fills bytes 16 l .. 16 l + 15 from the instruction's LDS base
         {
