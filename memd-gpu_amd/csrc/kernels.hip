@@ -1573,7 +1573,7 @@ template <bool STOP>
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
-    __shared__ double sG[STOP ? 2 * 4 * 64 : 1];   // [buffer][wave][sample]: group sums
+    __shared__ double sG[STOP ? 2 * 2 * 4 * 64 : 1];   // [buffer][pending direction][wave][sample]: group sums
     // spline records, segment s: [channel][stride] from its record base, as two
     // arrays of 16-byte halves (a, b) and (c, d): a wave's record reads and writes
     // are then 16 bytes apart, not every other 16 bytes of 32-byte records
@@ -1626,6 +1626,22 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
     int kc1 = 0;   // end of the staged chunk of formation inputs
+    // stop mode: directions whose group sums wait in sG[gbuf] for wave 0 (at most 2)
+    int gbuf = 0, npend = 0, kpend0 = 0, kpend1 = 0;
+    const int G16 = (n + 15) >> 4;
+    // wave 0 adds the 4 waves' group sums of the pending directions and stores the
+    // block sums; called right after a barrier
+    auto flush_blocks = [&]() {
+        if (tid < 64 && t0 + tid <= tlast) {
+            for (int p2 = 0; p2 < npend; ++p2) {
+                const double* gb = sG + ((gbuf * 2 + p2) * 4) * 64 + tid;
+                const double b = ((gb[0] + gb[64]) + gb[128]) + gb[192];
+                ed.ampb[((size_t)(p2 ? kpend1 : kpend0) * G16 + cg) * T + t0 + tid] = b;
+            }
+        }
+        gbuf ^= 1;
+        npend = 0;
+    };
     for (int kb = 0; kb < K;) {
         if (tid < 64) {   // formation
             if (kb >= kc1) {   // stage the next kEvalChunk directions: lane (direction, envelope)
@@ -1830,7 +1846,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             if (STOP) {
                 // lanes (channel 4w + j, quad q) of wave w: lane 16 j + q; samples past
                 // the tile and absent channels carry 0, which adds exactly
-                const int buf = d & 1, lw = tid & 63;
+                const int lw = tid & 63;
                 double g[4];
 #pragma unroll
                 for (int s2 = 0; s2 < 4; ++s2) {
@@ -1841,18 +1857,17 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 }
                 if (lw < 16) {
 #pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) sG[(buf * 4 + (tid >> 6)) * 64 + 4 * lw + s2] = g[s2];
+                    for (int s2 = 0; s2 < 4; ++s2) sG[((gbuf * 2 + npend) * 4 + (tid >> 6)) * 64 + 4 * lw + s2] = g[s2];
                 }
-                __syncthreads();
-                if (tid < 64 && t0 + tid <= tlast) {
-                    const double* gb = sG + buf * 4 * 64 + tid;
-                    const double b = ((gb[0] + gb[64]) + gb[128]) + gb[192];
-                    const int G16 = (n + 15) >> 4;
-                    ed.ampb[((size_t)(kb + d) * G16 + cg) * T + t0 + tid] = b;
+                if (npend == 0) kpend0 = kb + d; else kpend1 = kb + d;
+                if (++npend == 2) {   // two directions per barrier; the buffers alternate
+                    __syncthreads();
+                    flush_blocks();
                 }
             }
         }
         __syncthreads();
+        if (STOP && npend > 0) flush_blocks();   // the batch's last direction, after its barrier
         kb += nb;
     }
     if (ed.stats && br.y == 0 && tid == 0) {
