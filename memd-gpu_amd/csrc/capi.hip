@@ -1137,13 +1137,17 @@ memd_status memd_ctx_get_tuning(memd_ctx* ctx, int32_t n, memd_tuning* out) {
 }
 
 // Times every candidate shape on a synthetic batch of (n, T, K) epochs, sized to
-// fill the device (about 5.4e8 n*T*K per step, at most 256 epochs), with one IMF
-// of 4 sifts under the profiling path.  The block solve's time depends only on
-// block_groups and the evaluation's only on eval_tile_words, so the candidates
-// are run in pairs and each knob keeps its fastest value.
+// fill the device (about 5.4e8 n*T*K per step, at most 256 epochs), with one sift
+// of each of the first 5 IMFs under the profiling path: the knot densities of a
+// whole decomposition, from the first IMF's dense knots (where larger solver
+// groups win) to the sparse later ones (where a small batch needs more, smaller
+// work items).  The block solve's time depends only on block_groups and the
+// evaluation's only on eval_tile_words, so the candidates are run in pairs and
+// each knob keeps its fastest value.
 memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuning* out) {
     if (!ctx) return fail(nullptr, MEMD_ERR_ARG, "null context");
-    memd_status rs = validate_shape(ctx, n, T, K, 2, 4);
+    constexpr int kTuneSlots = 6, kTuneSifts = 1;   // 5 IMFs of one sift, and the residual
+    memd_status rs = validate_shape(ctx, n, T, K, kTuneSlots, kTuneSifts);
     if (rs != MEMD_OK) return rs;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const size_t work = n * T * K;
@@ -1162,14 +1166,15 @@ memd_status memd_autotune(memd_ctx* ctx, size_t n, size_t T, size_t K, memd_tuni
     Staging y, V, imfs;
     hipError_t e = dev_alloc(ctx, (void**)&y.p, n * T * sizeof(double));
     if (e == hipSuccess) e = dev_alloc(ctx, (void**)&V.p, K * n * sizeof(double));
-    if (e == hipSuccess) e = dev_alloc(ctx, (void**)&imfs.p, count * 2 * n * T * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(ctx, (void**)&imfs.p, count * kTuneSlots * n * T * sizeof(double));
     if (e != hipSuccess) return fail(ctx, MEMD_ERR_NOMEM, "autotune buffers: %s", hipGetErrorString(e));
     HIP_TRY(ctx, hipMemcpy(y.p, hy.data(), n * T * sizeof(double), hipMemcpyHostToDevice));
     rs = memd_directions_f64(ctx, n, K, nullptr, V.p);
     if (rs != MEMD_OK) return rs;
     std::vector<EpochIn> in(count);
     for (size_t i = 0; i < count; ++i)
-        in[i] = {(int)n, (int)T, (int)K, 2, 4, y.p, V.p, imfs.p + i * 2 * n * T, nullptr, nullptr};
+        in[i] = {(int)n, (int)T, (int)K, kTuneSlots, kTuneSifts, y.p, V.p, imfs.p + i * kTuneSlots * n * T,
+                 nullptr, nullptr};
     std::vector<int> bg, tw = {1, 2, 4};
     for (int b : {1, 2, 4, 8})
         if (block_bpw_fit((int)n, b) == b) bg.push_back(b);
