@@ -220,6 +220,9 @@ def test_workspace_bytes_follow_mode_and_slots():
     assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, None) == base
     assert lib.memd_workspace_bytes_opts(32, 4096, 64, 64, ctypes.byref(fix)) == base
     assert lib.memd_workspace_bytes_opts(32, 4096, 64, 65, ctypes.byref(fix)) > base
-    # stop mode's record array: 2K * n * (T/2 + 2) intervals of 32 bytes
-    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, ctypes.byref(stop)) >= base + 128 * 32 * 2050 * 32
+    # stop mode: the mean [n][T] and the per-block amplitude sums [K][ceil(n/16)][T]
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, ctypes.byref(stop)) >= base + 64 * 2 * 4096 * 8
+    # mirrored ends: the record array, 2K * n * (T/2 + 2) intervals of 32 bytes
+    mirror = memd_amd.default_opts(boundary=memd_amd.BOUNDARY_MIRROR)
+    assert lib.memd_workspace_bytes_opts(32, 4096, 64, 9, ctypes.byref(mirror)) >= base + 128 * 32 * 2050 * 32
     assert lib.memd_workspace_bytes_opts(32, 4096, 64, 0, None) == 0
