@@ -1573,6 +1573,7 @@ template <bool STOP>
 __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restrict__ eps,
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
+    constexpr int NT = 256;              // threads per workgroup
     __shared__ double sG[STOP ? 2 * 2 * 4 * 64 : 1];   // [buffer][pending direction][wave][sample]: group sums
     // spline records, segment s: [channel][stride] from its record base, as two
     // arrays of 16-byte halves (a, b) and (c, d): a wave's record reads and writes
@@ -1582,6 +1583,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     __shared__ int xk[kEvalXkCap];       // segment s: knot positions jlo .. jlo + nint
     __shared__ int4 sSegA[NS];           // item prefix, -, nint, jlo
     __shared__ int4 sSegB[NS];           // record base, row stride, E, knot-position base
+    // segment s: its rows of ysol, zsep and fac, so phase A addresses are a base plus a
+    // 32-bit offset (one (direction, envelope)'s rows stay far below 4 GB)
+    __shared__ const double* sSegP[NS][3];
     __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
     __shared__ int sCum[NS][4];          // knots before the end of each word of the tile, less jlo + 1
     __shared__ unsigned long long sMsk[NS][4];   // flag words; sample 0's flag cleared
@@ -1711,6 +1715,10 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             if (tid < 2 * nb) {
                 sSegA[tid] = make_int4(pit - nit, 0, ni, lo);
                 sSegB[tid] = make_int4(pneed - need, stride, E, pxk - nxk);
+                const size_t kes = (size_t)(2 * kb + tid);   // ke of the segment
+                sSegP[tid][0] = ysol + kes * Ecap * np;
+                sSegP[tid][1] = zsep + kes * (nbmax + 1) * np;
+                sSegP[tid][2] = fac + kes * kFacPlanes * Ecap * 2;
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
                     sCum[tid][w] = cum[w] - lo - 1;
@@ -1730,74 +1738,73 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
 #pragma unroll
         for (int s = 0; s <= NS; ++s) ip[s] = uniform(sIp[s]);
         const int total = ip[NS];
+        const int lwv = tid & 63;
+        const int ipl = sIp[min(lwv, NS)];   // lane s: the first item of segment s
         // phase A: items (interval, quad of <= 4 channels), quad fastest.  The
         // interval's positions, per-row records and reciprocals are loaded once for
         // its channels; the channel rows of the knot-major arrays are adjacent words.
-        for (int a0 = 0; wbase + 256 * a0 < total; ++a0) {   // wave-uniform bound
-            const int f = tid + 256 * a0;
-            double v0[4], v1[4], y0[4], y1[4], z00[4], z01[4], z10[4], z11[4];
-            double al0 = 0.0, be0 = 0.0, al1 = 0.0, be1 = 0.0, ih = 0.0, i3h = 0.0;
-            int x0 = 0, x1 = 0, seg = -1, j = 0, qd = 0, nch = 0;
-            if (f < total) {
-                // the item's segment: the last s with ip[s] <= f (ip in scalar registers:
-                // a fully unrolled count keeps them there)
-                seg = 0;
-#pragma unroll
-                for (int s2 = 1; s2 < NS; ++s2) seg += (f >= ip[s2]) ? 1 : 0;
-                const int4 A = sSegA[seg];
-                const int E = sSegB[seg].z;
-                const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
-                const int rem = f - A.x;
-                const int ii = (rem * qmag) >> 16;
-                qd = rem - ii * nq;
-                j = A.w + ii;
-                const int c0 = cg * CG + 4 * qd;
-                nch = min(4, cgn - 4 * qd);
-                const int J = (E - 3) / kSolverBlock;
-                // 4 channels from c0 (a multiple of 4) in rows of np (a multiple of 4)
-                // words: 32-byte aligned; channels past the group are padding or the
-                // next group's and are not used
-                const size_t kE = (size_t)ke * Ecap;
-                const GPtr<v4d> Yr = gptr(reinterpret_cast<const v4d*>(ysol + (kE + j) * np + c0));
-                const double* Zb = zsep + (size_t)ke * (nbmax + 1) * np + c0;
-                const int2 zj = zsep_index(j, E, J), zk = zsep_index(j + 1, E, J);
-                const double* Fb = fac + (size_t)ke * kFacPlanes * Ecap * 2;   // planes of (direction, envelope) ke
-                const GPtr<int> Xr = gptr(Xall + kE);
-                // the knot positions first: the knot-value loads below depend on them,
-                // and the other loads are issued while they are in flight
-                x0 = Xr[j];
-                x1 = Xr[j + 1];
-                const GPtr<v2d> ABr = gptr(reinterpret_cast<const v2d*>(Fb + (size_t)kFacAB * Ecap * 2));
-                const GPtr<v2d> IHr = gptr(reinterpret_cast<const v2d*>(Fb + (size_t)kFacIH * Ecap * 2));
-                const v2d ab0 = ABr[j], ab1 = ABr[j + 1], ihp = IHr[j];
-                const int nr = np >> 2;   // double4 per row
-                const v4d yh0 = Yr[0], yh1 = Yr[nr];
-                const v4d za = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zj.x * np));
-                const v4d zb = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zj.y * np));
-                const v4d zc = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zk.x * np));
-                const v4d zd = *gptr(reinterpret_cast<const v4d*>(Zb + (size_t)zk.y * np));
-                __builtin_amdgcn_sched_barrier(0);   // keep the independent loads above
-                // knot values: the signal at the two knots (sample-major rows), or the
-                // clamped end values (E >= 6), rows T + 2 ke and T + 2 ke + 1 of the same
-                // array
-                const int r0 = (E >= 6 && j == 0) ? T + 2 * ke : x0;
-                const int r1 = (E >= 6 && j + 1 == E - 1) ? T + 2 * ke + 1 : x1;
-                const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r0 * np + c0));
-                const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(curt + (size_t)r1 * np + c0));
-                al0 = ab0.x; be0 = ab0.y; ih = ihp.x; i3h = ihp.y;
-                al1 = ab1.x; be1 = ab1.y;
-                v0[0] = yv0.x; v0[1] = yv0.y; v0[2] = yv0.z; v0[3] = yv0.w;
-                v1[0] = yv1.x; v1[1] = yv1.y; v1[2] = yv1.z; v1[3] = yv1.w;
-                y0[0] = yh0.x; y0[1] = yh0.y; y0[2] = yh0.z; y0[3] = yh0.w;
-                y1[0] = yh1.x; y1[1] = yh1.y; y1[2] = yh1.z; y1[3] = yh1.w;
-                z00[0] = za.x; z00[1] = za.y; z00[2] = za.z; z00[3] = za.w;
-                z01[0] = zb.x; z01[1] = zb.y; z01[2] = zb.z; z01[3] = zb.w;
-                z10[0] = zc.x; z10[1] = zc.y; z10[2] = zc.z; z10[3] = zc.w;
-                z11[0] = zd.x; z11[1] = zd.y; z11[2] = zd.z; z11[3] = zd.w;
+        // Lanes past the last item redo it and store nothing, so every lane's loads
+        // are in range and no register needs a default.
+        for (int a0 = 0; wbase + NT * a0 < total; ++a0) {   // wave-uniform bound
+            const int fw = wbase + NT * a0;                  // the wave's first item
+            const int fr = tid + NT * a0;
+            const bool on = fr < total;
+            const int f = min(fr, total - 1);
+            // the item's segment (the last s with ip[s] <= f): lane s of the wave holds
+            // boundary ip[s]; those at or below the wave's first item are counted by a
+            // ballot, the few inside the wave's items per lane
+            const bool bin = lwv >= 1 && lwv < 2 * nb;
+            int seg = (int)__popcll(__ballot(bin && ipl <= fw));
+            for (unsigned long long in = __ballot(bin && ipl > fw && ipl <= fw + 63); in; in &= in - 1) {
+                const int bnd = __builtin_amdgcn_readlane(ipl, (int)__builtin_ctzll(in));
+                seg += (f >= bnd) ? 1 : 0;
             }
-            if (seg >= 0) {
-                const int4 A = sSegA[seg];
-                const int4 B = sSegB[seg];
+            const int4 A = sSegA[seg];
+            const int4 B = sSegB[seg];
+            const int E = B.z;
+            const int ke = 2 * (kb + (seg >> 1)) + (seg & 1);
+            const int rem = f - A.x;
+            const int ii = (rem * qmag) >> 16;
+            const int qd = rem - ii * nq;
+            const int j = A.w + ii;
+            const int c0 = cg * CG + 4 * qd;
+            const int nch = min(4, cgn - 4 * qd);
+            const int J = (E - 3) / kSolverBlock;
+            // 4 channels from c0 (a multiple of 4) in rows of np (a multiple of 4)
+            // words: 32-byte aligned; channels past the group are padding or the
+            // next group's and are not used
+            const double* const yb = sSegP[seg][0];
+            const double* const zb0 = sSegP[seg][1];
+            const double* const Fb = sSegP[seg][2];   // planes of (direction, envelope) ke
+            const GPtr<int> Xr = gptr(Xall + (size_t)ke * Ecap);
+            // the knot positions first: the knot-value loads below depend on them,
+            // and the other loads are issued while they are in flight
+            const int x0 = Xr[j];
+            const int x1 = Xr[j + 1];
+            const GPtr<v2d> ABr = gptr(reinterpret_cast<const v2d*>(Fb + (unsigned)(kFacAB * Ecap * 2 + 2 * j)));
+            const GPtr<v2d> IHr = gptr(reinterpret_cast<const v2d*>(Fb + (unsigned)(kFacIH * Ecap * 2 + 2 * j)));
+            const v2d ab0 = ABr[0], ab1 = ABr[1], ihp = IHr[0];
+            const int nr = np >> 2;   // double4 per row
+            const GPtr<v4d> Yr = gptr(reinterpret_cast<const v4d*>(yb + (unsigned)(j * np + c0)));
+            const v4d yh0 = Yr[0], yh1 = Yr[nr];
+            // separator values Z_b, Z_b+1 of the interval's block b: the pair of every
+            // block row among its two knots; a knot outside the blocks (an end row or a
+            // separator) has al = be = 0, so its M is its y^ whatever the pair
+            const int zb = min(j >> 5, J);
+            const GPtr<v4d> Zr = gptr(reinterpret_cast<const v4d*>(zb0 + (unsigned)(zb * np + c0)));
+            const v4d za = Zr[0], zc = Zr[nr];
+            __builtin_amdgcn_sched_barrier(0);   // keep the independent loads above
+            // knot values: the signal at the two knots (sample-major rows), or the
+            // clamped end values (E >= 6), rows T + 2 ke and T + 2 ke + 1 of the same
+            // array
+            const int r0 = (E >= 6 && j == 0) ? T + 2 * ke : x0;
+            const int r1 = (E >= 6 && j + 1 == E - 1) ? T + 2 * ke + 1 : x1;
+            const v4d yv0 = *gptr(reinterpret_cast<const v4d*>(curt + (unsigned)(r0 * np + c0)));
+            const v4d yv1 = *gptr(reinterpret_cast<const v4d*>(curt + (unsigned)(r1 * np + c0)));
+            if (on) {
+                const double v0[4] = {yv0.x, yv0.y, yv0.z, yv0.w}, v1[4] = {yv1.x, yv1.y, yv1.z, yv1.w};
+                const double y0[4] = {yh0.x, yh0.y, yh0.z, yh0.w}, y1[4] = {yh1.x, yh1.y, yh1.z, yh1.w};
+                const double z0[4] = {za.x, za.y, za.z, za.w}, z1[4] = {zc.x, zc.y, zc.z, zc.w};
                 const int i = j - A.w;
                 if (qd == 0) {   // first quad of the group: knot positions
                     xk[B.w + i] = x0;
@@ -1807,12 +1814,12 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
 #pragma unroll
                 for (int q2 = 0; q2 < 4; ++q2) {
                     if (q2 < nch) {
-                        const double mj = knot_m(y0[q2], al0, be0, z00[q2], z01[q2]);
-                        const double mk = knot_m(y1[q2], al1, be1, z10[q2], z11[q2]);
-                        const double slp = (v1[q2] - v0[q2]) * ih;
+                        const double mj = knot_m(y0[q2], ab0.x, ab0.y, z0[q2], z1[q2]);
+                        const double mk = knot_m(y1[q2], ab1.x, ab1.y, z0[q2], z1[q2]);
+                        const double slp = (v1[q2] - v0[q2]) * ihp.x;
                         const int at = B.x + (4 * qd + q2) * B.y + i;
                         recAB[at] = v2d{v0[q2], slp - (h * (2.0 * mj + mk)) * (1.0 / 3.0)};
-                        recCD[at] = v2d{mj, (mk - mj) * i3h};
+                        recCD[at] = v2d{mj, (mk - mj) * ihp.y};
                     }
                 }
             }
