@@ -12,6 +12,7 @@ this machine's cores, a bounded sample) are reported beside it.
     python bench.py [--steps K] [--warmup W] [--batch B]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -86,6 +87,14 @@ def measured_traffic(workload, kernel):
         return None
     ent = rec.get(workload, {}).get(kernel)
     return int(ent["bytes_per_launch"]) if ent else None
+
+
+def quiesce_gc():
+    """Collect once and move every object allocated so far (torch's modules, the
+    inputs) out of the collector's reach: a full collection over them takes about
+    40 ms of host time, and left enabled one lands inside the timed calls."""
+    gc.collect()
+    gc.freeze()
 
 
 def pct(xs, q):
@@ -175,6 +184,7 @@ def bench_eeg(a):
     V = ctx.directions(n, K)
     ys = [torch.from_numpy(synth.eeg_epoch(n, T, 3000 + i)).to("cuda:0") for i in range(max(a.steps, 1))]
     out = torch.empty((N_imf, n, T), dtype=torch.float64, device="cuda:0")
+    quiesce_gc()
     for _ in range(a.warmup):
         ctx.run(ys[0], V, N_imf, S, out=out)
     torch.cuda.synchronize()
@@ -232,6 +242,7 @@ def bench_wide(a):
     ys = [torch.from_numpy(synth.eeg_epoch(n, T, 4000 + i, gamma40=True)).to("cuda:0")
           for i in range(max(a.steps, 1))]
     out = torch.empty((N_imf, n, T), dtype=torch.float64, device="cuda:0")
+    quiesce_gc()
     for _ in range(a.warmup):
         ctx.run(ys[0], V, N_imf, S, out=out)
     torch.cuda.synchronize()
@@ -298,6 +309,7 @@ def bench_uneven(a):
         y = torch.from_numpy(synth.eeg_epoch(n, T, 78 + i)).to("cuda:0")
         eps.append({"y": y, "V": Vs[(n, K)], "n_imf": 9, "S": 10,
                     "out": torch.empty((9, n, T), dtype=torch.float64, device="cuda:0")})
+    quiesce_gc()
     for _ in range(a.warmup):
         ctx.run_batch(eps)
     torch.cuda.synchronize()
@@ -383,6 +395,7 @@ def bench_served(a):
 
     srv = memd_amd.MemdServer(ctx, pool_bytes=a.pool_gb << 30, max_active=a.max_active)
     results = []
+    quiesce_gc()
     for rate in a.rates:
         # arrival schedule
         arr, t = [], 0.0
@@ -525,6 +538,7 @@ def main():
     outs = [torch.empty((N_imf, n, T), dtype=torch.float64, device=dev) for _ in range(B)]
     eps = [{"y": ys[i], "V": V, "n_imf": N_imf, "S": S, "out": outs[i]} for i in range(B)]
 
+    quiesce_gc()
     for _ in range(a.warmup):
         ctx.run_batch(eps, mode=mode, boundary=bnd)
     torch.cuda.synchronize()
@@ -537,6 +551,7 @@ def main():
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     value = B * a.steps / total
+    print("batched call latencies (ms): " + " ".join(f"{1e3 * v:.1f}" for v in lat), file=sys.stderr, flush=True)
 
     # plain per-epoch path (one call per epoch), latency per epoch
     plain_lat = []

@@ -613,6 +613,13 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     }
     hipStream_t s = work_stream(ctx);
     HIP_TRY(ctx, launch_zero_ints(ndone_dev, 1, s));   // a kernel, not a memset node: callers may capture
+    // each epoch's first entry in the gathered sift counts (read back after the steps)
+    std::vector<int> sift_off(count);
+    for (int e = 0, at = 0; e < count; ++e) {
+        sift_off[e] = at;
+        at += in[e].N_imf;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(arena + P.siftoff_at, sift_off.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
     StepPlan plan{};
     std::vector<int2> tabs;
     rs = write_plan(ctx, eds, grp, P, arena, pk.stop_kernels ? kModeStop : kModeFix, records, tabs, s, &plan);
@@ -736,16 +743,18 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, hipEventRecord(ctx->t1, s));
     }
 
-    // results: per-epoch state and sift counts
+    // results: every epoch's state and sift counts gathered into the plan region by
+    // one kernel, then two copies (not two per epoch)
     std::vector<EpochState> sts(count);
-    std::vector<std::vector<int>> sc(count);
-    for (int e = 0; e < count; ++e) {
-        HIP_TRY(ctx, hipMemcpyAsync(&sts[e], eds[e].st, sizeof(EpochState), hipMemcpyDeviceToHost, s));
-        if (in[e].h_sift_counts) {
-            sc[e].resize(in[e].N_imf);
-            HIP_TRY(ctx, hipMemcpyAsync(sc[e].data(), eds[e].sift_counts, sizeof(int) * in[e].N_imf,
+    std::vector<int> sift_all((size_t)P.sift_total);
+    {
+        EpochState* dstates = (EpochState*)(arena + P.states_at);
+        int* dsift = (int*)(arena + P.sift_at);
+        HIP_TRY(ctx, launch_gather_probe(plan, dstates, dsift, (const int*)(arena + P.siftoff_at), s));
+        HIP_TRY(ctx, hipMemcpyAsync(sts.data(), dstates, sizeof(EpochState) * count, hipMemcpyDeviceToHost, s));
+        if (P.sift_total > 0)
+            HIP_TRY(ctx, hipMemcpyAsync(sift_all.data(), dsift, sizeof(int) * (size_t)P.sift_total,
                                         hipMemcpyDeviceToHost, s));
-        }
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
     if (elapsed) {
@@ -757,7 +766,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
         if (!sts[e].done)
             return fail(ctx, MEMD_ERR_HIP, "epoch %d did not finish (imf=%d nsift=%d)", e, sts[e].imf, sts[e].nsift);
         if (in[e].h_sift_counts)
-            for (int i = 0; i < in[e].N_imf; ++i) in[e].h_sift_counts[i] = sc[e][i];
+            for (int i = 0; i < in[e].N_imf; ++i) in[e].h_sift_counts[i] = sift_all[(size_t)sift_off[e] + i];
         if (in[e].h_n_imf_out) *in[e].h_n_imf_out = sts[e].n_imf_out;
     }
     return MEMD_OK;
