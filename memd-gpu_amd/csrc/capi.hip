@@ -355,7 +355,7 @@ memd_status plan_layout(memd_ctx* ctx, const std::vector<EpochDesc>& eds, const 
     P->bcnt_at = total;
     total += align_up(kBcountStride * (size_t)count);
     P->stats_at = total;
-    total += align_up(sizeof(unsigned long long) * 4);
+    total += align_up(sizeof(unsigned long long) * 4 * kStatSlots);
     P->ndone_at = total;
     total += align_up(sizeof(int));
     P->fresh_at = total;
@@ -636,7 +636,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
 
     if (prof) {
         std::memset(prof, 0, sizeof *prof);
-        HIP_TRY(ctx, hipMemsetAsync(arena + stats_at, 0, sizeof(unsigned long long) * 4, s));
+        HIP_TRY(ctx, hipMemsetAsync(arena + stats_at, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
         HIP_TRY(ctx, hipEventRecord(ctx->pev[0], s));
         HIP_TRY(ctx, launch_init(plan, s));
         HIP_TRY(ctx, hipEventRecord(ctx->pev[1], s));
@@ -669,8 +669,12 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             prof->ms_per_launch[kk] = steps ? acc[kk] / (double)steps : 0.0;
         }
         prof->steps = steps;
-        unsigned long long st4[4];
-        HIP_TRY(ctx, hipMemcpy(st4, arena + stats_at, sizeof st4, hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> slots(4 * (size_t)kStatSlots);
+        HIP_TRY(ctx, hipMemcpy(slots.data(), arena + stats_at, sizeof(unsigned long long) * slots.size(),
+                               hipMemcpyDeviceToHost));
+        unsigned long long st4[4] = {0, 0, 0, 0};
+        for (int sl = 0; sl < kStatSlots; ++sl)
+            for (int i = 0; i < 4; ++i) st4[i] += slots[4 * (size_t)sl + i];
         prof->knots = (int64_t)st4[0];
         prof->knot_channels = (int64_t)st4[1];
         prof->eval_points = (int64_t)st4[2];

@@ -142,7 +142,7 @@ struct EpochDesc {
     int max_passes;
     int* sift_counts;      // device [N_imf]
     EpochState* st;
-    unsigned long long* stats; // optional [4]: knots, knot*channels, eval points, samples
+    unsigned long long* stats; // optional [kStatSlots][4]: knots, knot*channels, eval points, samples
     int* ndone;                // shared by the batch: epochs finished (stop mode)
     int coef_records;          // 1: k_coef writes the record array (stop mode)
     int xcd;                   // XCD group: work items go to sub-list xcd (when the plan has groups);
@@ -213,6 +213,9 @@ long list_capacity(int list, const EpochDesc& e);
 // block, reduced, coef, eval | eval_stop, advance | stop_decide, stop_apply
 // (stop mode only); slot kNumStepKernels is the one-off init.
 constexpr int kNumStepKernels = 9;
+// Profiling counters are summed over this many slots (one per workgroup index
+// modulo kStatSlots), so the profiled launches do not serialise on one address.
+constexpr int kStatSlots = 64;
 int knot_capacity(int T);
 inline int channel_stride(int n) { return (n + 3) & ~3; }
 int solver_block_capacity(int Ecap);

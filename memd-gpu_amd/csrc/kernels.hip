@@ -685,8 +685,9 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
             if (ed.d_trace && st->pass < ed.max_passes)
                 ed.d_trace[(size_t)st->pass * 2 * ed.K + ke] = Etot;
             if (ed.stats) {
-                atomicAdd(&ed.stats[0], (unsigned long long)Etot);
-                atomicAdd(&ed.stats[1], (unsigned long long)Etot * (unsigned long long)ed.n);
+                unsigned long long* sl = ed.stats + 4 * (blockIdx.x % kStatSlots);
+                atomicAdd(&sl[0], (unsigned long long)Etot);
+                atomicAdd(&sl[1], (unsigned long long)Etot * (unsigned long long)ed.n);
             }
         }
     }
@@ -1878,8 +1879,9 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         kb += nb;
     }
     if (ed.stats && br.y == 0 && tid == 0) {
-        atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
-        atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
+        unsigned long long* sl = ed.stats + 4 * (blockIdx.x % kStatSlots);
+        atomicAdd(&sl[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
+        atomicAdd(&sl[3], (unsigned long long)n * (unsigned long long)T);
     }
     if (!(cl < cgn)) return;
     if (STOP) {   // mean = sum / nvalid (0/0 = NaN with no valid direction: the decision then stops)
@@ -1996,8 +1998,9 @@ __global__ __launch_bounds__(64) void k_eval_stop(const EpochDesc* __restrict__ 
         amps = amps + sqrt(s2) * 0.5;
     }
     if (ed.stats && br.y == 0 && lane == 0) {
-        atomicAdd(&ed.stats[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
-        atomicAdd(&ed.stats[3], (unsigned long long)n * (unsigned long long)T);
+        unsigned long long* sl = ed.stats + 4 * (blockIdx.x % kStatSlots);
+        atomicAdd(&sl[2], (unsigned long long)nv * (unsigned long long)n * (unsigned long long)T);
+        atomicAdd(&sl[3], (unsigned long long)n * (unsigned long long)T);
     }
     if (t >= T) return;
     double q = 0.0;
