@@ -995,10 +995,10 @@ __device__ __forceinline__ BlockRowIn block_row_in(const EpochDesc& ed, const in
 //      barrier, r_u = 3 (sl_u - sl_{u-1}) with sl_u = (Y_{u+1} - Y_u) * ih_u and the
 //      block's Thomas sweeps (y = A_b^{-1} r) run in place in the tile, a rolling
 //      window of Y values in registers keeping every read ahead of the writes.
-//   3. Lanes over (row, channel) pairs store the block rows' y (coalesced).
+//   3. The backward sweep stores each block row's y as it is formed (lanes of a
+//      wave are consecutive channels of one row, so the stores are coalesced).
 __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __restrict__ eps,
                                                          const int2* __restrict__ tab, int rmax, int tile_off) {
-    constexpr int L = kSolverBlock;
     extern __shared__ double lds[];
     // per tile row r (rmax + 3 rows; the entries past a row's use are written, never read)
     const int rw = rmax + 3;
@@ -1030,11 +1030,9 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const int E = it.z;
         const int bpw = ed.bpw;
         const BlockItemGeom geo = block_item_geom(it, bpw);
-        const int J = geo.J, b0 = geo.b0, nb = geo.nb, p0 = geo.p0, qlast = geo.qlast, R = geo.R, RW = geo.RW;
-        // idx / d as (idx * ceil(2^32 / d)) >> 32 in 64 bits: exact for idx * (d - 1) < 2^32
-        // (the tile bounds idx < np * RW by 160 KB / 8 and d <= 256 here); the constant
-        // is 2^32 itself when d = 1
-        const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
+        const int J = geo.J, b0 = geo.b0, nb = geo.nb, p0 = geo.p0, qlast = geo.qlast, RW = geo.RW;
+        // idx / np as (idx * ceil(2^32 / np)) >> 32 in 64 bits: exact for idx * (np - 1) <
+        // 2^32 (the tile bounds idx < np * RW by 160 KB / 8 and np <= 256 here)
         const unsigned long long pmagic = (0x100000000ull + (unsigned)np - 1) / (unsigned)np;
         const GPtr<int> X = gptr(ed.X + (size_t)ke * ed.Ecap);
         const double* curt = ed.curt;   // [T][np] current signal, sample-major
@@ -1157,28 +1155,19 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                 ym = y0;
                 y0 = yp;
             }
-            // backward sweep, y in place
+            // backward sweep: each row's y goes straight to ysol (a wave's lanes are
+            // consecutive channels of one row: coalesced), and the rows beside the
+            // separators to the separator inputs, separator-major for k_reduced
             double x = 0.0;
+            const int b = b0 + slot;
 #pragma unroll 4
             for (int i = len - 1; i >= 0; --i) {
                 const double z = Yc[(i + 1) * np];
                 x = (i == len - 1) ? z : fma(-s_cp[off + i], x, z);
-                Yc[(i + 1) * np] = x;
+                ys[(unsigned)(p + i) * (unsigned)np + (unsigned)c] = x;
+                if (i == len - 1 && b < J) sip[((size_t)(b + 1) * 3 + 0) * np + c] = x;   // row 32 (b+1) - 1
             }
-        }
-        __syncthreads();
-        // 3. block rows' y, coalesced (separator rows hold Y in the tile: skipped;
-        //    row N = 32(J+1) is a block row)
-        for (int idx = tid; idx < n * R; idx += kBlockThreads) {   // (row, channel)
-            const int r = (int)(((unsigned long long)idx * nmagic) >> 32);
-            const int c2 = idx - r * n;
-            const int u = p0 + r;
-            if ((u & (L - 1)) == 0 && (u >> 5) <= J) continue;
-            const double yv = sY[(r + 1) * np + c2];
-            ys[(unsigned)u * (unsigned)np + (unsigned)c2] = yv;
-            // the rows beside separator j = 32j -/+ 1, separator-major for k_reduced
-            if ((u & (L - 1)) == L - 1 && ((u + 1) >> 5) <= J) sip[((size_t)((u + 1) >> 5) * 3 + 0) * np + c2] = yv;
-            if ((u & (L - 1)) == 1 && u > L) sip[((size_t)(u >> 5) * 3 + 2) * np + c2] = yv;
+            if (b >= 1) sip[((size_t)b * 3 + 2) * np + c] = x;   // row 32 b + 1
         }
         it = nit;
     }
