@@ -683,12 +683,22 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
     HIP_TRY(ctx, launch_init(plan, s));
     HIP_TRY(ctx, hipEventRecord(ctx->t0, s));
     const bool graph = o.use_graph && steps > 1 && !where.fixed;
+    // fix mode: the graph holds G consecutive steps (the largest divisor of the step
+    // count up to 16), so replays, and the gaps between them, are G times fewer;
+    // stop mode polls between chunks and captures one step
+    int G = 1;
+    if (o.mode != MEMD_MODE_STOP)
+        for (int d = 16; d > 1; --d)
+            if (steps % d == 0) {
+                G = d;
+                break;
+            }
     if (graph) {
-        // capture one step on the owned stream.  The graph bakes in every launch
+        // capture G steps on the owned stream.  The graph bakes in every launch
         // argument, all of which come from the plan (descriptor array, block tables,
         // work lists and their counters, grids, LDS sizes): it is replayed only for
-        // a plan equal to the captured one in every field.
-        const std::string key = plan_key(plan);
+        // a plan equal to the captured one in every field, and the same G.
+        const std::string key = plan_key(plan) + "G" + std::to_string(G);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) {
                 HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
@@ -696,7 +706,8 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             }
             hipGraph_t g = nullptr;
             HIP_TRY(ctx, hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
-            hipError_t le = launch_step(plan, ctx->own);
+            hipError_t le = hipSuccess;
+            for (int i = 0; i < G && le == hipSuccess; ++i) le = launch_step(plan, ctx->own);
             hipError_t ce = hipStreamEndCapture(ctx->own, &g);
             if (le != hipSuccess || ce != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -717,7 +728,7 @@ memd_status run_epochs(memd_ctx* ctx, const std::vector<EpochIn>& in, const memd
             if (graph) {
                 HIP_TRY(ctx, hipEventRecord(ctx->ev_in, s));
                 HIP_TRY(ctx, hipStreamWaitEvent(ctx->own, ctx->ev_in, 0));
-                for (long i = 0; i < nsteps; ++i) HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->own));
+                for (long i = 0; i < nsteps; i += G) HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->own));
                 HIP_TRY(ctx, hipEventRecord(ctx->ev_out, ctx->own));
                 HIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_out, 0));
             } else {
