@@ -1197,11 +1197,19 @@ __device__ __forceinline__ double knot_M(const double* row, int rst, const doubl
 // no registers: a whole chunk is one round trip), and the chain reads LDS only;
 // the back substitution stages the forward values the same way.  Arithmetic per
 // channel and separator as the oracle's phase 2.
-constexpr int kRedStage = 3072;   // doubles per chunk: separators x 3 x np (forward), x np (back)
-__global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl) {
+// Staging doubles per chunk (separators x 3 x np forward, x np back): 48 np, within
+// [768, 3072], for the widest epoch of the launch.  A small stage keeps more of
+// these latency-bound waves resident; a wide epoch still gets 16 separators per
+// chunk up to 64 channels.
+int reduced_stage(int nmax) {
+    const int v = 48 * channel_stride(nmax);
+    return v < 768 ? 768 : (v > 3072 ? 3072 : v);
+}
+
+__global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__ eps, WorkLists wl, int kRedStage) {
     constexpr int L = kSolverBlock;
     constexpr int kCh = (kMaxChannels + 63) / 64;   // channels per lane
-    __shared__ __align__(16) double sIn[kRedStage];
+    extern __shared__ __align__(16) double sIn[];   // kRedStage doubles
     __shared__ double sA[64], sB[64];
     __shared__ int sLo[64], sUp[64];
     const ListCursor cu = list_cursor(wl, kListReduced);
@@ -2313,7 +2321,12 @@ static void launch_kernel(int which, const StepPlan& p, hipStream_t s) {
         hipLaunchKernelGGL(k_block, dim3(p.blocks[kMapBlock]), dim3(kBlockThreads), p.block_lds, s, p.eps,
                            p.tab[kMapBlock], p.block_rmax, (int)p.block_tile_off);
         break;
-    case 4: MEMD_LIST_LAUNCH(k_reduced, kListReduced, 64); break;
+    case 4: {
+        const int stage = reduced_stage(p.nmax);
+        hipLaunchKernelGGL(k_reduced, dim3(p.grid[kListReduced]), dim3(64), (size_t)stage * sizeof(double), s, p.eps,
+                           p.wl, stage);
+        break;
+    }
     case 5:   // the fused evaluation forms M itself; records only for stop mode's evaluation
         if (p.coef_records) MEMD_LIST_LAUNCH(k_coef, kListCoef, 256);
         break;
