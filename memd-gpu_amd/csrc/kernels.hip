@@ -1616,8 +1616,12 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int tlast = min(t0 + 64 * TW - 1, T - 1);
     const int tid = threadIdx.x;
     const int wbase = uniform(tid & ~63);
-    const int cl = tid >> lgq;
-    const int q = tid & ((1 << lgq) - 1);
+    // lane -> (channel cl, quad q).  Fix mode: channels fastest, so a wave's lanes
+    // share 64 / CG quads' crossing patterns and read CG channel rows of one interval
+    // (distinct banks: the row stride is odd).  Stop mode: quads fastest, the layout
+    // its per-wave channel-group sums assume.
+    const int cl = STOP ? tid >> lgq : tid & (CG - 1);
+    const int q = STOP ? tid & ((1 << lgq) - 1) : tid >> (8 - lgq);
     const int c = cg * CG + cl;
     const int ts = t0 + 4 * q;
     const int wi = q >> 4;               // tile word of the quad
