@@ -1492,6 +1492,15 @@ __device__ __forceinline__ void row_scan16(int& x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);   // row_shr:8
 }
 
+// The value of lane + K of the same 16-lane row (DPP row shift; lanes whose
+// source would leave the row get 0, and their results are not used).
+template <int K>
+__device__ __forceinline__ double row_from(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x100 + K, 0xF, 0xF, false);   // row_shl:K
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x100 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Phase B of one direction for a lane's 4 consecutive samples: both envelopes
 // (upper u, lower l) at each sample, the mean of the two added to the
 // accumulator.  Each sample uses the record of X_j < t <= X_{j+1}.  On entry ru /
@@ -1572,7 +1581,6 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                                                    const int2* __restrict__ tab) {
     constexpr int NS = 2 * kEvalBatch;   // segments (direction, envelope) per batch
     constexpr int NT = 256;              // threads per workgroup
-    __shared__ double sG[STOP ? 2 * 2 * 4 * 64 : 1];   // [buffer][pending direction][wave][sample]: group sums
     // spline records, segment s: [channel][stride] from its record base, as two
     // arrays of 16-byte halves (a, b) and (c, d): a wave's record reads and writes
     // are then 16 bytes apart, not every other 16 bytes of 32-byte records
@@ -1616,12 +1624,12 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int tlast = min(t0 + 64 * TW - 1, T - 1);
     const int tid = threadIdx.x;
     const int wbase = uniform(tid & ~63);
-    // lane -> (channel cl, quad q).  Fix mode: channels fastest, so a wave's lanes
-    // share 64 / CG quads' crossing patterns and read CG channel rows of one interval
-    // (distinct banks: the row stride is odd).  Stop mode: quads fastest, the layout
-    // its per-wave channel-group sums assume.
-    const int cl = STOP ? tid >> lgq : tid & (CG - 1);
-    const int q = STOP ? tid & ((1 << lgq) - 1) : tid >> (8 - lgq);
+    // lane -> (channel cl, quad q), channels fastest: a wave's lanes share 64 / CG
+    // quads' crossing patterns and read CG channel rows of one interval (distinct
+    // banks: the row stride is odd); in stop mode (CG = 16) a quad's channels are one
+    // 16-lane row for the amplitude sums
+    const int cl = tid & (CG - 1);
+    const int q = tid >> (8 - lgq);
     const int c = cg * CG + cl;
     const int ts = t0 + 4 * q;
     const int wi = q >> 4;               // tile word of the quad
@@ -1632,22 +1640,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double nv = 0.0;
     int kc1 = 0;   // end of the staged chunk of formation inputs
-    // stop mode: directions whose group sums wait in sG[gbuf] for wave 0 (at most 2)
-    int gbuf = 0, npend = 0, kpend0 = 0, kpend1 = 0;
     const int G16 = (n + 15) >> 4;
-    // wave 0 adds the 4 waves' group sums of the pending directions and stores the
-    // block sums; called right after a barrier
-    auto flush_blocks = [&]() {
-        if (tid < 64 && t0 + tid <= tlast) {
-            for (int p2 = 0; p2 < npend; ++p2) {
-                const double* gb = sG + ((gbuf * 2 + p2) * 4) * 64 + tid;
-                const double b = ((gb[0] + gb[64]) + gb[128]) + gb[192];
-                ed.ampb[((size_t)(p2 ? kpend1 : kpend0) * G16 + cg) * T + t0 + tid] = b;
-            }
-        }
-        gbuf ^= 1;
-        npend = 0;
-    };
     for (int kb = 0; kb < K;) {
         if (tid < 64) {   // formation
             if (kb >= kc1) {   // stage the next kEvalChunk directions: lane (direction, envelope)
@@ -1853,30 +1846,21 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                                 x1, ts, tlast, acc, dd);
             }
             if (STOP) {
-                // lanes (channel 4w + j, quad q) of wave w: lane 16 j + q; samples past
-                // the tile and absent channels carry 0, which adds exactly
-                const int lw = tid & 63;
-                double g[4];
+                // the block's amplitude sum per sample: the quad's 16 channels are one
+                // 16-lane row (lane = channel), summed in the fixed order: channels in
+                // groups of 4, ((c0 + c1) + c2) + c3, then the groups, ((g0 + g1) + g2) +
+                // g3 (row shifts; the sums of lanes not at a group start are unused).
+                // Samples past the tile and absent channels carry 0, which adds exactly.
 #pragma unroll
                 for (int s2 = 0; s2 < 4; ++s2) {
                     const double x = dd[s2] * dd[s2];
-                    const double x1 = __shfl(x, (lw & 15) + 16), x2 = __shfl(x, (lw & 15) + 32),
-                                 x3 = __shfl(x, (lw & 15) + 48);
-                    g[s2] = ((x + x1) + x2) + x3;
-                }
-                if (lw < 16) {
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) sG[((gbuf * 2 + npend) * 4 + (tid >> 6)) * 64 + 4 * lw + s2] = g[s2];
-                }
-                if (npend == 0) kpend0 = kb + d; else kpend1 = kb + d;
-                if (++npend == 2) {   // two directions per barrier; the buffers alternate
-                    __syncthreads();
-                    flush_blocks();
+                    const double g = ((x + row_from<1>(x)) + row_from<2>(x)) + row_from<3>(x);
+                    const double b = ((g + row_from<4>(g)) + row_from<8>(g)) + row_from<12>(g);
+                    if (cl == 0 && ts + s2 <= tlast) ed.ampb[((size_t)(kb + d) * G16 + cg) * T + ts + s2] = b;
                 }
             }
         }
         __syncthreads();
-        if (STOP && npend > 0) flush_blocks();   // the batch's last direction, after its barrier
         kb += nb;
     }
     if (ed.stats && br.y == 0 && tid == 0) {
