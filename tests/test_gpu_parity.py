@@ -135,6 +135,19 @@ def test_graph_and_direct_launch_agree(ctx, oracle):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n_imf,S", [(2, 17), (4, 6), (3, 16), (5, 4)])
+def test_graph_steps_per_replay(ctx, oracle, n_imf, S):
+    # fix mode captures the largest divisor (up to 16) of the (n_imf - 1) * S steps per
+    # graph: 17 steps -> 1 per replay, 18 -> 9, 32 -> 16, 16 -> 16; each against the
+    # oracle and against direct launches
+    import synth
+    y = synth.sines_epoch(8, 2000, 90 + S)
+    print(f"graph steps n_imf={n_imf} S={S}", flush=True)
+    a = _check_epoch(ctx, oracle, y, 16, n_imf, S, use_graph=True)
+    b = _check_epoch(ctx, oracle, y, 16, n_imf, S, use_graph=False)
+    assert np.array_equal(a, b)
+
+
 def test_graph_replays_only_for_an_equal_plan(ctx, oracle):
     # consecutive single-epoch calls with the same launch geometry (block counts,
     # grids, list capacities) but different workspace layouts: T = 1000 and 1010 move
