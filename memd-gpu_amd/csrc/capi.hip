@@ -126,7 +126,7 @@ struct EpochIn {
 
 // Byte layout of one epoch's workspace inside the arena.
 struct EpochLayout {
-    size_t running, curt, mask, pre, E, mir, X, fac, redsys, ysol, coef, sep, sepin, mean, ampb, tstat, red, sift, st,
+    size_t running, vt, curt, mask, pre, E, mir, X, fac, redsys, ysol, coef, sep, sepin, mean, ampb, tstat, red, sift, st,
         total;
 };
 
@@ -156,6 +156,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, const PathKind& pk) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t at = off; off += align_up(bytes); return at; };
     L.running = take((size_t)n * T * 8);
+    L.vt = take((size_t)n * dir_pad(K) * 8);
     L.curt = take(((size_t)T + 4 * (size_t)K) * channel_stride(n) * 8);   // then yend
     L.mask = take((size_t)K * 2 * W * 8);
     L.pre = take((size_t)K * 2 * W * 4);
@@ -272,6 +273,7 @@ EpochDesc desc_geometry(const memd_ctx* ctx, const EpochIn& I, const memd_opts& 
 // Point a descriptor at its workspace (layout L starting at b).
 void bind_workspace(EpochDesc& d, char* b, const EpochLayout& L) {
     d.running = (double*)(b + L.running);
+    d.vt = (double*)(b + L.vt);
     d.mask = (unsigned long long*)(b + L.mask);
     d.pre = (int*)(b + L.pre);
     d.E = (int*)(b + L.E);

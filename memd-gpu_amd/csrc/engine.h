@@ -29,13 +29,15 @@ constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
 constexpr int kEvalBatch = 8;      // directions per batch
 constexpr int kEvalChunk = 32;     // directions whose formation inputs are staged together
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles up to
-// kFlagLdsMaxCh channels; wider epochs pass kFlagChunk channels at a time through
-// the same tile, kFlagGroup directions per workgroup.
+// kFlagLdsMaxCh channels and runs all directions kFlagGroup at a time; wider
+// epochs pass kFlagChunk channels at a time through the same tile, kFlagGroup
+// directions per workgroup.
 constexpr int kFlagLdsMaxCh = 16;
 constexpr int kFlagChunk = 16;
 constexpr int kFlagGroup = 32;
-constexpr int kFlagDirs = 8;       // directions per k_flags pass
-constexpr int kFlagHalo = 128;     // directions whose halo projections are computed together
+// Directions rounded up to a whole k_flags group: the row length of the
+// channel-major copy of the directions (EpochDesc::vt).
+__host__ __device__ constexpr int dir_pad(int K) { return (K + kFlagGroup - 1) / kFlagGroup * kFlagGroup; }
 // Knot-value / right-hand-side kernel: 256 knots per workgroup, 4 channels per pass.
 // Coefficient kernel: items (channel, interval) per thread.
 // Solver blocks per k_block work item (EpochDesc::bpw).  Default: at most 4 (<= 128
@@ -105,6 +107,8 @@ struct EpochDesc {
     double sd, sd2, tol;
     const double* y;       // [n][T]
     const double* V;       // [K][n]
+    double* vt;            // [n][dir_pad(K)] the directions channel-major, zero past K (k_init):
+                           //   a k_flags group's coefficients of one channel are adjacent words
     double* imfs;          // [N_imf][n][T]
     double* running;       // [n][T]
     double* curt;          // [T][np] current signal, sample-major (written by k_flags every step),

@@ -210,6 +210,14 @@ __global__ __launch_bounds__(256) void k_init(const EpochDesc* __restrict__ eps,
         ed.running[i] = v;
         ed.imfs[i] = v;   // slot 0 holds the signal being sifted (or the residual if N_imf == 1)
     }
+    {   // the directions channel-major (the epoch's blocks stride over n * Kp entries)
+        const int Kp = dir_pad(ed.K);
+        const size_t nvt = (size_t)ed.n * Kp, stride = (nT + 255) / 256 * 256;
+        for (size_t j = i; j < nvt; j += stride) {
+            const int c = (int)(j / Kp), k = (int)(j - (size_t)c * Kp);
+            ed.vt[j] = k < ed.K ? ed.V[(size_t)k * ed.n + c] : 0.0;
+        }
+    }
     if (br.y == 0 && threadIdx.x == 0) {
         EpochState* st = ed.st;
         st->imf = 0;
@@ -241,21 +249,132 @@ __device__ __forceinline__ double wave_from_next(double v) {   // lane i <- lane
     return __hiloint2double(hi, lo);
 }
 
-// One workgroup per (epoch, 256-sample tile) and all directions: the tile of
-// the current signal (plus one sample each side) is staged in LDS once, then
-// every direction's projection, extremum tests and ballots run from LDS.
+// The projections beside each wave of a k_flags tile, per direction d of a pass:
+// Px[d][w + 1] = wave w's first and last, Px[d][0][1] = the projection at t0 - 1
+// and Px[d][kFlagWaves + 1][0] the one at t0 + kFlagTile, so wave w's lane 0 finds its
+// left neighbour at Px[d][w][1] and lane 63 its right one at Px[d][w + 2][0].
+constexpr int kFlagWaves = kFlagTile / 64;
+typedef double FlagEdges[kFlagWaves + 2][2];
+
+// Lane 0 and 63 of every wave store their projections, threads 2d + side below 2 kd
+// the halo projection `hacc` of direction d (side 0: t0 - 1, side 1: t0 + kFlagTile).
+template <int DG>
+__device__ __forceinline__ void put_edges(const double (&acc)[DG], double hacc, int kd, int tid, FlagEdges* Px) {
+    const int lane = tid & 63, wv = tid >> 6;
+    if (tid < 2 * kd) Px[tid >> 1][(tid & 1) ? kFlagWaves + 1 : 0][(tid & 1) ? 0 : 1] = hacc;
+    if (lane == 0 || lane == 63) {
+#pragma unroll
+        for (int d = 0; d < DG; ++d) Px[d][wv + 1][lane == 63 ? 1 : 0] = acc[d];
+    }
+}
+
+// Flag words of directions k0 .. k0 + kd - 1 from the tile's projections (acc[d]
+// at sample t0 + tid) and their edge values: end samples are both extrema, the
+// tested interior samples (within the window) strict extrema against both
+// neighbours; two ballots per (direction, 64 samples), stored by lane 0.
+template <int DG>
+__device__ __forceinline__ void store_flags(const EpochDesc& ed, const double (&acc)[DG], const FlagEdges* Px, int k0,
+                                            int kd, int t0, int tid) {
+    const int T = ed.T, W = ed.W;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int t = t0 + tid;
+    const int w = (t0 >> 6) + wv;
+    const int lim = window_limit(T, ed.window);
+    const bool endk = t == 0 || t == T - 1;
+    const bool test = t > 0 && t < T - 1 && t <= lim;
+    // global-address-space stores: a flat store would also hold up the LDS waits
+    const GMut<unsigned long long> mk = gmut(ed.mask);
+#pragma unroll
+    for (int d = 0; d < DG; ++d) {
+        if (d < kd) {   // uniform
+            const int k = k0 + d;
+            // neighbours: adjacent lanes; across waves and tiles the edge values (every
+            // lane reads the wave's two: broadcast reads).  The lane shifts run with
+            // every lane active, before the selects: under a lane condition the lane
+            // beside a disabled one would read no value
+            const double p = acc[d];
+            const double sa = wave_from_prev(p), sb = wave_from_next(p);
+            const double ea = Px[d][wv][1], eb = Px[d][wv + 2][0];
+            const double a = lane == 0 ? ea : sa;
+            const double b = lane == 63 ? eb : sb;
+            const bool fmax = endk | (test & (p > a) & (p > b));
+            const bool fmin = endk | (test & (p < a) & (p < b));
+            const unsigned long long mmax = __ballot(fmax);
+            const unsigned long long mmin = __ballot(fmin);
+            if (lane == 0 && w < W) {
+                mk[((size_t)k * 2 + 0) * W + w] = mmax;
+                mk[((size_t)k * 2 + 1) * W + w] = mmin;
+            }
+        }
+    }
+}
+
+// Stage channels c0 .. c0 + cn - 1 (cn <= kFlagChunk) of the signal at samples
+// t0 - 1 .. t0 + kFlagTile into LDS rows of kFlagTile + 2 doubles (samples outside
+// 0 .. T-1 are 0): thread i loads sample t0 + i of every channel in one round of
+// loads, threads below 2 cn the two halo columns.
+static_assert(kFlagLdsMaxCh <= kFlagChunk, "the whole-tile path stages its channels in one call");
+__device__ __forceinline__ void stage_columns(const GPtr<double>& cur, int T, int c0, int cn, int t0,
+                                              double* curT, int tid) {
+    constexpr int TW = kFlagTile + 2;
+    const int t = t0 + tid;
+    const unsigned tc = (unsigned)min(t, T - 1);
+    double v[kFlagChunk];
+#pragma unroll
+    for (int c = 0; c < kFlagChunk; ++c) v[c] = c < cn ? cur[(unsigned)(c0 + c) * (unsigned)T + tc] : 0.0;
+    double hv = 0.0;
+    const int hc = tid >> 1, th = (tid & 1) ? t0 + kFlagTile : t0 - 1;
+    const bool hin = tid < 2 * cn && th >= 0 && th < T;
+    if (hin) hv = cur[(unsigned)(c0 + hc) * (unsigned)T + (unsigned)th];
+#pragma unroll
+    for (int c = 0; c < kFlagChunk; ++c)
+        if (c < cn) curT[c * TW + tid + 1] = t < T ? v[c] : 0.0;
+    if (tid < 2 * cn) curT[hc * TW + ((tid & 1) ? TW - 1 : 0)] = hv;
+}
+
+// Sample-major copy of channels c0 .. c0 + cn - 1 of the staged tile for the
+// knot-value gathers: lanes over (sample, channel), channel fastest.
+__device__ __forceinline__ void copy_sample_major(const EpochDesc& ed, const double* curT, int c0, int cn, int t0,
+                                                  int tid) {
+    constexpr int TW = kFlagTile + 2;
+    const int tn = min(kFlagTile, ed.T - t0);
+    const unsigned long long cmagic = (0x100000000ull + (unsigned)cn - 1) / (unsigned)cn;   // idx / cn (as k_block)
+    const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np + c0);
+#pragma unroll 4
+    for (int idx = tid; idx < cn * tn; idx += kFlagTile) {
+        const int tl = (int)(((unsigned long long)idx * cmagic) >> 32);
+        const int c = idx - tl * cn;
+        ct[(unsigned)tl * (unsigned)ed.np + (unsigned)c] = curT[c * TW + tl + 1];
+    }
+}
+
+// Projections of the tile for DG directions from k0 (the chains of directions
+// past K run on the zero coefficients of the channel-major copy and are not
+// used): acc[d] += V[k0 + d][c0 + c] * x(c) for c in 0 .. cn-1, in ascending c.  The
+// DG coefficients of a channel are adjacent words: one base, immediate offsets.
+template <int DG>
+__device__ __forceinline__ void project(const EpochDesc& ed, const double* curT, int c0, int cn, int k0, int tid,
+                                        double (&acc)[DG]) {
+    constexpr int TW = kFlagTile + 2;
+    const int Kp = dir_pad(ed.K);
+    const CPtr<double> VT = cptr(ed.vt) + k0;
+    for (int c = 0; c < cn; ++c) {
+        const double x = curT[c * TW + tid + 1];
+        const CPtr<double> v = VT + (unsigned)(c0 + c) * (unsigned)Kp;
+#pragma unroll
+        for (int d = 0; d < DG; ++d) acc[d] = fma((double)v[d], x, acc[d]);
+    }
+}
+
 // Channel-chunked path of k_flags (n > kFlagLdsMaxCh): one workgroup per (group
 // of kFlagGroup directions, tile).  The tile's channels pass through the LDS tile
 // kFlagChunk at a time, and every direction's chain (and each halo chain) stays in
 // registers across the chunks in ascending channel order: the same fma chain as
 // with the whole tile staged.  The signal is read once per direction group.
-__device__ void flags_chunked(const EpochDesc& ed, const EpochState* st, int by, double* curT) {
+__device__ void flags_chunked(const EpochDesc& ed, const EpochState* st, int by, double* curT, FlagEdges* Px) {
     constexpr int TW = kFlagTile + 2;
     constexpr int DG = kFlagGroup;
     constexpr int CH = kFlagChunk;
-    constexpr int NW = kFlagTile / 64;
-    __shared__ double Qe[DG][NW][2];   // each wave's first and last projection
-    __shared__ double Qh[DG][2];       // projections at t0-1 and t0+kFlagTile
     const int n = ed.n, T = ed.T, K = ed.K;
     const int tiles = (T + kFlagTile - 1) / kFlagTile;
     const int grp = by / tiles;
@@ -263,17 +382,12 @@ __device__ void flags_chunked(const EpochDesc& ed, const EpochState* st, int by,
     const int kg0 = grp * DG;
     const int kd = min(DG, K - kg0);
     const int t0 = tile * kFlagTile;
-    const int tn = min(kFlagTile, T - t0);
     const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
     const CPtr<double> V = cptr(ed.V);
     const int tid = threadIdx.x;
     double acc[DG];
-    unsigned vrow[DG];   // chains past the last direction repeat it and are ignored
 #pragma unroll
-    for (int d = 0; d < DG; ++d) {
-        acc[d] = 0.0;
-        vrow[d] = (unsigned)min(kg0 + d, K - 1) * (unsigned)n;
-    }
+    for (int d = 0; d < DG; ++d) acc[d] = 0.0;
     // halo chains: thread 2d + side, for direction d and column 0 or TW - 1
     const unsigned hrow = (unsigned)min(kg0 + (tid >> 1), K - 1) * (unsigned)n;
     const int hcol = (tid & 1) ? TW - 1 : 0;
@@ -281,233 +395,62 @@ __device__ void flags_chunked(const EpochDesc& ed, const EpochState* st, int by,
     for (int c0 = 0; c0 < n; c0 += CH) {
         const int cn = min(CH, n - c0);
         __syncthreads();   // the previous chunk has been read
-        {   // stage the chunk: G loads per lane in flight before their LDS stores
-            constexpr int G = 8;
-            const int tot = cn * TW;
-            for (int i0 = tid; i0 < tot; i0 += kFlagTile * G) {
-                double v[G];
-#pragma unroll
-                for (int g = 0; g < G; ++g) {   // unconditional loads from clamped addresses
-                    const int idx = min(i0 + g * kFlagTile, tot - 1);
-                    const int c = idx / TW, i = idx - c * TW;
-                    v[g] = cur[(unsigned)(c0 + c) * (unsigned)T + (unsigned)min(max(t0 - 1 + i, 0), T - 1)];
-                }
-#pragma unroll
-                for (int g = 0; g < G; ++g) {   // samples outside 0 .. T-1 are 0
-                    const int idx = i0 + g * kFlagTile;
-                    const int c = idx / TW, t = t0 - 1 + (idx - c * TW);
-                    if (idx < tot) curT[idx] = (t >= 0 && t < T) ? v[g] : 0.0;
-                }
-            }
-        }
+        stage_columns(cur, T, c0, cn, t0, curT, tid);
         __syncthreads();
-        if (grp == 0) {   // sample-major copy of the chunk for the knot-value gathers
-            const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np + c0);
-            for (int idx = tid; idx < CH * tn; idx += kFlagTile) {
-                const int tl = idx / CH, c = idx - tl * CH;
-                if (c < cn) ct[(unsigned)tl * (unsigned)ed.np + (unsigned)c] = curT[c * TW + tl + 1];
-            }
-        }
+        if (grp == 0) copy_sample_major(ed, curT, c0, cn, t0, tid);
         if (tid < 2 * kd) {
             for (int c = 0; c < cn; ++c) hacc = fma((double)V[hrow + c0 + c], curT[c * TW + hcol], hacc);
         }
-        // every curT read feeds DG chains
-        int c = 0;
-        for (; c + 4 <= cn; c += 4) {
-            const double x0 = curT[(c + 0) * TW + tid + 1], x1 = curT[(c + 1) * TW + tid + 1];
-            const double x2 = curT[(c + 2) * TW + tid + 1], x3 = curT[(c + 3) * TW + tid + 1];
-#pragma unroll
-            for (int d = 0; d < DG; ++d) {
-                const CPtr<double> v = V + vrow[d] + c0 + c;
-                acc[d] = fma((double)v[0], x0, acc[d]);
-                acc[d] = fma((double)v[1], x1, acc[d]);
-                acc[d] = fma((double)v[2], x2, acc[d]);
-                acc[d] = fma((double)v[3], x3, acc[d]);
-            }
-        }
-        for (; c < cn; ++c) {
-            const double x = curT[c * TW + tid + 1];
-#pragma unroll
-            for (int d = 0; d < DG; ++d) acc[d] = fma((double)V[vrow[d] + c0 + c], x, acc[d]);
-        }
+        project(ed, curT, c0, cn, kg0, tid, acc);
     }
-    const int lane = tid & 63, wv = tid >> 6;
-    if (tid < 2 * kd) Qh[tid >> 1][tid & 1] = hacc;
-    if (lane == 0 || lane == 63) {
-#pragma unroll
-        for (int d = 0; d < DG; ++d) Qe[d][wv][lane == 63 ? 1 : 0] = acc[d];
-    }
+    put_edges(acc, hacc, kd, tid, Px);
     __syncthreads();
-    const int lim = window_limit(T, ed.window);
-    const int t = t0 + tid;
-    const int w = (t0 >> 6) + wv;
-#pragma unroll
-    for (int d = 0; d < DG; ++d) {
-        if (d < kd) {   // uniform
-            const int k = kg0 + d;
-            // neighbours: adjacent lanes; across waves and tiles through LDS
-            const double p = acc[d];
-            double a = wave_from_prev(p);
-            double b = wave_from_next(p);
-            if (lane == 0) a = wv > 0 ? Qe[d][wv - 1][1] : Qh[d][0];
-            if (lane == 63) b = wv < NW - 1 ? Qe[d][wv + 1][0] : Qh[d][1];
-            bool fmax = false, fmin = false;
-            if (t < T) {
-                if (t == 0 || t == T - 1) {
-                    fmax = fmin = true;
-                } else if (t <= lim) {
-                    fmax = (p > a) && (p > b);
-                    fmin = (p < a) && (p < b);
-                }
-            }
-            const unsigned long long mmax = __ballot(fmax);
-            const unsigned long long mmin = __ballot(fmin);
-            if (lane == 0 && w < ed.W) {
-                const GMut<unsigned long long> mk = gmut(ed.mask);
-                mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-                mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
-            }
-        }
-    }
+    store_flags(ed, acc, Px, kg0, kd, t0, tid);
 }
 
+// One workgroup per (epoch, 256-sample tile) and all directions (n <= kFlagLdsMaxCh):
+// the tile of the current signal (plus one sample each side) is staged in LDS
+// once, then every pass of kFlagGroup directions runs its projections, extremum
+// tests and ballots from LDS.
 __global__ __launch_bounds__(kFlagTile) void k_flags(const EpochDesc* __restrict__ eps,
                                                      const int2* __restrict__ tab) {
     constexpr int TW = kFlagTile + 2;
-    constexpr int KD = kFlagDirs;   // directions per pass: independent fma chains per thread
-    constexpr int NW = kFlagTile / 64;
+    constexpr int DG = kFlagGroup;   // directions per pass: independent fma chains per thread
     extern __shared__ double curT[];   // [n][TW]: samples t0-1 .. t0+kFlagTile
-    __shared__ double Pe[KD][NW][2];      // each wave's first and last projection
-    __shared__ double Ph[kFlagHalo][2];   // projections at t0-1 and t0+kFlagTile
+    __shared__ FlagEdges Px[DG];
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
     if (st->done) return;
     if (ed.n > kFlagLdsMaxCh) {   // its blocks are (direction group, tile) pairs (blocks_for)
-        flags_chunked(ed, st, br.y, curT);
+        flags_chunked(ed, st, br.y, curT, Px);
         return;
     }
     const int n = ed.n, T = ed.T, K = ed.K;
-    const int tile = br.y;
-    const int t0 = tile * kFlagTile;
+    const int t0 = br.y * kFlagTile;
     const GPtr<double> cur = gptr(cur_slot(ed, st->imf));
     const CPtr<double> V = cptr(ed.V);
     const int tid = threadIdx.x;
-    {   // stage the tile: G loads per lane in flight before their LDS stores
-        constexpr int G = 8;
-        const int tot = n * TW;
-        for (int i0 = tid; i0 < tot; i0 += kFlagTile * G) {
-            double v[G];
-#pragma unroll
-            for (int g = 0; g < G; ++g) {   // unconditional loads from clamped addresses
-                const int idx = min(i0 + g * kFlagTile, tot - 1);
-                const int c = idx / TW, i = idx - c * TW;
-                v[g] = cur[(unsigned)c * (unsigned)T + (unsigned)min(max(t0 - 1 + i, 0), T - 1)];
-            }
-#pragma unroll
-            for (int g = 0; g < G; ++g) {   // samples outside 0 .. T-1 are 0
-                const int idx = i0 + g * kFlagTile;
-                const int c = idx / TW, t = t0 - 1 + (idx - c * TW);
-                if (idx < tot) curT[idx] = (t >= 0 && t < T) ? v[g] : 0.0;
-            }
-        }
-    }
+    stage_columns(cur, T, 0, n, t0, curT, tid);
     __syncthreads();
-    {   // sample-major copy of the tile for the knot-value gathers: lanes over
-        // (sample, channel), channel fastest; idx / n as in k_block
-        const int tn = min(kFlagTile, T - t0);
-        const unsigned long long nmagic = (0x100000000ull + (unsigned)n - 1) / (unsigned)n;
-        const GMut<double> ct = gmut(ed.curt + (size_t)t0 * ed.np);
-#pragma unroll 4
-        for (int idx = tid; idx < n * tn; idx += kFlagTile) {
-            const int tl = (int)(((unsigned long long)idx * nmagic) >> 32);
-            const int c = idx - tl * n;
-            ct[(unsigned)tl * (unsigned)ed.np + (unsigned)c] = curT[c * TW + tl + 1];
-        }
-    }
-    const int lim = window_limit(T, ed.window);
-    const int t = t0 + tid;
-    const int lane = tid & 63;
-    const int w = (t0 >> 6) + (tid >> 6);
-    for (int kh = 0; kh < K; kh += kFlagHalo) {
-        const int khn = min(kFlagHalo, K - kh);
-        __syncthreads();
+    copy_sample_major(ed, curT, 0, n, t0, tid);
+    const int hcol = (tid & 1) ? TW - 1 : 0;
+    for (int k0 = 0; k0 < K; k0 += DG) {
+        const int kd = min(DG, K - k0);
+        double acc[DG];
+#pragma unroll
+        for (int d = 0; d < DG; ++d) acc[d] = 0.0;
+        project(ed, curT, 0, n, k0, tid, acc);
         // halo projections of the pass's directions, one chain per (direction, side)
-        if (tid < 2 * khn) {
-            const int kk = tid >> 1, side = tid & 1;
-            const int col = side ? TW - 1 : 0;
-            const unsigned vb = (unsigned)(kh + kk) * (unsigned)n;
-            double acc = 0.0;
-#pragma unroll 16
-            for (int c = 0; c < n; ++c) acc = fma((double)V[vb + c], curT[c * TW + col], acc);
-            Ph[kk][side] = acc;
+        double hacc = 0.0;
+        if (tid < 2 * kd) {
+            const unsigned hrow = (unsigned)(k0 + (tid >> 1)) * (unsigned)n;
+            for (int c = 0; c < n; ++c) hacc = fma((double)V[hrow + c], curT[c * TW + hcol], hacc);
         }
-        for (int k0 = kh; k0 < kh + khn; k0 += KD) {
-            const int kd = min(KD, kh + khn - k0);
-            // the tile's projections for kd directions: every curT read feeds kd chains,
-            // each chain in ascending channel order
-            double acc[KD];
-            unsigned vrow[KD];   // chains past the last direction repeat it and are ignored
-#pragma unroll
-            for (int d = 0; d < KD; ++d) {
-                acc[d] = 0.0;
-                vrow[d] = (unsigned)min(k0 + d, K - 1) * (unsigned)n;
-            }
-            int c = 0;
-            for (; c + 4 <= n; c += 4) {
-                const double x0 = curT[(c + 0) * TW + tid + 1], x1 = curT[(c + 1) * TW + tid + 1];
-                const double x2 = curT[(c + 2) * TW + tid + 1], x3 = curT[(c + 3) * TW + tid + 1];
-#pragma unroll
-                for (int d = 0; d < KD; ++d) {
-                    const CPtr<double> v = V + vrow[d] + c;
-                    acc[d] = fma((double)v[0], x0, acc[d]);
-                    acc[d] = fma((double)v[1], x1, acc[d]);
-                    acc[d] = fma((double)v[2], x2, acc[d]);
-                    acc[d] = fma((double)v[3], x3, acc[d]);
-                }
-            }
-            for (; c < n; ++c) {
-                const double x = curT[c * TW + tid + 1];
-#pragma unroll
-                for (int d = 0; d < KD; ++d) acc[d] = fma((double)V[vrow[d] + c], x, acc[d]);
-            }
-            const int wv = tid >> 6;
-            __syncthreads();   // Pe of the previous pass has been read; Ph is complete
-            if (lane == 0 || lane == 63) {
-#pragma unroll
-                for (int d = 0; d < KD; ++d) Pe[d][wv][lane == 63 ? 1 : 0] = acc[d];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int d = 0; d < KD; ++d) {
-                if (d >= kd) break;
-                const int k = k0 + d;
-                // neighbours: adjacent lanes; across waves and tiles through LDS
-                const double p = acc[d];
-                double a = wave_from_prev(p);
-                double b = wave_from_next(p);
-                if (lane == 0) a = wv > 0 ? Pe[d][wv - 1][1] : Ph[k - kh][0];
-                if (lane == 63) b = wv < NW - 1 ? Pe[d][wv + 1][0] : Ph[k - kh][1];
-                bool fmax = false, fmin = false;
-                if (t < T) {
-                    if (t == 0 || t == T - 1) {
-                        fmax = fmin = true;
-                    } else if (t <= lim) {
-                        fmax = (p > a) && (p > b);
-                        fmin = (p < a) && (p < b);
-                    }
-                }
-                const unsigned long long mmax = __ballot(fmax);
-                const unsigned long long mmin = __ballot(fmin);
-                if (lane == 0 && w < ed.W) {
-                    // global-address-space stores: a flat store would also hold up the
-                    // LDS waits of the next pass
-                    const GMut<unsigned long long> mk = gmut(ed.mask);
-                    mk[((size_t)k * 2 + 0) * ed.W + w] = mmax;
-                    mk[((size_t)k * 2 + 1) * ed.W + w] = mmin;
-                }
-            }
-        }
+        __syncthreads();   // the previous pass has read its edge values
+        put_edges(acc, hacc, kd, tid, Px);
+        __syncthreads();
+        store_flags(ed, acc, Px, k0, kd, t0, tid);
     }
 }
 
