@@ -26,8 +26,15 @@ __host__ __device__ inline int eval_max_int(int tw) { return 32 * tw + 1; }
 // stride, at most 2 x 16 x 33 (TW = 1), 2 x 8 x 65, 2 x 4 x 129.
 constexpr int kEvalCap = 1056;
 constexpr int kEvalXkCap = 384;    // knot positions (worst direction: 2 x 131)
-constexpr int kEvalBatch = 8;      // directions per batch
+constexpr int kEvalBatch = 16;     // directions per batch (2 segments each: 32 formation lanes)
 constexpr int kEvalChunk = 32;     // directions whose formation inputs are staged together
+// Flag words of the tile held for a batch's segments and for a staged chunk: a
+// tile of TW words takes batches of at most min(2 kEvalBatch, kEvalSegWords / TW)
+// segments and chunks of min(kEvalChunk, kEvalChunkWords / (2 TW)) directions
+// (TW = 1, 2: 16 directions per batch, 32 per chunk; TW = 4: 8 and 16)
+constexpr int kEvalSegWords = 64;
+constexpr int kEvalChunkWords = 128;
+static_assert(kEvalChunk >= kEvalBatch && kEvalChunkWords >= kEvalSegWords, "a batch fits in one staged chunk");
 // Projection/extrema: the LDS-staged kernel holds n x 258 doubles up to
 // kFlagLdsMaxCh channels and runs all directions kFlagGroup at a time; wider
 // epochs pass kFlagChunk channels at a time through the same tile, kFlagGroup

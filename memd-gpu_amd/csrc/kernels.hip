@@ -1435,6 +1435,15 @@ __device__ __forceinline__ void row_scan16(int& x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);   // row_shr:8
 }
 
+// Inclusive prefix sum over lanes 0 .. 31: within each 16-lane row, then lanes
+// 16 .. 31 add the first row's total (lanes above 31 are not used).
+__device__ __forceinline__ void seg_scan(int& x) {
+    row_scan16(x);
+    const int r0 = __builtin_amdgcn_readlane(x, 15);
+    if (threadIdx.x & 16) x += r0;
+}
+static_assert(2 * kEvalBatch <= 32, "the formation's prefix sums cover 32 segments");
+
 // The value of lane + K of the same 16-lane row (DPP row shift; lanes whose
 // source would leave the row get 0, and their results are not used).
 template <int K>
@@ -1536,12 +1545,14 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     // 32-bit offset (one (direction, envelope)'s rows stay far below 4 GB)
     __shared__ const double* sSegP[NS][3];
     __shared__ int sIp[NS + 1];          // item prefix; entries past the batch hold the total
-    __shared__ int sCum[NS][4];          // knots before the end of each word of the tile, less jlo + 1
-    __shared__ unsigned long long sMsk[NS][4];   // flag words; sample 0's flag cleared
+    // segment s, tile word w at [s * TW + w]: the knots before the end of the word,
+    // less jlo + 1, and the flag word (sample 0's flag cleared)
+    __shared__ int sCum[kEvalSegWords];
+    __shared__ unsigned long long sMsk[kEvalSegWords];
     __shared__ int sNb;
     __shared__ unsigned sValid;   // bit s: segment s of the batch is valid (both envelopes of its direction)
     __shared__ int cE[2 * kEvalChunk], cPr[2 * kEvalChunk];   // staged formation inputs
-    __shared__ unsigned long long cM[2 * kEvalChunk][4];
+    __shared__ unsigned long long cM[kEvalChunkWords];          // entry i, word w at [i * TW + w]
     const int2 br = tab[blockIdx.x];
     const EpochDesc& ed = eps[br.x];
     const EpochState* st = ed.st;
@@ -1558,6 +1569,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int TW = ed.etw;
     const int lgq = 4 + (TW >> 1);       // log2 quads per channel: 4, 5, 6 for TW = 1, 2, 4
     const int CG = 16 / TW;
+    const int nsmax = min(NS, kEvalSegWords / TW);                 // segments per batch
+    const int chunk = min(kEvalChunk, kEvalChunkWords / (2 * TW));  // directions per staged chunk
     const int groups = (n + CG - 1) / CG;
     const int tile = br.y / groups;
     const int cg = br.y - tile * groups;
@@ -1586,11 +1599,11 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
     const int G16 = (n + 15) >> 4;
     for (int kb = 0; kb < K;) {
         if (tid < 64) {   // formation
-            if (kb >= kc1) {   // stage the next kEvalChunk directions: lane (direction, envelope)
+            if (kb >= kc1) {   // stage the next chunk of directions: lane (direction, envelope)
                 const int kk = kb + (tid >> 1), ke = 2 * kk + (tid & 1);
                 int E = 0, pr = 0;
                 unsigned long long m[4] = {0, 0, 0, 0};
-                if (kk < K) {
+                if (tid < 2 * chunk && kk < K) {
                     E = gptr(ed.E)[ke];
                     pr = gptr(ed.pre)[(size_t)ke * W + w0];
                     const GPtr<unsigned long long> mk = gptr(ed.mask);
@@ -1598,16 +1611,19 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                     for (int w = 0; w < 4; ++w)
                         if (w < TW && w0 + w < W) m[w] = mk[(size_t)ke * W + w0 + w];
                 }
-                cE[tid] = E;
-                cPr[tid] = pr;
+                if (tid < 2 * chunk) {
+                    cE[tid] = E;
+                    cPr[tid] = pr;
 #pragma unroll
-                for (int w = 0; w < 4; ++w) cM[tid][w] = m[w];
-                kc1 = kb + kEvalChunk;
+                    for (int w = 0; w < 4; ++w)
+                        if (w < TW) cM[tid * TW + w] = m[w];
+                }
+                kc1 = kb + chunk;
             }
             const int k = kb + (tid >> 1), env = tid & 1;
             // a batch stays inside the staged chunk
-            const bool have = tid < NS && k < K && k < kc1;
-            const int ci = 2 * (k - (kc1 - kEvalChunk)) + env;   // chunk entry
+            const bool have = tid < nsmax && k < K && k < kc1;
+            const int ci = 2 * (k - (kc1 - chunk)) + env;   // chunk entry
             int E = 0, pr = 0, inside = 0;
             int cum[4];
             unsigned long long msk[4];
@@ -1620,7 +1636,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             for (int w = 0; w < 4; ++w) {
                 msk[w] = 0;
                 if (have && w < TW && w0 + w < W) {
-                    const unsigned long long m = cM[ci][w];
+                    const unsigned long long m = cM[ci * TW + w];
                     const int lim = tlast - 64 * (w0 + w);   // bits below tlast in this word
                     msk[w] = m;
                     inside += lim >= 64 ? (int)__popcll(m) : (lim > 0 ? (int)__popcll(m & ((1ull << lim) - 1ull)) : 0);
@@ -1641,10 +1657,10 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
             const int nxk = valid ? ni + 1 : 0;
             const int nit = valid ? nq * ni : 0;
             int pneed = need, pxk = nxk, pit = nit;
-            // inclusive prefix sums over the NS = 16 segments: one DPP row of lanes
-            row_scan16(pneed);
-            row_scan16(pxk);
-            row_scan16(pit);
+            // inclusive prefix sums over the NS segments (lanes 0 .. NS - 1)
+            seg_scan(pneed);
+            seg_scan(pxk);
+            seg_scan(pit);
             // leading directions whose records and knot positions fit (the first always does)
             const unsigned long long fit =
                 __ballot(have && (tid & 1) && pneed <= kEvalCap && pxk <= kEvalXkCap);
@@ -1659,8 +1675,10 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 sSegP[tid][2] = fac + kes * kFacPlanes * Ecap * 2;
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
-                    sCum[tid][w] = cum[w] - lo - 1;
-                    sMsk[tid][w] = msk[w];
+                    if (w < TW) {
+                        sCum[tid * TW + w] = cum[w] - lo - 1;
+                        sMsk[tid * TW + w] = msk[w];
+                    }
                 }
             }
             if (tid <= NS) sIp[tid] = tid < 2 * nb ? pit - nit : total;
@@ -1672,10 +1690,7 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
         }
         __syncthreads();
         const int nb = uniform(sNb);
-        int ip[NS + 1];
-#pragma unroll
-        for (int s = 0; s <= NS; ++s) ip[s] = uniform(sIp[s]);
-        const int total = ip[NS];
+        const int total = uniform(sIp[NS]);
         const int lwv = tid & 63;
         const int ipl = sIp[min(lwv, NS)];   // lane s: the first item of segment s
         // phase A: items (interval, quad of <= 4 channels), quad fastest.  The
@@ -1777,8 +1792,8 @@ __global__ __launch_bounds__(256, 4) void k_eval_fused(const EpochDesc* __restri
                 // both envelopes' flag words, knot counts and record offsets in one
                 // round of LDS reads; then the first interval (in the tile's window)
                 // and the crossing flags
-                const unsigned long long m0 = sMsk[2 * d][wi], m1 = sMsk[2 * d + 1][wi];
-                const int c0 = sCum[2 * d][wi], c1 = sCum[2 * d + 1][wi];
+                const unsigned long long m0 = sMsk[2 * d * TW + wi], m1 = sMsk[(2 * d + 1) * TW + wi];
+                const int c0 = sCum[2 * d * TW + wi], c1 = sCum[(2 * d + 1) * TW + wi];
                 const int4 B0 = sSegB[2 * d], B1 = sSegB[2 * d + 1];
                 __builtin_amdgcn_sched_barrier(0);
                 const unsigned long long h0 = m0 >> qsh, h1 = m1 >> qsh;
