@@ -1079,36 +1079,79 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
             const BlockGeom g = block_geom(E, b0 + slot);
             const int p = g.p, q = g.q, off = p - p0, len = q - p + 1;
             double* Yc = sY + off * np + c;   // Yc[i * np] = Y(p - 1 + i)
-            // forward sweep, z_i written over Y(p+i) once no read needs it
-            const bool nak = ed.spline && E >= 4;   // not-a-knot rows 1 and N = E-2
-            double ym = Yc[0], y0 = Yc[np], zp = 0.0;
-#pragma unroll 4
-            for (int i = 0; i < len; ++i) {
-                const int u = p + i;
-                const double yp = Yc[(i + 2) * np];
-                const double sl = (yp - y0) * s_ih[off + i + 1];
-                const double slm = (y0 - ym) * s_ih[off + i];
-                double r = 3.0 * (sl - slm);
+            // forward sweep, z_i written over Y(p+i) once no read needs it.  The first
+            // and last rows are peeled (the pivot of row 0, the not-a-knot rows u = 1 and
+            // u = N = E-2, which are only ever a block's first or last row), and each
+            // row's LDS inputs are read one row ahead: the tile, the reciprocals and
+            // the factors share one LDS array, so the compiler would not move a read
+            // above the previous row's store
+            const bool nak = ed.spline && E >= 4;
+            auto nak_scale = [&](int u, double r) {
                 if (nak) {
                     if (u == 1) r = r * (double)(X2 - X1);                 // h_1
                     else if (u == E - 2) r = r * (double)(XE2 - XE3);      // h_{N-1}
                 }
-                zp = (i == 0) ? r * s_rd[off] : fma(-s_lo[off + i], zp, r) * s_rd[off + i];
-                Yc[(i + 1) * np] = zp;
-                ym = y0;
-                y0 = yp;
+                return r;
+            };
+            double ym = Yc[0], y0 = Yc[np], yp = Yc[2 * np];
+            double zp;
+            {
+                const double sl = (yp - y0) * s_ih[off + 1];
+                const double slm = (y0 - ym) * s_ih[off];
+                zp = nak_scale(p, 3.0 * (sl - slm)) * s_rd[off];
+                Yc[np] = zp;
+            }
+            if (len > 1) {
+                // row i's inputs: yq = Y(p+i+1), ihq = ih of interval p+i, loq, rdq
+                double yq = Yc[3 * np], ihq = s_ih[off + 2], loq = s_lo[off + 1], rdq = s_rd[off + 1];
+                double ihp = s_ih[off + 1];
+#pragma unroll 2
+                for (int i = 1; i < len - 1; ++i) {
+                    const double yn = Yc[(i + 3) * np], ihn = s_ih[off + i + 2];
+                    const double lon = s_lo[off + i + 1], rdn = s_rd[off + i + 1];
+                    ym = y0;
+                    y0 = yp;
+                    yp = yq;
+                    const double sl = (yp - y0) * ihq;
+                    const double slm = (y0 - ym) * ihp;
+                    const double r = 3.0 * (sl - slm);
+                    zp = fma(-loq, zp, r) * rdq;
+                    Yc[(i + 1) * np] = zp;
+                    yq = yn;
+                    ihp = ihq;
+                    ihq = ihn;
+                    loq = lon;
+                    rdq = rdn;
+                }
+                {   // the last row
+                    const int i = len - 1;
+                    ym = y0;
+                    y0 = yp;
+                    yp = yq;
+                    const double sl = (yp - y0) * ihq;
+                    const double slm = (y0 - ym) * ihp;
+                    zp = fma(-loq, zp, nak_scale(p + i, 3.0 * (sl - slm))) * rdq;
+                    Yc[(i + 1) * np] = zp;
+                }
             }
             // backward sweep: each row's y goes straight to ysol (a wave's lanes are
             // consecutive channels of one row: coalesced), and the rows beside the
-            // separators to the separator inputs, separator-major for k_reduced
-            double x = 0.0;
+            // separators to the separator inputs, separator-major for k_reduced; the
+            // next row's z and factor are read one row ahead
             const int b = b0 + slot;
-#pragma unroll 4
-            for (int i = len - 1; i >= 0; --i) {
-                const double z = Yc[(i + 1) * np];
-                x = (i == len - 1) ? z : fma(-s_cp[off + i], x, z);
-                ys[(unsigned)(p + i) * (unsigned)np + (unsigned)c] = x;
-                if (i == len - 1 && b < J) sip[((size_t)(b + 1) * 3 + 0) * np + c] = x;   // row 32 (b+1) - 1
+            double x = zp;   // the last row: y = z
+            ys[(unsigned)(p + len - 1) * (unsigned)np + (unsigned)c] = x;
+            if (b < J) sip[((size_t)(b + 1) * 3 + 0) * np + c] = x;   // row 32 (b+1) - 1
+            if (len > 1) {
+                double zq = Yc[(len - 1) * np], cpq = s_cp[off + len - 2];
+#pragma unroll 2
+                for (int i = len - 2; i >= 0; --i) {
+                    const double zn = Yc[max(i, 0) * np], cpn = s_cp[off + max(i - 1, 0)];
+                    x = fma(-cpq, x, zq);
+                    ys[(unsigned)(p + i) * (unsigned)np + (unsigned)c] = x;
+                    zq = zn;
+                    cpq = cpn;
+                }
             }
             if (b >= 1) sip[((size_t)b * 3 + 2) * np + c] = x;   // row 32 b + 1
         }
