@@ -280,8 +280,10 @@ __device__ __forceinline__ void store_flags(const EpochDesc& ed, const double (&
     const int t = t0 + tid;
     const int w = (t0 >> 6) + wv;
     const int lim = window_limit(T, ed.window);
-    const bool endk = t == 0 || t == T - 1;
-    const bool test = t > 0 && t < T - 1 && t <= lim;
+    // the lanes' classes as wave masks (per direction only the comparisons are
+    // balloted, and the flag words are formed from masks in scalar registers)
+    const unsigned long long endk = __ballot(t == 0 || t == T - 1);
+    const unsigned long long test = __ballot(t > 0 && t < T - 1 && t <= lim);
     // global-address-space stores: a flat store would also hold up the LDS waits
     const GMut<unsigned long long> mk = gmut(ed.mask);
 #pragma unroll
@@ -297,10 +299,8 @@ __device__ __forceinline__ void store_flags(const EpochDesc& ed, const double (&
             const double ea = Px[d][wv][1], eb = Px[d][wv + 2][0];
             const double a = lane == 0 ? ea : sa;
             const double b = lane == 63 ? eb : sb;
-            const bool fmax = endk | (test & (p > a) & (p > b));
-            const bool fmin = endk | (test & (p < a) & (p < b));
-            const unsigned long long mmax = __ballot(fmax);
-            const unsigned long long mmin = __ballot(fmin);
+            const unsigned long long mmax = endk | (test & __ballot(p > a) & __ballot(p > b));
+            const unsigned long long mmin = endk | (test & __ballot(p < a) & __ballot(p < b));
             if (lane == 0 && w < W) {
                 mk[((size_t)k * 2 + 0) * W + w] = mmax;
                 mk[((size_t)k * 2 + 1) * W + w] = mmin;
