@@ -1105,7 +1105,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                 // row i's inputs: yq = Y(p+i+1), ihq = ih of interval p+i, loq, rdq
                 double yq = Yc[3 * np], ihq = s_ih[off + 2], loq = s_lo[off + 1], rdq = s_rd[off + 1];
                 double ihp = s_ih[off + 1];
-#pragma unroll 2
+#pragma unroll 4
                 for (int i = 1; i < len - 1; ++i) {
                     const double yn = Yc[(i + 3) * np], ihn = s_ih[off + i + 2];
                     const double lon = s_lo[off + i + 1], rdn = s_rd[off + i + 1];
@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
             if (b < J) sip[((size_t)(b + 1) * 3 + 0) * np + c] = x;   // row 32 (b+1) - 1
             if (len > 1) {
                 double zq = Yc[(len - 1) * np], cpq = s_cp[off + len - 2];
-#pragma unroll 2
+#pragma unroll 4
                 for (int i = len - 2; i >= 0; --i) {
                     const double zn = Yc[max(i, 0) * np], cpn = s_cp[off + max(i - 1, 0)];
                     x = fma(-cpq, x, zq);
