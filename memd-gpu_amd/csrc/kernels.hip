@@ -744,8 +744,17 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_factor(const EpochDesc* __restrict__ eps, WorkLists wl) {
     constexpr int L = kSolverBlock;
     __shared__ double s_rlo[NT], s_rdi[NT], s_rup[NT];
+    // the first NT blocks' boundary rows for the reduced system: the positions X[p-1],
+    // X[p] and X[q] and the (al, be) of rows p and q
+    __shared__ int s_bx[NT][3];
+    __shared__ v2d s_bab[NT][2];
     const ListCursor cu = list_cursor(wl, kListFactor);
     const int lane = threadIdx.x;
+    // A wave's loads and stores share one in-order completion counter, so a load
+    // issued after stores waits for them too: each phase's loads are issued before
+    // the previous phase's stores (the row phase one round ahead, the first block's
+    // positions before the row phase), and the reduced system takes the blocks'
+    // boundary rows from LDS instead of reloading what the block phase stored.
     for (int w = cu.start; w < cu.cnt; w += cu.stride) {
         const int4 it = cu.base[w];
         const EpochDesc& ed = eps[it.x];
@@ -757,29 +766,52 @@ __global__ __launch_bounds__(NT) void k_factor(const EpochDesc* __restrict__ eps
         const GMut<v2d> IH = gmut(reinterpret_cast<v2d*>(fac_plane(ed, ke, kFacIH)));   // (ih, i3h)
         const GMut<v2d> RC = gmut(reinterpret_cast<v2d*>(fac_plane(ed, ke, kFacRC)));   // (rd, cp)
         const int J = block_geom(E, 0).J;
+        // the lane's first block's positions X[p-1 .. q+1] (lanes past the last block
+        // load the last block's and do not use them)
+        int xs[L + 2];
+        {
+            const BlockGeom g = block_geom(E, min(lane, J));
+            const int len = g.q - g.p + 1;
+#pragma unroll
+            for (int i = 0; i < L + 2; ++i) xs[i] = X[g.p - 1 + min(i, len + 1)];
+        }
         // every row: the reciprocals of interval u (u < E-1) and, at the rows outside
-        // the blocks, al = be = 0; all lanes, 4 rows per lane per round with their
-        // loads issued together
-        for (int u0 = lane; u0 < E; u0 += NT * 4) {
+        // the blocks, al = be = 0; all lanes, 4 rows per lane per round, the next
+        // round's positions loaded (clamped in range) before this round's stores
+        {
             int xa[4], xb[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int u = min(u0 + NT * g, E - 2);
+                const int u = min(lane + NT * g, E - 2);
                 xa[g] = X[u];
                 xb[g] = X[u + 1];
             }
+            for (int u0 = lane; u0 < E; u0 += NT * 4) {
+                int na[4], nb[4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int u = u0 + NT * g;
-                if (u < E) {
-                    if (u < E - 1) {
-                        const double h = (double)(xb[g] - xa[g]);
-                        v2d v;
-                        v.x = 1.0 / h;
-                        v.y = 1.0 / (3.0 * h);
-                        IH[u] = v;
+                for (int g = 0; g < 4; ++g) {
+                    const int u = min(u0 + NT * (4 + g), E - 2);
+                    na[g] = X[u];
+                    nb[g] = X[u + 1];
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int u = u0 + NT * g;
+                    if (u < E) {
+                        if (u < E - 1) {
+                            const double h = (double)(xb[g] - xa[g]);
+                            v2d v;
+                            v.x = 1.0 / h;
+                            v.y = 1.0 / (3.0 * h);
+                            IH[u] = v;
+                        }
+                        if (outside_blocks(u, E, J)) AB[u] = v2d{0.0, 0.0};
                     }
-                    if (outside_blocks(u, E, J)) AB[u] = v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    xa[g] = na[g];
+                    xb[g] = nb[g];
                 }
             }
         }
@@ -787,12 +819,14 @@ __global__ __launch_bounds__(NT) void k_factor(const EpochDesc* __restrict__ eps
         for (int b = lane; b <= J; b += NT) {
             const BlockGeom g = block_geom(E, b);
             const int p = g.p, len = g.q - g.p + 1;
-            int xs[L + 2];   // X[p-1 .. q+1]
+            if (b != lane) {   // blocks past the first NT (long epochs)
 #pragma unroll
-            for (int i = 0; i < L + 2; ++i) xs[i] = X[p - 1 + min(i, len + 1)];
+                for (int i = 0; i < L + 2; ++i) xs[i] = X[p - 1 + min(i, len + 1)];
+            }
             const bool left = b >= 1, right = b < J;
             double cpr[L], zl[L];
             double cprev = 0.0, z = 0.0, zr = 0.0;
+            int xq = 0;   // X[q]
 #pragma unroll
             for (int i = 0; i < L; ++i) {
                 if (i < len) {
@@ -808,37 +842,67 @@ __global__ __launch_bounds__(NT) void k_factor(const EpochDesc* __restrict__ eps
                     zl[i] = z;
                     // right spike A_b^{-1}(up_q e_last): the forward sweep is +0 above the
                     // last row (every pivot is positive) and up_q * rd_q there
-                    if (i == len - 1) zr = up * r;
+                    if (i == len - 1) {
+                        zr = up * r;
+                        xq = xs[i + 1];
+                    }
                 }
             }
             // phase-3 coefficients: the spikes, 0 where the block has none (the
             // first block has no left spike, the last no right spike)
             double x = 0.0, y = 0.0;
+            v2d abq = v2d{0.0, 0.0}, abp = v2d{0.0, 0.0};   // (al, be) of rows q and p
 #pragma unroll
             for (int i = L - 1; i >= 0; --i) {
                 if (i < len) {
                     x = (i == len - 1) ? zl[i] : fma(-cpr[i], x, zl[i]);
                     y = (i == len - 1) ? zr : fma(-cpr[i], y, 0.0);
-                    AB[p + i] = v2d{left ? x : 0.0, right ? y : 0.0};
+                    const v2d ab = v2d{left ? x : 0.0, right ? y : 0.0};
+                    AB[p + i] = ab;
+                    if (i == len - 1) abq = ab;
+                    abp = ab;
                 }
+            }
+            if (b < NT) {
+                s_bx[b][0] = xs[0];
+                s_bx[b][1] = xs[1];
+                s_bx[b][2] = xq;
+                s_bab[b][0] = abp;
+                s_bab[b][1] = abq;
             }
         }
         __syncthreads();
         if (J > 0) {
             // reduced system: coefficients of every separator in parallel (LDS), then one
             // lane runs the factorisation chain
-            double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
-            double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
-            double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
+            // global-address-space stores (flat stores also count against the LDS wait
+            // counter that the chain's LDS reads wait on)
+            const GMut<double> Rlo = gmut(ed.red + ((size_t)ke * 3 + 0) * ed.nbmax);
+            const GMut<double> Rrd = gmut(ed.red + ((size_t)ke * 3 + 1) * ed.nbmax);
+            const GMut<double> Rcp = gmut(ed.red + ((size_t)ke * 3 + 2) * ed.nbmax);
             double cprev = 0.0;
             for (int j0 = 1; j0 <= J; j0 += NT) {
                 const int j = j0 + lane;
                 if (j <= J) {
+                    // separator j = row sp: the last row of block j-1 and the first of block j
                     const int sp = j * L;
-                    int xm = X[sp - 1], x0 = X[sp], xp = X[sp + 1];
+                    int xm, x0, xp;
+                    v2d abm, abp;
+                    if (j < NT) {
+                        xm = s_bx[j - 1][2];
+                        abm = s_bab[j - 1][1];
+                        x0 = s_bx[j][0];
+                        xp = s_bx[j][1];
+                        abp = s_bab[j][0];
+                    } else {
+                        xm = X[sp - 1];
+                        x0 = X[sp];
+                        xp = X[sp + 1];
+                        abm = AB[sp - 1];
+                        abp = AB[sp + 1];
+                    }
                     double lo = (double)(x0 - xm), up = (double)(xp - x0);
                     double di = (double)(2 * ((x0 - xm) + (xp - x0)));
-                    const v2d abm = AB[sp - 1], abp = AB[sp + 1];   // (al, be) beside the separator
                     s_rlo[lane] = (j >= 2) ? -(lo * abm.x) : 0.0;
                     s_rdi[lane] = fma(-up, abp.x, fma(-lo, abm.y, di));
                     s_rup[lane] = (j <= J - 1) ? -(up * abp.y) : 0.0;
