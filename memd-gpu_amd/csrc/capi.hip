@@ -169,7 +169,7 @@ EpochLayout layout_for(int n, int T, int K, int N_imf, const PathKind& pk) {
     L.ysol = take((size_t)K * 2 * np * Ecap * 8);
     L.coef = take(records ? (size_t)K * 2 * n * Ecap * 32 : 0);   // the record path's array only
     L.sep = take((size_t)K * 2 * np * (nb + 1) * 8);
-    L.sepin = take((size_t)K * 2 * nb * 3 * np * 8);
+    L.sepin = take((size_t)K * 2 * nb * np * 8);
     L.mean = take((size_t)n * T * 8);
     L.ampb = take(pk.ampb ? (size_t)K * ((n + 15) / 16) * T * 8 : 0);
     L.tstat = take((size_t)T * 16);

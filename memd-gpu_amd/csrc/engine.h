@@ -143,7 +143,8 @@ struct EpochDesc {
                            //   not-a-knot end value) at the end rows (i0 = i1 = 0 at both)
     double* coef;          // [K][2][n][Ecap][4] spline record (a,b,c,d) per interval
     double* sep;           // [K][2][nbmax + 1][np] Z: Z[0] = Z[J+1] = 0, Z[j] = separator value j
-    double* sepin;         // [K][2][nbmax][3][np] y before, right-hand side, y after each separator
+    double* sepin;         // [K][2][nbmax][np] right-hand side of each separator's row (the y^ rows
+                           //   beside it are read from ysol)
     double* mean;          // [n][T] stop mode: envelope mean
     double* ampb;          // [K][ceil(n/16)][T] stop mode, fused evaluation: per valid direction and
                            //   block of 16 channels the sum of (e+ - e-)^2 (groups of 4 in order)

@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
         const double* curt = ed.curt;   // [T][np] current signal, sample-major
         const GMut<double> ye = gmut(ed.yend + (size_t)ke * 2 * np);   // [2][np]: Y_0, Y_{E-1} clamped
         const GMut<double> ys = gmut(ed.ysol + (size_t)ke * ed.Ecap * np);
-        const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * 3 * np);   // [separator][3][np]
+        const GMut<double> sip = gmut(ed.sepin + (size_t)ke * ed.nbmax * np);   // [separator][np]: right-hand side
         __syncthreads();   // the previous item's tile is no longer read
         // 1a. knot positions of tile rows 0..RW-1, factors of rows p0..qlast, reciprocals of
         //     intervals p0-1..qlast+1 (the last one: the next separator's right-hand side;
@@ -1135,7 +1135,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
             const double* Yc = sY + off * np + c;   // Yc[i * np] = Y(p - 1 + i)
             const double sl = (Yc[(len + 2) * np] - Yc[(len + 1) * np]) * s_ih[off + len + 1];
             const double slm = (Yc[(len + 1) * np] - Yc[len * np]) * s_ih[off + len];
-            sip[((size_t)(b0 + slot + 1) * 3 + 1) * np + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
+            sip[(size_t)(b0 + slot + 1) * np + c] = 3.0 * (sl - slm);   // r of separator j = b + 1
         }
         __syncthreads();
         for (int lane = tid; lane < n * nb; lane += kBlockThreads) {
@@ -1199,13 +1199,11 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                 }
             }
             // backward sweep: each row's y goes straight to ysol (a wave's lanes are
-            // consecutive channels of one row: coalesced), and the rows beside the
-            // separators to the separator inputs, separator-major for k_reduced; the
-            // next row's z and factor are read one row ahead
-            const int b = b0 + slot;
+            // consecutive channels of one row: coalesced; k_reduced reads the rows beside
+            // the separators from there); the next row's z and factor are read one row
+            // ahead
             double x = zp;   // the last row: y = z
             ys[(unsigned)(p + len - 1) * (unsigned)np + (unsigned)c] = x;
-            if (b < J) sip[((size_t)(b + 1) * 3 + 0) * np + c] = x;   // row 32 (b+1) - 1
             if (len > 1) {
                 double zq = Yc[(len - 1) * np], cpq = s_cp[off + len - 2];
 #pragma unroll 4
@@ -1217,7 +1215,6 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_block(const EpochDesc* __r
                     cpq = cpn;
                 }
             }
-            if (b >= 1) sip[((size_t)b * 3 + 2) * np + c] = x;   // row 32 b + 1
         }
         it = nit;
     }
@@ -1275,7 +1272,7 @@ __global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__
         const double* Rlo = ed.red + ((size_t)ke * 3 + 0) * ed.nbmax;
         const double* Rrd = ed.red + ((size_t)ke * 3 + 1) * ed.nbmax;
         const double* Rcp = ed.red + ((size_t)ke * 3 + 2) * ed.nbmax;
-        const double* sip = ed.sepin + (size_t)ke * ed.nbmax * 3 * np;   // [j][3][np]: y before, r, y after
+        const double* sip = ed.sepin + (size_t)ke * ed.nbmax * np;   // [j][np]: right-hand side of separator j
         double* sv = zsep_base(ed, ke);                                    // [j][np]: Z[j]
         double* row = ed.ysol + (size_t)ke * ed.Ecap * np;                 // [u][np]: y^ (separator rows)
         const GPtr<int> Xg = gptr(X);
@@ -1287,14 +1284,25 @@ __global__ __launch_bounds__(64, 2) void k_reduced(const EpochDesc* __restrict__
         for (int j0 = 1; j0 <= J; j0 += CH) {
             const int cnt = min(CH, J - j0 + 1);
             __syncthreads();   // the previous chunk is no longer read
-            {
-                const double* src = sip + (size_t)3 * j0 * np;
-                const int pieces = cnt * 3 * np / 2;   // np is a multiple of 4
+            {   // per separator j the three rows [y^ of row 32 j - 1, right-hand side, y^ of
+                // row 32 j + 1] (the block solution beside the separator, from ysol), 16
+                // bytes (two channels of one row) per lane; np is a multiple of 4
+                const int rowd = 3 * np;
+                const unsigned long long rmagic = (0x100000000ull + (unsigned)rowd - 1) / (unsigned)rowd;
+                const int pieces = cnt * rowd / 2;
                 for (int q0 = 0; q0 < pieces; q0 += 64) {
                     const int q = q0 + lane;
-                    if (q < pieces)
-                        __builtin_amdgcn_global_load_lds(src + 2 * q, (__attribute__((address_space(3))) void*)(sIn + 2 * q0),
-                                                         16, 0, 0);
+                    if (q < pieces) {
+                        const int f = 2 * q;
+                        const int jj = (int)(((unsigned long long)f * rmagic) >> 32);
+                        const int rem = f - jj * rowd;
+                        const int comp = rem >= 2 * np ? 2 : (rem >= np ? 1 : 0);
+                        const int c2 = rem - comp * np;
+                        const int j = j0 + jj;
+                        const double* src = comp == 1 ? sip + (size_t)j * np + c2
+                                                      : row + (size_t)(j * L + comp - 1) * np + c2;
+                        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(sIn + 2 * q0), 16, 0, 0);
+                    }
                 }
             }
             if (lane < cnt) {
