@@ -607,18 +607,42 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
         }
         const int Etot = __shfl(incl, 63);
         int base = incl - mine;
-        for (int w = w0; w < w1; ++w) {
-            unsigned long long bits = m[w];
-            pre[w] = base;
-            while (bits) {
-                const int b = __ffsll((long long)bits) - 1;
-                const int pos = (w << 6) + b;
-                if (mirror) {
-                    if (base >= 1 && base <= 3) sEnds[env][base - 1] = pos;
-                    if (base >= Etot - 4 && base <= Etot - 2) sEnds[env][3 + base - (Etot - 4)] = pos;
+        if (per == 1 && Etot > 8 * 64) {
+            // dense envelope, one word per lane: the wave takes the words one at a time,
+            // lane l storing the position of bit l, so consecutive knots come from
+            // consecutive lanes (coalesced stores instead of one scattered store per
+            // knot and lane)
+            if (lane < W) pre[lane] = base;
+            const unsigned long long mw = lane < W ? m[lane] : 0ull;
+            for (int w = 0; w < W; ++w) {
+                const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)mw, w);
+                const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(mw >> 32), w);
+                const int bw = __builtin_amdgcn_readlane(base, w);
+                const unsigned long long bits = ((unsigned long long)hi << 32) | lo;
+                if ((bits >> lane) & 1ull) {
+                    const int at = bw + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                    const int pos = (w << 6) + lane;
+                    if (mirror) {
+                        if (at >= 1 && at <= 3) sEnds[env][at - 1] = pos;
+                        if (at >= Etot - 4 && at <= Etot - 2) sEnds[env][3 + at - (Etot - 4)] = pos;
+                    }
+                    X[at] = pos;
                 }
-                X[base++] = pos;
-                bits &= bits - 1;
+            }
+        } else {
+            for (int w = w0; w < w1; ++w) {
+                unsigned long long bits = m[w];
+                pre[w] = base;
+                while (bits) {
+                    const int b = __ffsll((long long)bits) - 1;
+                    const int pos = (w << 6) + b;
+                    if (mirror) {
+                        if (base >= 1 && base <= 3) sEnds[env][base - 1] = pos;
+                        if (base >= Etot - 4 && base <= Etot - 2) sEnds[env][3 + base - (Etot - 4)] = pos;
+                    }
+                    X[base++] = pos;
+                    bits &= bits - 1;
+                }
             }
         }
         if (lane == 63) {
