@@ -607,26 +607,45 @@ __global__ __launch_bounds__(128) void k_compact(const EpochDesc* __restrict__ e
         }
         const int Etot = __shfl(incl, 63);
         int base = incl - mine;
-        if (per == 1 && Etot > 8 * 64) {
-            // dense envelope, one word per lane: the wave takes the words one at a time,
-            // lane l storing the position of bit l, so consecutive knots come from
-            // consecutive lanes (coalesced stores instead of one scattered store per
-            // knot and lane)
-            if (lane < W) pre[lane] = base;
-            const unsigned long long mw = lane < W ? m[lane] : 0ull;
-            for (int w = 0; w < W; ++w) {
-                const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)mw, w);
-                const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(mw >> 32), w);
-                const int bw = __builtin_amdgcn_readlane(base, w);
-                const unsigned long long bits = ((unsigned long long)hi << 32) | lo;
-                if ((bits >> lane) & 1ull) {
-                    const int at = bw + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-                    const int pos = (w << 6) + lane;
-                    if (mirror) {
-                        if (at >= 1 && at <= 3) sEnds[env][at - 1] = pos;
-                        if (at >= Etot - 4 && at <= Etot - 2) sEnds[env][3 + at - (Etot - 4)] = pos;
+        if (per <= 2 && Etot > 8 * W) {
+            // dense envelope (more than 8 knots per flag word) of at most 2 words per lane:
+            // the wave takes the words one at a time, lane l storing the position of bit
+            // l, so consecutive knots come from consecutive lanes (coalesced stores instead
+            // of one scattered store per knot and lane).  Longer epochs keep the per-lane
+            // walk: a wave then runs W word steps, which cost more than the scattered
+            // stores save when few epochs share the device
+            unsigned long long mw[2];
+            int bs[2];
+            int run = base;
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int w = w0 + sl;
+                const bool own = sl < per && w < W;
+                mw[sl] = own ? m[w] : 0ull;
+                bs[sl] = run;
+                if (own) pre[w] = run;
+                run += (int)__popcll(mw[sl]);
+            }
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                if (sl < per) {   // uniform
+                    for (int src = 0; src < 64; ++src) {
+                        const int w = src * per + sl;
+                        if (w >= W) break;   // uniform; later lanes' words are past W too
+                        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)mw[sl], src);
+                        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(mw[sl] >> 32), src);
+                        const int bw = __builtin_amdgcn_readlane(bs[sl], src);
+                        const unsigned long long bits = ((unsigned long long)hi << 32) | lo;
+                        if ((bits >> lane) & 1ull) {
+                            const int at = bw + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                            const int pos = (w << 6) + lane;
+                            if (mirror) {
+                                if (at >= 1 && at <= 3) sEnds[env][at - 1] = pos;
+                                if (at >= Etot - 4 && at <= Etot - 2) sEnds[env][3 + at - (Etot - 4)] = pos;
+                            }
+                            X[at] = pos;
+                        }
                     }
-                    X[at] = pos;
                 }
             }
         } else {
